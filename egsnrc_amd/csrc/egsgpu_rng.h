@@ -1,0 +1,186 @@
+// egsgpu_rng.h -- device random number generators (gfx950).
+//
+// ranmar: the Marsaglia-Zaman generator of EGS_Ranmar
+//         (egs++/egs_rndm.cpp:303-398, src/ranmar.mortran), one stream per
+//         lane.  The 97-word lag table lives in HBM (one 400 B record per
+//         stream, so a refill touches two short contiguous runs of it);
+//         the outputs are produced RNG_NB at a time into an LDS ring, the
+//         device analogue of the 128-entry rng_array of $RANDOMSET
+//         (interface/egs_c_interface2.macros:73-80).
+// ranlux: Luescher's generator as in src/ranlux.mortran; the 24-word state
+//         is kept rotated so that i24 is always word 0 and the per-call
+//         update is a fully unrolled register computation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace egsgpu {
+
+// Layout of one ranmar stream record (int32 words).
+struct RanmarRec {
+    int u[97];
+    int c, ix, jx;
+};
+static_assert(sizeof(RanmarRec) == 400, "ranmar record is 400 bytes");
+
+// Seed pair of stream s (createRNG(input, s) for jxx+s <= 30080,
+// egs_rndm.cpp:438-446; the excess carries into ixx).
+__host__ __device__ inline void stream_seeds(int ixx, int jxx, int64_t s,
+                                             int &is, int &js) {
+    if (ixx <= 0 || ixx >= 31328) ixx = 1802;
+    if (jxx <= 0 || jxx >= 30081) jxx = 9373;
+    int64_t t = (int64_t)jxx + s;
+    int64_t carry = (t - 1) / 30080;
+    js = (int)(1 + (t - 1) % 30080);
+    is = (int)(1 + ((int64_t)ixx - 1 + carry) % 31327);
+}
+
+// EGS_Ranmar::setState (egs_rndm.cpp:303-336).
+__device__ inline void ranmar_seed(RanmarRec *r, int ixx, int jxx) {
+    if (ixx <= 0 || ixx >= 31328) ixx = 1802;
+    if (jxx <= 0 || jxx >= 30081) jxx = 9373;
+    int i = (ixx / 177) % 177 + 2;
+    int j = ixx % 177 + 2;
+    int k = (jxx / 169) % 178 + 1;
+    int l = jxx % 169;
+    for (int ii = 0; ii < 97; ii++) {
+        int s = 0, t = 8388608;
+        for (int jj = 0; jj < 24; jj++) {
+            int m = (((i * j) % 179) * k) % 179;
+            i = j; j = k; k = m;
+            l = (53 * l + 1) % 169;
+            if ((l * m) % 64 >= 32) s += t;
+            t >>= 1;
+        }
+        r->u[ii] = s;
+    }
+    r->c = 362436;
+    r->ix = 96;
+    r->jx = 32;
+}
+
+// Produce NB consecutive outputs of the stream (EGS_Ranmar::fillArray,
+// egs_rndm.cpp:338-370) as exact floats r*2^-24 into buf[k*stride].
+template <int NB>
+__device__ inline void ranmar_refill(RanmarRec *__restrict__ r,
+                                     float *__restrict__ buf, int stride) {
+    int c = r->c, ix = r->ix, jx = r->jx;
+    const float twom24 = 1.0f / 16777216.0f;
+#pragma unroll 4
+    for (int k = 0; k < NB; k++) {
+        int v = r->u[ix] - r->u[jx];
+        v += (v < 0) ? 16777216 : 0;
+        r->u[ix] = v;
+        ix = (ix == 0) ? 96 : ix - 1;
+        jx = (jx == 0) ? 96 : jx - 1;
+        c -= 7654321;
+        c += (c < 0) ? 16777213 : 0;
+        v -= c;
+        v += (v < 0) ? 16777216 : 0;
+        buf[k * stride] = (float)v * twom24;
+    }
+    r->c = c; r->ix = ix; r->jx = jx;
+}
+
+// ---- ranlux ------------------------------------------------------------
+// Stream record: w[k] = seeds((i24-1-k) mod 24 + 1) (Fortran indices), so
+// that the lag-14 partner of word k is word (k+14) mod 24 and the word
+// read for the low bits after step k is (k+15) mod 24.
+struct RanluxRec {
+    int w[24];
+    int carry;
+    int lux;
+    int pad[2];
+};
+static_assert(sizeof(RanluxRec) == 112, "ranlux record is 112 bytes");
+
+// init_ranlux (ranlux.mortran): i24 = 24 at start, so w[k] = seeds(24-k).
+__device__ inline void ranlux_seed(RanluxRec *r, int luxury, int seed) {
+    int jseed = seed <= 0 ? 314159265 : seed;
+    if (luxury < 0 || luxury > 4) luxury = 1;
+    int seeds[24];
+    for (int j = 0; j < 24; j++) {
+        int k = jseed / 53668;
+        jseed = 40014 * (jseed - k * 53668) - k * 12211;
+        if (jseed < 0) jseed += 2147483563;
+        seeds[j] = jseed % 16777216;
+    }
+    for (int k = 0; k < 24; k++) r->w[k] = seeds[23 - k];
+    r->carry = seeds[23] == 0 ? 1 : 0;
+    r->lux = luxury;
+}
+
+// Steps k0..k0+23 of the subtract-with-borrow recurrence on the rotated
+// registers s[]; OUT selects whether outputs are written.
+template <bool OUT>
+__device__ inline void ranlux_block(int (&s)[24], int &carry,
+                                    float *__restrict__ buf, int stride) {
+    const float twom24 = 1.0f / 16777216.0f;
+#pragma unroll
+    for (int k = 0; k < 24; k++) {
+        int uni = s[(k + 14) % 24] - s[k] - carry;
+        carry = uni < 0 ? 1 : 0;
+        uni += uni < 0 ? 16777216 : 0;
+        s[k] = uni;
+        if (OUT) {
+            float v;
+            if (uni >= 4096) {
+                v = (float)uni * twom24;
+            } else {
+                float a = (float)uni * twom24;
+                float b = (float)s[(k + 15) % 24] * twom24 * twom24;
+                v = __fadd_rn(a, b);
+            }
+            buf[k * stride] = v;
+        }
+    }
+}
+
+template <int REM>
+__device__ inline void ranlux_tail(int (&s)[24], int &carry) {
+#pragma unroll
+    for (int k = 0; k < REM; k++) {
+        int uni = s[(k + 14) % 24] - s[k] - carry;
+        carry = uni < 0 ? 1 : 0;
+        uni += uni < 0 ? 16777216 : 0;
+        s[k] = uni;
+    }
+    // rotate so that the next i24 is word 0 again
+    int t[24];
+#pragma unroll
+    for (int k = 0; k < 24; k++) t[k] = s[(k + REM) % 24];
+#pragma unroll
+    for (int k = 0; k < 24; k++) s[k] = t[k];
+}
+
+// ranlux(rng_array): 24 outputs then nskip = {0,24,73,199,365}[lux]
+// further steps (ranlux.mortran).  nskip = 24*q + rem.
+__device__ inline void ranlux_refill(RanluxRec *__restrict__ r,
+                                     float *__restrict__ buf, int stride) {
+    int s[24];
+#pragma unroll
+    for (int k = 0; k < 24; k++) s[k] = r->w[k];
+    int carry = r->carry;
+    ranlux_block<true>(s, carry, buf, stride);
+    switch (r->lux) {
+        case 0: break;
+        case 1: ranlux_block<false>(s, carry, buf, stride); break;
+        case 2:
+            for (int q = 0; q < 3; q++) ranlux_block<false>(s, carry, buf, stride);
+            ranlux_tail<1>(s, carry);
+            break;
+        case 3:
+            for (int q = 0; q < 8; q++) ranlux_block<false>(s, carry, buf, stride);
+            ranlux_tail<7>(s, carry);
+            break;
+        default:
+            for (int q = 0; q < 15; q++) ranlux_block<false>(s, carry, buf, stride);
+            ranlux_tail<5>(s, carry);
+            break;
+    }
+#pragma unroll
+    for (int k = 0; k < 24; k++) r->w[k] = s[k];
+    r->carry = carry;
+}
+
+}  // namespace egsgpu

@@ -1,0 +1,185 @@
+/*
+ * egsgpu.h -- C ABI of the MI355X (gfx950) voxel transport engine.
+ *
+ * The engine replaces the EGSnrc shower loop reached through egsShower()
+ * (HEN_HOUSE/interface/egs_interface2.h:761-771, egs_interface2.c:92-99)
+ * together with the per-step egs++ callbacks it makes into
+ * EGS_XYZGeometry::howfar/hownear/isWhere (egs++/geometry/egs_nd_geometry/
+ * egs_nd_geometry.h:791,963,1106), EGS_DoseScoring::processEvent
+ * (egs++/ausgab_objects/egs_dose_scoring/egs_dose_scoring.h:203) and the
+ * RNG refill egsFillRandomArray (egs++/egs_advanced_application.cpp:1405).
+ *
+ * Conventions
+ *  - every function returns int status: 0 = OK, < 0 = error code;
+ *    egsgpu_last_error() gives the text of the last error of the calling
+ *    thread.  Nothing in the library calls exit().
+ *  - all pointers are host pointers to caller-owned memory unless stated;
+ *    the handle owns its device memory; tables/geometry/source are copied
+ *    at create time.
+ *  - one handle per host thread; calls on one handle are not re-entrant.
+ *  - region index = ix + iy*nx + iz*nx*ny (x fastest), as in
+ *    EGS_XYZGeometry::isWhere (egs_nd_geometry.h:791-804).
+ */
+#ifndef EGSGPU_H
+#define EGSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define EGSGPU_OK              0
+#define EGSGPU_E_ARG          -1   /* invalid argument                   */
+#define EGSGPU_E_HIP          -2   /* HIP runtime error                  */
+#define EGSGPU_E_IO           -3   /* file could not be read/written     */
+#define EGSGPU_E_FORMAT       -4   /* malformed table blob / input       */
+#define EGSGPU_E_UNSUPPORTED  -5   /* option not supported by the engine */
+#define EGSGPU_E_OVERFLOW     -6   /* particle stack / tally overflow    */
+#define EGSGPU_E_NODEVICE     -7   /* no gfx950 device                   */
+
+/* ---- geometry: EGS_XYZGeometry (egs_nd_geometry.h:769-1381) ---------- */
+typedef struct EgsXYZGeometry {
+    int32_t nx, ny, nz;          /* voxels per axis                        */
+    const double *xb;            /* nx+1 increasing plane positions (cm)   */
+    const double *yb;            /* ny+1                                   */
+    const double *zb;            /* nz+1                                   */
+    const int32_t *medium;       /* nx*ny*nz medium indices (0-based, into
+                                    the media list of the table blob)      */
+    const double *rhor;          /* nx*ny*nz relative density
+                                    (EGS_BaseGeometry::setRelativeRho,
+                                    egs_base_geometry.cpp:813-860); NULL=1 */
+    double boundary_tolerance;   /* EGS_BaseGeometry boundaryTolerance;
+                                    <=0 selects distanceEpsilon
+                                    (egs_functions.h:80-87)                */
+} EgsXYZGeometry;
+
+/* ---- source: EGS_ParallelBeam + spectrum (egs_parallel_beam.h:150,
+ *      egs_spectra.cpp:74-100,374-435, egs_alias_table.cpp:268-300) ----- */
+#define EGSGPU_SPEC_MONO       0   /* EGS_MonoEnergy                       */
+#define EGSGPU_SPEC_HISTOGRAM  1   /* tabulated spectrum type 1 (per MeV)  */
+#define EGSGPU_SPEC_LINE       2   /* tabulated spectrum type 0 (lines)    */
+#define EGSGPU_SHAPE_POINT     0   /* EGS_PointShape                       */
+#define EGSGPU_SHAPE_RECTANGLE 1   /* EGS_RectangleShape (z = shape_z)     */
+typedef struct EgsSource {
+    int32_t charge;              /* 0 photon, -1 electron, +1 positron     */
+    int32_t spectrum;            /* EGSGPU_SPEC_*                          */
+    double energy;               /* kinetic energy for MONO (MeV)          */
+    int32_t nbins;               /* HISTOGRAM: bins; LINE: lines           */
+    const double *spec_e;        /* HISTOGRAM: nbins+1 edges; LINE: nbins  */
+    const double *spec_w;        /* nbins probabilities                    */
+    int32_t shape;               /* EGSGPU_SHAPE_*                         */
+    double x0, x1, y0, y1, z;    /* point = (x0,y0,z); rect [x0,x1]x[y0,y1]*/
+    double ux, uy, uz;           /* beam direction (normalised)            */
+} EgsSource;
+
+/* ---- transport parameters (":start MC transport parameter:",
+ *      egs_advanced_application.cpp:482-639) --------------------------- */
+typedef struct EgsTransport {
+    double ecut;                 /* total energy (MeV); <=0 -> max AE      */
+    double pcut;                 /* MeV; <=0 -> max AP                     */
+    int32_t rayleigh;            /* IRAYLR (egsnrc.macros:2299)            */
+    int32_t bound_compton;       /* IBCMP 0 KN, 3 norej (default)          */
+    int32_t photoelectron_angle; /* IPHTER                                 */
+    int32_t atomic_relaxation;   /* IEDGFL                                 */
+    int32_t pair_angle;          /* IPRDST 0..2                            */
+    int32_t brems_angle;         /* IBRDST 0/1                             */
+    int32_t spin;                /* spin effects (needs spinms data)       */
+    int32_t exact_bca;           /* 1 exact boundary crossing (default)    */
+    double estepe;               /* ESTEPE (default 0.25)                  */
+    double ximax;                /* XIMAX (default 0.5)                    */
+    double skin_depth;           /* skindepth_for_bca (default 3)          */
+    int32_t kerma;               /* 1: deposit every electron locally      */
+    int32_t fp64;                /* 1: run the FP64 build of the kernels   */
+} EgsTransport;
+
+/* ---- tables ----------------------------------------------------------- */
+typedef struct egsgpu_tables egsgpu_tables;
+typedef struct egsgpu_handle egsgpu_handle;
+
+/* Text of the last error on this thread (always NUL terminated). */
+int egsgpu_last_error(char *buf, size_t n);
+
+/* Number of visible gfx950 devices (0 when none). */
+int egsgpu_device_count(int32_t *n);
+
+/* Load a cross-section table blob written by the table builder
+ * (PEGS4 + XCOM + msnew data prepared at initialisation, the state HATCH
+ * leaves behind: egsnrc.mortran:1828-2320). */
+int egsgpu_tables_load(const char *path, egsgpu_tables **out);
+int egsgpu_tables_free(egsgpu_tables *t);
+int egsgpu_tables_nmedia(const egsgpu_tables *t, int32_t *n);
+
+/* ---- run control ------------------------------------------------------ */
+/* Create a simulation: EGS_AdvancedApplication::initSimulation analogue
+ * (egs_application.cpp:753-802).  Nothing runs yet. */
+int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
+                  const EgsSource *src, const EgsTransport *tp,
+                  egsgpu_handle **out);
+int egsgpu_destroy(egsgpu_handle *h);
+
+/* Launch shape: number of independent RNG streams ("parallel jobs").
+ * Stream s is seeded like EGS_RandomGenerator::createRNG(input, s):
+ * ranmar (ixx, jxx + s) (egs_rndm.cpp:409-463).  0 = engine default. */
+int egsgpu_set_streams(egsgpu_handle *h, int32_t nstreams,
+                       int32_t block_size);
+
+/* Seed all streams: ranmar initial seeds (ixx, jxx).  Re-seeding resets
+ * the streams; tallies are kept. */
+int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx);
+
+/* Simulate nhist further primary histories (runSimulation,
+ * egs_application.cpp:839-917).  Histories are dealt round-robin to the
+ * streams; tallies accumulate history-by-history (egs_scoring.h:63-125). */
+int egsgpu_run(egsgpu_handle *h, int64_t nhist);
+
+/* Per-voxel FP64 sums of energy deposited (MeV, weight included) and of
+ * its per-history squares, plus the number of histories so far.  Arrays
+ * of nx*ny*nz, caller-owned. */
+int egsgpu_get_tallies(egsgpu_handle *h, double *sum, double *sum2,
+                       int64_t *nhist);
+int egsgpu_reset_tallies(egsgpu_handle *h);
+
+/* Engine counters: [0] photon steps, [1] voxel crossings, [2] deposits,
+ * [3] history flushes (voxel,history pairs), [4] electron steps,
+ * [5] stack overflows, [6] tally-table overflows, [7] reserved. */
+int egsgpu_get_counters(egsgpu_handle *h, int64_t counters[8]);
+
+/* Average duration (ms) of the transport kernel launches of the last
+ * egsgpu_run, measured with HIP events on the launch stream. */
+int egsgpu_last_kernel_ms(egsgpu_handle *h, double *ms, int32_t *launches);
+
+/* ---- test hooks -------------------------------------------------------- */
+/* Fill out[s*n + i] with the first n numbers of stream s of ranmar seeded
+ * (ixx, jxx + s), s < nstreams, computed on the device
+ * (EGS_Ranmar::setState/fillArray, egs_rndm.cpp:303-398). */
+int egsgpu_ranmar_fill(int32_t ixx, int32_t jxx, int32_t nstreams,
+                       int64_t n, double *out);
+
+/* Same for ranlux (ranlux.mortran) at luxury level lux with stream s seeded
+ * with seed + s. */
+int egsgpu_ranlux_fill(int32_t lux, int32_t seed, int32_t nstreams,
+                       int64_t n, double *out);
+
+/* Trace rays through the XYZ geometry on the device: starting at x, u
+ * (3 doubles each per ray) find the region (isWhere, or howfar from
+ * outside), then step with howfar(t = 1e30) until the ray leaves.  For ray
+ * r, regions[r*max_steps+k] is the k-th region entered (-1 = left) and
+ * steps[r*max_steps+k] the step length to it; nsteps[r] the number of
+ * entries.  precision is 32 or 64 (the FP32 transport build or the FP64
+ * build of the same code). */
+int egsgpu_trace_rays(const EgsXYZGeometry *g, int32_t nrays,
+                      const double *x, const double *u, int32_t max_steps,
+                      int32_t precision, int32_t *regions, double *steps,
+                      int32_t *nsteps);
+
+/* hownear for nrays points with known regions (egs_nd_geometry.h:1106). */
+int egsgpu_hownear(const EgsXYZGeometry *g, int32_t n, const double *x,
+                   const int32_t *ireg, int32_t precision, double *tperp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EGSGPU_H */
