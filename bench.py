@@ -1,0 +1,152 @@
+"""Benchmark of the MI355X voxel transport engine.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one pass of the transport path over one batch of primary
+histories, with tables, phantom and source already resident in HBM.  The
+timed region is K steps between two device synchronisations.  Prints one
+JSON line with histories/s, the efficiency 1/(sigma^2 T), the roofline of
+the transport kernel and a CPU baseline (the FP64 oracle run on this host's
+cores over a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # photon-only kerma phantom: 64^3 x 4 mm, density field in [0.2, 2.0],
+    # 2 MeV 8x8 cm^2 parallel beam, ECUT 20 MeV (electrons deposited locally)
+    "kerma64": dict(blob="521icru.egstab", hist_per_step=4_000_000),
+}
+
+
+def build_workload(name):
+    from egsnrc_amd import _lib
+    from egsnrc_amd.engine import make_source
+    from tests import workloads
+    if name == "kerma64":
+        xb, yb, zb, med, rhor = workloads.kerma_phantom(64, 0.4)
+        src = make_source(0, 2.0, x0=-4, x1=4, y0=-4, y1=4, z=-1.0)
+        tp = _lib.default_transport(ecut=20.0, kerma=1)
+        cfg = {"workload": "kerma64: 64^3 x 0.4 cm water, density GRF [0.2,2.0] g/cm3, "
+                           "2 MeV 8x8 cm2 parallel photon beam, ECUT 20 MeV (kerma), PCUT 0.01",
+               "voxels": 64 ** 3}
+        return xb, yb, zb, med, rhor, src, tp, cfg
+    raise SystemExit("unknown workload " + name)
+
+
+def efficiency(sum_, sum2, n, seconds):
+    """1/(sigma^2 T): sigma^2 = mean relative variance over voxels with
+    D > 0.5 Dmax."""
+    r = sum_ / n
+    v = sum2 / n - r * r
+    v = np.where(v > 0, v / (n - 1), 0.0)
+    m = r > 0.5 * r.max()
+    rel2 = float(np.mean(v[m] / (r[m] * r[m])))
+    return 1.0 / (rel2 * seconds), rel2
+
+
+def cpu_baseline(name, seconds_target=15.0):
+    """The FP64 oracle on the host cores, run on a bounded sample."""
+    from egsnrc_amd import _lib
+    from tests import oracle_lib
+    xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
+    g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
+    cores = min(os.cpu_count() or 1, 16)
+    blob = os.path.join(ROOT, "egsnrc_amd", "data", WORKLOADS[name]["blob"])
+    o = oracle_lib.OracleSimulation(blob, g, src[0], tp, nstreams=4 * cores, nthreads=cores)
+    n = 20000
+    t = time.perf_counter()
+    o.run(n)
+    dt = time.perf_counter() - t
+    # scale the sample to about seconds_target of CPU time
+    n2 = int(min(5e7, max(n, n * seconds_target / max(dt, 1e-3))))
+    t = time.perf_counter()
+    o.run(n2)
+    dt = time.perf_counter() - t
+    return {"value": n2 / dt, "unit": "histories/s", "cores": cores, "kind": "port",
+            "sample": "%d histories of the same workload (FP64 oracle, %d threads, %.1f s)"
+                      % (n2, cores, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="kerma64")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(histories would shard across GPUs with one all-reduce of the tallies)",
+              file=sys.stderr)
+        sys.exit(2)
+    from egsnrc_amd.engine import Simulation, Tables
+    name = a.workload
+    xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
+    W = WORKLOADS[name]
+    tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", W["blob"]))
+    sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp)
+    hps = W["hist_per_step"]
+    for _ in range(a.warmup):
+        sim.run(hps)
+    from egsnrc_amd import _lib
+    _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
+    c0 = sim.counters()
+    kms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sim.run(hps)
+        kms += sim.kernel_ms()[0]
+    t1 = time.perf_counter()
+    T = t1 - t0
+    c1 = sim.counters()
+    s, s2, n = sim.tallies()
+    hist_s = a.steps * hps / T
+    eff, rel2 = efficiency(s, s2, n, T)
+    dc = c1 - c0
+    # algorithmic HBM bytes: 5 B per voxel crossing, 16 B per deposit,
+    # 32 B per (history, voxel) flush
+    alg_bytes = 5.0 * dc[1] + 16.0 * dc[2] + 32.0 * dc[3]
+    kernel_s = kms / 1e3
+    achieved = alg_bytes / kernel_s / 1e9 / a.steps * a.steps
+    out = {
+        "metric": "histories/s",
+        "value": hist_s,
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * T / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 transport, fp64 tallies",
+        "data": "synthetic phantom and source",
+        "config": dict(cfg, histories_per_step=hps, parallelism="none"),
+        "efficiency": {"value": eff, "unit": "1/(sigma^2 s)",
+                       "sigma2": rel2, "definition": "mean relative variance over D > 0.5 Dmax"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_transport<float>",
+                     "kernel_ms_per_step": kms / a.steps,
+                     "counters": {"voxel_crossings": int(dc[1]), "deposits": int(dc[2]),
+                                  "flushes": int(dc[3]), "photon_steps": int(dc[0])}},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(name)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

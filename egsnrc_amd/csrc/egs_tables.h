@@ -1,0 +1,66 @@
+// egs_tables.h -- cross-section table blob: builder (from PEGS4 + EGSnrc
+// data files) and reader.
+//
+// The blob is the state HATCH leaves behind for the media of a run
+// (src/egsnrc.mortran:1828-2320): PEGS4 electron data scaled to cm,
+// photon cross sections rebuilt on the 2000-point log grid from the XCOM
+// element files (egs_init_user_photon :7973-8284), Rayleigh form-factor
+// sampling tables (egs_init_rayleigh/prepare_rayleigh_data :8284-8720),
+// bound Compton shells (init_compton :2442-2598), photo-absorption edges
+// (EDGSET :6312-6436), brems/pair screening fits (fix_brems :2322-2420),
+// the multiple-scattering step tables (mscati :2750-2981) and the screened
+// Rutherford q-surface (init_ms_SR :3267-3306, data/msnew.data).
+//
+// Format: "EGSGPUT1", uint32 version, uint32 nrec, then nrec records of
+// {char name[48]; uint32 type (1=f64, 2=i32); uint32 pad; uint64 count;
+//  payload padded to 8 bytes}.  Keys are "<field>" for global data and
+// "m<k>.<field>" for medium k.
+#pragma once
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace egsgpu {
+
+struct Blob {
+    std::map<std::string, std::vector<double>> f;
+    std::map<std::string, std::vector<int32_t>> i;
+
+    bool has(const std::string &k) const { return f.count(k) || i.count(k); }
+    const std::vector<double> &F(const std::string &k) const;
+    const std::vector<int32_t> &I(const std::string &k) const;
+    double f1(const std::string &k) const { return F(k).at(0); }
+    int32_t i1(const std::string &k) const { return I(k).at(0); }
+    static std::string mk(int m, const char *field) {
+        return "m" + std::to_string(m) + "." + field;
+    }
+};
+
+// Returns 0 or a negative EGSGPU_E_* code (message through set_error).
+int blob_write(const Blob &b, const std::string &path);
+int blob_read(const std::string &path, Blob &b);
+
+struct BuildOptions {
+    double estepe = 0.25;     // ESTEPE
+    double ximax = 0.5;       // XIMAX
+    int bound_compton = 3;    // IBCMP
+    int rayleigh = 1;         // IRAYLR
+};
+
+int tables_build(const std::string &hen_house, const std::string &pegs_file,
+                 const std::vector<std::string> &media, const BuildOptions &o,
+                 Blob &out);
+
+// Dimensions fixed by the reference data layout.
+constexpr int kMXGE = 2000;      // photon log-energy grid ($MXGE)
+constexpr int kMXRAYFF = 100;    // Rayleigh form factor points
+constexpr int kRAYCDF = 100;     // Rayleigh CDF bins ($RAYCDFSIZE)
+constexpr int kMXTOTSH = 1538;   // Compton shells Z=1..100
+constexpr int kMXSHXSEC = 30;    // binding energies per element
+constexpr int kMXEDGE = 16;      // photo-absorption fit edges
+constexpr int kMXSHELL = 6;
+constexpr int kMS_L = 64, kMS_Q = 8, kMS_U = 32;   // msnew.data surface
+constexpr double kRM = 0.5109989461;               // egsnrc.macros:1140
+
+}  // namespace egsgpu

@@ -1,0 +1,98 @@
+// egsgpu_device.h -- device-side data layout of the transport engine.
+//
+// Everything the kernels read lives in HBM in structure-of-arrays form:
+//  * cross-section tables (PWLF coefficient pairs on the photon/electron
+//    log-energy grids, egsnrc.macros:1226-1248) in precision R; the
+//    cumulative-probability tables (Rayleigh CDF, source alias table) in
+//    FP64;
+//  * the voxel grid: medium id (uint8) + relative density (R) per voxel,
+//    plane positions in R;
+//  * per-lane state: ranmar record (400 B), particle stack (SoA, lane
+//    fastest), history tally hash (keys int32 + values FP64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "egsgpu_geom.h"
+#include "egsgpu_rng.h"
+
+namespace egsgpu {
+
+constexpr int kMaxMed = 16;
+constexpr int kMaxNE = 16;        // elements per medium
+constexpr int kMaxShMed = 200;    // Compton shells per medium ($MXMDSH)
+constexpr int kGE = 2000;         // $MXGE
+constexpr int kMaxEKE = 500;      // $MXEKE
+constexpr int kRNB = 16;          // ranmar outputs per refill (LDS ring)
+
+// Per-medium scalars.
+template <typename R>
+struct MedScalars {
+    R ge0, ge1, eke0, eke1;
+    R rho, ae, ap, te, thmoll;
+    R delcm, bpar0, bpar1, zbrang, lzbrang, log_ap;
+    R blcc, xcc, expeke1, esig_e, psig_e;
+    int nne, n_shell, meke, sig_mono_e, sig_mono_p;
+    int pad;
+};
+
+template <typename R>
+struct DevTables {
+    int nmed;
+    const MedScalars<R> *ms;          // [nmed]
+    // photon PWLF pairs [m*kGE + L-1]
+    const R *gmfp0, *gmfp1, *gbr10, *gbr11, *gbr20, *gbr21, *cohe0, *cohe1;
+    const R *pmax0, *pmax1;
+    // Rayleigh sampling [m*100 + i]
+    const double *ray_x, *ray_fcum, *ray_b, *ray_c;
+    const int *ray_i;
+    // Compton shells: per medium [m*kMaxShMed + i]; global [j]
+    const R *eno;
+    const int *eno_atbin, *shell;
+    const R *cs_be, *cs_Jo, *cs_erfJo;
+    // elements [m*kMaxNE + k]
+    const R *z, *pz;
+    // EDGE data per element [iz*30 + k], [iz*6 + k], [iz*16 + k]
+    const R *bind, *iprob, *ea, *eb, *ec, *ed, *ee;
+    const int *enumb;
+    // brems/pair fits [m*48 + k*8 + l]
+    const R *dl;
+    // electron PWLF pairs [m*kMaxEKE + L-1]
+    const R *esig0, *esig1, *psig0, *psig1, *ededx0, *ededx1, *pdedx0,
+        *pdedx1, *ebr10, *ebr11, *pbr10, *pbr11, *pbr20, *pbr21, *tmxs0,
+        *tmxs1, *range_ep0, *range_ep1, *e_array;
+};
+
+template <typename R>
+struct DevSource {
+    int charge, spectrum, shape, np, type;
+    R energy, x0, x1, y0, y1, z, ux, uy, uz;
+    const double *xi, *wi;   // alias table (FP64 cumulative data)
+    const int *bin;
+};
+
+struct DevParams {
+    int rayleigh, ibcmp, iphter, iedgfl, iprdst, ibrdst, exact_bca, kerma;
+    double ecut, pcut;
+};
+
+// Per-lane state arrays (lane-fastest SoA for the stack).
+template <typename R>
+struct DevLanes {
+    int nlanes;
+    RanmarRec *rng;                 // [nlanes]
+    int mxstk;
+    R *se, *sx, *sy, *sz, *su, *sv, *sw, *swt, *sdn;   // [k*nlanes + lane]
+    int *siq, *sir;
+    int hcap;                       // hash capacity per lane (power of 2)
+    int *hkey;                      // [lane*hcap + slot], voxel+1, 0 = empty
+    double *hval;                   // [lane*hcap + slot]
+    int *hlist;                     // touched slots [lane*hcap + k]
+};
+
+struct DevTally {
+    double *sum, *sum2;
+    unsigned long long *counters;   // 8 engine counters
+};
+
+}  // namespace egsgpu

@@ -1,0 +1,587 @@
+// egsgpu_engine.hip -- the transport engine behind egsgpu_create/run:
+// device residency of tables, voxel grid and source, per-lane state, the
+// transport kernel launch and the FP64 tallies.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/egsgpu.h"
+#include "egs_tables.h"
+#include "egsgpu_common.h"
+#include "egsgpu_device.h"
+#include "egsgpu_engine.h"
+#include "egsgpu_transport.h"
+
+namespace egsgpu {
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_seed(RanmarRec *recs, int nlanes, int ixx, int jxx) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nlanes) return;
+    int is, js;
+    stream_seeds(ixx, jxx, s, is, js);
+    ranmar_seed(recs + s, is, js);
+}
+
+template <typename R>
+struct KArgs {
+    DevTables<R> T;
+    GeomView<R> G;
+    const uint8_t *vmed;
+    const R *vrho;
+    DevParams P;
+    DevLanes<R> L;
+    DevTally tally;
+    DevSource<R> S;
+    int64_t base;      // histories per lane
+    int rem;           // lanes < rem do one more
+};
+
+// EGS_AliasTable::sample (egs_alias_table.cpp:268-300), types 0 and 1
+template <typename R>
+__device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R> &sh) {
+    if (S.spectrum == 0) return S.energy;
+    double r1 = (double)sh.rnd();
+    double aj = r1 * S.np;
+    int j = (int)aj;
+    aj -= j;
+    if (aj > S.wi[j]) j = S.bin[j];
+    if (!S.type) return (R)S.xi[j];
+    double x = S.xi[j], dx = S.xi[j + 1] - x;
+    double r2 = (double)sh.rnd();
+    return (R)(x + dx * r2);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
+    extern __shared__ float rbuf_all[];
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.L.nlanes) return;
+    Shower<R> sh(a.T, a.G, a.vmed, a.vrho, a.P, a.L, a.tally, lane,
+                 rbuf_all + threadIdx.x);
+    const R RM = (R)0.5109989461;
+    int64_t nh = a.base + (lane < a.rem ? 1 : 0);
+    for (int64_t k = 0; k < nh; k++) {
+        // EGS_Application::simulateSingleShower (egs_application.cpp:919-959)
+        Part<R> q;
+        int ireg = -1;
+        for (int ntry = 0; ntry < 100000 && ireg < 0; ntry++) {
+            R E = source_energy(a.S, sh);
+            R x = a.S.x0, y = a.S.y0, z = a.S.z;
+            if (a.S.shape == 1) {
+                x = a.S.x0 + (a.S.x1 - a.S.x0) * sh.rnd();
+                y = a.S.y0 + (a.S.y1 - a.S.y0) * sh.rnd();
+            }
+            ireg = geom_is_where(a.G, x, y, z);
+            if (ireg < 0) {
+                R t = (R)1e30;
+                ireg = geom_howfar_out(a.G, x, y, z, a.S.ux, a.S.uy, a.S.uz, t);
+                if (ireg >= 0) { x += a.S.ux * t; y += a.S.uy * t; z += a.S.uz * t; }
+            }
+            q.e = a.S.charge ? E + RM : E;
+            q.x = x; q.y = y; q.z = z;
+            q.u = a.S.ux; q.v = a.S.uy; q.w = a.S.uz;
+            q.wt = 1; q.dnear = 0; q.iq = a.S.charge; q.ir = ireg;
+        }
+        if (ireg < 0) continue;
+        sh.p = q;
+        sh.np = 1;
+        while (sh.np > 0) {
+            if (sh.p.iq == 0) sh.photon();
+            else sh.electron();
+        }
+        sh.end_history();
+    }
+    unsigned long long *c = a.tally.counters;
+    atomicAdd(c + 0, sh.c_psteps);
+    atomicAdd(c + 1, sh.c_cross);
+    atomicAdd(c + 2, sh.c_dep);
+    atomicAdd(c + 3, sh.c_flush);
+    atomicAdd(c + 4, sh.c_esteps);
+    if (sh.c_ovf) atomicAdd(c + 5, sh.c_ovf);
+    if (sh.c_hovf) atomicAdd(c + 6, sh.c_hovf);
+}
+
+// ------------------------------------------------------------ host side
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+template <typename R>
+struct Precision {
+    DevTables<R> T;
+    GeomView<R> G;
+    const R *vrho = nullptr;
+    DevSource<R> S;
+    DevLanes<R> L;
+};
+
+}  // namespace egsgpu
+
+using namespace egsgpu;
+
+struct egsgpu_handle {
+    int fp64 = 0;
+    int nvox = 0;
+    std::vector<void *> allocs;      // everything to hipFree
+    Precision<float> pf;
+    Precision<double> pd;
+    const uint8_t *vmed = nullptr;
+    DevParams P{};
+    DevTally tally{};
+    RanmarRec *rng = nullptr;
+    int nlanes = 0, block = 256, mxstk = 48, hcap = 256;
+    int ixx = 1802, jxx = 9373;
+    bool lanes_ready = false;
+    int64_t nhist = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    int last_launches = 0;
+    ~egsgpu_handle() {
+        for (void *p : allocs) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+int upload(egsgpu_handle *h, const T *src, size_t n, const T **dst) {
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+    EGS_HIP(hipMalloc(&p, bytes));
+    h->allocs.push_back(p);
+    if (n) EGS_HIP(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = (const T *)p;
+    return EGSGPU_OK;
+}
+
+template <typename T>
+int alloc(egsgpu_handle *h, size_t n, T **dst) {
+    void *p = nullptr;
+    EGS_HIP(hipMalloc(&p, std::max<size_t>(n * sizeof(T), 16)));
+    h->allocs.push_back(p);
+    *dst = (T *)p;
+    return EGSGPU_OK;
+}
+
+#define CK(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+// Pack medium arrays [nmed][stride] in precision R.
+template <typename R>
+int pack_med(egsgpu_handle *h, const Blob &b, int nmed, const char *field,
+             int stride, const R **dst) {
+    std::vector<R> v((size_t)nmed * stride, (R)0);
+    for (int m = 0; m < nmed; m++) {
+        const auto &a = b.F(Blob::mk(m, field));
+        if ((int)a.size() > stride) { set_error("table %s too long", field); return EGSGPU_E_FORMAT; }
+        for (size_t i = 0; i < a.size(); i++) v[(size_t)m * stride + i] = (R)a[i];
+    }
+    return upload(h, v.data(), v.size(), dst);
+}
+int pack_med_i(egsgpu_handle *h, const Blob &b, int nmed, const char *field,
+               int stride, const int **dst) {
+    std::vector<int> v((size_t)nmed * stride, 0);
+    for (int m = 0; m < nmed; m++) {
+        const auto &a = b.I(Blob::mk(m, field));
+        if ((int)a.size() > stride) { set_error("table %s too long", field); return EGSGPU_E_FORMAT; }
+        for (size_t i = 0; i < a.size(); i++) v[(size_t)m * stride + i] = a[i];
+    }
+    return upload(h, v.data(), v.size(), dst);
+}
+int pack_med_d(egsgpu_handle *h, const Blob &b, int nmed, const char *field,
+               int stride, const double **dst) {
+    std::vector<double> v((size_t)nmed * stride, 0.0);
+    for (int m = 0; m < nmed; m++) {
+        if (!b.has(Blob::mk(m, field))) continue;
+        const auto &a = b.F(Blob::mk(m, field));
+        for (size_t i = 0; i < a.size() && (int)i < stride; i++) v[(size_t)m * stride + i] = a[i];
+    }
+    return upload(h, v.data(), v.size(), dst);
+}
+template <typename R>
+int pack_glob(egsgpu_handle *h, const Blob &b, const char *field, const R **dst) {
+    const auto &a = b.F(field);
+    std::vector<R> v(a.begin(), a.end());
+    return upload(h, v.data(), v.size(), dst);
+}
+
+template <typename R>
+int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
+    int nmed = b.i1("nmed");
+    T.nmed = nmed;
+    std::vector<MedScalars<R>> ms(nmed);
+    for (int m = 0; m < nmed; m++) {
+        auto f = [&](const char *k) { return (R)b.f1(Blob::mk(m, k)); };
+        MedScalars<R> &s = ms[m];
+        s.ge0 = f("ge0"); s.ge1 = f("ge1"); s.eke0 = f("eke0"); s.eke1 = f("eke1");
+        s.rho = f("rho"); s.ae = f("ae"); s.ap = f("ap"); s.te = f("te"); s.thmoll = f("thmoll");
+        s.delcm = f("delcm");
+        s.bpar0 = (R)b.F(Blob::mk(m, "bpar"))[0];
+        s.bpar1 = (R)b.F(Blob::mk(m, "bpar"))[1];
+        s.zbrang = f("zbrang"); s.lzbrang = f("lzbrang"); s.log_ap = f("log_ap");
+        s.blcc = f("blcc"); s.xcc = f("xcc"); s.expeke1 = f("expeke1");
+        s.esig_e = f("esig_e"); s.psig_e = f("psig_e");
+        s.nne = b.i1(Blob::mk(m, "nne"));
+        s.n_shell = b.i1(Blob::mk(m, "n_shell"));
+        s.meke = b.i1(Blob::mk(m, "meke"));
+        s.sig_mono_e = b.I(Blob::mk(m, "sig_monotone"))[0];
+        s.sig_mono_p = b.I(Blob::mk(m, "sig_monotone"))[1];
+        if (s.nne > kMaxNE || s.n_shell > kMaxShMed || s.meke > kMaxEKE ||
+            b.i1(Blob::mk(m, "mge")) != kGE) {
+            set_error("medium %d exceeds the engine's table dimensions", m);
+            return EGSGPU_E_UNSUPPORTED;
+        }
+    }
+    CK(upload(h, ms.data(), ms.size(), &T.ms));
+    CK(pack_med<R>(h, b, nmed, "gmfp0", kGE, &T.gmfp0));
+    CK(pack_med<R>(h, b, nmed, "gmfp1", kGE, &T.gmfp1));
+    CK(pack_med<R>(h, b, nmed, "gbr10", kGE, &T.gbr10));
+    CK(pack_med<R>(h, b, nmed, "gbr11", kGE, &T.gbr11));
+    CK(pack_med<R>(h, b, nmed, "gbr20", kGE, &T.gbr20));
+    CK(pack_med<R>(h, b, nmed, "gbr21", kGE, &T.gbr21));
+    CK(pack_med<R>(h, b, nmed, "cohe0", kGE, &T.cohe0));
+    CK(pack_med<R>(h, b, nmed, "cohe1", kGE, &T.cohe1));
+    bool ray = b.has(Blob::mk(0, "ray_x"));
+    if (ray) {
+        CK(pack_med<R>(h, b, nmed, "ray_pmax0", kGE, &T.pmax0));
+        CK(pack_med<R>(h, b, nmed, "ray_pmax1", kGE, &T.pmax1));
+        CK(pack_med_i(h, b, nmed, "ray_i", 100, &T.ray_i));
+    } else {
+        T.pmax0 = T.pmax1 = nullptr;
+        T.ray_i = nullptr;
+    }
+    CK(pack_med_d(h, b, nmed, "ray_x", 100, &T.ray_x));
+    CK(pack_med_d(h, b, nmed, "ray_fcum", 100, &T.ray_fcum));
+    CK(pack_med_d(h, b, nmed, "ray_b", 100, &T.ray_b));
+    CK(pack_med_d(h, b, nmed, "ray_c", 100, &T.ray_c));
+    CK(pack_med<R>(h, b, nmed, "eno", kMaxShMed, &T.eno));
+    CK(pack_med_i(h, b, nmed, "eno_atbin", kMaxShMed, &T.eno_atbin));
+    CK(pack_med_i(h, b, nmed, "shell", kMaxShMed, &T.shell));
+    CK(pack_glob<R>(h, b, "cs_be", &T.cs_be));
+    CK(pack_glob<R>(h, b, "cs_Jo", &T.cs_Jo));
+    CK(pack_glob<R>(h, b, "cs_erfJo", &T.cs_erfJo));
+    CK(pack_med<R>(h, b, nmed, "z", kMaxNE, &T.z));
+    CK(pack_med<R>(h, b, nmed, "pz", kMaxNE, &T.pz));
+    CK(pack_glob<R>(h, b, "binding_energies", &T.bind));
+    CK(pack_glob<R>(h, b, "interaction_prob", &T.iprob));
+    CK(pack_glob<R>(h, b, "edge_a", &T.ea));
+    CK(pack_glob<R>(h, b, "edge_b", &T.eb));
+    CK(pack_glob<R>(h, b, "edge_c", &T.ec));
+    CK(pack_glob<R>(h, b, "edge_d", &T.ed));
+    CK(pack_glob<R>(h, b, "edge_energies", &T.ee));
+    {
+        const auto &en = b.I("edge_number");
+        std::vector<int> v(en.begin(), en.end());
+        CK(upload(h, v.data(), v.size(), &T.enumb));
+    }
+    CK(pack_med<R>(h, b, nmed, "dl", 48, &T.dl));
+    const char *ef[] = {"esig0", "esig1", "psig0", "psig1", "ededx0", "ededx1",
+                        "pdedx0", "pdedx1", "ebr10", "ebr11", "pbr10", "pbr11",
+                        "pbr20", "pbr21", "tmxs0", "tmxs1", "range_ep0",
+                        "range_ep1", "e_array"};
+    const R **ed[] = {&T.esig0, &T.esig1, &T.psig0, &T.psig1, &T.ededx0, &T.ededx1,
+                      &T.pdedx0, &T.pdedx1, &T.ebr10, &T.ebr11, &T.pbr10, &T.pbr11,
+                      &T.pbr20, &T.pbr21, &T.tmxs0, &T.tmxs1, &T.range_ep0,
+                      &T.range_ep1, &T.e_array};
+    for (int k = 0; k < 19; k++) CK(pack_med<R>(h, b, nmed, ef[k], kMaxEKE, ed[k]));
+    return EGSGPU_OK;
+}
+
+template <typename R>
+int build_geom(egsgpu_handle *h, const EgsXYZGeometry *g, Precision<R> &P) {
+    int n = (g->nx + 1) + (g->ny + 1) + (g->nz + 1);
+    std::vector<R> pl(n);
+    int k = 0;
+    for (int i = 0; i <= g->nx; i++) pl[k++] = (R)g->xb[i];
+    for (int i = 0; i <= g->ny; i++) pl[k++] = (R)g->yb[i];
+    for (int i = 0; i <= g->nz; i++) pl[k++] = (R)g->zb[i];
+    const R *dpl;
+    CK(upload(h, pl.data(), pl.size(), &dpl));
+    GeomView<R> &v = P.G;
+    v.nx = g->nx; v.ny = g->ny; v.nz = g->nz; v.nxy = g->nx * g->ny;
+    v.xp = dpl; v.yp = dpl + g->nx + 1; v.zp = dpl + g->nx + 1 + g->ny + 1;
+    double tol = g->boundary_tolerance;
+    if (tol <= 0) tol = sizeof(R) == 4 ? 1.0 / (1 << 21) : 1.0 / 4294967296.0;
+    v.btol = (R)tol;
+    v.hbtol = v.btol / (R)2;
+    std::vector<R> rho((size_t)h->nvox, (R)1);
+    if (g->rhor)
+        for (int i = 0; i < h->nvox; i++) rho[i] = (R)g->rhor[i];
+    CK(upload(h, rho.data(), rho.size(), &P.vrho));
+    return EGSGPU_OK;
+}
+
+template <typename R>
+int build_source(egsgpu_handle *h, const EgsSource *s, Precision<R> &P) {
+    DevSource<R> &S = P.S;
+    S.charge = s->charge; S.spectrum = s->spectrum; S.shape = s->shape;
+    S.energy = (R)s->energy;
+    S.x0 = (R)s->x0; S.x1 = (R)s->x1; S.y0 = (R)s->y0; S.y1 = (R)s->y1; S.z = (R)s->z;
+    S.ux = (R)s->ux; S.uy = (R)s->uy; S.uz = (R)s->uz;
+    S.np = 0; S.type = 0;
+    S.xi = S.wi = nullptr;
+    S.bin = nullptr;
+    if (s->spectrum == EGSGPU_SPEC_MONO) return EGSGPU_OK;
+    // EGS_AliasTable::make (egs_alias_table.cpp:102-171)
+    int type = s->spectrum == EGSGPU_SPEC_HISTOGRAM ? 1 : 0;
+    int n = type ? s->nbins + 1 : s->nbins;
+    int np = type ? n - 1 : n;
+    std::vector<double> xi(s->spec_e, s->spec_e + n), fi(s->spec_w, s->spec_w + np), wi(np, 1.0), fcum(np);
+    std::vector<int> bin(np, 0);
+    std::vector<bool> not_done(np, true);
+    double sum = 0;
+    for (int i = 0; i < np; i++) {
+        fcum[i] = type == 0 ? fi[i] : fi[i] * (xi[i + 1] - xi[i]);
+        sum += fcum[i];
+    }
+    sum /= np;
+    for (int i = 0; i < np - 1; i++) {
+        int jh, jl, high = -1, low = -1;
+        for (jh = 0; jh < np; jh++) if (not_done[jh] && fcum[jh] > sum) { high = jh; break; }
+        if (high < 0) break;
+        for (jl = 0; jl < np; jl++) if (not_done[jl] && fcum[jl] < sum) { low = jl; break; }
+        if (low < 0) break;
+        fcum[high] -= sum - fcum[low];
+        not_done[jl] = false;
+        wi[low] = fcum[low] / sum;
+        bin[low] = high;
+    }
+    S.np = np; S.type = type;
+    CK(upload(h, xi.data(), xi.size(), &S.xi));
+    CK(upload(h, wi.data(), wi.size(), &S.wi));
+    CK(upload(h, bin.data(), bin.size(), &S.bin));
+    return EGSGPU_OK;
+}
+
+template <typename R>
+int build_lanes(egsgpu_handle *h, Precision<R> &P) {
+    DevLanes<R> &L = P.L;
+    L.nlanes = h->nlanes;
+    L.rng = h->rng;
+    L.mxstk = h->mxstk;
+    size_t ns = (size_t)h->mxstk * h->nlanes;
+    R **rf[] = {&L.se, &L.sx, &L.sy, &L.sz, &L.su, &L.sv, &L.sw, &L.swt, &L.sdn};
+    for (R **f : rf) CK(alloc(h, ns, f));
+    CK(alloc(h, ns, &L.siq));
+    CK(alloc(h, ns, &L.sir));
+    L.hcap = h->hcap;
+    size_t nh = (size_t)h->hcap * h->nlanes;
+    CK(alloc(h, nh, &L.hkey));
+    CK(alloc(h, nh, &L.hval));
+    CK(alloc(h, nh, &L.hlist));
+    EGS_HIP(hipMemset(L.hkey, 0, nh * sizeof(int)));
+    return EGSGPU_OK;
+}
+
+int ensure_lanes(egsgpu_handle *h) {
+    if (h->lanes_ready) return EGSGPU_OK;
+    CK(alloc(h, (size_t)h->nlanes, &h->rng));
+    if (h->fp64) CK(build_lanes(h, h->pd));
+    else CK(build_lanes(h, h->pf));
+    hipLaunchKernelGGL(k_seed, dim3((h->nlanes + 255) / 256), dim3(256), 0, h->stream,
+                       h->rng, h->nlanes, h->ixx, h->jxx);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    h->lanes_ready = true;
+    return EGSGPU_OK;
+}
+
+template <typename R>
+int launch(egsgpu_handle *h, Precision<R> &P, int64_t n) {
+    KArgs<R> a;
+    a.T = P.T; a.G = P.G; a.vmed = h->vmed; a.vrho = P.vrho; a.P = h->P;
+    a.L = P.L; a.tally = h->tally; a.S = P.S;
+    a.base = n / h->nlanes;
+    a.rem = (int)(n % h->nlanes);
+    size_t lds = (size_t)kRNB * h->block * sizeof(float);
+    hipLaunchKernelGGL(k_transport<R>, dim3((h->nlanes + h->block - 1) / h->block),
+                       dim3(h->block), lds, h->stream, a);
+    EGS_HIP(hipGetLastError());
+    return EGSGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
+                  const EgsSource *src, const EgsTransport *tp,
+                  egsgpu_handle **out) {
+    EGS_ARG(tab && geom && src && tp && out, "tables, geometry, source, transport, out");
+    EGS_ARG(geom->nx > 0 && geom->ny > 0 && geom->nz > 0 && geom->xb && geom->yb && geom->zb,
+            "geometry planes");
+    EGS_ARG(src->charge >= -1 && src->charge <= 1, "source charge");
+    EGS_ARG(src->spectrum == EGSGPU_SPEC_MONO || ((src->spectrum == EGSGPU_SPEC_HISTOGRAM ||
+            src->spectrum == EGSGPU_SPEC_LINE) && src->nbins > 0 && src->spec_e && src->spec_w),
+            "source spectrum");
+    int rc = require_device();
+    if (rc) return rc;
+    if (tp->spin) {
+        set_error("spin effects need the spinms data, which the engine does not have");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    if (!tp->kerma) {
+        set_error("charged-particle transport above ECUT is not available in this build; use kerma=1");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    const Blob &b = tab->blob;
+    auto *h = new egsgpu_handle();
+    auto fail = [&](int code) { delete h; return code; };
+    h->fp64 = tp->fp64 ? 1 : 0;
+    h->nvox = geom->nx * geom->ny * geom->nz;
+    int nmed = b.i1("nmed");
+    if (nmed > kMaxMed) { set_error("too many media"); return fail(EGSGPU_E_UNSUPPORTED); }
+    if (tp->rayleigh && !b.has(Blob::mk(0, "ray_x"))) {
+        set_error("table blob has no Rayleigh data"); return fail(EGSGPU_E_FORMAT);
+    }
+    if (tp->bound_compton != 0 && tp->bound_compton != 3) {
+        set_error("bound Compton option %d not supported (0 or 3)", tp->bound_compton);
+        return fail(EGSGPU_E_UNSUPPORTED);
+    }
+    // media map
+    std::vector<uint8_t> med(h->nvox, 0);
+    for (int i = 0; i < h->nvox; i++) {
+        int m = geom->medium ? geom->medium[i] : 0;
+        if (m < 0 || m >= nmed) { set_error("voxel %d has medium %d outside the table", i, m); return fail(EGSGPU_E_ARG); }
+        med[i] = (uint8_t)m;
+    }
+    if (hipStreamCreate(&h->stream) != hipSuccess || hipEventCreate(&h->ev0) != hipSuccess ||
+        hipEventCreate(&h->ev1) != hipSuccess) {
+        set_error("stream/event creation failed");
+        return fail(EGSGPU_E_HIP);
+    }
+    if ((rc = upload(h, med.data(), med.size(), &h->vmed))) return fail(rc);
+    // cut-offs: $adjust_rhor_ecut_pcut
+    double ecut = 0, pcut = 0;
+    for (int m = 0; m < nmed; m++) {
+        ecut = std::max(ecut, b.f1(Blob::mk(m, "ae")));
+        pcut = std::max(pcut, b.f1(Blob::mk(m, "ap")));
+    }
+    h->P.ecut = std::max(ecut, tp->ecut);
+    h->P.pcut = std::max(pcut, tp->pcut);
+    h->P.rayleigh = tp->rayleigh; h->P.ibcmp = tp->bound_compton;
+    h->P.iphter = tp->photoelectron_angle; h->P.iedgfl = tp->atomic_relaxation;
+    h->P.iprdst = tp->pair_angle; h->P.ibrdst = tp->brems_angle;
+    h->P.exact_bca = tp->exact_bca; h->P.kerma = tp->kerma;
+    try {
+        if (h->fp64) {
+            if ((rc = build_tables(h, b, h->pd.T))) return fail(rc);
+            if ((rc = build_geom(h, geom, h->pd))) return fail(rc);
+            if ((rc = build_source(h, src, h->pd))) return fail(rc);
+        } else {
+            if ((rc = build_tables(h, b, h->pf.T))) return fail(rc);
+            if ((rc = build_geom(h, geom, h->pf))) return fail(rc);
+            if ((rc = build_source(h, src, h->pf))) return fail(rc);
+        }
+    } catch (const std::exception &e) {
+        set_error("%s", e.what());
+        return fail(EGSGPU_E_FORMAT);
+    }
+    if ((rc = alloc(h, (size_t)h->nvox, &h->tally.sum))) return fail(rc);
+    if ((rc = alloc(h, (size_t)h->nvox, &h->tally.sum2))) return fail(rc);
+    if ((rc = alloc(h, 8, &h->tally.counters))) return fail(rc);
+    if (hipMemset(h->tally.sum, 0, sizeof(double) * h->nvox) != hipSuccess ||
+        hipMemset(h->tally.sum2, 0, sizeof(double) * h->nvox) != hipSuccess ||
+        hipMemset(h->tally.counters, 0, 64) != hipSuccess) {
+        set_error("tally clear failed");
+        return fail(EGSGPU_E_HIP);
+    }
+    hipDeviceProp_t prop;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipGetDeviceProperties(&prop, dev);
+    h->nlanes = std::max(1, prop.multiProcessorCount) * 512;
+    *out = h;
+    return EGSGPU_OK;
+}
+
+int egsgpu_destroy(egsgpu_handle *h) {
+    delete h;
+    return EGSGPU_OK;
+}
+
+int egsgpu_set_streams(egsgpu_handle *h, int32_t nstreams, int32_t block_size) {
+    EGS_ARG(h, "handle");
+    EGS_ARG(nstreams >= 0 && (block_size == 0 || (block_size % 64 == 0 && block_size <= 256)),
+            "nstreams >= 0, block size a multiple of 64 up to 256");
+    if (h->lanes_ready) {
+        set_error("launch shape must be set before the first run");
+        return EGSGPU_E_ARG;
+    }
+    if (nstreams > 0) h->nlanes = nstreams;
+    if (block_size > 0) h->block = block_size;
+    return EGSGPU_OK;
+}
+
+int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx) {
+    EGS_ARG(h, "handle");
+    h->ixx = ixx;
+    h->jxx = jxx;
+    if (h->lanes_ready) {
+        hipLaunchKernelGGL(k_seed, dim3((h->nlanes + 255) / 256), dim3(256), 0, h->stream,
+                           h->rng, h->nlanes, h->ixx, h->jxx);
+        EGS_HIP(hipGetLastError());
+        EGS_HIP(hipStreamSynchronize(h->stream));
+    }
+    return EGSGPU_OK;
+}
+
+int egsgpu_run(egsgpu_handle *h, int64_t nhist) {
+    EGS_ARG(h && nhist >= 0, "handle, nhist >= 0");
+    int rc = ensure_lanes(h);
+    if (rc) return rc;
+    EGS_HIP(hipEventRecord(h->ev0, h->stream));
+    rc = h->fp64 ? launch(h, h->pd, nhist) : launch(h, h->pf, nhist);
+    if (rc) return rc;
+    EGS_HIP(hipEventRecord(h->ev1, h->stream));
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    float ms = 0;
+    EGS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    h->last_launches = 1;
+    h->nhist += nhist;
+    return EGSGPU_OK;
+}
+
+int egsgpu_get_tallies(egsgpu_handle *h, double *sum, double *sum2, int64_t *nhist) {
+    EGS_ARG(h, "handle");
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    if (sum) EGS_HIP(hipMemcpy(sum, h->tally.sum, sizeof(double) * h->nvox, hipMemcpyDeviceToHost));
+    if (sum2) EGS_HIP(hipMemcpy(sum2, h->tally.sum2, sizeof(double) * h->nvox, hipMemcpyDeviceToHost));
+    if (nhist) *nhist = h->nhist;
+    return EGSGPU_OK;
+}
+
+int egsgpu_reset_tallies(egsgpu_handle *h) {
+    EGS_ARG(h, "handle");
+    EGS_HIP(hipMemsetAsync(h->tally.sum, 0, sizeof(double) * h->nvox, h->stream));
+    EGS_HIP(hipMemsetAsync(h->tally.sum2, 0, sizeof(double) * h->nvox, h->stream));
+    EGS_HIP(hipMemsetAsync(h->tally.counters, 0, 64, h->stream));
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    h->nhist = 0;
+    return EGSGPU_OK;
+}
+
+int egsgpu_get_counters(egsgpu_handle *h, int64_t counters[8]) {
+    EGS_ARG(h && counters, "handle, counters");
+    EGS_HIP(hipMemcpy(counters, h->tally.counters, 64, hipMemcpyDeviceToHost));
+    return EGSGPU_OK;
+}
+
+int egsgpu_last_kernel_ms(egsgpu_handle *h, double *ms, int32_t *launches) {
+    EGS_ARG(h && ms && launches, "handle, outputs");
+    *ms = h->last_ms;
+    *launches = h->last_launches;
+    return EGSGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// egsgpu_tables_api.cpp -- C ABI for the cross-section table blob.
+#include <string>
+#include <vector>
+
+#include "../../include/egsgpu.h"
+#include "egs_tables.h"
+#include "egsgpu_common.h"
+#include "egsgpu_engine.h"
+
+using namespace egsgpu;
+
+extern "C" {
+
+int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
+                        int32_t nmedia, const char *const *media,
+                        const EgsTransport *tp, const char *out_path) {
+    EGS_ARG(hen_house && pegs_file && media && out_path, "paths and media");
+    EGS_ARG(nmedia > 0, "nmedia > 0");
+    std::vector<std::string> names;
+    for (int i = 0; i < nmedia; i++) {
+        EGS_ARG(media[i], "medium name");
+        names.push_back(media[i]);
+    }
+    BuildOptions o;
+    if (tp) {
+        if (tp->estepe > 0 && tp->estepe < 1) o.estepe = tp->estepe;
+        if (tp->ximax > 0 && tp->ximax < 1) o.ximax = tp->ximax;
+        o.bound_compton = tp->bound_compton;
+        o.rayleigh = tp->rayleigh;
+    }
+    Blob b;
+    int rc = tables_build(hen_house, pegs_file, names, o, b);
+    if (rc) return rc;
+    return blob_write(b, out_path);
+}
+
+int egsgpu_tables_load(const char *path, egsgpu_tables **out) {
+    EGS_ARG(path && out, "path and out");
+    auto *t = new egsgpu_tables();
+    int rc = blob_read(path, t->blob);
+    if (rc) { delete t; return rc; }
+    if (!t->blob.has("nmed")) {
+        delete t;
+        set_error("%s has no medium count", path);
+        return EGSGPU_E_FORMAT;
+    }
+    *out = t;
+    return EGSGPU_OK;
+}
+
+int egsgpu_tables_free(egsgpu_tables *t) {
+    delete t;
+    return EGSGPU_OK;
+}
+
+int egsgpu_tables_nmedia(const egsgpu_tables *t, int32_t *n) {
+    EGS_ARG(t && n, "tables and n");
+    *n = t->blob.i1("nmed");
+    return EGSGPU_OK;
+}
+
+}  // extern "C"

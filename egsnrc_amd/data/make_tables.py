@@ -1,0 +1,49 @@
+"""Regenerate the committed cross-section table blobs from the EGSnrc data.
+
+Runs where an EGSnrc HEN_HOUSE tree is available (default
+/root/reference/HEN_HOUSE); the blobs it writes are what the engine loads.
+"""
+import argparse
+import ctypes as C
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from egsnrc_amd import _lib  # noqa: E402
+
+BLOBS = {
+    # name: (pegs4 file, media)
+    "521icru.egstab": ("521icru.pegs4dat",
+                       ["H2O521ICRU", "LUNG521ICRU", "ICRPBONE521ICRU",
+                        "AIR521ICRU", "AL521ICRU"]),
+    "700icru.egstab": ("700icru.pegs4dat", ["H2O700ICRU"]),
+}
+
+
+def build(hen_house, name, pegs, media, out_dir=HERE, estepe=0.25, ximax=0.5):
+    L = _lib.lib()
+    L.egsgpu_tables_build.argtypes = [C.c_char_p, C.c_char_p, C.c_int32,
+                                      C.POINTER(C.c_char_p), C.c_void_p,
+                                      C.c_char_p]
+    arr = (C.c_char_p * len(media))(*[m.encode() for m in media])
+    tp = _lib.default_transport(estepe=estepe, ximax=ximax)
+    out = os.path.join(out_dir, name)
+    _lib.check(L.egsgpu_tables_build(
+        hen_house.encode(), os.path.join(hen_house, "pegs4", "data", pegs).encode(),
+        len(media), arr, C.cast(C.pointer(tp), C.c_void_p), out.encode()))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--hen-house", default="/root/reference/HEN_HOUSE")
+    ap.add_argument("--out-dir", default=HERE)
+    a = ap.parse_args()
+    for name, (pegs, media) in BLOBS.items():
+        print("wrote", build(a.hen_house, name, pegs, media, a.out_dir))
+
+
+if __name__ == "__main__":
+    main()

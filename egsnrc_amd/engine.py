@@ -1,0 +1,121 @@
+"""Host-side Python mirror of the engine's C ABI (include/egsgpu.h).
+
+``Simulation`` plays the role of EGS_AdvancedApplication for the voxel
+dose path: initSimulation (egsgpu_create), runSimulation (egsgpu_run) and
+the dose-scoring results (EGS_DoseScoring, egs_dose_scoring.cpp:416-479).
+Every call goes to the HIP library; nothing here computes transport.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import EgsSource, check, ptr
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+class Tables:
+    """A loaded cross-section table blob (egsgpu_tables_load)."""
+
+    def __init__(self, path):
+        self.path = path
+        self.h = C.c_void_p()
+        check(_lib.lib().egsgpu_tables_load(path.encode(), C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            _lib.lib().egsgpu_tables_free(self.h)
+            self.h = C.c_void_p()
+
+
+def make_source(charge=0, energy=1.0, spectrum=None, shape="rectangle",
+                x0=-5.0, x1=5.0, y0=-5.0, y1=5.0, z=0.0, u=(0.0, 0.0, 1.0)):
+    """EGS_ParallelBeam-like source.  spectrum = (edges, weights) for a
+    histogram spectrum (per-MeV weights), or None for monoenergetic."""
+    keep = []
+    if spectrum is None:
+        s = EgsSource(charge, 0, energy, 0, None, None, 0, x0, x1, y0, y1, z, *u)
+    else:
+        e = np.ascontiguousarray(spectrum[0], dtype=np.float64)
+        w = np.ascontiguousarray(spectrum[1], dtype=np.float64)
+        keep += [e, w]
+        s = EgsSource(charge, 1, 0.0, len(w), ptr(e).value, ptr(w).value, 0,
+                      x0, x1, y0, y1, z, *u)
+    s.shape = 0 if shape == "point" else 1
+    return s, keep
+
+
+class Simulation:
+    def __init__(self, tables, xb, yb, zb, medium, rhor, source, transport,
+                 nstreams=0, block=0):
+        self.tables = tables
+        self.geom, self._gk = _lib.make_geometry(xb, yb, zb, medium, rhor)
+        self.src, self._sk = source
+        self.tp = transport
+        self.nvox = self.geom.nx * self.geom.ny * self.geom.nz
+        self.h = C.c_void_p()
+        L = _lib.lib()
+        check(L.egsgpu_create(tables.h, C.byref(self.geom), C.byref(self.src),
+                              C.byref(self.tp), C.byref(self.h)))
+        if nstreams or block:
+            check(L.egsgpu_set_streams(self.h, nstreams, block))
+
+    def seed(self, ixx, jxx):
+        check(_lib.lib().egsgpu_seed(self.h, ixx, jxx))
+
+    def run(self, nhist):
+        check(_lib.lib().egsgpu_run(self.h, int(nhist)))
+
+    def tallies(self):
+        s = np.zeros(self.nvox)
+        s2 = np.zeros(self.nvox)
+        n = C.c_int64(0)
+        check(_lib.lib().egsgpu_get_tallies(self.h, ptr(s), ptr(s2), C.byref(n)))
+        return s, s2, n.value
+
+    def counters(self):
+        c = np.zeros(8, dtype=np.int64)
+        check(_lib.lib().egsgpu_get_counters(self.h, ptr(c)))
+        return c
+
+    def kernel_ms(self):
+        ms = C.c_double(0)
+        n = C.c_int32(0)
+        check(_lib.lib().egsgpu_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if self.h.value:
+            _lib.lib().egsgpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dose_stats(sum_, sum2, n):
+    """Per-history mean and its standard error (EGS_ScoringSingle::
+    currentResult, egs_scoring.h:115-125)."""
+    r = sum_ / n
+    var = sum2 / n - r * r
+    var = np.where(var > 0, var / (n - 1), 0.0)
+    return r, np.sqrt(var)
+
+
+def write_3ddose(path, xb, yb, zb, edep, dedep, mass, norm=1.602e-10):
+    """.3ddose writer (EGS_DoseScoring::outputDoseFile,
+    egs_dose_scoring.cpp:416-479): dims, bounds, dose = r*norm/mass,
+    relative uncertainty (1 where r = 0); x fastest."""
+    dose = edep * norm / mass
+    rel = np.where(edep > 0, dedep / np.where(edep > 0, edep, 1), 1.0)
+    with open(path, "w") as f:
+        f.write("%d %d %d\n" % (len(xb) - 1, len(yb) - 1, len(zb) - 1))
+        for b in (xb, yb, zb):
+            f.write(" ".join("%g" % np.float32(v) for v in b) + " \n")
+        f.write(" ".join("%g" % np.float32(v) for v in dose) + " \n")
+        f.write(" ".join("%g" % np.float32(v) for v in rel) + " \n")

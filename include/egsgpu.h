@@ -109,6 +109,15 @@ int egsgpu_device_count(int32_t *n);
  * (PEGS4 + XCOM + msnew data prepared at initialisation, the state HATCH
  * leaves behind: egsnrc.mortran:1828-2320). */
 int egsgpu_tables_load(const char *path, egsgpu_tables **out);
+
+/* Build a table blob from the reference data files: the PEGS4 file
+ * pegs_file (media looked up by name, HATCH egsnrc.mortran:2011-2176) and
+ * the EGSnrc data directory hen_house/data + hen_house/pegs4/pgs4form.dat.
+ * tp supplies ESTEPE/XIMAX (mscati, egsnrc.mortran:2750-2981), IBCMP and
+ * IRAYLR.  Writes the blob to out_path. */
+int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
+                        int32_t nmedia, const char *const *media,
+                        const EgsTransport *tp, const char *out_path);
 int egsgpu_tables_free(egsgpu_tables *t);
 int egsgpu_tables_nmedia(const egsgpu_tables *t, int32_t *n);
 

@@ -84,3 +84,56 @@ def is_where(geom, x, precision):
     assert load().oracle_is_where(C.byref(geom), x.shape[0], _p(x), precision,
                                   _p(out)) == 0
     return out
+
+
+class OracleSimulation:
+    """The oracle's transport run with the same inputs as
+    egsnrc_amd.engine.Simulation (checker only)."""
+
+    def __init__(self, blob_path, geom, source, transport, nstreams=1024,
+                 nthreads=8):
+        L = load()
+        for name, args in {
+            "oracle_create": [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+            "oracle_set_streams": [C.c_void_p, C.c_int32, C.c_int32],
+            "oracle_seed": [C.c_void_p, C.c_int32, C.c_int32],
+            "oracle_run": [C.c_void_p, C.c_int64],
+            "oracle_get_tallies": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+            "oracle_destroy": [C.c_void_p],
+            "oracle_last_error": [C.c_void_p, C.c_size_t],
+        }.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        self.L = L
+        self.geom = geom
+        self.nvox = geom.nx * geom.ny * geom.nz
+        self.h = C.c_void_p()
+        self._check(L.oracle_create(blob_path.encode(), C.byref(geom),
+                                    C.byref(source), C.byref(transport),
+                                    C.byref(self.h)))
+        self._check(L.oracle_set_streams(self.h, nstreams, nthreads))
+
+    def _check(self, rc):
+        if rc != 0:
+            buf = C.create_string_buffer(1024)
+            self.L.oracle_last_error(buf, 1024)
+            raise RuntimeError("oracle: " + buf.value.decode())
+
+    def seed(self, ixx, jxx):
+        self._check(self.L.oracle_seed(self.h, ixx, jxx))
+
+    def run(self, n):
+        self._check(self.L.oracle_run(self.h, int(n)))
+
+    def tallies(self):
+        s = np.zeros(self.nvox)
+        s2 = np.zeros(self.nvox)
+        n = C.c_int64(0)
+        self._check(self.L.oracle_get_tallies(self.h, _p(s), _p(s2), C.byref(n)))
+        return s, s2, n.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.oracle_destroy(self.h)
+            self.h = C.c_void_p()
