@@ -36,6 +36,22 @@ def _compile(src):
     return obj
 
 
+HOST_APP = os.path.join(HERE, "egs_gpu_app")
+
+
+def build_app():
+    """The C++ host application linked against libegsgpu.so."""
+    srcs = [os.path.join(HERE, "host", f) for f in ("egs_input.cpp", "egs_gpu_app.cpp")]
+    cmd = ["g++", "-O2", "-std=c++17", "-o", HOST_APP + ".tmp"] + srcs + [
+        "-I", os.path.join(HERE, "..", "include"), "-L", HERE, "-legsgpu",
+        "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host app build failed:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(HOST_APP + ".tmp", HOST_APP)
+    return HOST_APP
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
@@ -48,8 +64,9 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_app()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", HOST_APP)
     return LIB
 
 

@@ -61,6 +61,10 @@ struct DevTables {
     const R *esig0, *esig1, *psig0, *psig1, *ededx0, *ededx1, *pdedx0,
         *pdedx1, *ebr10, *ebr11, *pbr10, *pbr11, *pbr20, *pbr21, *tmxs0,
         *tmxs1, *range_ep0, *range_ep1, *e_array;
+    // screened Rutherford q^(2+) surface (msnew.data), [(i*8 + j)*32 + k]
+    const float *ms_u, *ms_f, *ms_w;
+    const int *ms_i;
+    float ms_dllambi, ms_dqmsi;
 };
 
 template <typename R>
@@ -73,7 +77,7 @@ struct DevSource {
 
 struct DevParams {
     int rayleigh, ibcmp, iphter, iedgfl, iprdst, ibrdst, exact_bca, kerma;
-    double ecut, pcut;
+    double ecut, pcut, skindepth;
 };
 
 // Per-lane state arrays (lane-fastest SoA for the stack).

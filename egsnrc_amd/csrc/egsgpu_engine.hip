@@ -293,6 +293,20 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
                       &T.pbr20, &T.pbr21, &T.tmxs0, &T.tmxs1, &T.range_ep0,
                       &T.range_ep1, &T.e_array};
     for (int k = 0; k < 19; k++) CK(pack_med<R>(h, b, nmed, ef[k], kMaxEKE, ed[k]));
+    {
+        const char *mf[] = {"ms_u", "ms_f", "ms_w"};
+        const float **md[] = {&T.ms_u, &T.ms_f, &T.ms_w};
+        for (int k = 0; k < 3; k++) {
+            const auto &a = b.F(mf[k]);
+            std::vector<float> v(a.begin(), a.end());
+            CK(upload(h, v.data(), v.size(), md[k]));
+        }
+        const auto &mi = b.I("ms_i");
+        std::vector<int> v(mi.begin(), mi.end());
+        CK(upload(h, v.data(), v.size(), &T.ms_i));
+        T.ms_dllambi = (float)b.F("ms_const")[3];
+        T.ms_dqmsi = (float)b.F("ms_const")[5];
+    }
     return EGSGPU_OK;
 }
 
@@ -429,8 +443,8 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         set_error("spin effects need the spinms data, which the engine does not have");
         return EGSGPU_E_UNSUPPORTED;
     }
-    if (!tp->kerma) {
-        set_error("charged-particle transport above ECUT is not available in this build; use kerma=1");
+    if (!tp->kerma && !tp->exact_bca) {
+        set_error("only the exact boundary-crossing algorithm is implemented");
         return EGSGPU_E_UNSUPPORTED;
     }
     const Blob &b = tab->blob;
@@ -472,6 +486,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->P.iphter = tp->photoelectron_angle; h->P.iedgfl = tp->atomic_relaxation;
     h->P.iprdst = tp->pair_angle; h->P.ibrdst = tp->brems_angle;
     h->P.exact_bca = tp->exact_bca; h->P.kerma = tp->kerma;
+    h->P.skindepth = tp->skin_depth > 1e-4 ? tp->skin_depth : 3.0;
     try {
         if (h->fp64) {
             if ((rc = build_tables(h, b, h->pd.T))) return fail(rc);
@@ -500,6 +515,10 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     (void)hipGetDevice(&dev);
     (void)hipGetDeviceProperties(&prop, dev);
     h->nlanes = std::max(1, prop.multiProcessorCount) * 512;
+    // per-history tally hash: a kerma history touches a few voxels, a
+    // coupled electron shower up to several hundred
+    h->hcap = tp->kerma ? 256 : 2048;
+    h->mxstk = tp->kerma ? 32 : 64;
     *out = h;
     return EGSGPU_OK;
 }

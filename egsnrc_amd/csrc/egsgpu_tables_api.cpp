@@ -1,4 +1,5 @@
 // egsgpu_tables_api.cpp -- C ABI for the cross-section table blob.
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,20 @@ int egsgpu_tables_free(egsgpu_tables *t) {
 int egsgpu_tables_nmedia(const egsgpu_tables *t, int32_t *n) {
     EGS_ARG(t && n, "tables and n");
     *n = t->blob.i1("nmed");
+    return EGSGPU_OK;
+}
+
+int egsgpu_tables_medium_info(const egsgpu_tables *t, int32_t i, char *name,
+                              size_t n, double *rho) {
+    EGS_ARG(t, "tables");
+    int nmed = t->blob.i1("nmed");
+    EGS_ARG(i >= 0 && i < nmed, "medium index");
+    if (name && n > 0) {
+        std::string s;
+        for (int c : t->blob.I(Blob::mk(i, "name"))) s.push_back((char)c);
+        snprintf(name, n, "%s", s.c_str());
+    }
+    if (rho) *rho = t->blob.f1(Blob::mk(i, "rho"));
     return EGSGPU_OK;
 }
 

@@ -714,17 +714,673 @@ struct Shower {
         set_below_iq(iq1);
     }
 
-    // ---------------------------------------------------- charged (kerma)
-    __device__ void electron() {
+    // ------------------------------------------------------------ ELECTR
+    // egsnrc.mortran:985-1823: PRESTA-II condensed history with exact
+    // boundary crossing (single elastic scattering within skindepth of a
+    // boundary), spin effects off, EII off.
+    __device__ __forceinline__ int lel(const MedScalars<R> &M, R elke) const {
+        int L = (int)(M.eke1 * elke + M.eke0);
+        return L < 1 ? 1 : (L > M.meke ? M.meke : L);
+    }
+    __device__ __forceinline__ R ev(const R *a0, const R *a1, int m, int L, R x) const {
+        int i = m * kMaxEKE + L - 1;
+        return a1[i] * x + a0[i];
+    }
+    // $COMPUTE-DRANGE (egsnrc.macros:3214-3243)
+    __device__ R drange(int m, bool pos, R e1, R e2, int l1, R el1, R el2) const {
+        R fedep = 1 - e2 / e1;
+        R elktmp = (R)0.5 * (el1 + el2 + (R)0.25 * fedep * fedep * (1 + fedep * (1 + (R)0.875 * fedep)));
+        R dedxmid, aux;
+        if (!pos) {
+            dedxmid = 1 / ev(T.ededx0, T.ededx1, m, l1, elktmp);
+            aux = T.ededx1[m * kMaxEKE + l1 - 1] * dedxmid;
+        } else {
+            dedxmid = 1 / ev(T.pdedx0, T.pdedx1, m, l1, elktmp);
+            aux = T.pdedx1[m * kMaxEKE + l1 - 1] * dedxmid;
+        }
+        R f = fedep / (2 - fedep);
+        aux = aux * (1 + 2 * aux) * f * f / 6;
+        return fedep * e1 * dedxmid * (1 + aux);
+    }
+    // $COMPUTE-ELOSS (egsnrc.macros:3290-3313)
+    __device__ R eloss(int m, bool pos, R rhof, R step, R e, R el, int L) const {
+        R dedxmid, aux;
+        if (!pos) {
+            dedxmid = ev(T.ededx0, T.ededx1, m, L, el);
+            aux = T.ededx1[m * kMaxEKE + L - 1] / dedxmid;
+        } else {
+            dedxmid = ev(T.pdedx0, T.pdedx1, m, L, el);
+            aux = T.pdedx1[m * kMaxEKE + L - 1] / dedxmid;
+        }
+        R de = dedxmid * step * rhof;
+        R fedep = de / e;
+        return de * (1 - (R)0.5 * fedep * aux *
+                             (1 - (R)0.333333 * fedep * (aux - 1 - (R)0.25 * fedep * (2 - aux * (4 - aux)))));
+    }
+    // $COMPUTE-ELOSS-G (egsnrc.macros:3315-3352)
+    __device__ R eloss_g(int m, const MedScalars<R> &M, bool pos, R rhof, R range, R step,
+                         R eke, R elke, int lelke) const {
+        const R *rep = (pos ? T.range_ep1 : T.range_ep0) + m * kMaxEKE;
+        R tuss = range - rep[lelke - 1] / rhof;
+        if (tuss >= step) return eloss(m, pos, rhof, step, eke, elke, lelke);
+        int lt = lelke;
+        tuss = (range - step) * rhof;
+        if (tuss <= 0) return eke - M.te * (R)0.99;
+        while (lt > 1 && tuss < rep[lt - 1]) lt--;
+        R elktmp = (R)(lt + 1 - M.eke0) / M.eke1;
+        R eketmp = T.e_array[m * kMaxEKE + lt];
+        tuss = (rep[lt] - tuss) / rhof;
+        return eloss(m, pos, rhof, tuss, eketmp, elktmp, lt) + eke - eketmp;
+    }
+
+    __device__ void ecut_discard(int lelec, R peie) {
         const R RM = (R)0.5109989461;
-        int lelec = p.iq;
-        R peie = p.e;
-        // :ECUT-DISCARD: (every charged particle is below ECUT in kerma mode)
-        c_esteps++;
         R edep = peie - RM;
         score(p.ir, edep, p.wt);
         if (lelec > 0 && edep < peie) { annih_at_rest(); return; }
         pop();
+    }
+
+    __device__ void electron() {
+        const R RM = (R)0.5109989461, RMT2 = 2 * RM, RMSQ = RM * RM;
+        const R ecut = (R)P.ecut;
+        if (P.kerma) { c_esteps++; ecut_discard(p.iq, p.e); return; }
+        int irl = p.ir;
+        int medium = vmed[irl];
+        R rhof = vrho[irl];
+        for (;;) {   // :NEWELECTRON:
+            int lelec = p.iq;
+            bool pos = lelec > 0;
+            R peie = p.e, eie = peie;
+            if (eie <= ecut) { ecut_discard(lelec, peie); return; }
+            if (p.wt == 0) { pop(); return; }
+            bool interact = false;
+            R elke = 0;
+            int lelke = 1;
+            while (!interact) {   // :TSTEP:
+                int m = medium;
+                const MedScalars<R> &M = T.ms[m];
+                R eke = eie - RM;
+                R rnne1 = rnd();
+                if (rnne1 == 0) rnne1 = (R)1.e-30;
+                R demfp = -dlog(rnne1);
+                demfp = demfp > (R)1.e-8 ? demfp : (R)1.e-8;
+                elke = dlog(eke);
+                lelke = lel(M, elke);
+                R sig0, sigf;
+                bool mono = pos ? M.sig_mono_p : M.sig_mono_e;
+                if (mono) {
+                    sigf = pos ? ev(T.psig0, T.psig1, m, lelke, elke) / ev(T.pdedx0, T.pdedx1, m, lelke, elke)
+                               : ev(T.esig0, T.esig1, m, lelke, elke) / ev(T.ededx0, T.ededx1, m, lelke, elke);
+                    sig0 = sigf;
+                } else {
+                    sig0 = pos ? M.psig_e : M.esig_e;
+                }
+                bool compute_tstep = true;
+                R total_tstep = 0;
+                bool restart = false;
+                do {   // :USTEP:
+                    if (++hsteps > kMaxStepsPerHistory) { c_ovf++; np = 0; return; }
+                    c_esteps++;
+                    R sig = sig0, tstep;
+                    const R *rep = (pos ? T.range_ep1 : T.range_ep0) + m * kMaxEKE;
+                    if (sig <= 0) {
+                        tstep = (R)1e8;
+                        sig0 = (R)1.E-15;
+                    } else {
+                        if (compute_tstep) {
+                            R total_de = demfp / sig;
+                            R ekef = eke - total_de;
+                            if (ekef <= T.e_array[m * kMaxEKE]) {
+                                tstep = (R)1e8;
+                            } else {
+                                R elkef = dlog(ekef);
+                                int lelkef = lel(M, elkef);
+                                if (lelkef == lelke) {
+                                    tstep = drange(m, pos, eke, ekef, lelke, elke, elkef);
+                                } else {
+                                    R ekei = T.e_array[m * kMaxEKE + lelke - 1];
+                                    R elkei = (R)(lelke - M.eke0) / M.eke1;
+                                    R tuss = drange(m, pos, eke, ekei, lelke, elke, elkei);
+                                    ekei = T.e_array[m * kMaxEKE + lelkef];
+                                    elkei = (R)(lelkef + 1 - M.eke0) / M.eke1;
+                                    tstep = drange(m, pos, ekei, ekef, lelkef, elkei, elkef);
+                                    tstep = tstep + tuss + rep[lelke - 1] - rep[lelkef];
+                                }
+                            }
+                            total_tstep = tstep;
+                            compute_tstep = false;
+                        }
+                        tstep = total_tstep / rhof;
+                    }
+                    R dedx0 = pos ? ev(T.pdedx0, T.pdedx1, m, lelke, elke) : ev(T.ededx0, T.ededx1, m, lelke, elke);
+                    R dedx = rhof * dedx0;
+                    R tmxs = ev(T.tmxs0, T.tmxs1, m, lelke, elke) / rhof;
+                    R range;
+                    {
+                        R ekei = T.e_array[m * kMaxEKE + lelke - 1];
+                        R elkei = (R)(lelke - M.eke0) / M.eke1;
+                        range = (drange(m, pos, eke, ekei, lelke, elke, elkei) + rep[lelke - 1]) / rhof;
+                    }
+                    R tustep = tstep < tmxs ? tstep : tmxs;
+                    tustep = tustep < range ? tustep : range;
+                    R tperp = geom_hownear(G, p.ir, p.x, p.y, p.z);
+                    p.dnear = tperp;
+                    R blccl = rhof * M.blcc;
+                    R p2 = eke * (eke + RMT2), beta2 = p2 / (p2 + RMSQ);
+                    R ssmfp = beta2 / blccl;
+                    R skindepth = (R)P.skindepth * ssmfp;
+                    R mx = tperp > skindepth ? tperp : skindepth;
+                    tustep = tustep < mx ? tustep : mx;
+                    bool callhowfar, dosingle = false, callmsdist;
+                    R ustep, de = 0;
+                    R us = 0, vs = 0, ws = 0, xt = 0, yt = 0, zt = 0;
+                    // the CH-versus-boundary decision is taken in FP64
+                    if ((double)tustep <= (double)tperp && (double)tustep > (double)skindepth) {
+                        callhowfar = false;
+                        callmsdist = true;
+                        de = eloss_g(m, M, pos, rhof, range, tustep, eke, elke, lelke);
+                        msdist_pII(M, eke, de, tustep, rhof, us, vs, ws, xt, yt, zt, ustep);
+                    } else {
+                        callmsdist = false;
+                        R rnnoss = rnd();
+                        if (rnnoss < (R)1.e-30) rnnoss = (R)1.e-30;
+                        R lambda = -dlog(1 - rnnoss);
+                        R e1 = eke / RM + 1;
+                        R lambda_max = (R)0.5 * blccl * RM / dedx * e1 * e1 * e1;
+                        if (lambda >= 0 && lambda_max > 0) {
+                            R tuss = lambda < lambda_max ? lambda * ssmfp * (1 - (R)0.5 * lambda / lambda_max)
+                                                         : (R)0.5 * lambda * ssmfp;
+                            if (tuss < tustep) { tustep = tuss; dosingle = true; }
+                        } else {
+                            pop();
+                            return;
+                        }
+                        ustep = tustep;
+                        callhowfar = !(ustep < tperp);
+                    }
+                    int irold = p.ir, irnew = p.ir, idisc = 0;
+                    if (callhowfar || p.wt <= 0) {
+                        int inew = geom_howfar_in(G, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+                        if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
+                    }
+                    if (ustep <= 0) ustep = 0;
+                    if (ustep == 0) {
+                        if (irnew != irold) {
+                            c_cross++;
+                            p.ir = irnew; irl = irnew;
+                            if (irnew >= 0) { medium = vmed[irnew]; rhof = vrho[irnew]; }
+                        }
+                        if (eie <= ecut) { ecut_discard(lelec, peie); return; }
+                        restart = true;
+                        break;
+                    }
+                    R vstep = ustep, tvstep;
+                    if (callhowfar) {
+                        tvstep = vstep;
+                        if (tvstep != tustep) dosingle = false;
+                        de = eloss_g(m, M, pos, rhof, range, tvstep, eke, elke, lelke);
+                    } else {
+                        tvstep = tustep;
+                    }
+                    R save_de = de;
+                    R edep = de;
+                    R ekef = eke - de;
+                    R xf, yf, zf, uf, vf, wf;
+                    if (callmsdist) {
+                        uf = us; vf = vs; wf = ws; xf = xt; yf = yt; zf = zt;
+                    } else {
+                        xf = p.x + p.u * vstep;
+                        yf = p.y + p.v * vstep;
+                        zf = p.z + p.w * vstep;
+                        if (dosingle) {
+                            R ekems = ekef > ecut - RM ? ekef : ecut - RM;
+                            R p2s = ekems * (ekems + RMT2);
+                            R chia2 = M.xcc / (4 * M.blcc * p2s);
+                            R xi = rnd();
+                            xi = 2 * chia2 * xi / (1 - xi + chia2);
+                            R costhe = 1 - xi, sinthe = dsqrt(xi * (2 - xi));
+                            R ut = p.u, vt = p.v, wt = p.w;
+                            uphi21(costhe, sinthe);
+                            uf = p.u; vf = p.v; wf = p.w;
+                            p.u = ut; p.v = vt; p.w = wt;
+                        } else {
+                            uf = p.u; vf = p.v; wf = p.w;
+                        }
+                    }
+                    score(p.ir, edep, p.wt);
+                    p.x = xf; p.y = yf; p.z = zf;
+                    p.u = uf; p.v = vf; p.w = wf;
+                    p.dnear = p.dnear - vstep;
+                    irold = p.ir;
+                    peie = peie - edep;
+                    eie = peie;
+                    p.e = peie;
+                    if (irnew == irl && eie <= ecut) { ecut_discard(lelec, peie); return; }
+                    int medold = medium;
+                    eke = eie - RM;
+                    elke = dlog(eke);
+                    lelke = lel(M, elke);
+                    if (irnew != irold) {
+                        c_cross++;
+                        p.ir = irnew; irl = irnew;
+                        if (irnew >= 0) { medium = vmed[irnew]; rhof = vrho[irnew]; }
+                    }
+                    if (eie <= ecut) { ecut_discard(lelec, peie); return; }
+                    if (idisc < 0) { pop(); return; }
+                    if (medium != medold) { restart = true; break; }
+                    demfp = demfp - save_de * sig;
+                    total_tstep = total_tstep - tvstep * rhof;
+                    if (total_tstep < (R)1e-9) demfp = 0;
+                } while (!(demfp < (R)1.e-8));
+                if (restart) continue;
+                const R dd = pos ? ev(T.pdedx0, T.pdedx1, m, lelke, elke) : ev(T.ededx0, T.ededx1, m, lelke, elke);
+                sigf = (pos ? ev(T.psig0, T.psig1, m, lelke, elke) : ev(T.esig0, T.esig1, m, lelke, elke)) / dd;
+                R rfict = rnd();
+                if (rfict <= sigf / sig0) interact = true;
+            }
+            // ---- discrete interaction (:1652-1744)
+            int m = medium;
+            const MedScalars<R> &M = T.ms[m];
+            if (!pos) {
+                R ebr1 = ev(T.ebr10, T.ebr11, m, lelke, elke);
+                R rnno24 = rnd();
+                if (rnno24 <= ebr1 || p.e <= M.thmoll) {
+                    if (!(rnno24 <= ebr1) && ebr1 <= 0) continue;
+                    brems(m, M, elke);
+                    if (p.iq == 0) return;
+                    continue;
+                }
+                moller(M);
+                if (p.iq == 0) return;
+                continue;
+            }
+            R pbr1 = ev(T.pbr10, T.pbr11, m, lelke, elke);
+            R rnno25 = rnd();
+            if (rnno25 < pbr1) {
+                brems(m, M, elke);
+                if (p.iq == 0) return;
+                continue;
+            }
+            R pbr2 = ev(T.pbr20, T.pbr21, m, lelke, elke);
+            if (rnno25 < pbr2) {
+                bhabha(M);
+                if (p.iq == 0) return;
+                continue;
+            }
+            annih();
+            return;
+        }
+    }
+
+    // msdist_pII (egsnrc.mortran:4050-4308), spin effects off
+    __device__ void msdist_pII(const MedScalars<R> &M, R e0, R el, R tustep, R rhof,
+                               R &us, R &vs, R &ws, R &xf, R &yf, R &zf, R &ustep) {
+        const R PRM = (R)0.5109989461, RMT2 = 2 * PRM, RMSQ = PRM * PRM;
+        R blccc = M.blcc, xcccc = M.xcc;
+        R e = e0 - (R)0.5 * el;
+        R tau = e / PRM, tau2 = tau * tau;
+        R epsilon = el / e0, epsilonp = el / e;
+        e = e * (1 - epsilonp * epsilonp * (6 + 10 * tau + 5 * tau2) / (24 * tau2 + 72 * tau + 48));
+        R p2 = e * (e + RMT2);
+        R beta2 = p2 / (p2 + RMSQ);
+        R chia2 = xcccc / (4 * p2 * blccc);
+        R lambda = (R)0.5 * tustep * rhof * blccc / beta2;
+        R f = epsilonp / ((tau + 1) * (tau + 2));
+        R temp2 = (R)0.166666 * (4 + tau * (6 + tau * (7 + tau * (4 + tau)))) * f * f;
+        lambda = lambda * (1 - temp2);
+        lambda = lambda / (1 + chia2);
+        R chilog = dlog(1 + 1 / chia2);
+        R q1 = 2 * chia2 * (chilog * (1 + chia2) - 1);
+        R gamma = 6 * chia2 * (1 + chia2) * (chilog * (1 + 2 * chia2) - 2) / q1;
+        R xi = q1 * lambda;
+        bool find_index = true;
+        int msi = 0, msj = 0;
+        R omega2 = 0;
+        R w1, sint1, w2, sint2, cphi1, sphi1, cphi2, sphi2;
+        mscat(lambda, chia2, xi, find_index, msi, msj, omega2, w1, sint1);
+        azimuth(cphi1, sphi1);
+        mscat(lambda, chia2, xi, find_index, msi, msj, omega2, w2, sint2);
+        azimuth(cphi2, sphi2);
+        R u2 = sint2 * cphi2, v2 = sint2 * sphi2;
+        R u2p = w1 * u2 + sint1 * w2;
+        us = u2p * cphi1 - v2 * sphi1;
+        vs = u2p * sphi1 + v2 * cphi1;
+        ws = w1 * w2 - sint1 * u2;
+        xi = 2 * xi;
+        R eta = dsqrt(rnd());
+        R eta1 = (R)0.5 * (1 - eta);
+        R delta = (R)0.9082483 - ((R)0.1020621 - (R)0.0263747 * gamma) * xi;
+        R temp1 = 2 + tau;
+        R temp = (2 + tau * temp1) / ((tau + 1) * temp1);
+        temp = temp - (tau + 1) / ((tau + 2) * (chilog * (1 + chia2) - 1));
+        temp = temp * epsilonp;
+        temp1 = 1 - temp;
+        delta = delta + (R)0.40824829 * (epsilon * (tau + 1) / ((tau + 2) * (chilog * (1 + chia2) - 1) *
+                                                                (chilog * (1 + 2 * chia2) - 2)) -
+                                         (R)0.25 * temp * temp);
+        R b = eta * delta, c = eta * (1 - delta);
+        R w1v2 = w1 * v2;
+        R ut = b * sint1 * cphi1 + c * (cphi1 * u2 - sphi1 * w1v2) + eta1 * us * temp1;
+        R vt = b * sint1 * sphi1 + c * (sphi1 * u2 + cphi1 * w1v2) + eta1 * vs * temp1;
+        R wt = eta1 * (1 + temp) + b * w1 + c * w2 + eta1 * ws * temp1;
+        ustep = tustep * dsqrt(ut * ut + vt * vt + wt * wt);
+        R u0 = p.u, v0 = p.v, w0 = p.w;
+        R sint02 = u0 * u0 + v0 * v0;
+        if (sint02 > (R)1e-20) {
+            R sint0 = dsqrt(sint02), sint0i = 1 / sint0;
+            R cphi0 = sint0i * u0, sphi0 = sint0i * v0;
+            u2p = w0 * us + sint0 * ws;
+            ws = w0 * ws - sint0 * us;
+            us = u2p * cphi0 - vs * sphi0;
+            vs = u2p * sphi0 + vs * cphi0;
+            u2p = w0 * ut + sint0 * wt;
+            wt = w0 * wt - sint0 * ut;
+            ut = u2p * cphi0 - vt * sphi0;
+            vt = u2p * sphi0 + vt * cphi0;
+        } else {
+            wt = w0 * wt;
+            ws = w0 * ws;
+        }
+        xf = p.x + tustep * ut;
+        yf = p.y + tustep * vt;
+        zf = p.z + tustep * wt;
+    }
+
+    // mscat (egsnrc.mortran:2984-3160), spin effects off
+    __device__ void mscat(R lambda, R chia2, R q1, bool &find_index, int &msi, int &msj,
+                          R &omega2, R &cost, R &sint) {
+        if (lambda <= (R)13.8) {
+            R sprob = rnd();
+            R explambda = dexp(-lambda);
+            if (sprob < explambda) { cost = 1; sint = 0; return; }
+            R wsum = (1 + lambda) * explambda;
+            if (sprob < wsum) {
+                R xi = rnd();
+                xi = 2 * chia2 * xi / (1 - xi + chia2);
+                cost = 1 - xi;
+                sint = dsqrt(xi * (2 - xi));
+                return;
+            }
+            if (lambda <= 1) {
+                R wprob = explambda;
+                wsum = explambda;
+                cost = 1; sint = 0;
+                int icount = 0;
+                do {
+                    icount++;
+                    if (icount > 20) break;
+                    wprob = wprob * lambda / icount;
+                    wsum = wsum + wprob;
+                    R xi = rnd();
+                    xi = 2 * chia2 * xi / (1 - xi + chia2);
+                    R cosz = 1 - xi;
+                    R sinz = xi * (2 - xi);
+                    if (sinz > (R)1.e-20) {
+                        sinz = dsqrt(sinz);
+                        xi = rnd();
+                        R phi = xi * (R)6.2831853;
+                        cost = cost * cosz - sint * sinz * cos(phi);
+                        R s2 = (1 - cost) * (1 + cost);
+                        sint = dsqrt(s2 > 0 ? s2 : (R)0);
+                    }
+                } while (!(wsum > sprob));
+                return;
+            }
+        }
+        if (find_index) {
+            R llmbda = dlog(lambda);
+            R ai = llmbda * (R)T.ms_dllambi;
+            int i = (int)ai;
+            ai = ai - i;
+            R xi = rnd();
+            if (xi < ai) i = i + 1;
+            int j;
+            if (q1 < (R)1e-3) j = 0;
+            else if (q1 < (R)0.5) {
+                R aj = q1 * (R)T.ms_dqmsi;
+                j = (int)aj;
+                aj = aj - j;
+                xi = rnd();
+                if (xi < aj) j = j + 1;
+            } else j = 7;
+            i = i < 0 ? 0 : (i > 63 ? 63 : i);
+            j = j > 7 ? 7 : j;
+            if (llmbda < (R)2.2299)
+                omega2 = chia2 * (lambda + 4) *
+                         ((R)1.347006 + llmbda * ((R)0.209364 - llmbda * ((R)0.45525 - llmbda * ((R)0.50142 - (R)0.081234 * llmbda))));
+            else
+                omega2 = chia2 * (lambda + 4) *
+                         ((R)-2.77164 + llmbda * ((R)2.94874 - llmbda * ((R)0.1535754 - llmbda * (R)0.00552888)));
+            msi = i; msj = j;
+            find_index = false;
+        }
+        R xi = rnd();
+        R ak = xi * 31;
+        int k = (int)ak;
+        k = k > 30 ? 30 : k;
+        ak = ak - k;
+        int base = (msi * 8 + msj) * 32;
+        if (ak > (R)T.ms_w[base + k]) k = T.ms_i[base + k];
+        R a = (R)T.ms_f[base + k], u = (R)T.ms_u[base + k];
+        R du = (R)T.ms_u[base + k + 1] - u;
+        xi = rnd();
+        if (fabs(a) < (R)0.2) {
+            R x1 = (R)0.5 * (1 - xi) * a;
+            u = u + xi * du * (1 + x1 * (1 - xi * a));
+        } else {
+            u = u - du / a * (1 - dsqrt(1 + xi * a * (2 + a)));
+        }
+        xi = omega2 * u / (1 + (R)0.5 * omega2 - u);
+        if (xi > (R)1.99999) xi = (R)1.99999;
+        cost = 1 - xi;
+        sint = dsqrt(xi * (2 - xi));
+    }
+
+    // MOLLER (egsnrc.mortran:2599-2746), EII off
+    __device__ void moller(const MedScalars<R> &M) {
+        const R RM = (R)0.5109989461;
+        R peie = p.e, pekin = peie - RM, ekin = pekin;
+        if (ekin <= 2 * M.te) return;
+        R t0 = ekin / RM, e0 = t0 + 1;
+        R extrae = peie - M.thmoll;
+        R e02 = e0 * e0;
+        R g2 = t0 * t0 / e02, g3 = (2 * t0 + 1) / e02;
+        R gmax = 1 + (R)1.25 * g2;
+        R br, rejf4, rnno28;
+        do {
+            R rnno27 = rnd();
+            br = M.te / (ekin - extrae * rnno27);
+            R r = br / (1 - br);
+            rnno28 = rnd();
+            rejf4 = 1 + g2 * br * br + r * (r - g3);
+            rnno28 = gmax * rnno28;
+        } while (!(rnno28 <= rejf4));
+        R pekse2 = br * ekin;
+        R pese1 = peie - pekse2, pese2 = pekse2 + RM;
+        p.e = pese1;
+        R h1 = (peie + RM) / pekin;
+        R dcosth = h1 * (pese1 - RM) / (pese1 + RM);
+        uphi21(dsqrt(dcosth), dsqrt(1 - dcosth));
+        if (!push_copy()) return;
+        p.iq = -1;
+        p.e = pese2;
+        dcosth = h1 * (pese2 - RM) / (pese2 + RM);
+        uphi32(dsqrt(dcosth), -dsqrt(1 - dcosth));
+    }
+
+    // BHABHA (egsnrc.mortran:229-307)
+    __device__ void bhabha(const MedScalars<R> &M) {
+        const R RM = (R)0.5109989461;
+        R peip = p.e, pekin = peip - RM, ekin = pekin;
+        R t0 = ekin / RM, e0 = t0 + 1;
+        R yy = 1 / (t0 + 2), e02 = e0 * e0;
+        R beta2 = (e02 - 1) / e02;
+        R ep0 = M.te / ekin, ep0c = 1 - ep0;
+        R y2 = yy * yy, yp = 1 - 2 * yy, yp2 = yp * yp;
+        R b4 = yp2 * yp, b3 = b4 + yp2, b2 = yp * (3 + y2), b1 = 2 - y2;
+        R br, rejf2, rnno04;
+        do {
+            R rnno03 = rnd();
+            br = ep0 / (1 - ep0c * rnno03);
+            rnno04 = rnd();
+            rejf2 = 1 - beta2 * br * (b1 - br * (b2 - br * (b3 - br * b4)));
+        } while (!(rnno04 <= rejf2));
+        int iq_np = 1, iq_new = -1;
+        if (!(br < (R)0.5)) { iq_np = -1; iq_new = 1; br = 1 - br; }
+        br = br > 0 ? br : (R)0;
+        R pekse2 = br * ekin;
+        R pese1 = peip - pekse2, pese2 = pekse2 + RM;
+        p.e = pese1;
+        p.iq = iq_np;
+        R h1 = (peip + RM) / pekin;
+        R dcosth = h1 * (pese1 - RM) / (pese1 + RM);
+        dcosth = dcosth < 1 ? dcosth : (R)1;
+        uphi21(dsqrt(dcosth), dsqrt(1 - dcosth));
+        if (!push_copy()) return;
+        p.e = pese2;
+        p.iq = iq_new;
+        dcosth = h1 * (pese2 - RM) / (pese2 + RM);
+        uphi32(dsqrt(dcosth), -dsqrt(1 - dcosth));
+    }
+
+    // BREMS (egsnrc.mortran:312-526): Bethe-Heitler, Koch-Motz 2BS angle
+    __device__ void brems(int m, const MedScalars<R> &M, R elke) {
+        const R RM = (R)0.5109989461;
+        const R *dl = T.dl + m * 48;
+        R peie = p.e, eie = peie;
+        int l = eie < 50 ? 1 : 3, l1 = l + 1;
+        R ekin = peie - RM;
+        R brmin = M.ap / ekin;
+        R waux = elke - M.log_ap;
+        R a = p.u, b = p.v, c = p.w;
+        R sinpsi = a * a + b * b, sindel = 0, cosdel = 0;
+        if (sinpsi > (R)1e-20) {
+            sinpsi = dsqrt(sinpsi);
+            sindel = b / sinpsi;
+            cosdel = a / sinpsi;
+        }
+        R ztarg = M.zbrang, tteie = eie / RM;
+        R beta = dsqrt((tteie - 1) * (tteie + 1)) / tteie;
+        R y2max = 2 * beta * (1 + beta) * tteie * tteie, y2maxi = 1 / y2max;
+        R z2max = y2max + 1, z2maxi = dsqrt(z2max);
+        R esg, pese, ese;
+        for (;;) {
+            R rnno06 = rnd(), rnno07 = rnd();
+            R br = brmin * dexp(rnno06 * waux);
+            esg = ekin * br;
+            pese = peie - esg;
+            ese = pese;
+            R delta = esg / eie / ese * M.delcm, aux = ese / eie, phi1, phi2;
+            if (delta < 1) {
+                phi1 = dl[l - 1] + delta * (dl[8 + l - 1] + delta * dl[16 + l - 1]);
+                phi2 = dl[l1 - 1] + delta * (dl[8 + l1 - 1] + delta * dl[16 + l1 - 1]);
+            } else {
+                phi1 = dl[24 + l - 1] + dl[32 + l - 1] * dlog(delta + dl[40 + l - 1]);
+                phi2 = phi1;
+            }
+            R rejf = (1 + aux * aux) * phi1 - 2 * aux * phi2 / 3;
+            if (rnno07 < rejf) break;
+        }
+        // the electron keeps pese; the photon goes on top
+        p.e = pese;
+        if (!push_copy()) return;
+        p.e = esg;
+        p.iq = 0;
+        R y2tst;
+        if (P.ibrdst == 1) {
+            R ttese = ese / RM, esedei = ttese / tteie;
+            R rjarg1 = 1 + esedei * esedei, rjarg2 = rjarg1 + 2 * esedei;
+            R aux = 2 * ese * tteie / esg;
+            aux = aux * aux;
+            R aux1 = aux * ztarg, rjarg3;
+            if (aux1 > 10) rjarg3 = M.lzbrang + (1 - aux1) / (aux1 * aux1);
+            else rjarg3 = dlog(aux / (1 + aux1));
+            R rejmax = rjarg1 * rjarg3 - rjarg2;
+            for (;;) {
+                y2tst = rnd();
+                R rtest = rnd();
+                R aux3 = z2maxi / (y2tst + (1 - y2tst) * z2maxi);
+                rtest = rtest * aux3 * rejmax;
+                y2tst = aux3 * aux3 - 1;
+                R a4 = aux3 * aux3 * aux3 * aux3;
+                R y2tst1 = esedei * y2tst / a4;
+                R aux4 = 16 * y2tst1 - rjarg2, aux5 = rjarg1 - 4 * y2tst1;
+                if (rtest < aux4 + aux5 * rjarg3) break;
+                R aux2 = dlog(aux / (1 + aux1 / a4));
+                if (rtest < aux4 + aux5 * aux2) break;
+            }
+        } else {
+            y2tst = rnd();
+            y2tst = y2tst / (1 - y2tst + y2maxi);
+        }
+        R costhe = 1 - 2 * y2tst * y2maxi;
+        R s2 = (1 - costhe) * (1 + costhe);
+        R sinthe = dsqrt(s2 > 0 ? s2 : (R)0);
+        R cphi, sphi;
+        azimuth(cphi, sphi);
+        if (sinpsi >= (R)1e-10) {
+            R us = sinthe * cphi, vs = sinthe * sphi;
+            p.u = c * cosdel * us - sindel * vs + a * costhe;
+            p.v = c * sindel * us + cosdel * vs + b * costhe;
+            p.w = c * costhe - sinpsi * us;
+        } else {
+            p.u = sinthe * cphi;
+            p.v = sinthe * sphi;
+            p.w = c * costhe;
+        }
+    }
+
+    // ANNIH (egsnrc.mortran:70-184)
+    __device__ void annih() {
+        const R RM = (R)0.5109989461;
+        R pavip = p.e + RM, avip = pavip;
+        R a = avip / RM, g = a - 1, t = g - 1;
+        R pp = dsqrt(a * t), pot = pp / t;
+        R ep0 = 1 / (a + pp);
+        R wsamp = dlog((1 - ep0) / ep0);
+        R aa = p.u, bb = p.v, cc = p.w;
+        R sinpsi = aa * aa + bb * bb, sindel = 0, cosdel = 0;
+        if (sinpsi > (R)1e-20) {
+            sinpsi = dsqrt(sinpsi);
+            sindel = bb / sinpsi;
+            cosdel = aa / sinpsi;
+        }
+        R ep, rejf, rnno02;
+        do {
+            R rnno01 = rnd();
+            ep = ep0 * dexp(rnno01 * wsamp);
+            rnno02 = rnd();
+            R q = ep * a - 1;
+            rejf = 1 - q * q / (ep * (a * a - 2));
+        } while (!(rnno02 <= rejf));
+        R esg1 = avip * ep;
+        R cphi, sphi;
+        R costhe = (esg1 - RM) * pot / esg1;
+        costhe = costhe > 1 ? (R)1 : (costhe < -1 ? (R)-1 : costhe);
+        R sinthe = dsqrt(1 - costhe * costhe);
+        azimuth(cphi, sphi);
+        p.e = esg1;
+        p.iq = 0;
+        auto set_dir = [&](R ct, R st) {
+            if (sinpsi >= (R)1e-10) {
+                R us = st * cphi, vs = st * sphi;
+                p.u = cc * cosdel * us - sindel * vs + aa * ct;
+                p.v = cc * sindel * us + cosdel * vs + bb * ct;
+                p.w = cc * ct - sinpsi * us;
+            } else {
+                p.u = st * cphi;
+                p.v = st * sphi;
+                p.w = cc * ct;
+            }
+        };
+        set_dir(costhe, sinthe);
+        R esg2 = pavip - esg1;
+        if (!push_copy()) return;
+        p.e = esg2;
+        costhe = (esg2 - RM) * pot / esg2;
+        costhe = costhe > 1 ? (R)1 : (costhe < -1 ? (R)-1 : costhe);
+        set_dir(costhe, -dsqrt(1 - costhe * costhe));
     }
 
     __device__ void annih_at_rest() {

@@ -1,0 +1,430 @@
+// egs_gpu_app.cpp -- host application for the voxel dose path.
+//
+// The egs++ application surface the engine drops into: an
+// EGS_AdvancedApplication-like object with initSimulation / runSimulation /
+// finishSimulation (egs++/egs_application.cpp:753-802, 839-917, 1048-1085)
+// reading the reference's .egsinp blocks:
+//   geometry definition   EGS_XYZGeometry (egs_nd_geometry.cpp:1162-1300):
+//                         x/y/z-planes or x/y/z-slabs, media input with
+//                         "set medium" and "set relative density"
+//                         (egs_base_geometry.cpp:742-860)
+//   source definition     egs_parallel_beam (egs_parallel_beam.cpp:40-117)
+//                         with point/rectangle shape and monoenergetic or
+//                         tabulated spectrum (egs_spectra.cpp:482-760)
+//   MC transport parameter (egs_advanced_application.cpp:482-639)
+//   run control           ncase / nbatch (egs_run_control.cpp:64-120)
+//   rng definition        ranmar initial seeds (egs_rndm.cpp:409-463)
+//   ausgab object         egs_dose_scoring .3ddose output
+//                         (egs_dose_scoring.cpp:416-479)
+// Transport runs through the C ABI of include/egsgpu.h.
+//
+// usage: egs_gpu_app -i input.egsinp -p tables.egstab [-o output]
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/egsgpu.h"
+#include "egs_input.h"
+
+using namespace egshost;
+
+namespace {
+
+std::string engine_error() {
+    char buf[1024];
+    egsgpu_last_error(buf, sizeof(buf));
+    return buf;
+}
+
+void check(int rc, const char *what) {
+    if (rc) throw std::runtime_error(std::string(what) + ": " + engine_error());
+}
+
+struct Planes {
+    std::vector<double> p;
+    // EGS_PlanesT(xo, dx, np): p[j+1] = p[j] + dx (egs_planes.h:264-283)
+    static Planes slabs(const std::vector<double> &s) {
+        if (s.size() != 3) throw std::runtime_error("slab input needs 3 values");
+        int n = (int)(s[2] + 0.1);
+        if (n < 1 || s[1] <= 0) throw std::runtime_error("bad slab input");
+        Planes r;
+        r.p.resize(n + 1);
+        r.p[0] = s[0];
+        for (int j = 0; j < n; j++) r.p[j + 1] = r.p[j] + s[1];
+        return r;
+    }
+};
+
+}  // namespace
+
+class EGS_GPUApplication {
+  public:
+    std::string input_file, tables_file, output;
+    std::unique_ptr<InputItem> input;
+    egsgpu_tables *tab = nullptr;
+    egsgpu_handle *h = nullptr;
+    Planes xp, yp, zp;
+    std::vector<int32_t> medium;
+    std::vector<double> rhor;
+    std::vector<std::string> media;    // geometry media names
+    std::vector<double> spec_e, spec_w;
+    EgsSource src{};
+    EgsTransport tp{};
+    int64_t ncase = 0;
+    int nbatch = 10;
+    int ixx = 1802, jxx = 9373;
+    double area = 0;
+    std::string dose_file;
+
+    ~EGS_GPUApplication() {
+        if (h) egsgpu_destroy(h);
+        if (tab) egsgpu_tables_free(tab);
+    }
+
+    int parse_args(int argc, char **argv) {
+        for (int i = 1; i < argc; i++) {
+            std::string a = argv[i];
+            auto next = [&]() -> std::string {
+                if (i + 1 >= argc) throw std::runtime_error("missing value for " + a);
+                return argv[++i];
+            };
+            if (a == "-i") input_file = next();
+            else if (a == "-p") tables_file = next();
+            else if (a == "-o") output = next();
+            else throw std::runtime_error("unknown argument " + a);
+        }
+        if (input_file.empty() || tables_file.empty())
+            throw std::runtime_error("usage: egs_gpu_app -i input.egsinp -p tables.egstab [-o output]");
+        if (output.empty()) {
+            output = input_file;
+            size_t d = output.rfind(".egsinp");
+            if (d != std::string::npos) output = output.substr(0, d);
+        }
+        return 0;
+    }
+
+    void init_geometry() {
+        const InputItem *gd = input->block("geometry definition");
+        if (!gd) throw std::runtime_error("no geometry definition");
+        std::string simname;
+        gd->get("simulation geometry", simname);
+        const InputItem *g = nullptr;
+        for (auto *b : gd->blocks("geometry")) {
+            std::string nm;
+            b->get("name", nm);
+            if (simname.empty() || nm == simname) g = b;
+        }
+        if (!g) throw std::runtime_error("simulation geometry not found");
+        std::string type;
+        g->get("type", type);
+        if (!InputItem::same(type, "EGS_XYZGeometry"))
+            throw std::runtime_error("only EGS_XYZGeometry is supported, got " + type);
+        std::vector<double> v;
+        const char *axes[3][2] = {{"x-planes", "x-slabs"}, {"y-planes", "y-slabs"}, {"z-planes", "z-slabs"}};
+        Planes *P[3] = {&xp, &yp, &zp};
+        for (int a = 0; a < 3; a++) {
+            if (!g->get(axes[a][0], v)) P[a]->p = v;
+            else if (!g->get(axes[a][1], v)) *P[a] = Planes::slabs(v);
+            else throw std::runtime_error(std::string("missing ") + axes[a][0]);
+        }
+        int nx = (int)xp.p.size() - 1, ny = (int)yp.p.size() - 1, nz = (int)zp.p.size() - 1;
+        int nreg = nx * ny * nz;
+        medium.assign(nreg, 0);
+        rhor.assign(nreg, 1.0);
+        const InputItem *mi = g->block("media input");
+        if (!mi) throw std::runtime_error("geometry has no media input");
+        mi->get("media", media);
+        if (media.empty()) throw std::runtime_error("media input lists no media");
+        // setMedia (egs_base_geometry.cpp:786-812)
+        for (auto &c : mi->children) {
+            if (c->is_block) continue;
+            if (InputItem::same(c->key, "set medium")) {
+                std::string val = c->value;
+                for (char &ch : val) if (ch == ',') ch = ' ';
+                std::istringstream is(val);
+                std::vector<int> s;
+                int x;
+                while (is >> x) s.push_back(x);
+                int start, end, med, delta = 1;
+                if (s.size() == 2) { start = end = s[0]; med = s[1]; }
+                else if (s.size() == 3) { start = s[0]; end = s[1]; med = s[2]; }
+                else if (s.size() == 4) { start = s[0]; end = s[1]; med = s[2]; delta = s[3]; }
+                else throw std::runtime_error("set medium needs 2-4 values");
+                if (med < 0 || med >= (int)media.size()) throw std::runtime_error("set medium index out of range");
+                start = std::max(start, 0);
+                end = std::min(end, nreg - 1);
+                for (int j = start; j <= end; j += std::max(delta, 1)) medium[j] = med;
+            } else if (InputItem::same(c->key, "set relative density")) {
+                std::string val = c->value;
+                for (char &ch : val) if (ch == ',') ch = ' ';
+                std::istringstream is(val);
+                std::vector<double> s;
+                double x;
+                while (is >> x) s.push_back(x);
+                int start, end;
+                double rho;
+                if (s.size() == 2) { start = end = (int)(s[0] + 0.1); rho = s[1]; }
+                else if (s.size() == 3) { start = (int)(s[0] + 0.1); end = (int)(s[1] + 0.1); rho = s[2]; }
+                else throw std::runtime_error("set relative density needs 2 or 3 values");
+                start = std::max(start, 0);
+                end = std::min(end, nreg - 1);
+                for (int j = start; j <= end; j++) rhor[j] = rho;
+            }
+        }
+    }
+
+    void init_source() {
+        const InputItem *sd = input->block("source definition");
+        if (!sd) throw std::runtime_error("no source definition");
+        std::string simname;
+        sd->get("simulation source", simname);
+        const InputItem *s = nullptr;
+        for (auto *b : sd->blocks("source")) {
+            std::string nm;
+            b->get("name", nm);
+            if (simname.empty() || nm == simname) s = b;
+        }
+        if (!s) throw std::runtime_error("simulation source not found");
+        std::string lib;
+        s->get("library", lib);
+        if (!InputItem::same(lib, "egs_parallel_beam"))
+            throw std::runtime_error("only egs_parallel_beam is supported, got " + lib);
+        int q = 0;
+        s->get("charge", q);
+        src.charge = q;
+        std::vector<double> dir{0, 0, 1};
+        s->get("direction", dir);
+        double nrm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        src.ux = dir[0] / nrm; src.uy = dir[1] / nrm; src.uz = dir[2] / nrm;
+        const InputItem *sh = s->block("shape");
+        if (!sh) throw std::runtime_error("parallel beam needs a shape");
+        std::string st;
+        sh->get("type", st);
+        if (st.empty()) sh->get("library", st);
+        std::vector<double> pos;
+        if (InputItem::same(st, "point")) {
+            if (sh->get("position", pos) || pos.size() != 3) throw std::runtime_error("point shape needs position");
+            src.shape = EGSGPU_SHAPE_POINT;
+            src.x0 = src.x1 = pos[0]; src.y0 = src.y1 = pos[1]; src.z = pos[2];
+            area = 0;
+        } else if (InputItem::same(st, "egs_rectangle") || InputItem::same(st, "rectangle")) {
+            if (sh->get("rectangle", pos) || pos.size() != 4) throw std::runtime_error("rectangle needs 4 values");
+            src.shape = EGSGPU_SHAPE_RECTANGLE;
+            src.x0 = std::min(pos[0], pos[2]); src.x1 = std::max(pos[0], pos[2]);
+            src.y0 = std::min(pos[1], pos[3]); src.y1 = std::max(pos[1], pos[3]);
+            std::vector<double> tr;
+            src.z = 0;
+            if (const InputItem *t = sh->block("transformation"))
+                if (!t->get("translation", tr) && tr.size() == 3) {
+                    src.x0 += tr[0]; src.x1 += tr[0]; src.y0 += tr[1]; src.y1 += tr[1]; src.z = tr[2];
+                }
+            area = (src.x1 - src.x0) * (src.y1 - src.y0);
+        } else {
+            throw std::runtime_error("unsupported shape " + st);
+        }
+        const InputItem *sp = s->block("spectrum");
+        if (!sp) throw std::runtime_error("source needs a spectrum");
+        std::string stype;
+        sp->get("type", stype);
+        if (InputItem::same(stype, "monoenergetic")) {
+            src.spectrum = EGSGPU_SPEC_MONO;
+            if (sp->get("energy", src.energy)) throw std::runtime_error("monoenergetic spectrum needs energy");
+        } else if (InputItem::same(stype, "tabulated spectrum")) {
+            int mode = 1;
+            if (sp->get("energies", spec_e) || sp->get("probabilities", spec_w))
+                throw std::runtime_error("tabulated spectrum needs energies and probabilities");
+            if (sp->get("spectrum mode", mode)) sp->get("spectrum type", mode);
+            if (mode == 1 || mode == 0) {
+                if (spec_e.size() != spec_w.size() + 1 && spec_e.size() == spec_w.size()) {
+                    // per-bin upper edges given with a lower first edge missing
+                    throw std::runtime_error("histogram spectrum needs nbins+1 energies");
+                }
+                if (mode == 0)   // counts per bin -> per MeV
+                    for (size_t k = 0; k < spec_w.size(); k++) spec_w[k] /= (spec_e[k + 1] - spec_e[k]);
+                src.spectrum = EGSGPU_SPEC_HISTOGRAM;
+            } else if (mode == 2) {
+                src.spectrum = EGSGPU_SPEC_LINE;
+            } else {
+                throw std::runtime_error("spectrum mode 3 (interpolated) is not supported");
+            }
+            src.nbins = (int)spec_w.size();
+            src.spec_e = spec_e.data();
+            src.spec_w = spec_w.data();
+        } else {
+            throw std::runtime_error("unsupported spectrum type " + stype);
+        }
+    }
+
+    void init_transport() {
+        tp = EgsTransport{};
+        tp.rayleigh = 1; tp.bound_compton = 3; tp.photoelectron_angle = 1;
+        tp.atomic_relaxation = 1; tp.pair_angle = 1; tp.brems_angle = 1;
+        tp.spin = 1; tp.exact_bca = 1; tp.estepe = 0.25; tp.ximax = 0.5;
+        tp.skin_depth = 3; tp.kerma = 0; tp.fp64 = 0;
+        const InputItem *t = input->block("MC transport parameter");
+        if (!t) return;
+        auto onoff = [&](const char *k, int &dst) {
+            std::string v;
+            if (!t->get(k, v)) dst = !(InputItem::same(v, "off") || InputItem::same(v, "no") || v == "0");
+        };
+        t->get("Global ECUT", tp.ecut);
+        t->get("Global PCUT", tp.pcut);
+        onoff("Rayleigh scattering", tp.rayleigh);
+        onoff("Spin effects", tp.spin);
+        onoff("Photoelectron angular sampling", tp.photoelectron_angle);
+        onoff("Atomic relaxations", tp.atomic_relaxation);
+        std::string v;
+        if (!t->get("Bound Compton scattering", v)) {
+            if (InputItem::same(v, "Off")) tp.bound_compton = 0;
+            else if (InputItem::same(v, "norej")) tp.bound_compton = 3;
+            else throw std::runtime_error("Bound Compton scattering = " + v + " not supported");
+        }
+        if (!t->get("Boundary crossing algorithm", v) && !InputItem::same(v, "Exact"))
+            throw std::runtime_error("only the Exact boundary crossing algorithm is supported");
+        if (!t->get("Electron-step algorithm", v) && !InputItem::same(v, "PRESTA-II"))
+            throw std::runtime_error("only PRESTA-II is supported");
+        if (!t->get("Brems angular sampling", v)) tp.brems_angle = InputItem::same(v, "Simple") ? 0 : 1;
+        if (!t->get("Pair angular sampling", v)) tp.pair_angle = InputItem::same(v, "Off") ? 0 : 1;
+        if (!t->get("Brems cross sections", v) && !InputItem::same(v, "BH"))
+            throw std::runtime_error("only BH bremsstrahlung cross sections are supported");
+        t->get("ESTEPE", tp.estepe);
+        t->get("Ximax", tp.ximax);
+        t->get("Skin depth for BCA", tp.skin_depth);
+        int kerma = 0;
+        if (!t->get("kerma approximation", v)) kerma = InputItem::same(v, "yes") || InputItem::same(v, "on");
+        tp.kerma = kerma;
+    }
+
+    void init_run() {
+        if (const InputItem *rc = input->block("run control")) {
+            double nc = 0;
+            if (rc->get("number of histories", nc)) rc->get("ncase", nc);
+            ncase = (int64_t)nc;
+            rc->get("nbatch", nbatch);
+        }
+        if (ncase <= 0) throw std::runtime_error("run control needs ncase > 0");
+        if (nbatch < 1) nbatch = 1;
+        if (const InputItem *r = input->block("rng definition")) {
+            std::vector<int> s;
+            if (!r->get("initial seeds", s) && s.size() == 2) { ixx = s[0]; jxx = s[1]; }
+        }
+        dose_file = output + ".3ddose";
+    }
+
+    // EGS_AdvancedApplication::initSimulation analogue
+    int initSimulation() {
+        input = parse_input_file(input_file);
+        init_geometry();
+        init_source();
+        init_transport();
+        init_run();
+        check(egsgpu_tables_load(tables_file.c_str(), &tab), "loading tables");
+        // map geometry media names to table indices (egsAddMedium)
+        int32_t ntab = 0;
+        check(egsgpu_tables_nmedia(tab, &ntab), "table media");
+        std::vector<int> map(media.size(), -1);
+        mdens.assign(media.size(), 1.0);
+        for (size_t k = 0; k < media.size(); k++) {
+            for (int t = 0; t < ntab; t++) {
+                char nm[64];
+                double rho;
+                check(egsgpu_tables_medium_info(tab, t, nm, sizeof(nm), &rho), "medium info");
+                if (media[k] == nm) { map[k] = t; mdens[k] = rho; }
+            }
+            if (map[k] < 0) throw std::runtime_error("medium " + media[k] + " is not in the table blob");
+        }
+        std::vector<int32_t> tmed(medium.size());
+        for (size_t i = 0; i < medium.size(); i++) tmed[i] = map[medium[i]];
+        EgsXYZGeometry g{};
+        g.nx = (int)xp.p.size() - 1; g.ny = (int)yp.p.size() - 1; g.nz = (int)zp.p.size() - 1;
+        g.xb = xp.p.data(); g.yb = yp.p.data(); g.zb = zp.p.data();
+        g.medium = tmed.data();
+        g.rhor = rhor.data();
+        g.boundary_tolerance = 0;
+        if (tp.spin) {
+            fprintf(stderr, "note: spin effects requested but the spin data are not available; running with Spin effects = Off\n");
+            tp.spin = 0;
+        }
+        check(egsgpu_create(tab, &g, &src, &tp, &h), "creating the simulation");
+        check(egsgpu_seed(h, ixx, jxx), "seeding");
+        return 0;
+    }
+
+    // runSimulation: nbatch batches of ncase/nbatch histories
+    int runSimulation() {
+        int64_t per = std::max<int64_t>(1, ncase / nbatch);
+        int64_t done = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        printf("\n  batch   histories   elapsed (s)\n");
+        for (int b = 0; b < nbatch && done < ncase; b++) {
+            int64_t n = (b == nbatch - 1) ? ncase - done : std::min(per, ncase - done);
+            check(egsgpu_run(h, n), "run");
+            done += n;
+            double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            printf("  %5d  %10lld  %10.3f\n", b + 1, (long long)done, el);
+        }
+        return 0;
+    }
+
+    // finishSimulation: .3ddose output (EGS_DoseScoring::outputDoseFile)
+    int finishSimulation() {
+        int nx = (int)xp.p.size() - 1, ny = (int)yp.p.size() - 1, nz = (int)zp.p.size() - 1;
+        int nreg = nx * ny * nz;
+        std::vector<double> s(nreg), s2(nreg);
+        int64_t n = 0;
+        check(egsgpu_get_tallies(h, s.data(), s2.data(), &n), "tallies");
+        double normE = area > 0 ? area : 1.0;   // per fluence for beams with an area
+        double normD = 1.602e-10 * normE;
+        std::ofstream f(dose_file);
+        f << nx << " " << ny << " " << nz << "\n";
+        for (auto *P : {&xp, &yp, &zp}) {
+            for (double b : P->p) f << (float)b << " ";
+            f << "\n";
+        }
+        std::vector<float> dose(nreg), unc(nreg);
+        double etot = 0;
+        for (int i = 0; i < nreg; i++) {
+            double r = s[i] / n, r2 = s2[i] / n;
+            double dr = r2 - r * r;
+            dr = dr > 0 ? std::sqrt(dr / (n - 1)) : 0;
+            int iz = i / (nx * ny), iy = (i / nx) % ny, ix = i % nx;
+            double vol = (xp.p[ix + 1] - xp.p[ix]) * (yp.p[iy + 1] - yp.p[iy]) * (zp.p[iz + 1] - zp.p[iz]);
+            double mass = vol * rhor[i] * medium_density(medium[i]);
+            dose[i] = (float)(r * normD / mass);
+            unc[i] = (float)(r > 0 ? dr / r : 1.0);
+            etot += r;
+        }
+        for (float d : dose) f << d << " ";
+        f << "\n";
+        for (float u : unc) f << u << " ";
+        f << "\n";
+        printf("\n  histories: %lld   energy deposited per history: %.6g MeV\n", (long long)n, etot);
+        printf("  dose written to %s\n", dose_file.c_str());
+        return 0;
+    }
+
+    std::vector<double> mdens;   // nominal density of each geometry medium
+    double medium_density(int m) { return m < (int)mdens.size() ? mdens[m] : 1.0; }
+};
+
+int main(int argc, char **argv) {
+    try {
+        EGS_GPUApplication app;
+        app.parse_args(argc, argv);
+        app.initSimulation();
+        app.runSimulation();
+        app.finishSimulation();
+    } catch (const std::exception &e) {
+        fprintf(stderr, "egs_gpu_app: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

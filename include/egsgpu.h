@@ -121,6 +121,12 @@ int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
 int egsgpu_tables_free(egsgpu_tables *t);
 int egsgpu_tables_nmedia(const egsgpu_tables *t, int32_t *n);
 
+/* Name (PEGS4 medium name) and nominal density (g/cm^3) of medium i of a
+ * table blob, for mapping the media of a geometry onto the table
+ * (egsAddMedium, egs_advanced_application.cpp:460-464). */
+int egsgpu_tables_medium_info(const egsgpu_tables *t, int32_t i, char *name,
+                              size_t n, double *rho);
+
 /* ---- run control ------------------------------------------------------ */
 /* Create a simulation: EGS_AdvancedApplication::initSimulation analogue
  * (egs_application.cpp:753-802).  Nothing runs yet. */

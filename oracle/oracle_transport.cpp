@@ -68,6 +68,7 @@ struct Shower {
     // UPHI saved state
     double A = 0, B = 0, Cc = 0, cosphi = 1, sinphi = 0;
     int64_t n_overflow = 0;
+    int64_t c_esteps = 0;
 
     Shower(const Problem &p, std::vector<double> &t, std::vector<int> &tl)
         : pb(p), T(p.T), tmp(t), touched(tl) {}
@@ -671,14 +672,726 @@ struct Shower {
     }
 
     // ------------------------------------------------------------ ELECTR
+    // egsnrc.mortran:985-1823 with PRESTA-II, exact boundary crossing
+    // (skin depth in elastic MFPs), spin effects off, EII off, and the
+    // egs++ C-interface callbacks.
+    int lel(const Medium &M, double elke) const {
+        int L = (int)(M.eke1 * elke + M.eke0);
+        return L;
+    }
+    static double ev(const std::vector<double> &a0, const std::vector<double> &a1, int L, double x) {
+        return a1[L - 1] * x + a0[L - 1];
+    }
+    // $COMPUTE-DRANGE (egsnrc.macros:3214-3243)
+    double compute_drange(const Medium &M, int lelec, double e1, double e2, int lel1,
+                          double el1, double el2) const {
+        double fedep = 1 - e2 / e1;
+        double elktmp = 0.5 * (el1 + el2 + 0.25 * fedep * fedep * (1 + fedep * (1 + 0.875 * fedep)));
+        int lelktmp = lel1;
+        double dedxmid, aux;
+        if (lelec < 0) {
+            dedxmid = ev(M.ededx0, M.ededx1, lelktmp, elktmp);
+            dedxmid = 1 / dedxmid;
+            aux = M.ededx1[lelktmp - 1] * dedxmid;
+        } else {
+            dedxmid = ev(M.pdedx0, M.pdedx1, lelktmp, elktmp);
+            dedxmid = 1 / dedxmid;
+            aux = M.pdedx1[lelktmp - 1] * dedxmid;
+        }
+        aux = aux * (1 + 2 * aux) * std::pow(fedep / (2 - fedep), 2) / 6;
+        return fedep * e1 * dedxmid * (1 + aux);
+    }
+    // $COMPUTE-ELOSS (egsnrc.macros:3290-3313)
+    double compute_eloss(const Medium &M, int lelec, double rhof, double step, double e,
+                         double el, int L) const {
+        double dedxmid, aux;
+        if (lelec < 0) {
+            dedxmid = ev(M.ededx0, M.ededx1, L, el);
+            aux = M.ededx1[L - 1] / dedxmid;
+        } else {
+            dedxmid = ev(M.pdedx0, M.pdedx1, L, el);
+            aux = M.pdedx1[L - 1] / dedxmid;
+        }
+        double de = dedxmid * step * rhof;
+        double fedep = de / e;
+        de = de * (1 - 0.5 * fedep * aux * (1 - 0.333333 * fedep * (aux - 1 - 0.25 * fedep * (2 - aux * (4 - aux)))));
+        return de;
+    }
+    // $COMPUTE-ELOSS-G (egsnrc.macros:3315-3352)
+    double compute_eloss_g(const Medium &M, int lelec, int qel, double rhof, double range,
+                           double step, double eke, double elke, int lelke) const {
+        const std::vector<double> &rep = qel ? M.range_ep1 : M.range_ep0;
+        double tuss = range - rep[lelke - 1] / rhof;
+        if (tuss >= step) return compute_eloss(M, lelec, rhof, step, eke, elke, lelke);
+        int lelktmp = lelke;
+        tuss = (range - step) * rhof;
+        if (tuss <= 0) return eke - M.te * 0.99;
+        while (tuss < rep[lelktmp - 1]) lelktmp--;
+        double elktmp = (lelktmp + 1 - M.eke0) / M.eke1;
+        double eketmp = M.e_array[lelktmp];
+        tuss = (rep[lelktmp] - tuss) / rhof;
+        double de = compute_eloss(M, lelec, rhof, tuss, eketmp, elktmp, lelktmp);
+        return de + eke - eketmp;
+    }
+
     void electron() {
+        const double ecut = pb.P.ecut;
+        if (pb.P.kerma) {
+            Particle &p = top();
+            ecut_discard(p.iq, p.e);
+            return;
+        }
+        Particle *pp = &top();
+        int irl = pp->ir;
+        int medium = pb.med[irl];
+        double rhof = pb.rhor[irl];
+    newelectron:
+        pp = &top();
+        int lelec = pp->iq;
+        int qel = (1 + lelec) / 2;
+        double peie = pp->e, eie = peie;
+        if (eie <= ecut) { ecut_discard(lelec, peie); return; }
+        if (pp->wt == 0.0) { user_electron_discard(lelec, 3); return; }
+        for (;;) {   // :TSTEP:
+            bool compute_tstep = true;
+            double eke = eie - RM;
+            const Medium *M = &T.med[medium];
+            double rnne1 = rnd();
+            if (rnne1 == 0.0) rnne1 = 1.e-30;
+            double demfp = std::max(-std::log(rnne1), 1.e-8);
+            double elke = std::log(eke);
+            int lelke = lel(*M, elke);
+            double sig0, sigf, dedx0;
+            auto eval_sigf = [&]() {
+                if (lelec < 0) {
+                    sigf = ev(M->esig0, M->esig1, lelke, elke);
+                    dedx0 = ev(M->ededx0, M->ededx1, lelke, elke);
+                } else {
+                    sigf = ev(M->psig0, M->psig1, lelke, elke);
+                    dedx0 = ev(M->pdedx0, M->pdedx1, lelke, elke);
+                }
+                sigf = sigf / dedx0;
+            };
+            // $EVALUATE-SIG0
+            if (M->sig_mono[qel]) { eval_sigf(); sig0 = sigf; }
+            else sig0 = lelec < 0 ? M->esig_e : M->psig_e;
+            double total_de = 0, total_tstep = 0;
+            bool restart_tstep = false;
+            do {   // :USTEP:
+                Particle &p = top();
+                double sig = sig0, tstep;
+                if (sig <= 0) {
+                    tstep = 1e8;   // vacdst
+                    sig0 = 1.E-15;
+                } else {
+                    // $CALCULATE-TSTEP-FROM-DEMFP
+                    if (compute_tstep) {
+                        total_de = demfp / sig;
+                        double fedep = total_de;
+                        double ekef = eke - fedep;
+                        if (ekef <= M->e_array[0]) {
+                            tstep = 1e8;
+                        } else {
+                            double elkef = std::log(ekef);
+                            int lelkef = lel(*M, elkef);
+                            if (lelkef == lelke) {
+                                tstep = compute_drange(*M, lelec, eke, ekef, lelke, elke, elkef);
+                            } else {
+                                double ekei = M->e_array[lelke - 1];
+                                double elkei = (lelke - M->eke0) / M->eke1;
+                                double tuss = compute_drange(*M, lelec, eke, ekei, lelke, elke, elkei);
+                                ekei = M->e_array[lelkef];
+                                elkei = (lelkef + 1 - M->eke0) / M->eke1;
+                                tstep = compute_drange(*M, lelec, ekei, ekef, lelkef, elkei, elkef);
+                                const std::vector<double> &rep = qel ? M->range_ep1 : M->range_ep0;
+                                tstep = tstep + tuss + rep[lelke - 1] - rep[lelkef];
+                            }
+                        }
+                        total_tstep = tstep;
+                        compute_tstep = false;
+                    }
+                    tstep = total_tstep / rhof;
+                }
+                if (lelec < 0) dedx0 = ev(M->ededx0, M->ededx1, lelke, elke);
+                else dedx0 = ev(M->pdedx0, M->pdedx1, lelke, elke);
+                double dedx = rhof * dedx0;
+                double tmxs = ev(M->tmxs0, M->tmxs1, lelke, elke);
+                tmxs = tmxs / rhof;
+                // $COMPUTE-RANGE
+                double range;
+                {
+                    double ekei = M->e_array[lelke - 1];
+                    double elkei = (lelke - M->eke0) / M->eke1;
+                    range = compute_drange(*M, lelec, eke, ekei, lelke, elke, elkei);
+                    const std::vector<double> &rep = qel ? M->range_ep1 : M->range_ep0;
+                    range = (range + rep[lelke - 1]) / rhof;
+                }
+                tmxs = std::min(tmxs, 1e10);   // smaxir
+                double tustep = std::min(tstep, std::min(tmxs, range));
+                double tperp = pb.geo.hownear(p.ir, p.x, p.y, p.z);
+                p.dnear = tperp;
+                // $SET-SKINDEPTH / $CALCULATE-ELASTIC-SCATTERING-MFP
+                double blccl = rhof * M->blcc;
+                double p2 = eke * (eke + RMT2), beta2 = p2 / (p2 + RMSQ);
+                double ssmfp = beta2 / blccl;
+                double skindepth = pb.P.skindepth * ssmfp;
+                tustep = std::min(tustep, std::max(tperp, skindepth));
+                bool callhowfar, dosingle = false, callmsdist;
+                double ustep, de;
+                double uscat = 0, vscat = 0, wscat = 0, xtrans = 0, ytrans = 0, ztrans = 0;
+                if (tustep <= tperp && tustep > skindepth) {
+                    callhowfar = false;
+                    dosingle = false;
+                    callmsdist = true;
+                    de = compute_eloss_g(*M, lelec, qel, rhof, range, tustep, eke, elke, lelke);
+                    msdist_pII(*M, eke, de, tustep, rhof, p.u, p.v, p.w, p.x, p.y, p.z,
+                               uscat, vscat, wscat, xtrans, ytrans, ztrans, ustep);
+                } else {
+                    callmsdist = false;
+                    double rnnoss = rnd();
+                    if (rnnoss < 1.e-30) rnnoss = 1.e-30;
+                    double lambda = -std::log(1 - rnnoss);
+                    double lambda_max = 0.5 * blccl * RM / dedx * std::pow(eke / RM + 1, 3);
+                    if (lambda >= 0 && lambda_max > 0) {
+                        double tuss = lambda < lambda_max ? lambda * ssmfp * (1 - 0.5 * lambda / lambda_max)
+                                                          : 0.5 * lambda * ssmfp;
+                        if (tuss < tustep) { tustep = tuss; dosingle = true; }
+                        else dosingle = false;
+                    } else {
+                        stk.pop_back();
+                        return;
+                    }
+                    ustep = tustep;
+                    callhowfar = !(ustep < tperp);
+                }
+                int irold = p.ir, irnew = p.ir, idisc = 0;
+                if (callhowfar || p.wt <= 0) {
+                    int inew = pb.geo.howfar(p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+                    if (inew != p.ir) {
+                        irnew = inew;
+                        if (inew < 0) idisc = -1;
+                    }
+                }
+                if (ustep <= 0) ustep = 0;   // $CHECK-NEGATIVE-USTEP
+                if (ustep == 0) {
+                    if (irnew != irold) {
+                        p.ir = irnew; irl = irnew;
+                        if (irnew >= 0) { medium = pb.med[irnew]; rhof = pb.rhor[irnew]; }
+                    }
+                    if (eie <= ecut) { ecut_discard(lelec, peie); return; }
+                    restart_tstep = true;
+                    break;
+                }
+                double vstep = ustep, tvstep;
+                if (callhowfar) {
+                    tvstep = vstep;
+                    if (tvstep != tustep) dosingle = false;
+                    de = compute_eloss_g(*M, lelec, qel, rhof, range, tvstep, eke, elke, lelke);
+                } else {
+                    tvstep = tustep;
+                    if (!callmsdist)
+                        de = compute_eloss_g(*M, lelec, qel, rhof, range, tvstep, eke, elke, lelke);
+                }
+                double save_de = de;
+                double edep = de;
+                double ekef = eke - de;
+                double costhe = 1, sinthe = 0;
+                if (!callmsdist && dosingle) {
+                    double ekems = std::max(ekef, ecut - RM);
+                    double p2s = ekems * (ekems + RMT2);
+                    double beta2s = p2s / (p2s + RMSQ);
+                    double chia2 = M->xcc / (4 * M->blcc * p2s);
+                    (void)beta2s;
+                    sscat(chia2, costhe, sinthe);
+                }
+                double xf, yf, zf, uf, vf, wf;
+                if (callmsdist) {
+                    uf = uscat; vf = vscat; wf = wscat;
+                    xf = xtrans; yf = ytrans; zf = ztrans;
+                } else {
+                    xf = p.x + p.u * vstep;
+                    yf = p.y + p.v * vstep;
+                    zf = p.z + p.w * vstep;
+                    if (dosingle) {
+                        double ut = p.u, vt = p.v, wt = p.w;
+                        uphi21(costhe, sinthe);
+                        uf = p.u; vf = p.v; wf = p.w;
+                        p.u = ut; p.v = vt; p.w = wt;
+                    } else {
+                        uf = p.u; vf = p.v; wf = p.w;
+                    }
+                }
+                score(p.ir, edep, p.wt);   // ausgab(TRANAUSB): energy lost along the step
+                c_esteps++;
+                p.x = xf; p.y = yf; p.z = zf;
+                p.u = uf; p.v = vf; p.w = wf;
+                p.dnear = p.dnear - vstep;
+                irold = p.ir;
+                peie = peie - edep;
+                eie = peie;
+                p.e = peie;
+                if (irnew == irl && eie <= ecut) { ecut_discard(lelec, peie); return; }
+                int medold = medium;
+                eke = eie - RM;
+                elke = std::log(eke);
+                lelke = lel(*M, elke);
+                if (irnew != irold) {   // $electron_region_change
+                    p.ir = irnew; irl = irnew;
+                    if (irnew >= 0) { medium = pb.med[irnew]; rhof = pb.rhor[irnew]; }
+                }
+                if (eie <= ecut) { ecut_discard(lelec, peie); return; }
+                if (idisc < 0) { user_electron_discard(lelec, 1); return; }
+                if (medium != medold) { restart_tstep = true; break; }
+                // $UPDATE-DEMFP
+                demfp = demfp - save_de * sig;
+                total_de = total_de - save_de;
+                total_tstep = total_tstep - tvstep * rhof;
+                if (total_tstep < 1e-9) demfp = 0;
+            } while (!(demfp < 1.e-8));
+            if (restart_tstep) continue;
+            eval_sigf();
+            double sigratio = sigf / sig0;
+            double rfict = rnd();
+            if (rfict <= sigratio) {
+                // ---- discrete interaction
+                Particle &p = top();
+                if (lelec < 0) {
+                    double ebr1 = ev(M->ebr10, M->ebr11, lelke, elke);
+                    double rnno24 = rnd();
+                    if (rnno24 <= ebr1) {
+                        brems(*M, elke);
+                        if (top().iq == 0) return;
+                        goto newelectron;
+                    }
+                    if (p.e <= M->thmoll) {
+                        if (ebr1 <= 0) goto newelectron;
+                        brems(*M, elke);
+                        if (top().iq == 0) return;
+                        goto newelectron;
+                    }
+                    moller(*M);
+                    if (top().iq == 0) return;
+                    goto newelectron;
+                }
+                double pbr1 = ev(M->pbr10, M->pbr11, lelke, elke);
+                double rnno25 = rnd();
+                if (rnno25 < pbr1) {
+                    brems(*M, elke);
+                    if (top().iq == 0) return;
+                    goto newelectron;
+                }
+                double pbr2 = ev(M->pbr20, M->pbr21, lelke, elke);
+                if (rnno25 < pbr2) {
+                    bhabha(*M);
+                    if (top().iq == 0) return;
+                    goto newelectron;
+                }
+                annih();
+                return;
+            }
+        }
+    }
+
+    // :USER-ELECTRON-DISCARD: (egsnrc.mortran:1802-1812)
+    void user_electron_discard(int lelec, int idisc) {
         Particle &p = top();
-        int irl = p.ir;
-        int lelec = p.iq;
+        double edep = (lelec < 0 || idisc == 99) ? p.e - PRM : p.e + PRM;
+        score(p.ir, edep, p.wt);   // outside the grid (ir < 0) nothing is scored
+        stk.pop_back();
+    }
+
+    // msdist_pII (egsnrc.mortran:4050-4308), spin effects off
+    void msdist_pII(const Medium &M, double e0, double eloss, double tustep, double rhof,
+                    double u0, double v0, double w0, double x0, double y0, double z0,
+                    double &us, double &vs, double &ws, double &xf, double &yf,
+                    double &zf, double &ustep) {
+        double blccc = M.blcc, xcccc = M.xcc;
+        double e = e0 - 0.5 * eloss;
+        double tau = e / PRM, tau2 = tau * tau;
+        double epsilon = eloss / e0, epsilonp = eloss / e;
+        e = e * (1 - epsilonp * epsilonp * (6 + 10 * tau + 5 * tau2) / (24 * tau2 + 72 * tau + 48));
+        double p2 = e * (e + RMT2);
+        double beta2 = p2 / (p2 + RMSQ);
+        double chia2 = xcccc / (4 * p2 * blccc);
+        double lambda = 0.5 * tustep * rhof * blccc / beta2;
+        double temp2 = 0.166666 * (4 + tau * (6 + tau * (7 + tau * (4 + tau)))) *
+                       std::pow(epsilonp / ((tau + 1) * (tau + 2)), 2);
+        lambda = lambda * (1 - temp2);
+        double etap = 1, xi_corr = 1, gamma = 1, ms_corr = 1;
+        chia2 = chia2 * etap;
+        lambda = lambda / (etap * (1 + chia2)) * ms_corr;
+        double chilog = std::log(1 + 1 / chia2);
+        double q1 = 2 * chia2 * (chilog * (1 + chia2) - 1);
+        gamma = 6 * chia2 * (1 + chia2) * (chilog * (1 + 2 * chia2) - 2) / q1 * gamma;
+        double xi = q1 * lambda;
+        bool find_index = true;
+        double w1, sint1, w2, sint2, cphi1, sphi1, cphi2, sphi2;
+        mscat(lambda, chia2, xi, find_index, w1, sint1);
+        azimuth(cphi1, sphi1);
+        mscat(lambda, chia2, xi, find_index, w2, sint2);
+        azimuth(cphi2, sphi2);
+        double u2 = sint2 * cphi2, v2 = sint2 * sphi2;
+        double u2p = w1 * u2 + sint1 * w2;
+        us = u2p * cphi1 - v2 * sphi1;
+        vs = u2p * sphi1 + v2 * cphi1;
+        ws = w1 * w2 - sint1 * u2;
+        xi = 2 * xi * xi_corr;
+        double eta = rnd();
+        eta = std::sqrt(eta);
+        double eta1 = 0.5 * (1 - eta);
+        double delta = 0.9082483 - (0.1020621 - 0.0263747 * gamma) * xi;
+        double temp1 = 2 + tau;
+        double temp = (2 + tau * temp1) / ((tau + 1) * temp1);
+        temp = temp - (tau + 1) / ((tau + 2) * (chilog * (1 + chia2) - 1));
+        temp = temp * epsilonp;
+        temp1 = 1 - temp;
+        delta = delta + 0.40824829 * (epsilon * (tau + 1) / ((tau + 2) * (chilog * (1 + chia2) - 1) *
+                                                             (chilog * (1 + 2 * chia2) - 2)) -
+                                      0.25 * temp * temp);
+        double b = eta * delta, c = eta * (1 - delta);
+        double w1v2 = w1 * v2;
+        double ut = b * sint1 * cphi1 + c * (cphi1 * u2 - sphi1 * w1v2) + eta1 * us * temp1;
+        double vt = b * sint1 * sphi1 + c * (sphi1 * u2 + cphi1 * w1v2) + eta1 * vs * temp1;
+        double wt = eta1 * (1 + temp) + b * w1 + c * w2 + eta1 * ws * temp1;
+        ustep = tustep * std::sqrt(ut * ut + vt * vt + wt * wt);
+        double sint02 = u0 * u0 + v0 * v0;
+        if (sint02 > 1e-20) {
+            double sint0 = std::sqrt(sint02), sint0i = 1 / sint0;
+            double cphi0 = sint0i * u0, sphi0 = sint0i * v0;
+            u2p = w0 * us + sint0 * ws;
+            ws = w0 * ws - sint0 * us;
+            us = u2p * cphi0 - vs * sphi0;
+            vs = u2p * sphi0 + vs * cphi0;
+            u2p = w0 * ut + sint0 * wt;
+            wt = w0 * wt - sint0 * ut;
+            ut = u2p * cphi0 - vt * sphi0;
+            vt = u2p * sphi0 + vt * cphi0;
+        } else {
+            wt = w0 * wt;
+            ws = w0 * ws;
+        }
+        xf = x0 + tustep * ut;
+        yf = y0 + tustep * vt;
+        zf = z0 + tustep * wt;
+    }
+
+    // mscat (egsnrc.mortran:2984-3160), spin effects off.  ms_i/ms_j/
+    // ms_omega2 persist between the two calls of one msdist_pII step.
+    int ms_i = 0, ms_j = 0;
+    double ms_omega2 = 0;
+    void mscat(double lambda, double chia2, double q1, bool &find_index, double &cost,
+               double &sint) {
+        if (lambda <= 13.8) {
+            double sprob = rnd();
+            double explambda = std::exp(-lambda);
+            if (sprob < explambda) { cost = 1; sint = 0; return; }
+            double wsum = (1 + lambda) * explambda;
+            if (sprob < wsum) {
+                double xi = rnd();
+                xi = 2 * chia2 * xi / (1 - xi + chia2);
+                cost = 1 - xi;
+                sint = std::sqrt(xi * (2 - xi));
+                return;
+            }
+            if (lambda <= 1) {
+                double wprob = explambda;
+                wsum = explambda;
+                cost = 1; sint = 0;
+                int icount = 0;
+                do {
+                    icount++;
+                    if (icount > 20) break;
+                    wprob = wprob * lambda / icount;
+                    wsum = wsum + wprob;
+                    double xi = rnd();
+                    xi = 2 * chia2 * xi / (1 - xi + chia2);
+                    double cosz = 1 - xi;
+                    double sinz = xi * (2 - xi);
+                    if (sinz > 1.e-20) {
+                        sinz = std::sqrt(sinz);
+                        xi = rnd();
+                        double phi = xi * 6.2831853;
+                        cost = cost * cosz - sint * sinz * std::cos(phi);
+                        sint = std::sqrt(std::max(0.0, (1 - cost) * (1 + cost)));
+                    }
+                } while (!(wsum > sprob));
+                return;
+            }
+        }
+        if (lambda <= 1e5) {
+            const std::vector<double> &mc = T.ms_const;   // llammin..dqmsi (real*4)
+            double dllambi = mc[3], dqmsi = mc[5];
+            if (find_index) {
+                double llmbda = std::log(lambda);
+                double ai = llmbda * dllambi;
+                int i = (int)ai;
+                ai = ai - i;
+                double xi = rnd();
+                if (xi < ai) i = i + 1;
+                int j;
+                if (q1 < 1e-3) j = 0;
+                else if (q1 < 0.5) {
+                    double aj = q1 * dqmsi;
+                    j = (int)aj;
+                    aj = aj - j;
+                    xi = rnd();
+                    if (xi < aj) j = j + 1;
+                } else j = 7;
+                if (llmbda < 2.2299)
+                    ms_omega2 = chia2 * (lambda + 4) *
+                                (1.347006 + llmbda * (0.209364 - llmbda * (0.45525 - llmbda * (0.50142 - 0.081234 * llmbda))));
+                else
+                    ms_omega2 = chia2 * (lambda + 4) *
+                                (-2.77164 + llmbda * (2.94874 - llmbda * (0.1535754 - llmbda * 0.00552888)));
+                ms_i = i; ms_j = j;
+                find_index = false;
+            }
+            double xi = rnd();
+            double ak = xi * 31;
+            int k = (int)ak;
+            ak = ak - k;
+            int base = (ms_i * 8 + ms_j) * 32;
+            if (ak > T.ms_w[base + k]) k = T.ms_i[base + k];
+            double a = T.ms_f[base + k], u = T.ms_u[base + k];
+            double du = T.ms_u[base + k + 1] - u;
+            xi = rnd();
+            if (std::fabs(a) < 0.2) {
+                double x1 = 0.5 * (1 - xi) * a;
+                u = u + xi * du * (1 + x1 * (1 - xi * a));
+            } else {
+                u = u - du / a * (1 - std::sqrt(1 + xi * a * (2 + a)));
+            }
+            xi = ms_omega2 * u / (1 + 0.5 * ms_omega2 - u);
+            if (xi > 1.99999) xi = 1.99999;
+            cost = 1 - xi;
+            sint = std::sqrt(xi * (2 - xi));
+            return;
+        }
+        throw std::runtime_error("maximum step size in mscat exceeded");
+    }
+
+    // sscat (egsnrc.mortran:3226-3262), spin effects off
+    void sscat(double chia2, double &cost, double &sint) {
+        double xi = rnd();
+        xi = 2 * chia2 * xi / (1 - xi + chia2);
+        cost = 1 - xi;
+        sint = std::sqrt(xi * (2 - xi));
+    }
+
+    // MOLLER (egsnrc.mortran:2599-2746), EII off
+    void moller(const Medium &M) {
+        Particle &p = top();
+        double peie = p.e, pekin = peie - PRM, ekin = pekin;
+        if (ekin <= 2 * M.te) return;
+        double t0 = ekin / RM, e0 = t0 + 1.0;
+        double extrae = peie - M.thmoll;
+        double e02 = e0 * e0;
+        double g2 = t0 * t0 / e02, g3 = (2. * t0 + 1.) / e02;
+        double gmax = (1. + 1.25 * g2);
+        double br, rejf4, rnno28;
+        do {
+            double rnno27 = rnd();
+            br = M.te / (ekin - extrae * rnno27);
+            double r = br / (1. - br);
+            rnno28 = rnd();
+            rejf4 = (1. + g2 * br * br + r * (r - g3));
+            rnno28 = gmax * rnno28;
+        } while (!(rnno28 <= rejf4));
+        double pekse2 = br * ekin;
+        double pese1 = peie - pekse2, pese2 = pekse2 + PRM;
+        p.e = pese1;
+        double h1 = (peie + PRM) / pekin;
+        double dcosth = h1 * (pese1 - PRM) / (pese1 + PRM);
+        double sinthe = std::sqrt(1.0 - dcosth), costhe = std::sqrt(dcosth);
+        uphi21(costhe, sinthe);
+        push_copy();
+        top().iq = -1;
+        top().e = pese2;
+        dcosth = h1 * (pese2 - PRM) / (pese2 + PRM);
+        sinthe = -std::sqrt(1.0 - dcosth);
+        costhe = std::sqrt(dcosth);
+        uphi32(costhe, sinthe);
+    }
+
+    // BHABHA (egsnrc.mortran:229-307)
+    void bhabha(const Medium &M) {
+        Particle &p = top();
+        double peip = p.e, pekin = peip - PRM, ekin = pekin;
+        double t0 = ekin / RM, e0 = t0 + 1.;
+        double yy = 1. / (t0 + 2.), e02 = e0 * e0;
+        double beta2 = (e02 - 1.) / e02;
+        double ep0 = M.te / ekin, ep0c = 1. - ep0;
+        double y2 = yy * yy, yp = 1. - 2. * yy, yp2 = yp * yp;
+        double b4 = yp2 * yp, b3 = b4 + yp2, b2 = yp * (3. + y2), b1 = 2. - y2;
+        double br, rejf2, rnno04;
+        do {
+            double rnno03 = rnd();
+            br = ep0 / (1. - ep0c * rnno03);
+            rnno04 = rnd();
+            rejf2 = (1.0 - beta2 * br * (b1 - br * (b2 - br * (b3 - br * b4))));
+        } while (!(rnno04 <= rejf2));
+        int iq_np = 1, iq_new = -1;
+        if (!(br < 0.5)) { iq_np = -1; iq_new = 1; br = 1. - br; }
+        br = std::max(br, 0.0);
+        double pekse2 = br * ekin;
+        double pese1 = peip - pekse2, pese2 = pekse2 + PRM;
+        p.e = pese1;
+        p.iq = iq_np;
+        double h1 = (peip + PRM) / pekin;
+        double dcosth = std::min(1.0, h1 * (pese1 - PRM) / (pese1 + PRM));
+        double sinthe = std::sqrt(1.0 - dcosth), costhe = std::sqrt(dcosth);
+        uphi21(costhe, sinthe);
+        push_copy();
+        top().e = pese2;
+        top().iq = iq_new;
+        dcosth = h1 * (pese2 - PRM) / (pese2 + PRM);
+        sinthe = -std::sqrt(1.0 - dcosth);
+        costhe = std::sqrt(dcosth);
+        uphi32(costhe, sinthe);
+    }
+
+    // BREMS (egsnrc.mortran:312-526): Bethe-Heitler (ibr_nist = 0), no
+    // splitting, photon angle from the Koch-Motz 2BS (ibrdst = 1) or the
+    // leading term (ibrdst = 0)
+    void brems(const Medium &M, double elke) {
+        Particle &p = top();
         double peie = p.e, eie = peie;
-        if (eie <= pb.P.ecut) { ecut_discard(lelec, peie); return; }
-        if (pb.P.kerma) { ecut_discard(lelec, peie); return; }
-        throw std::runtime_error("charged-particle transport above ECUT is not restated yet");
+        int l = eie < 50.0 ? 1 : 3, l1 = l + 1;
+        double ekin = peie - PRM;
+        double brmin = M.ap / ekin;
+        double waux = elke - M.log_ap;
+        double a = p.u, b = p.v, c = p.w;
+        double sinpsi = a * a + b * b, sindel = 0, cosdel = 0;
+        if (sinpsi > 1e-20) {
+            sinpsi = std::sqrt(sinpsi);
+            sindel = b / sinpsi;
+            cosdel = a / sinpsi;
+        }
+        double ztarg = M.zbrang, tteie = eie / RM;
+        double beta = std::sqrt((tteie - 1) * (tteie + 1)) / tteie;
+        double y2max = 2 * beta * (1 + beta) * tteie * tteie, y2maxi = 1 / y2max;
+        double z2max = y2max + 1, z2maxi = std::sqrt(z2max);
+        double esg, pesg, pese, ese;
+        const double(*dl)[8] = M.dl;
+        for (;;) {
+            double rnno06 = rnd(), rnno07 = rnd();
+            double br = brmin * std::exp(rnno06 * waux);
+            esg = ekin * br;
+            pesg = esg;
+            pese = peie - pesg;
+            ese = pese;
+            double delta = esg / eie / ese * M.delcm, aux = ese / eie, phi1, phi2;
+            if (delta < 1) {
+                phi1 = dl[0][l - 1] + delta * (dl[1][l - 1] + delta * dl[2][l - 1]);
+                phi2 = dl[0][l1 - 1] + delta * (dl[1][l1 - 1] + delta * dl[2][l1 - 1]);
+            } else {
+                phi1 = dl[3][l - 1] + dl[4][l - 1] * std::log(delta + dl[5][l - 1]);
+                phi2 = phi1;
+            }
+            double rejf = (1 + aux * aux) * phi1 - 2 * aux * phi2 / 3;
+            if (rnno07 < rejf) break;
+        }
+        double weight = p.wt;
+        push_copy();
+        Particle &g = top();
+        g.e = pesg;
+        g.iq = 0;
+        g.wt = weight;
+        double y2tst;
+        if (pb.P.ibrdst == 1) {
+            double ttese = ese / RM, esedei = ttese / tteie;
+            double rjarg1 = 1 + esedei * esedei, rjarg2 = rjarg1 + 2 * esedei;
+            double aux = 2 * ese * tteie / esg;
+            aux = aux * aux;
+            double aux1 = aux * ztarg, rjarg3;
+            if (aux1 > 10) rjarg3 = M.lzbrang + (1 - aux1) / (aux1 * aux1);
+            else rjarg3 = std::log(aux / (1 + aux1));
+            double rejmax = rjarg1 * rjarg3 - rjarg2;
+            for (;;) {
+                y2tst = rnd();
+                double rtest = rnd();
+                double aux3 = z2maxi / (y2tst + (1 - y2tst) * z2maxi);
+                rtest = rtest * aux3 * rejmax;
+                y2tst = aux3 * aux3 - 1;
+                double y2tst1 = esedei * y2tst / (aux3 * aux3 * aux3 * aux3);
+                double aux4 = 16 * y2tst1 - rjarg2, aux5 = rjarg1 - 4 * y2tst1;
+                if (rtest < aux4 + aux5 * rjarg3) break;
+                double aux2 = std::log(aux / (1 + aux1 / (aux3 * aux3 * aux3 * aux3)));
+                double rejtst = aux4 + aux5 * aux2;
+                if (rtest < rejtst) break;
+            }
+        } else {
+            y2tst = rnd();
+            y2tst = y2tst / (1 - y2tst + y2maxi);
+        }
+        double costhe = 1 - 2 * y2tst * y2maxi;
+        double sinthe = std::sqrt(std::max((1 - costhe) * (1 + costhe), 0.0));
+        double cphi, sphi;
+        azimuth(cphi, sphi);
+        Particle &gg = top();
+        if (sinpsi >= 1e-10) {
+            double us = sinthe * cphi, vs = sinthe * sphi;
+            gg.u = c * cosdel * us - sindel * vs + a * costhe;
+            gg.v = c * sindel * us + cosdel * vs + b * costhe;
+            gg.w = c * costhe - sinpsi * us;
+        } else {
+            gg.u = sinthe * cphi;
+            gg.v = sinthe * sphi;
+            gg.w = c * costhe;
+        }
+        stk[stk.size() - 2].e = pese;
+    }
+
+    // ANNIH (egsnrc.mortran:70-184), two-photon in-flight annihilation
+    void annih() {
+        Particle &p = top();
+        double pavip = p.e + PRM, avip = pavip;
+        double a = avip / RM, g = a - 1.0, t = g - 1.0;
+        double pp = std::sqrt(a * t), pot = pp / t;
+        double ep0 = 1.0 / (a + pp);
+        double wsamp = std::log((1.0 - ep0) / ep0);
+        double aa = p.u, bb = p.v, cc = p.w;
+        double sinpsi = aa * aa + bb * bb, sindel = 0, cosdel = 0;
+        if (sinpsi > 1e-20) {
+            sinpsi = std::sqrt(sinpsi);
+            sindel = bb / sinpsi;
+            cosdel = aa / sinpsi;
+        }
+        double ep, rejf, rnno02;
+        do {
+            double rnno01 = rnd();
+            ep = ep0 * std::exp(rnno01 * wsamp);
+            rnno02 = rnd();
+            rejf = 1 - (ep * a - 1) * (ep * a - 1) / (ep * (a * a - 2));
+        } while (!(rnno02 <= rejf));
+        double esg1 = avip * ep;
+        p.e = esg1;
+        p.iq = 0;
+        double costhe = std::max(-1.0, std::min(1.0, (esg1 - RM) * pot / esg1));
+        double sinthe = std::sqrt(1.0 - costhe * costhe);
+        double cphi, sphi;
+        azimuth(cphi, sphi);
+        auto set_dir = [&](Particle &q, double ct, double st) {
+            if (sinpsi >= 1e-10) {
+                double us = st * cphi, vs = st * sphi;
+                q.u = cc * cosdel * us - sindel * vs + aa * ct;
+                q.v = cc * sindel * us + cosdel * vs + bb * ct;
+                q.w = cc * ct - sinpsi * us;
+            } else {
+                q.u = st * cphi;
+                q.v = st * sphi;
+                q.w = cc * ct;
+            }
+        };
+        set_dir(p, costhe, sinthe);
+        double pesg2 = pavip - esg1, esg2 = pesg2;
+        push_copy();
+        Particle &q = top();
+        q.e = pesg2;
+        q.iq = 0;
+        costhe = std::max(-1.0, std::min(1.0, (esg2 - RM) * pot / esg2));
+        sinthe = -std::sqrt(1.0 - costhe * costhe);
+        set_dir(q, costhe, sinthe);
     }
 
     // :ECUT-DISCARD: / :POSITRON-ANNIHILATION: (egsnrc.mortran:1773-1800)

@@ -81,3 +81,56 @@ def test_kerma_dose_parity(engine, fp64):
     # total energy deposited agrees to within 4 combined sigma
     ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
     assert abs(ta - tb) < 0.01 * ta
+
+
+def electron_case(n=12, dx=1.0, energy=6.0):
+    """Electron beam on a water phantom with a bone slab (coupled transport,
+    ECUT 0.521 MeV total, PCUT 0.01 MeV)."""
+    xb = np.linspace(-n * dx / 2, n * dx / 2, n + 1)
+    zb = np.linspace(0.0, n * dx, n + 1)
+    zc = 0.5 * (zb[1:] + zb[:-1])
+    mz = np.where((zc > 1.0) & (zc < 2.0), 2, 0)
+    med = np.broadcast_to(mz[:, None, None], (n, n, n)).reshape(-1).astype(np.int32)
+    src = make_source(-1, energy, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    tp = _lib.default_transport(kerma=0)
+    return xb, xb.copy(), zb, med, None, src, tp
+
+
+def test_oracle_electron_depth_dose_20mev():
+    """20 MeV broad electron beam in water: R50 ~ 8.3 cm (E0 ~ 2.33 R50)
+    and energy deposited below E0 by the escaping bremsstrahlung."""
+    xb = np.array([-20.0, 20.0]); yb = xb.copy(); zb = np.linspace(0, 15, 31)
+    med = np.zeros(30, dtype=np.int32)
+    g, keep = _lib.make_geometry(xb, yb, zb, med, None)
+    src = make_source(-1, 20.0, x0=-5, x1=5, y0=-5, y1=5, z=-1.0)
+    tp = _lib.default_transport(kerma=0)
+    o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=64, nthreads=8)
+    o.run(3000)
+    s, s2, n = o.tallies()
+    d = s / n / 0.5
+    assert 17.5 < d.sum() * 0.5 < 19.5
+    i = int(np.argmax(d))
+    zc = zb[:-1] + 0.25
+    r50 = np.interp(0.5 * d.max(), d[i:][::-1], zc[i:][::-1])
+    assert 7.9 < r50 < 8.8, r50
+
+
+@pytest.mark.gpu
+def test_electron_dose_parity(engine):
+    xb, yb, zb, med, rhor, src, tp = electron_case()
+    g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
+    o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
+    o.run(40000)
+    a = o.tallies()
+    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192)
+    sim.run(40000)
+    b = sim.tallies()
+    c = sim.counters()
+    assert c[5] == 0, "stack overflow"
+    chi2, out2, nv = workloads.compare_doses(a, b)
+    print("electrons: chi2/dof %.3f  >2sigma %.3f  voxels %d  tally overflows %d" % (chi2, out2, nv, c[6]))
+    assert nv > 100
+    assert chi2 < 1.3
+    assert out2 < 0.09
+    ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
+    assert abs(ta - tb) < 0.01 * ta
