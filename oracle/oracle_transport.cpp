@@ -1683,3 +1683,112 @@ int oracle_destroy(void *h) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------
+// Sampler hooks for the oracle's own pinning tests: run one interaction
+// routine n times on a particle moving along +z in an infinite medium and
+// return per-sample outputs.  kind:
+//   0 COMPT   -> [k'/k, cos(theta_gamma), T_electron, Uj_deposit]
+//   1 MOLLER  -> [T_secondary, T_primary, cos theta_1, cos theta_2]
+//   2 BHABHA  -> [T_of_particle_2, T_of_particle_1, q_2, q_1]
+//   3 ANNIH   -> [E_gamma1, E_gamma2, cos1, cos2]
+//   4 PAIR    -> [E_total_top, E_total_below, q_top, q_below]
+//   5 PHOTO   -> [E_electron_total, deposit, cos, 0]
+//   6 mscat   -> [cos theta, sin theta, lambda, chia2] at energy E and
+//                step of param cm (one call per sample, find_index reset)
+//   7 BREMS   -> [E_gamma, E_electron_total, cos theta_gamma, 0]
+//   8 RAYLEIGH-> [cos theta, 0, 0, 0]
+extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
+                             double energy, double param, int64_t n,
+                             int32_t seed, double *out) {
+    try {
+        static thread_local std::unique_ptr<Problem> pbp;
+        static thread_local std::string loaded;
+        if (!pbp || loaded != blob) {
+            pbp.reset(new Problem());
+            pbp->T.load(blob);
+            loaded = blob;
+        }
+        Problem &pb = *pbp;
+        double xb[2] = {-1e6, 1e6};
+        pb.geo.setup(1, 1, 1, xb, xb, xb, 0);
+        pb.nvox = 1;
+        pb.med.assign(1, medium);
+        pb.rhor.assign(1, 1.0);
+        Params &P = pb.P;
+        double ecut = 0, pcut = 0;
+        for (auto &m : pb.T.med) { ecut = std::max(ecut, m.ae); pcut = std::max(pcut, m.ap); }
+        P = Params{ecut, pcut, 1, 3, 1, 1, 1, 1, 0, 1, 0.25, 0.5, 3.0, 0};
+        if (kind == 0 && param == 0) P.ibcmp = 0;      // Klein-Nishina
+        std::vector<double> tmp(1, 0.0);
+        std::vector<int> touched;
+        Shower sh(pb, tmp, touched);
+        int is, js;
+        stream_seeds(1802, seed, 0, is, js);
+        sh.rng.set_state(is, js);
+        const Medium &M = pb.T.med[medium];
+        for (int64_t k = 0; k < n; k++) {
+            Particle p{};
+            p.w = 1; p.wt = 1; p.ir = 0;
+            double *o = out + 4 * k;
+            tmp[0] = 0;
+            touched.clear();
+            sh.stk.clear();
+            if (kind == 0) {
+                p.e = energy; p.iq = 0; sh.stk.push_back(p);
+                sh.compt(M);
+                if (sh.stk.size() == 2) {
+                    o[0] = sh.stk[0].e / energy; o[1] = sh.stk[0].w;
+                    o[2] = sh.stk[1].e - PRM; o[3] = tmp[0];
+                } else { o[0] = o[1] = o[2] = o[3] = -9; }
+            } else if (kind == 1 || kind == 2) {
+                p.e = energy + PRM; p.iq = kind == 1 ? -1 : 1; sh.stk.push_back(p);
+                if (kind == 1) sh.moller(M); else sh.bhabha(M);
+                if (sh.stk.size() == 2) {
+                    o[0] = sh.stk[1].e - PRM; o[1] = sh.stk[0].e - PRM;
+                    o[2] = kind == 1 ? sh.stk[0].w : sh.stk[1].iq;
+                    o[3] = kind == 1 ? sh.stk[1].w : sh.stk[0].iq;
+                } else { o[0] = o[1] = -9; o[2] = o[3] = 0; }
+            } else if (kind == 3) {
+                p.e = energy + PRM; p.iq = 1; sh.stk.push_back(p);
+                sh.annih();
+                o[0] = sh.stk[0].e; o[1] = sh.stk[1].e; o[2] = sh.stk[0].w; o[3] = sh.stk[1].w;
+            } else if (kind == 4) {
+                p.e = energy; p.iq = 0; sh.stk.push_back(p);
+                sh.pair(M, medium, std::log(energy));
+                o[0] = sh.stk[1].e; o[1] = sh.stk[0].e; o[2] = sh.stk[1].iq; o[3] = sh.stk[0].iq;
+            } else if (kind == 5) {
+                p.e = energy; p.iq = 0; sh.stk.push_back(p);
+                sh.photo(M, std::log(energy));
+                o[0] = sh.stk[0].e; o[1] = tmp[0]; o[2] = sh.stk[0].w; o[3] = 0;
+            } else if (kind == 6) {
+                // msdist-like inputs for one substep of length param
+                double e = energy, p2 = e * (e + RMT2), beta2 = p2 / (p2 + RMSQ);
+                double chia2 = M.xcc / (4 * p2 * M.blcc);
+                double lambda = param * M.blcc / beta2 / (1 + chia2);
+                double chilog = std::log(1 + 1 / chia2);
+                double q1 = 2 * chia2 * (chilog * (1 + chia2) - 1);
+                bool fi = true;
+                double c, s;
+                sh.mscat(lambda, chia2, q1 * lambda, fi, c, s);
+                o[0] = c; o[1] = s; o[2] = lambda; o[3] = chia2;
+            } else if (kind == 7) {
+                p.e = energy + PRM; p.iq = -1; sh.stk.push_back(p);
+                sh.brems(M, std::log(energy));
+                o[0] = sh.stk[1].e; o[1] = sh.stk[0].e; o[2] = sh.stk[1].w; o[3] = 0;
+            } else if (kind == 8) {
+                double gle = std::log(energy);
+                int lgle = (int)(M.ge1 * gle + M.ge0);
+                double c, s;
+                sh.rayleigh_sample(M, energy, gle, lgle, c, s);
+                o[0] = c; o[1] = s; o[2] = 0; o[3] = 0;
+            } else {
+                throw std::runtime_error("unknown sampler kind");
+            }
+        }
+        return 0;
+    } catch (const std::exception &ex) {
+        g_oerr = ex.what();
+        return -1;
+    }
+}

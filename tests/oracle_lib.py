@@ -137,3 +137,15 @@ class OracleSimulation:
         if getattr(self, "h", None) and self.h.value:
             self.L.oracle_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def sample(blob, kind, medium, energy, n, param=0.0, seed=9373):
+    """Run one interaction sampler of the oracle n times (oracle_sample)."""
+    L = load()
+    f = L.oracle_sample
+    f.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                  C.c_int64, C.c_int32, C.c_void_p]
+    f.restype = C.c_int
+    out = np.zeros((n, 4))
+    assert f(blob.encode(), kind, medium, energy, param, n, seed, _p(out)) == 0
+    return out
