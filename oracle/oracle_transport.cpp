@@ -67,6 +67,11 @@ struct Shower {
     std::vector<int> &touched;
     // UPHI saved state
     double A = 0, B = 0, Cc = 0, cosphi = 1, sinphi = 0;
+    // score-function event log (density-gradient replay): deposits
+    // (type 0, weight e*wt) and log-likelihood terms d log p / d rho_k
+    // (type 1) with the stack depth of the particle that made them
+    struct Ev { int type, vox, depth; double val; };
+    std::vector<Ev> *evlog = nullptr;
     int64_t n_overflow = 0;
     int64_t c_esteps = 0;
 
@@ -77,6 +82,7 @@ struct Shower {
 
     void score(int ir, double edep, double wt) {
         // EGS_DoseScoring::processEvent: iarg <= 4, ir >= 0, edep > 0
+        if (evlog && ir >= 0) evlog->push_back({0, ir, (int)stk.size(), edep > 0 ? edep * wt : 0.0});
         if (ir < 0 || !(edep > 0)) return;
         if (tmp[ir] == 0) touched.push_back(ir);
         tmp[ir] += edep * wt;
@@ -178,6 +184,8 @@ struct Shower {
                         }
                     }
                     double vstep = ustep;
+                    if (evlog)   // survival over the step: -ustep/(gmfp rho_rel)
+                        evlog->push_back({1, q.ir, (int)stk.size(), -ustep / (gmfp * rhof)});
                     q.x = q.x + q.u * vstep;
                     q.y = q.y + q.v * vstep;
                     q.z = q.z + q.w * vstep;
@@ -202,6 +210,8 @@ struct Shower {
             }
             // ---- interaction (:6545-6612)
             Particle &p1 = top();
+            if (evlog)   // interaction density proportional to rho_rel
+                evlog->push_back({1, p1.ir, (int)stk.size(), 1.0 / rhof});
             if (pb.P.rayleigh) {
                 double rnno37 = rnd();
                 if (rnno37 <= (1.0 - cohfac)) {
@@ -1524,6 +1534,90 @@ struct Engine {
         sh.shower(p);
     }
 
+    // Score-function gradient of L = sum_v (D_v - D*_v)^2 with respect to
+    // the relative densities, photon kerma mode: forward pass for D, then
+    // a replay of the same histories (same streams) with the event log and
+    // a reverse sweep per history (downstream deposit weights per depth).
+    void grad(int64_t n, const double *target, double *g_out, double *dose, double *loss,
+              int mode = 0) {
+        std::vector<Ranmar> start = streams;
+        std::fill(sum.begin(), sum.end(), 0.0);
+        std::fill(sum2.begin(), sum2.end(), 0.0);
+        nhist = 0;
+        run(n);
+        std::vector<double> mass(pb.nvox), w(pb.nvox);
+        double L = 0;
+        for (int i = 0; i < pb.nvox; i++) {
+            int iz = i / pb.geo.nxy, iy = (i / pb.geo.nx) % pb.geo.ny, ix = i % pb.geo.nx;
+            double vol = (pb.geo.xp[ix + 1] - pb.geo.xp[ix]) * (pb.geo.yp[iy + 1] - pb.geo.yp[iy]) *
+                         (pb.geo.zp[iz + 1] - pb.geo.zp[iz]);
+            mass[i] = vol * pb.rhor[i] * pb.T.med[pb.med[i]].rho;
+            dose[i] = sum[i] / (double)n / mass[i];
+            double r = dose[i] - target[i];
+            L += r * r;
+            w[i] = mode == 0 ? 2 * r / ((double)n * mass[i]) : 1.0 / (double)n;
+        }
+        if (mode == 1) {   // linear functional: total energy per history
+            L = 0;
+            for (int i = 0; i < pb.nvox; i++) L += sum[i] / (double)n;
+        }
+        *loss = L;
+        streams = start;
+        int nt = std::max(1, std::min(nthreads, nstreams));
+        std::vector<std::vector<double>> gt(nt);
+        std::vector<std::thread> th;
+        std::vector<std::string> errs(nt);
+        for (int t = 0; t < nt; t++) {
+            th.emplace_back([&, t]() {
+                try {
+                    std::vector<double> tmp(pb.nvox, 0.0);
+                    std::vector<int> touched;
+                    std::vector<Shower::Ev> ev;
+                    std::vector<double> acc(256, 0.0);
+                    gt[t].assign(pb.nvox, 0.0);
+                    Shower sh(pb, tmp, touched);
+                    sh.evlog = &ev;
+                    for (int s = t; s < nstreams; s += nt) {
+                        sh.rng = streams[s];
+                        int64_t cnt = n / nstreams + (s < n % nstreams ? 1 : 0);
+                        for (int64_t k = 0; k < cnt; k++) {
+                            ev.clear();
+                            history(sh);
+                            for (int ir : touched) tmp[ir] = 0;
+                            touched.clear();
+                            int dmax = 0;
+                            for (auto &e : ev) dmax = std::max(dmax, e.depth);
+                            if (dmax >= (int)acc.size()) acc.resize(dmax + 1);
+                            std::fill(acc.begin(), acc.begin() + dmax + 1, 0.0);
+                            for (size_t q = ev.size(); q-- > 0;) {
+                                const auto &e = ev[q];
+                                for (int d = e.depth + 1; d <= dmax; d++) acc[d] = 0;
+                                if (e.type == 0) {
+                                    double wd = w[e.vox] * e.val;
+                                    for (int d = 0; d <= e.depth; d++) acc[d] += wd;
+                                } else {
+                                    gt[t][e.vox] += e.val * acc[e.depth];
+                                }
+                            }
+                        }
+                        streams[s] = sh.rng;
+                    }
+                } catch (const std::exception &ex) {
+                    errs[t] = ex.what();
+                }
+            });
+        }
+        for (auto &x : th) x.join();
+        for (int t = 0; t < nt; t++)
+            if (!errs[t].empty()) throw std::runtime_error(errs[t]);
+        for (int i = 0; i < pb.nvox; i++) {
+            double gsum = 0;
+            for (int t = 0; t < nt; t++) gsum += gt[t][i];
+            // explicit mass term: D_v = E_v / (rho_v V_v)
+            g_out[i] = mode == 0 ? gsum - 2 * (dose[i] - target[i]) * dose[i] / pb.rhor[i] : gsum;
+        }
+    }
+
     void run(int64_t n) {
         int nt = std::max(1, std::min(nthreads, nstreams));
         std::vector<std::vector<double>> s1(nt), s2(nt);
@@ -1662,6 +1756,19 @@ int oracle_seed(void *h, int32_t ixx, int32_t jxx) {
 int oracle_run(void *h, int64_t n) {
     try {
         ((Engine *)h)->run(n);
+        return 0;
+    } catch (const std::exception &ex) {
+        g_oerr = ex.what();
+        return -1;
+    }
+}
+
+int oracle_grad(void *h, int64_t n, const double *target, double *grad,
+                double *dose, double *loss, int32_t mode) {
+    try {
+        auto *e = (Engine *)h;
+        if (!e->pb.P.kerma) throw std::runtime_error("the density gradient needs kerma mode");
+        e->grad(n, target, grad, dose, loss, mode);
         return 0;
     } catch (const std::exception &ex) {
         g_oerr = ex.what();

@@ -149,3 +149,19 @@ def sample(blob, kind, medium, energy, n, param=0.0, seed=9373):
     out = np.zeros((n, 4))
     assert f(blob.encode(), kind, medium, energy, param, n, seed, _p(out)) == 0
     return out
+
+
+def grad(sim, n, target, mode=0):
+    """Score-function density gradient of the oracle (oracle_grad).  mode 0:
+    L = sum (D - D*)^2; mode 1: total energy deposited per history."""
+    L = sim.L
+    f = L.oracle_grad
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_int32]
+    f.restype = C.c_int
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    g = np.zeros(sim.nvox)
+    d = np.zeros(sim.nvox)
+    loss = C.c_double(0)
+    sim._check(f(sim.h, int(n), _p(target), _p(g), _p(d), C.byref(loss), mode))
+    return g, d, loss.value
