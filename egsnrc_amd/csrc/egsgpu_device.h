@@ -99,4 +99,17 @@ struct DevTally {
     unsigned long long *counters;   // 8 engine counters
 };
 
+// Score-function event log of the density-gradient replay: per lane a
+// history's events (vox, depth*2 + type, value); type 0 = deposit
+// (value = energy * weight), type 1 = d log p / d rho of the voxel.
+struct DevGrad {
+    int cap;                        // events per lane
+    int2 *meta;                     // [lane*cap + i]
+    float *val;
+    double *acc;                    // [lane*kGradDepth + d] sweep accumulators
+    const double *w;                // per-voxel loss weights 2(D-D*)/(N m)
+    double *g;                      // gradient accumulator [nvox]
+};
+constexpr int kGradDepth = 32;
+
 }  // namespace egsgpu

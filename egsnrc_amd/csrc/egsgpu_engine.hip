@@ -39,6 +39,7 @@ struct KArgs {
     DevSource<R> S;
     int64_t base;      // histories per lane
     int rem;           // lanes < rem do one more
+    const DevGrad *grad;   // non-null: density-gradient replay
 };
 
 // EGS_AliasTable::sample (egs_alias_table.cpp:268-300), types 0 and 1
@@ -63,6 +64,7 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     if (lane >= a.L.nlanes) return;
     Shower<R> sh(a.T, a.G, a.vmed, a.vrho, a.P, a.L, a.tally, lane,
                  rbuf_all + threadIdx.x);
+    sh.grad = a.grad;
     const R RM = (R)0.5109989461;
     int64_t nh = a.base + (lane < a.rem ? 1 : 0);
     for (int64_t k = 0; k < nh; k++) {
@@ -94,6 +96,7 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
             if (sh.p.iq == 0) sh.photon();
             else sh.electron();
         }
+        if (a.grad) sh.grad_sweep();
         sh.end_history();
     }
     unsigned long long *c = a.tally.counters;
@@ -143,6 +146,11 @@ struct egsgpu_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
     int last_launches = 0;
+    // host copies for dose normalisation and the gradient
+    std::vector<double> hx, hy, hz, hrhor, hmass;
+    RanmarRec *rng_saved = nullptr;
+    DevGrad dgrad{};
+    DevGrad *dgrad_dev = nullptr;
     ~egsgpu_handle() {
         for (void *p : allocs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -410,12 +418,13 @@ int ensure_lanes(egsgpu_handle *h) {
 }
 
 template <typename R>
-int launch(egsgpu_handle *h, Precision<R> &P, int64_t n) {
+int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = nullptr) {
     KArgs<R> a;
     a.T = P.T; a.G = P.G; a.vmed = h->vmed; a.vrho = P.vrho; a.P = h->P;
     a.L = P.L; a.tally = h->tally; a.S = P.S;
     a.base = n / h->nlanes;
     a.rem = (int)(n % h->nlanes);
+    a.grad = grad;
     size_t lds = (size_t)kRNB * h->block * sizeof(float);
     hipLaunchKernelGGL(k_transport<R>, dim3((h->nlanes + h->block - 1) / h->block),
                        dim3(h->block), lds, h->stream, a);
@@ -474,6 +483,18 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         return fail(EGSGPU_E_HIP);
     }
     if ((rc = upload(h, med.data(), med.size(), &h->vmed))) return fail(rc);
+    h->hx.assign(geom->xb, geom->xb + geom->nx + 1);
+    h->hy.assign(geom->yb, geom->yb + geom->ny + 1);
+    h->hz.assign(geom->zb, geom->zb + geom->nz + 1);
+    h->hrhor.assign(h->nvox, 1.0);
+    h->hmass.assign(h->nvox, 0.0);
+    for (int i = 0; i < h->nvox; i++) {
+        if (geom->rhor) h->hrhor[i] = geom->rhor[i];
+        int iz = i / (geom->nx * geom->ny), iy = (i / geom->nx) % geom->ny, ix = i % geom->nx;
+        double vol = (geom->xb[ix + 1] - geom->xb[ix]) * (geom->yb[iy + 1] - geom->yb[iy]) *
+                     (geom->zb[iz + 1] - geom->zb[iz]);
+        h->hmass[i] = vol * h->hrhor[i] * b.f1(Blob::mk(med[i], "rho"));
+    }
     // cut-offs: $adjust_rhor_ecut_pcut
     double ecut = 0, pcut = 0;
     for (int m = 0; m < nmed; m++) {
@@ -568,6 +589,66 @@ int egsgpu_run(egsgpu_handle *h, int64_t nhist) {
     h->last_ms = ms;
     h->last_launches = 1;
     h->nhist += nhist;
+    return EGSGPU_OK;
+}
+
+int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
+                        double *grad_out, double *dose_out, double *loss) {
+    EGS_ARG(h && nhist > 0 && target && grad_out, "handle, nhist > 0, target, grad");
+    if (!h->P.kerma) {
+        set_error("the density gradient is defined for the photon kerma mode (kerma = 1)");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    int rc = ensure_lanes(h);
+    if (rc) return rc;
+    size_t nv = (size_t)h->nvox;
+    if (!h->rng_saved) {
+        CK(alloc(h, (size_t)h->nlanes, &h->rng_saved));
+        DevGrad &G = h->dgrad;
+        G.cap = 2048;
+        CK(alloc(h, (size_t)G.cap * h->nlanes, &G.meta));
+        CK(alloc(h, (size_t)G.cap * h->nlanes, &G.val));
+        CK(alloc(h, (size_t)kGradDepth * h->nlanes, &G.acc));
+        double *w = nullptr;
+        CK(alloc(h, nv, &w));
+        G.w = w;
+        CK(alloc(h, nv, &G.g));
+        CK(alloc(h, 1, &h->dgrad_dev));
+    }
+    // forward pass from the current stream states
+    EGS_HIP(hipMemcpyAsync(h->rng_saved, h->rng, sizeof(RanmarRec) * h->nlanes,
+                           hipMemcpyDeviceToDevice, h->stream));
+    CK(egsgpu_reset_tallies(h));
+    CK(egsgpu_run(h, nhist));
+    std::vector<double> s(nv), w(nv);
+    EGS_HIP(hipMemcpy(s.data(), h->tally.sum, sizeof(double) * nv, hipMemcpyDeviceToHost));
+    double L = 0;
+    std::vector<double> dose(nv);
+    for (size_t i = 0; i < nv; i++) {
+        dose[i] = s[i] / (double)nhist / h->hmass[i];
+        double r = dose[i] - target[i];
+        L += r * r;
+        w[i] = 2 * r / ((double)nhist * h->hmass[i]);
+    }
+    // replay of the same histories with the event log
+    EGS_HIP(hipMemcpyAsync((void *)h->dgrad.w, w.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->stream));
+    EGS_HIP(hipMemsetAsync(h->dgrad.g, 0, sizeof(double) * nv, h->stream));
+    EGS_HIP(hipMemcpyAsync(h->dgrad_dev, &h->dgrad, sizeof(DevGrad), hipMemcpyHostToDevice, h->stream));
+    EGS_HIP(hipMemcpyAsync(h->rng, h->rng_saved, sizeof(RanmarRec) * h->nlanes,
+                           hipMemcpyDeviceToDevice, h->stream));
+    EGS_HIP(hipEventRecord(h->ev0, h->stream));
+    rc = h->fp64 ? launch(h, h->pd, nhist, h->dgrad_dev) : launch(h, h->pf, nhist, h->dgrad_dev);
+    if (rc) return rc;
+    EGS_HIP(hipEventRecord(h->ev1, h->stream));
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    float ms = 0;
+    EGS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    EGS_HIP(hipMemcpy(grad_out, h->dgrad.g, sizeof(double) * nv, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nv; i++)
+        grad_out[i] -= 2 * (dose[i] - target[i]) * dose[i] / h->hrhor[i];
+    if (dose_out) std::copy(dose.begin(), dose.end(), dose_out);
+    if (loss) *loss = L;
     return EGSGPU_OK;
 }
 

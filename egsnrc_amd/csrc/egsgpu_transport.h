@@ -60,6 +60,45 @@ struct Shower {
     // counters
     unsigned long long c_psteps, c_cross, c_dep, c_flush, c_esteps, c_ovf, c_hovf;
     unsigned hsteps;   // steps of the current history (runaway guard)
+    // density-gradient replay log (null when not replaying)
+    const DevGrad *grad = nullptr;
+    int nlog = 0;
+    bool log_ovf = false;
+    __device__ __forceinline__ void log_event(int type, int vox, R v) {
+        if (nlog >= grad->cap) { log_ovf = true; return; }
+        size_t i = (size_t)lane * grad->cap + nlog++;
+        grad->meta[i] = make_int2(vox, np * 2 + type);
+        grad->val[i] = (float)v;
+    }
+    // reverse sweep over the history's events: every log-likelihood term
+    // is multiplied by the weighted deposits downstream of it (later events
+    // of the same particle and its descendants: stack depth >= its own)
+    __device__ void grad_sweep() {
+        if (log_ovf) { c_hovf++; nlog = 0; log_ovf = false; return; }
+        double *acc = grad->acc + (size_t)lane * kGradDepth;
+        int dmax = 0;
+        for (int q = 0; q < nlog; q++) {
+            int d = grad->meta[(size_t)lane * grad->cap + q].y >> 1;
+            dmax = d > dmax ? d : dmax;
+        }
+        dmax = dmax < kGradDepth - 1 ? dmax : kGradDepth - 1;
+        for (int d = 0; d <= dmax; d++) acc[d] = 0;
+        for (int q = nlog - 1; q >= 0; q--) {
+            int2 m = grad->meta[(size_t)lane * grad->cap + q];
+            int depth = m.y >> 1;
+            depth = depth < dmax ? depth : dmax;
+            for (int d = depth + 1; d <= dmax; d++) acc[d] = 0;
+            double v = (double)grad->val[(size_t)lane * grad->cap + q];
+            if ((m.y & 1) == 0) {
+                double wd = grad->w[m.x] * v;
+                for (int d = 0; d <= depth; d++) acc[d] += wd;
+            } else {
+                double c = v * acc[depth];
+                if (c != 0) atomicAdd(grad->g + m.x, c);
+            }
+        }
+        nlog = 0;
+    }
 
     __device__ Shower(const DevTables<R> &t, const GeomView<R> &g,
                       const uint8_t *vm, const R *vr, const DevParams &pp,
@@ -125,6 +164,10 @@ struct Shower {
     }
     // ausgab with iarg <= 4: score edep*wt in region ir (>= 0)
     __device__ __forceinline__ void score(int ir, R edep, R wt) {
+        if (grad) {   // replay: log only, the tallies were made by the forward pass
+            if (ir >= 0) log_event(0, ir, edep > 0 ? edep * wt : (R)0);
+            return;
+        }
         if (ir < 0 || !(edep > 0)) return;
         c_dep++;
         double e = (double)edep * (double)wt;
@@ -242,6 +285,7 @@ struct Shower {
                         int inew = geom_howfar_in(G, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
                         if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
                     }
+                    if (grad) log_event(1, p.ir, -ustep / (gmfp * rhof));
                     p.x = p.x + p.u * ustep;
                     p.y = p.y + p.v * ustep;
                     p.z = p.z + p.w * ustep;
@@ -262,6 +306,7 @@ struct Shower {
             }
             const MedScalars<R> &M = T.ms[medium];
             int li = medium * kGE + lgle - 1;
+            if (grad) log_event(1, p.ir, 1 / rhof);
             if (P.rayleigh) {
                 R rnno37 = rnd();
                 if (rnno37 <= (R)1.0 - cohfac) {

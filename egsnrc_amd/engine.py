@@ -75,6 +75,17 @@ class Simulation:
         check(_lib.lib().egsgpu_get_tallies(self.h, ptr(s), ptr(s2), C.byref(n)))
         return s, s2, n.value
 
+    def density_grad(self, nhist, target):
+        """Score-function gradient of sum_v (D_v - target_v)^2 w.r.t. the
+        relative voxel densities (forward + replay on the device)."""
+        target = np.ascontiguousarray(target, dtype=np.float64)
+        g = np.zeros(self.nvox)
+        d = np.zeros(self.nvox)
+        loss = C.c_double(0)
+        check(_lib.lib().egsgpu_density_grad(self.h, int(nhist), ptr(target), ptr(g), ptr(d),
+                                             C.byref(loss)))
+        return g, d, loss.value
+
     def counters(self):
         c = np.zeros(8, dtype=np.int64)
         check(_lib.lib().egsgpu_get_counters(self.h, ptr(c)))

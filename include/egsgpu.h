@@ -157,6 +157,20 @@ int egsgpu_get_tallies(egsgpu_handle *h, double *sum, double *sum2,
                        int64_t *nhist);
 int egsgpu_reset_tallies(egsgpu_handle *h);
 
+/* Density gradient (photon kerma mode): runs nhist histories forward from
+ * the current stream states to get the dose D_v = E_v/(N m_v), then replays
+ * the same histories from the saved RNG states and accumulates the
+ * score-function (likelihood-ratio) gradient of
+ *   L = sum_v (D_v - target_v)^2
+ * with respect to the relative voxel densities (FP64 tallies).  Writes
+ * dL/drho to grad (nvox), the dose to dose (nvox, may be NULL) and L to
+ * loss.  The explicit 1/m_v dependence of the dose is included.  No
+ * counterpart in the reference (egs_chamber's correlated sampling with
+ * RNG save/restore, user_codes/egs_chamber/egs_chamber.cpp:2424-2446, is
+ * the precedent for the replay). */
+int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
+                        double *grad, double *dose, double *loss);
+
 /* Engine counters: [0] photon steps, [1] voxel crossings, [2] deposits,
  * [3] history flushes (voxel,history pairs), [4] electron steps,
  * [5] stack overflows, [6] tally-table overflows, [7] reserved. */
