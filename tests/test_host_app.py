@@ -52,7 +52,8 @@ def test_app_runs_c0_input(tmp_path):
     assert dims == (30, 30, 30)
     assert np.allclose(b[0], np.arange(-15, 16))
     d3 = d.reshape(30, 30, 30)       # [iz, iy, ix]
-    # the pencil beam deposits along the central axis voxels (ix = iy = 15)
-    axis = d3[:, 15, 15]
+    # the pencil beam enters at x = y = 0, a voxel boundary: isWhere puts a
+    # point on a plane in the lower voxel (findRegion), i.e. ix = iy = 14
+    axis = d3[:, 14, 14]
     assert axis.max() == d.max()
     assert np.all(axis[:25] > 0)
