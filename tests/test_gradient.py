@@ -89,34 +89,62 @@ def test_oracle_score_function_dose_functional():
 @pytest.mark.gpu
 @pytest.mark.parametrize("fp64", [0, 1])
 def test_device_density_gradient_matches_oracle(engine, fp64):
-    """GPU forward + replay gradient against the oracle's estimator on the
-    same phantom and target: the directional derivatives along smooth
-    directions agree within their statistical errors and the gradient
-    vectors agree in relative L2."""
+    """GPU forward + replay against the oracle's estimator.  The target is
+    each side's own forward dose minus a/2, so that the loss weights
+    2(D - D*) are exactly a and both sides estimate the gradient of the same
+    linear dose functional sum_v a_v D_v (incl. the explicit 1/m term); the
+    directional derivatives along smooth directions then agree to within a
+    few percent, the estimators' statistical spread."""
+    from egsnrc_amd.engine import Simulation, Tables
+    n = 16
+    xb, yb, zb, med, rhor = workloads.kerma_phantom(n, 0.5, seed=4)
+    src = make_source(0, 2.0, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    tp = _lib.default_transport(ecut=20.0, kerma=1, fp64=fp64)
+    N = 4_000_000
+    iz = np.arange(n ** 3) // (n * n)
+    a = (iz + 1.0) / n
+    g1, keep1 = _lib.make_geometry(xb, yb, zb, med, rhor)
+    o = oracle_lib.OracleSimulation(BLOB, g1, src[0], tp, nstreams=64, nthreads=8)
+    _, d0o, _ = oracle_lib.grad(o, N, np.zeros(n ** 3))
+    o.seed(1802, 9373)
+    go, do, lo = oracle_lib.grad(o, N, d0o - a / 2)
+    sim = Simulation(Tables(BLOB), xb, yb, zb, med, rhor, src, tp, nstreams=16384)
+    sim.seed(1802, 9373)
+    _, d0g, _ = sim.density_grad(N, np.zeros(n ** 3))
+    sim.seed(1802, 9373)
+    gg, dg, lg = sim.density_grad(N, d0g - a / 2)
+    # the replayed forward pass repeats the histories (FP64 atomics may
+    # reorder the last bits of the sums)
+    assert np.allclose(dg, d0g, rtol=1e-12, atol=0)
+    c = sim.counters()
+    assert c[6] == 0, "event log overflow"
+    for delta in (rhor, rhor * (iz < n // 2), rhor * (iz >= n // 2)):
+        x, y = go @ delta, gg @ delta
+        print("directional derivative oracle %.6g gpu %.6g" % (x, y))
+        assert abs(x - y) < 0.03 * abs(x), (x, y)
+    big = np.abs(go) > 0.2 * np.abs(go).max()
+    rel = np.linalg.norm(gg[big] - go[big]) / np.linalg.norm(go[big])
+    print("gradient rel L2 over |g| > 0.2 max: %.3f" % rel)
+    assert rel < 0.15
+
+
+@pytest.mark.gpu
+def test_device_density_gradient_loss_runs(engine):
+    """The loss gradient of a perturbed target runs, is finite and its loss
+    matches the oracle's within the noise of the estimate."""
     from egsnrc_amd.engine import Simulation, Tables
     n = 16
     xb, yb, zb, med, rhor = workloads.kerma_phantom(n, 0.5, seed=4)
     _, _, _, _, rho2 = workloads.kerma_phantom(n, 0.5, seed=5)
     trho = rhor * (1 + 0.3 * (rho2 / rho2.mean() - 1))
     src = make_source(0, 2.0, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
-    tp = _lib.default_transport(ecut=20.0, kerma=1, fp64=fp64)
-    N = 4_000_000
-    g0, keep = _lib.make_geometry(xb, yb, zb, med, trho)
-    ot = oracle_lib.OracleSimulation(BLOB, g0, src[0], tp, nstreams=64, nthreads=8)
-    _, dstar, _ = oracle_lib.grad(ot, N, np.zeros(n ** 3))
+    tp = _lib.default_transport(ecut=20.0, kerma=1)
+    sim_t = Simulation(Tables(BLOB), xb, yb, zb, med, trho, src, tp)
+    _, dstar, _ = sim_t.density_grad(4_000_000, np.zeros(n ** 3))
+    sim = Simulation(Tables(BLOB), xb, yb, zb, med, rhor, src, tp)
+    g, d, L = sim.density_grad(4_000_000, dstar)
+    assert np.all(np.isfinite(g)) and L > 0
     g1, keep1 = _lib.make_geometry(xb, yb, zb, med, rhor)
     o = oracle_lib.OracleSimulation(BLOB, g1, src[0], tp, nstreams=64, nthreads=8)
-    go, do, lo = oracle_lib.grad(o, N, dstar)
-    sim = Simulation(Tables(BLOB), xb, yb, zb, med, rhor, src, tp, nstreams=16384)
-    gg, dg, lg = sim.density_grad(N, dstar)
-    c = sim.counters()
-    assert c[6] == 0, "event log overflow"
-    iz = np.arange(n ** 3) // (n * n)
-    for delta in (rhor, rhor * (iz < n // 2)):
-        a, b = go @ delta, gg @ delta
-        assert abs(a - b) < 0.05 * abs(a) + 1e-12, (a, b)
-    big = np.abs(go) > 0.2 * np.abs(go).max()
-    rel = np.linalg.norm(gg[big] - go[big]) / np.linalg.norm(go[big])
-    print("gradient rel L2 (|g| > 0.2 max): %.3f  loss gpu %.4g oracle %.4g" % (rel, lg, lo))
-    assert rel < 0.25
-    assert abs(lg - lo) < 0.1 * lo
+    _, _, lo = oracle_lib.grad(o, 4_000_000, dstar)
+    assert abs(L - lo) < 0.5 * lo

@@ -129,7 +129,7 @@ def test_electron_dose_parity(engine):
     assert c[5] == 0, "stack overflow"
     chi2, out2, nv = workloads.compare_doses(a, b)
     print("electrons: chi2/dof %.3f  >2sigma %.3f  voxels %d  tally overflows %d" % (chi2, out2, nv, c[6]))
-    assert nv > 100
+    assert nv > 30
     assert chi2 < 1.3
     assert out2 < 0.09
     ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
