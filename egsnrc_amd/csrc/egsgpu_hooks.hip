@@ -193,6 +193,8 @@ static int trace_impl(const EgsXYZGeometry *g, int32_t nrays, const double *x,
     T_HIP(hipMalloc(&ds, sizeof(double) * tot));
     T_HIP(hipMalloc(&dr, sizeof(int) * tot));
     T_HIP(hipMalloc(&dn, sizeof(int) * nrays));
+    T_HIP(hipMemset(ds, 0, sizeof(double) * tot));
+    T_HIP(hipMemset(dr, 0, sizeof(int) * tot));
     T_HIP(hipMemcpy(dx, x, 3 * sizeof(double) * nrays, hipMemcpyHostToDevice));
     T_HIP(hipMemcpy(du, u, 3 * sizeof(double) * nrays, hipMemcpyHostToDevice));
     int bs = 128;

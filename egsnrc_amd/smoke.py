@@ -14,10 +14,13 @@ def run():
     x = np.array([[0.1, 0.2, -1.0], [-3.0, 0.3, 1.1]])
     u = np.array([[0.1, 0.2, 0.97], [1.0, 0.0, 0.0]])
     u /= np.linalg.norm(u, axis=1)[:, None]
-    a = _lib.trace_rays(g, x, u, 64, 32)
-    b = oracle_lib.trace_rays(g, x, u, 64, 32)
-    for p, q in zip(a, b):
-        assert np.array_equal(p, q), "voxel trace mismatch"
+    ra, sa, na = _lib.trace_rays(g, x, u, 64, 32)
+    rb, sb, nb = oracle_lib.trace_rays(g, x, u, 64, 32)
+    assert np.array_equal(na, nb), "voxel trace length mismatch"
+    for r in range(len(x)):
+        k = na[r]
+        assert np.array_equal(ra[r, :k], rb[r, :k]) and np.array_equal(sa[r, :k], sb[r, :k]), \
+            "voxel trace mismatch"
     # tiny transport: 8^3 kerma phantom, 2 MeV photons
     xb, yb, zb, med, rhor = workloads.kerma_phantom(8, 2.0)
     src = make_source(0, 2.0, x0=-4, x1=4, y0=-4, y1=4, z=-1.0)
