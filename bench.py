@@ -104,6 +104,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="slab128")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="transport lanes (ranmar streams); 0 = library default")
+    ap.add_argument("--block", type=int, default=0, help="workgroup size; 0 = default")
+    ap.add_argument("--hist", type=int, default=0, help="histories per step; 0 = workload default")
+    ap.add_argument("--schedule", choices=["history", "event"], default=None,
+                    help="transport schedule; default = the library's choice")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -115,30 +121,39 @@ def main():
     xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
     W = WORKLOADS[name]
     tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", W["blob"]))
-    sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp)
-    hps = W["hist_per_step"]
+    sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=a.lanes, block=a.block,
+                     schedule=a.schedule)
+    hps = a.hist or W["hist_per_step"]
     for _ in range(a.warmup):
         sim.run(hps)
     from egsnrc_amd import _lib
     _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
-    c0 = sim.counters()
-    kms = 0.0
+    kms = np.zeros(4)
+    klaunch = np.zeros(4, dtype=np.int64)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         sim.run(hps)
-        kms += sim.kernel_ms()[0]
+        for k in range(4):
+            ms, nl = sim.kernel_time(k)
+            kms[k] += ms
+            klaunch[k] += nl
     t1 = time.perf_counter()
     T = t1 - t0
-    c1 = sim.counters()
     s, s2, n = sim.tallies()
     hist_s = a.steps * hps / T
     eff, rel2 = efficiency(s, s2, n, T)
-    dc = c1 - c0
+    # the dominant kernel (the tallies, and so the counters, were reset
+    # after the warm-up)
+    kdom = int(np.argmax(kms))
+    dc = sim.kernel_counters(kdom)
+    call = sim.counters()
     # algorithmic HBM bytes: 5 B per voxel crossing, 16 B per deposit,
     # 32 B per (history, voxel) flush
     alg_bytes = 5.0 * dc[1] + 16.0 * dc[2] + 32.0 * dc[3]
-    kernel_s = kms / 1e3
-    achieved = alg_bytes / kernel_s / 1e9 / a.steps * a.steps
+    kernel_s = kms[kdom] / 1e3
+    achieved = alg_bytes / kernel_s / 1e9
+    knames = ["k_transport<float>", "k_ev_source<float>", "k_ev_photon<float>",
+              "k_ev_electron<float>"]
     out = {
         "metric": METRIC,
         "value": hist_s,
@@ -153,15 +168,22 @@ def main():
         "dtype": "f32",
         "tally_dtype": "f64",
         "data": "synthetic phantom and source",
-        "config": dict(cfg, histories_per_step=hps, parallelism="none"),
+        "config": dict(cfg, histories_per_step=hps, parallelism="none",
+                       schedule=a.schedule or "default"),
         "efficiency": {"value": eff, "unit": "1/(sigma^2 s)",
                        "sigma2": rel2, "definition": "mean relative variance over D > 0.5 Dmax"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_transport<float>",
-                     "kernel_ms_per_step": kms / a.steps,
-                     "counters": {"voxel_crossings": int(dc[1]), "deposits": int(dc[2]),
-                                  "flushes": int(dc[3]), "photon_steps": int(dc[0])}},
+                     "kernel": knames[kdom],
+                     "kernel_launches": int(klaunch[kdom]),
+                     "kernel_ms_avg": kms[kdom] / max(1, klaunch[kdom]),
+                     "kernel_counters": {"voxel_crossings": int(dc[1]), "deposits": int(dc[2]),
+                                         "flushes": int(dc[3])}},
+        "kernels_ms_per_step": {knames[k]: kms[k] / a.steps for k in range(4) if klaunch[k]},
+        "counters": {"photon_steps": int(call[0]), "voxel_crossings": int(call[1]),
+                     "deposits": int(call[2]), "flushes": int(call[3]),
+                     "electron_steps": int(call[4]), "overflows": int(call[5]),
+                     "tally_overflows": int(call[6])},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(name)

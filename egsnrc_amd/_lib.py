@@ -54,6 +54,10 @@ def default_transport(**kw):
     return t
 
 
+SCHED_HISTORY = 0
+SCHED_EVENT = 1
+
+
 class EngineError(RuntimeError):
     pass
 
@@ -88,6 +92,9 @@ def lib():
         "egsgpu_last_kernel_ms": [vp, vp, vp],
         "egsgpu_density_grad": [vp, i64, vp, vp, vp, vp],
         "egsgpu_tables_medium_info": [vp, i32, vp, C.c_size_t, vp],
+        "egsgpu_set_schedule": [vp, i32],
+        "egsgpu_kernel_time": [vp, i32, vp, vp],
+        "egsgpu_kernel_counters": [vp, i32, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name, None)

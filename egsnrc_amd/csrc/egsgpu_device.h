@@ -99,6 +99,56 @@ struct DevTally {
     unsigned long long *counters;   // 8 engine counters
 };
 
+// ---- event schedule -----------------------------------------------------
+// Particles waiting for a transport kernel (SoA, index fastest).  tag =
+// slot*4 + (iq+1): the history slot the particle belongs to and its charge.
+template <typename R>
+struct DevBank {
+    int cap;
+    R *e, *x, *y, *z, *u, *v, *w, *wt, *dn;
+    int *ir, *tag;
+};
+
+// History slots of the event schedule.  Every history in flight owns one
+// slot: an open-addressing table of its per-voxel energy (the
+// history-by-history accumulator of egs_scoring.h:63-125, shared by all
+// lanes working on particles of that history, updated with atomics), the
+// list of claimed entries, and the number of pending particle groups
+// (particles in a bank plus lanes holding particles of the history).  The
+// lane that takes pending to zero adds the history's sums into the tally
+// and returns the slot to the free ring.
+struct DevSlots {
+    int nslots;
+    int hcap;                 // entries per slot (power of 2)
+    int *key;                 // [slot*hcap + i]: voxel+1, 0 = empty
+    double *val;              // [slot*hcap + i]
+    int *list;                // [slot*hcap + k]: claimed entry indices
+    int *used;                // [slot]: number of claimed entries
+    int *pending;             // [slot]
+    int *ring;                // free slots, ring of nslots entries
+};
+
+// Control words of the event schedule (device memory).
+struct EvCtl {
+    unsigned long long free_tail;   // ring append position
+    int nP, nE;                     // bank fill counts
+    int hP, hE;                     // bank consume heads
+    int pad[2];
+};
+
+// All active lanes of the wave claim consecutive indices from *counter
+// with one atomic (ballot + prefix count); returns this lane's index.
+__device__ __forceinline__ int wave_claim(int *counter) {
+    unsigned long long mask = __ballot(1);
+    int lid = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int rank = __popcll(mask & ((1ull << lid) - 1ull));
+    int base = 0;
+    if (lid == leader) base = atomicAdd(counter, (int)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + rank;
+}
+
 // Score-function event log of the density-gradient replay: per lane a
 // history's events (vox, depth*2 + type, value); type 0 = deposit
 // (value = energy * weight), type 1 = d log p / d rho of the voxel.

@@ -43,8 +43,8 @@ struct KArgs {
 };
 
 // EGS_AliasTable::sample (egs_alias_table.cpp:268-300), types 0 and 1
-template <typename R>
-__device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R> &sh) {
+template <typename R, bool EV>
+__device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R, EV> &sh) {
     if (S.spectrum == 0) return S.energy;
     double r1 = (double)sh.rnd();
     double aj = r1 * S.np;
@@ -57,6 +57,51 @@ __device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R> &sh)
     return (R)(x + dx * r2);
 }
 
+// One primary of EGS_Application::simulateSingleShower
+// (egs_application.cpp:919-959) from the parallel-beam source
+// (egs_parallel_beam.cpp:40-117): position on the source shape, entry into
+// the grid along the beam direction.  Returns false if it misses the grid.
+template <typename R, bool EV>
+__device__ bool sample_primary(const DevSource<R> &S, const GeomView<R> &G,
+                               Shower<R, EV> &sh, Part<R> &q) {
+    const R RM = (R)0.5109989461;
+    int ireg = -1;
+    for (int ntry = 0; ntry < 100000 && ireg < 0; ntry++) {
+        R E = source_energy(S, sh);
+        R x = S.x0, y = S.y0, z = S.z;
+        if (S.shape == 1) {
+            x = S.x0 + (S.x1 - S.x0) * sh.rnd();
+            y = S.y0 + (S.y1 - S.y0) * sh.rnd();
+        }
+        ireg = geom_is_where(G, x, y, z);
+        if (ireg < 0) {
+            R t = (R)1e30;
+            ireg = geom_howfar_out(G, x, y, z, S.ux, S.uy, S.uz, t);
+            if (ireg >= 0) { x += S.ux * t; y += S.uy * t; z += S.uz * t; }
+        }
+        q.e = S.charge ? E + RM : E;
+        q.x = x; q.y = y; q.z = z;
+        q.u = S.ux; q.v = S.uy; q.w = S.uz;
+        q.wt = 1; q.dnear = 0; q.iq = S.charge; q.ir = ireg;
+    }
+    return ireg >= 0;
+}
+
+template <typename R, bool EV>
+__device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, EV> &sh,
+                                             int kernel) {
+    unsigned long long *c = t.counters + 8 * kernel;
+    if (sh.c_psteps) atomicAdd(c + 0, sh.c_psteps);
+    if (sh.c_cross) atomicAdd(c + 1, sh.c_cross);
+    if (sh.c_dep) atomicAdd(c + 2, sh.c_dep);
+    if (sh.c_flush) atomicAdd(c + 3, sh.c_flush);
+    if (sh.c_esteps) atomicAdd(c + 4, sh.c_esteps);
+    if (sh.c_ovf) atomicAdd(c + 5, sh.c_ovf);
+    if (sh.c_hovf) atomicAdd(c + 6, sh.c_hovf);
+}
+
+// History schedule: lane s runs its share of the histories one after the
+// other with ranmar stream s, like egs-parallel job s.
 template <typename R>
 __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     extern __shared__ float rbuf_all[];
@@ -65,48 +110,199 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     Shower<R> sh(a.T, a.G, a.vmed, a.vrho, a.P, a.L, a.tally, lane,
                  rbuf_all + threadIdx.x);
     sh.grad = a.grad;
-    const R RM = (R)0.5109989461;
     int64_t nh = a.base + (lane < a.rem ? 1 : 0);
     for (int64_t k = 0; k < nh; k++) {
-        // EGS_Application::simulateSingleShower (egs_application.cpp:919-959)
         Part<R> q;
-        int ireg = -1;
-        for (int ntry = 0; ntry < 100000 && ireg < 0; ntry++) {
-            R E = source_energy(a.S, sh);
-            R x = a.S.x0, y = a.S.y0, z = a.S.z;
-            if (a.S.shape == 1) {
-                x = a.S.x0 + (a.S.x1 - a.S.x0) * sh.rnd();
-                y = a.S.y0 + (a.S.y1 - a.S.y0) * sh.rnd();
-            }
-            ireg = geom_is_where(a.G, x, y, z);
-            if (ireg < 0) {
-                R t = (R)1e30;
-                ireg = geom_howfar_out(a.G, x, y, z, a.S.ux, a.S.uy, a.S.uz, t);
-                if (ireg >= 0) { x += a.S.ux * t; y += a.S.uy * t; z += a.S.uz * t; }
-            }
-            q.e = a.S.charge ? E + RM : E;
-            q.x = x; q.y = y; q.z = z;
-            q.u = a.S.ux; q.v = a.S.uy; q.w = a.S.uz;
-            q.wt = 1; q.dnear = 0; q.iq = a.S.charge; q.ir = ireg;
-        }
-        if (ireg < 0) continue;
+        if (!sample_primary(a.S, a.G, sh, q)) continue;
         sh.p = q;
         sh.np = 1;
+        sh.eph = Shower<R>::kNew;
         while (sh.np > 0) {
-            if (sh.p.iq == 0) sh.photon();
-            else sh.electron();
+            if (sh.p.iq == 0) {
+                sh.photon();
+                sh.eph = Shower<R>::kNew;
+            } else {
+                sh.electron_step();
+            }
         }
         if (a.grad) sh.grad_sweep();
         sh.end_history();
     }
-    unsigned long long *c = a.tally.counters;
-    atomicAdd(c + 0, sh.c_psteps);
-    atomicAdd(c + 1, sh.c_cross);
-    atomicAdd(c + 2, sh.c_dep);
-    atomicAdd(c + 3, sh.c_flush);
-    atomicAdd(c + 4, sh.c_esteps);
-    if (sh.c_ovf) atomicAdd(c + 5, sh.c_ovf);
-    if (sh.c_hovf) atomicAdd(c + 6, sh.c_hovf);
+    add_counters(a.tally, sh, 0);
+}
+
+// ---- event schedule ------------------------------------------------------
+template <typename R>
+struct EvArgs {
+    KArgs<R> k;
+    DevBank<R> P, E;        // photon and charged-particle banks
+    DevSlots slots;
+    EvCtl *ctl;
+    unsigned long long free_head;   // first free-ring entry for new histories
+    int ninject;                    // new histories of this source launch
+};
+
+template <typename R>
+__device__ __forceinline__ void bank_store(const DevBank<R> &b, int i, const Part<R> &q, int slot) {
+    b.e[i] = q.e; b.x[i] = q.x; b.y[i] = q.y; b.z[i] = q.z;
+    b.u[i] = q.u; b.v[i] = q.v; b.w[i] = q.w; b.wt[i] = q.wt; b.dn[i] = q.dnear;
+    b.ir[i] = q.ir; b.tag[i] = slot * 4 + q.iq + 1;
+}
+
+template <typename R>
+__device__ __forceinline__ int bank_load(const DevBank<R> &b, int i, Part<R> &q) {
+    q.e = b.e[i]; q.x = b.x[i]; q.y = b.y[i]; q.z = b.z[i];
+    q.u = b.u[i]; q.v = b.v[i]; q.w = b.w[i]; q.wt = b.wt[i]; q.dnear = b.dn[i];
+    q.ir = b.ir[i];
+    int tag = b.tag[i];
+    q.iq = (tag & 3) - 1;
+    return tag >> 2;
+}
+
+// Move the particle on top of the lane's stack into bank b.  The history's
+// pending count goes up before the particle becomes visible.  A full bank
+// absorbs the particle where it is (counted as an overflow).
+template <typename R>
+__device__ void bank_push(Shower<R, true> &sh, const DevBank<R> &b, int *count) {
+    atomicAdd(sh.slots->pending + sh.slot, 1);
+    int i = wave_claim(count);
+    if (i < b.cap) {
+        bank_store(b, i, sh.p, sh.slot);
+        return;
+    }
+    atomicSub(sh.slots->pending + sh.slot, 1);
+    sh.c_ovf++;
+    const R RM = (R)0.5109989461;
+    R edep = sh.p.iq == 0 ? sh.p.e : (sh.p.iq < 0 ? sh.p.e - RM : sh.p.e + RM);
+    sh.score(sh.p.ir, edep, sh.p.wt);
+}
+
+template <typename R>
+__device__ __forceinline__ void ev_setup(Shower<R, true> &sh, const EvArgs<R> &a) {
+    sh.slots = &a.slots;
+    sh.ev_ctl = a.ctl;
+    sh.np = 0;
+    sh.slot = -1;
+}
+
+// New histories into free slots: thread i samples primaries i, i+nlanes, ...
+template <typename R>
+__global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
+    extern __shared__ float rbuf_all[];
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.k.L.nlanes) return;
+    Shower<R, true> sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
+                       rbuf_all + threadIdx.x);
+    ev_setup(sh, a);
+    const unsigned long long ns = (unsigned long long)a.slots.nslots;
+    for (int i = lane; i < a.ninject; i += a.k.L.nlanes) {
+        int slot = a.slots.ring[(a.free_head + (unsigned long long)i) % ns];
+        Part<R> q;
+        if (!sample_primary(a.k.S, a.k.G, sh, q)) {
+            unsigned long long at = atomicAdd(&a.ctl->free_tail, 1ull);
+            a.slots.ring[at % ns] = slot;
+            continue;
+        }
+        atomicExch(a.slots.pending + slot, 1);
+        const DevBank<R> &b = q.iq == 0 ? a.P : a.E;
+        int j = wave_claim(q.iq == 0 ? &a.ctl->nP : &a.ctl->nE);
+        if (j < b.cap) {
+            bank_store(b, j, q, slot);
+        } else {   // the host leaves room for every new primary
+            sh.c_ovf++;
+            atomicExch(a.slots.pending + slot, 0);
+            unsigned long long at = atomicAdd(&a.ctl->free_tail, 1ull);
+            a.slots.ring[at % ns] = slot;
+        }
+    }
+    add_counters(a.k.tally, sh, 1);
+}
+
+// Photon kernel: every lane takes photons from bank P until it is empty
+// and runs PHOTON on each; charged secondaries go to bank E.
+template <typename R>
+__global__ void __launch_bounds__(256) k_ev_photon(EvArgs<R> a) {
+    extern __shared__ float rbuf_all[];
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.k.L.nlanes) return;
+    Shower<R, true> sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
+                       rbuf_all + threadIdx.x);
+    ev_setup(sh, a);
+    const int n = min(a.ctl->nP, a.P.cap);
+    for (;;) {
+        if (sh.np == 0) {
+            if (sh.slot >= 0) sh.slot_release();
+            int i = wave_claim(&a.ctl->hP);
+            if (i >= n) break;
+            sh.slot = bank_load(a.P, i, sh.p);
+            sh.np = 1;
+            sh.hsteps = 0;
+            continue;
+        }
+        if (sh.p.iq != 0) {
+            bank_push(sh, a.E, &a.ctl->nE);
+            sh.pop();
+            continue;
+        }
+        sh.photon();
+    }
+    add_counters(a.k.tally, sh, 2);
+}
+
+// Charged-particle kernel: every lane takes electrons/positrons from bank
+// E and advances them one condensed-history step per loop iteration, so
+// that lanes whose particle ends pick up the next one while the others
+// keep stepping.  Photons created on the way go to bank P.
+template <typename R>
+__global__ void __launch_bounds__(256) k_ev_electron(EvArgs<R> a) {
+    extern __shared__ float rbuf_all[];
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.k.L.nlanes) return;
+    Shower<R, true> sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
+                       rbuf_all + threadIdx.x);
+    ev_setup(sh, a);
+    const int n = min(a.ctl->nE, a.E.cap);
+    for (;;) {
+        if (sh.np == 0) {
+            if (sh.slot >= 0) sh.slot_release();
+            int i = wave_claim(&a.ctl->hE);
+            if (i >= n) break;
+            sh.slot = bank_load(a.E, i, sh.p);
+            sh.np = 1;
+            sh.hsteps = 0;
+            sh.eph = Shower<R, true>::kNew;
+            continue;
+        }
+        if (sh.p.iq == 0) {
+            bank_push(sh, a.P, &a.ctl->nP);
+            sh.pop();
+            sh.eph = Shower<R, true>::kNew;
+            continue;
+        }
+        sh.electron_step();
+    }
+    add_counters(a.k.tally, sh, 3);
+}
+
+// Bank bookkeeping between kernels: which = 0 empties P, 1 empties E.
+__global__ void k_ev_ctl(EvCtl *c, int which) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (which == 0) { c->nP = 0; c->hP = 0; }
+        else { c->nE = 0; c->hE = 0; }
+    }
+}
+
+__global__ void k_ev_init(DevSlots S, EvCtl *c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S.nslots) {
+        S.ring[i] = i;
+        S.used[i] = 0;
+        S.pending[i] = 0;
+    }
+    if (i == 0) {
+        c->free_tail = (unsigned long long)S.nslots;
+        c->nP = c->nE = c->hP = c->hE = 0;
+    }
 }
 
 // ------------------------------------------------------------ host side
@@ -122,6 +318,7 @@ struct Precision {
     const R *vrho = nullptr;
     DevSource<R> S;
     DevLanes<R> L;
+    DevBank<R> BP{}, BE{};   // event-schedule banks
 };
 
 }  // namespace egsgpu
@@ -151,8 +348,23 @@ struct egsgpu_handle {
     RanmarRec *rng_saved = nullptr;
     DevGrad dgrad{};
     DevGrad *dgrad_dev = nullptr;
+    // event schedule
+    int schedule = EGSGPU_SCHED_EVENT;
+    bool ev_ready = false;
+    int nslots = 0, slot_hcap = 256;
+    DevSlots slots{};
+    EvCtl *ctl = nullptr;
+    EvCtl *ctl_host = nullptr;          // pinned
+    unsigned long long free_head = 0;
+    hipEvent_t kev[4][2] = {};          // per-kernel timing events
+    double kms[4] = {};
+    int klaunch[4] = {};
     ~egsgpu_handle() {
         for (void *p : allocs) (void)hipFree(p);
+        if (ctl_host) (void)hipHostFree(ctl_host);
+        for (auto &e : kev)
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -395,12 +607,13 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
     for (R **f : rf) CK(alloc(h, ns, f));
     CK(alloc(h, ns, &L.siq));
     CK(alloc(h, ns, &L.sir));
-    L.hcap = h->hcap;
-    size_t nh = (size_t)h->hcap * h->nlanes;
+    // per-lane history tables: history schedule only
+    L.hcap = h->schedule == EGSGPU_SCHED_HISTORY ? h->hcap : 0;
+    size_t nh = (size_t)L.hcap * h->nlanes;
     CK(alloc(h, nh, &L.hkey));
     CK(alloc(h, nh, &L.hval));
     CK(alloc(h, nh, &L.hlist));
-    EGS_HIP(hipMemset(L.hkey, 0, nh * sizeof(int)));
+    if (nh) EGS_HIP(hipMemset(L.hkey, 0, nh * sizeof(int)));
     return EGSGPU_OK;
 }
 
@@ -418,17 +631,156 @@ int ensure_lanes(egsgpu_handle *h) {
 }
 
 template <typename R>
-int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = nullptr) {
+KArgs<R> kargs(egsgpu_handle *h, Precision<R> &P) {
     KArgs<R> a;
     a.T = P.T; a.G = P.G; a.vmed = h->vmed; a.vrho = P.vrho; a.P = h->P;
     a.L = P.L; a.tally = h->tally; a.S = P.S;
+    a.base = 0;
+    a.rem = 0;
+    a.grad = nullptr;
+    return a;
+}
+
+int timing_events(egsgpu_handle *h) {
+    for (auto &e : h->kev)
+        for (hipEvent_t &x : e)
+            if (!x) EGS_HIP(hipEventCreate(&x));
+    for (int k = 0; k < 4; k++) { h->kms[k] = 0; h->klaunch[k] = 0; }
+    return EGSGPU_OK;
+}
+
+// add the duration of the last launch of kernel k (its events completed)
+int take_time(egsgpu_handle *h, int k) {
+    float ms = 0;
+    EGS_HIP(hipEventElapsedTime(&ms, h->kev[k][0], h->kev[k][1]));
+    h->kms[k] += ms;
+    h->klaunch[k]++;
+    return EGSGPU_OK;
+}
+
+template <typename R>
+int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = nullptr) {
+    KArgs<R> a = kargs(h, P);
     a.base = n / h->nlanes;
     a.rem = (int)(n % h->nlanes);
     a.grad = grad;
     size_t lds = (size_t)kRNB * h->block * sizeof(float);
+    EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
     hipLaunchKernelGGL(k_transport<R>, dim3((h->nlanes + h->block - 1) / h->block),
                        dim3(h->block), lds, h->stream, a);
     EGS_HIP(hipGetLastError());
+    EGS_HIP(hipEventRecord(h->kev[0][1], h->stream));
+    return EGSGPU_OK;
+}
+
+template <typename R>
+int alloc_bank(egsgpu_handle *h, int cap, DevBank<R> &b) {
+    b.cap = cap;
+    R **f[] = {&b.e, &b.x, &b.y, &b.z, &b.u, &b.v, &b.w, &b.wt, &b.dn};
+    for (R **x : f) CK(alloc(h, (size_t)cap, x));
+    CK(alloc(h, (size_t)cap, &b.ir));
+    CK(alloc(h, (size_t)cap, &b.tag));
+    return EGSGPU_OK;
+}
+
+// Event-schedule state: history slots, banks, control words.
+template <typename R>
+int ensure_event(egsgpu_handle *h, Precision<R> &P) {
+    if (h->ev_ready) return EGSGPU_OK;
+    DevSlots &S = h->slots;
+    S.nslots = h->nslots > 0 ? h->nslots : 4 * h->nlanes;
+    S.hcap = h->slot_hcap;
+    size_t ne = (size_t)S.nslots * S.hcap;
+    CK(alloc(h, ne, &S.key));
+    CK(alloc(h, ne, &S.val));
+    CK(alloc(h, ne, &S.list));
+    CK(alloc(h, (size_t)S.nslots, &S.used));
+    CK(alloc(h, (size_t)S.nslots, &S.pending));
+    CK(alloc(h, (size_t)S.nslots, &S.ring));
+    EGS_HIP(hipMemset(S.key, 0, ne * sizeof(int)));
+    EGS_HIP(hipMemset(S.val, 0, ne * sizeof(double)));
+    CK(alloc(h, 1, &h->ctl));
+    EGS_HIP(hipHostMalloc((void **)&h->ctl_host, sizeof(EvCtl), hipHostMallocDefault));
+    CK(alloc_bank(h, 2 * S.nslots, P.BP));
+    CK(alloc_bank(h, 8 * S.nslots, P.BE));
+    hipLaunchKernelGGL(k_ev_init, dim3((S.nslots + 255) / 256), dim3(256), 0, h->stream,
+                       S, h->ctl);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    h->free_head = 0;
+    h->ev_ready = true;
+    return EGSGPU_OK;
+}
+
+int read_ctl(egsgpu_handle *h, EvCtl &c) {
+    EGS_HIP(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, h->stream));
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    c = *h->ctl_host;
+    return EGSGPU_OK;
+}
+
+// Event schedule: inject new histories into free slots while banks drain.
+// One iteration = source, photon kernel, charged-particle kernel; the host
+// reads the bank counts after each iteration.
+template <typename R>
+int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
+    CK(ensure_event(h, P));
+    EvArgs<R> a;
+    a.k = kargs(h, P);
+    a.P = P.BP;
+    a.E = P.BE;
+    a.slots = h->slots;
+    a.ctl = h->ctl;
+    const dim3 grid((h->nlanes + h->block - 1) / h->block), block(h->block);
+    const size_t lds = (size_t)kRNB * h->block * sizeof(float);
+    const bool photon_src = P.S.charge == 0;
+    EvCtl c;
+    CK(read_ctl(h, c));
+    int64_t remaining = nhist;
+    const unsigned long long ns = (unsigned long long)h->slots.nslots;
+    for (int64_t iter = 0;; iter++) {
+        unsigned long long freeslots = c.free_tail - h->free_head;
+        int64_t inject = std::min<int64_t>((int64_t)freeslots, remaining);
+        int64_t room = photon_src ? (int64_t)a.P.cap - c.nP : (int64_t)a.E.cap - c.nE;
+        inject = std::max<int64_t>(0, std::min(inject, room));
+        if (inject == 0 && c.nP == 0 && c.nE == 0) {
+            set_error("event schedule: no free history slot with empty banks");
+            return EGSGPU_E_OVERFLOW;
+        }
+        bool timed_src = false;
+        if (inject > 0) {
+            a.free_head = h->free_head;
+            a.ninject = (int)inject;
+            EGS_HIP(hipEventRecord(h->kev[1][0], h->stream));
+            hipLaunchKernelGGL(k_ev_source<R>, grid, block, lds, h->stream, a);
+            EGS_HIP(hipGetLastError());
+            EGS_HIP(hipEventRecord(h->kev[1][1], h->stream));
+            timed_src = true;
+            h->free_head += (unsigned long long)inject;
+            remaining -= inject;
+        }
+        EGS_HIP(hipEventRecord(h->kev[2][0], h->stream));
+        hipLaunchKernelGGL(k_ev_photon<R>, grid, block, lds, h->stream, a);
+        EGS_HIP(hipGetLastError());
+        EGS_HIP(hipEventRecord(h->kev[2][1], h->stream));
+        hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, 0);
+        EGS_HIP(hipEventRecord(h->kev[3][0], h->stream));
+        hipLaunchKernelGGL(k_ev_electron<R>, grid, block, lds, h->stream, a);
+        EGS_HIP(hipGetLastError());
+        EGS_HIP(hipEventRecord(h->kev[3][1], h->stream));
+        hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, 1);
+        EGS_HIP(hipGetLastError());
+        CK(read_ctl(h, c));
+        if (timed_src) CK(take_time(h, 1));
+        CK(take_time(h, 2));
+        CK(take_time(h, 3));
+        if (remaining == 0 && c.nP == 0 && c.nE == 0) break;
+    }
+    if (c.free_tail - h->free_head != ns) {
+        set_error("event schedule ended with %llu of %llu history slots in use",
+                  (unsigned long long)(ns - (c.free_tail - h->free_head)), ns);
+        return EGSGPU_E_OVERFLOW;
+    }
     return EGSGPU_OK;
 }
 
@@ -524,10 +876,10 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     }
     if ((rc = alloc(h, (size_t)h->nvox, &h->tally.sum))) return fail(rc);
     if ((rc = alloc(h, (size_t)h->nvox, &h->tally.sum2))) return fail(rc);
-    if ((rc = alloc(h, 8, &h->tally.counters))) return fail(rc);
+    if ((rc = alloc(h, 32, &h->tally.counters))) return fail(rc);
     if (hipMemset(h->tally.sum, 0, sizeof(double) * h->nvox) != hipSuccess ||
         hipMemset(h->tally.sum2, 0, sizeof(double) * h->nvox) != hipSuccess ||
-        hipMemset(h->tally.counters, 0, 64) != hipSuccess) {
+        hipMemset(h->tally.counters, 0, 256) != hipSuccess) {
         set_error("tally clear failed");
         return fail(EGSGPU_E_HIP);
     }
@@ -540,6 +892,9 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     // coupled electron shower up to several hundred
     h->hcap = tp->kerma ? 256 : 2048;
     h->mxstk = tp->kerma ? 32 : 64;
+    // the event schedule pays off where charged particles are transported
+    h->schedule = tp->kerma ? EGSGPU_SCHED_HISTORY : EGSGPU_SCHED_EVENT;
+    h->slot_hcap = tp->kerma ? 64 : 256;
     *out = h;
     return EGSGPU_OK;
 }
@@ -575,20 +930,45 @@ int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx) {
     return EGSGPU_OK;
 }
 
+int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule) {
+    EGS_ARG(h, "handle");
+    EGS_ARG(schedule == EGSGPU_SCHED_HISTORY || schedule == EGSGPU_SCHED_EVENT, "schedule");
+    if (h->lanes_ready) {
+        set_error("the schedule must be chosen before the first run");
+        return EGSGPU_E_ARG;
+    }
+    h->schedule = schedule;
+    return EGSGPU_OK;
+}
+
 int egsgpu_run(egsgpu_handle *h, int64_t nhist) {
     EGS_ARG(h && nhist >= 0, "handle, nhist >= 0");
     int rc = ensure_lanes(h);
     if (rc) return rc;
+    CK(timing_events(h));
     EGS_HIP(hipEventRecord(h->ev0, h->stream));
-    rc = h->fp64 ? launch(h, h->pd, nhist) : launch(h, h->pf, nhist);
-    if (rc) return rc;
+    if (h->schedule == EGSGPU_SCHED_EVENT) {
+        rc = h->fp64 ? run_event(h, h->pd, nhist) : run_event(h, h->pf, nhist);
+        if (rc) return rc;
+    } else {
+        rc = h->fp64 ? launch(h, h->pd, nhist) : launch(h, h->pf, nhist);
+        if (rc) return rc;
+    }
     EGS_HIP(hipEventRecord(h->ev1, h->stream));
     EGS_HIP(hipStreamSynchronize(h->stream));
+    if (h->schedule != EGSGPU_SCHED_EVENT) CK(take_time(h, 0));
     float ms = 0;
     EGS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
-    h->last_launches = 1;
+    h->last_launches = h->klaunch[0] + h->klaunch[1] + h->klaunch[2] + h->klaunch[3];
     h->nhist += nhist;
+    return EGSGPU_OK;
+}
+
+int egsgpu_kernel_time(egsgpu_handle *h, int32_t kernel, double *ms, int32_t *launches) {
+    EGS_ARG(h && kernel >= 0 && kernel < 4 && ms && launches, "handle, kernel 0-3, outputs");
+    *ms = h->kms[kernel];
+    *launches = h->klaunch[kernel];
     return EGSGPU_OK;
 }
 
@@ -597,6 +977,10 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
     EGS_ARG(h && nhist > 0 && target && grad_out, "handle, nhist > 0, target, grad");
     if (!h->P.kerma) {
         set_error("the density gradient is defined for the photon kerma mode (kerma = 1)");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    if (h->schedule != EGSGPU_SCHED_HISTORY) {
+        set_error("the density gradient replays histories: it needs the history schedule");
         return EGSGPU_E_UNSUPPORTED;
     }
     int rc = ensure_lanes(h);
@@ -665,7 +1049,7 @@ int egsgpu_reset_tallies(egsgpu_handle *h) {
     EGS_ARG(h, "handle");
     EGS_HIP(hipMemsetAsync(h->tally.sum, 0, sizeof(double) * h->nvox, h->stream));
     EGS_HIP(hipMemsetAsync(h->tally.sum2, 0, sizeof(double) * h->nvox, h->stream));
-    EGS_HIP(hipMemsetAsync(h->tally.counters, 0, 64, h->stream));
+    EGS_HIP(hipMemsetAsync(h->tally.counters, 0, 256, h->stream));
     EGS_HIP(hipStreamSynchronize(h->stream));
     h->nhist = 0;
     return EGSGPU_OK;
@@ -673,7 +1057,15 @@ int egsgpu_reset_tallies(egsgpu_handle *h) {
 
 int egsgpu_get_counters(egsgpu_handle *h, int64_t counters[8]) {
     EGS_ARG(h && counters, "handle, counters");
-    EGS_HIP(hipMemcpy(counters, h->tally.counters, 64, hipMemcpyDeviceToHost));
+    int64_t all[32];
+    EGS_HIP(hipMemcpy(all, h->tally.counters, sizeof(all), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 8; i++) counters[i] = all[i] + all[8 + i] + all[16 + i] + all[24 + i];
+    return EGSGPU_OK;
+}
+
+int egsgpu_kernel_counters(egsgpu_handle *h, int32_t kernel, int64_t counters[8]) {
+    EGS_ARG(h && counters && kernel >= 0 && kernel < 4, "handle, kernel 0-3, counters");
+    EGS_HIP(hipMemcpy(counters, h->tally.counters + 8 * kernel, 64, hipMemcpyDeviceToHost));
     return EGSGPU_OK;
 }
 

@@ -61,24 +61,46 @@ __device__ inline void ranmar_seed(RanmarRec *r, int ixx, int jxx) {
 
 // Produce NB consecutive outputs of the stream (EGS_Ranmar::fillArray,
 // egs_rndm.cpp:338-370) as exact floats r*2^-24 into buf[k*stride].
+//
+// ix and jx step down together and always differ by 33 (mod 97): the NB
+// words updated, u[ix-k], and the NB words read, u[jx-k], k < NB, are
+// disjoint sets for NB <= 33.  So all 2*NB loads are issued before any
+// arithmetic (one memory round trip per refill instead of NB dependent
+// ones), and the updated words are stored at the end.
 template <int NB>
 __device__ inline void ranmar_refill(RanmarRec *__restrict__ r,
                                      float *__restrict__ buf, int stride) {
+    static_assert(NB >= 1 && NB <= 33, "ranmar refill runs must not overlap");
     int c = r->c, ix = r->ix, jx = r->jx;
     const float twom24 = 1.0f / 16777216.0f;
-#pragma unroll 4
+    int a[NB], b[NB];
+#pragma unroll
     for (int k = 0; k < NB; k++) {
-        int v = r->u[ix] - r->u[jx];
+        int i = ix - k, j = jx - k;
+        i += (i < 0) ? 97 : 0;
+        j += (j < 0) ? 97 : 0;
+        a[k] = r->u[i];
+        b[k] = r->u[j];
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        int v = a[k] - b[k];
         v += (v < 0) ? 16777216 : 0;
-        r->u[ix] = v;
-        ix = (ix == 0) ? 96 : ix - 1;
-        jx = (jx == 0) ? 96 : jx - 1;
+        a[k] = v;
         c -= 7654321;
         c += (c < 0) ? 16777213 : 0;
         v -= c;
         v += (v < 0) ? 16777216 : 0;
         buf[k * stride] = (float)v * twom24;
     }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        int i = ix - k;
+        i += (i < 0) ? 97 : 0;
+        r->u[i] = a[k];
+    }
+    ix -= NB; ix += (ix < 0) ? 97 : 0;
+    jx -= NB; jx += (jx < 0) ? 97 : 0;
     r->c = c; r->ix = ix; r->jx = jx;
 }
 

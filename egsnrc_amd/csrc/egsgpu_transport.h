@@ -31,7 +31,10 @@ __device__ __forceinline__ R dexp(R x) { return exp(x); }
 template <typename R>
 __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 
-template <typename R>
+// EV = false: the history schedule (a lane owns its histories; per-lane
+// tally hash).  EV = true: the event schedule (particles of one history
+// may be spread over lanes and kernels; per-slot shared tally tables).
+template <typename R, bool EV = false>
 struct Shower {
     const DevTables<R> &T;
     const GeomView<R> &G;
@@ -57,6 +60,10 @@ struct Shower {
     int cvox;          // run-length cache: voxel and energy
     double cval;
     const DevTally &tally;
+    // event schedule: the history slot of the particles on this lane
+    const DevSlots *slots = nullptr;
+    EvCtl *ev_ctl = nullptr;
+    int slot = -1;
     // counters
     unsigned long long c_psteps, c_cross, c_dep, c_flush, c_esteps, c_ovf, c_hovf;
     unsigned hsteps;   // steps of the current history (runaway guard)
@@ -119,11 +126,13 @@ struct Shower {
         hsteps = 0;
     }
 
+    // the refill is kept out of line: rnd() is called from ~100 places
+    __device__ __attribute__((noinline)) void refill() {
+        ranmar_refill<kRNB>(rrec, rbuf, blockDim.x);
+        rip = 0;
+    }
     __device__ __forceinline__ R rnd() {
-        if (rip >= kRNB) {
-            ranmar_refill<kRNB>(rrec, rbuf, blockDim.x);
-            rip = 0;
-        }
+        if (rip >= kRNB) refill();
         return (R)rbuf[(rip++) * blockDim.x];
     }
 
@@ -172,9 +181,66 @@ struct Shower {
         c_dep++;
         double e = (double)edep * (double)wt;
         if (ir == cvox) { cval += e; return; }
-        if (cvox >= 0) hash_add(cvox, cval);
+        if (cvox >= 0) {
+            if constexpr (EV) slot_add(cvox, cval);
+            else hash_add(cvox, cval);
+        }
         cvox = ir;
         cval = e;
+    }
+    // event schedule: add e to voxel vox of the current slot's table
+    __device__ void slot_add(int vox, double e) {
+        const DevSlots &S = *slots;
+        int key = vox + 1;
+        size_t base = (size_t)slot * S.hcap;
+        unsigned mask = (unsigned)S.hcap - 1u;
+        unsigned h = ((unsigned)key * 2654435761u) & mask;
+        for (int probe = 0; probe < S.hcap; probe++) {
+            int k = atomicCAS(S.key + base + h, 0, key);
+            if (k == 0) {
+                int idx = atomicAdd(S.used + slot, 1);
+                __hip_atomic_store(S.list + base + idx, (int)h, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                atomicAdd(S.val + base + h, e);
+                return;
+            }
+            if (k == key) {
+                atomicAdd(S.val + base + h, e);
+                return;
+            }
+            h = (h + 1) & mask;
+        }
+        // table full: score this part outside the history grouping (counted)
+        c_hovf++;
+        atomicAdd(tally.sum + vox, e);
+        atomicAdd(tally.sum2 + vox, e * e);
+    }
+    // event schedule: this lane is done with its particles of the slot
+    __device__ void slot_release() {
+        if (cvox >= 0) slot_add(cvox, cval);
+        cvox = -1;
+        cval = 0;
+        const DevSlots &S = *slots;
+        __threadfence();
+        int left = atomicSub(S.pending + slot, 1);
+        if (left == 1) {   // the history is complete
+            __threadfence();
+            size_t base = (size_t)slot * S.hcap;
+            int n = atomicExch(S.used + slot, 0);
+            for (int k = 0; k < n; k++) {
+                int h = __hip_atomic_load(S.list + base + k, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                int key = atomicExch(S.key + base + h, 0);
+                double e = atomicExch(S.val + base + h, 0.0);
+                atomicAdd(tally.sum + key - 1, e);
+                atomicAdd(tally.sum2 + key - 1, e * e);
+            }
+            c_flush += n;
+            unsigned long long at = atomicAdd(&ev_ctl->free_tail, 1ull);
+            __hip_atomic_store(S.ring + (at % (unsigned long long)S.nslots), slot,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        slot = -1;
     }
     __device__ void end_history() {
         hsteps = 0;
@@ -826,236 +892,250 @@ struct Shower {
         pop();
     }
 
-    __device__ void electron() {
+    // ELECTR (egsnrc.mortran:985-1823) cut into single condensed-history
+    // steps so that a wave re-converges after every step.  The loop state
+    // of ELECTR that lives across steps is kept in the e_* members:
+    //   kNew   -> the top of the stack is an electron not yet started
+    //             (:NEWELECTRON:);
+    //   kTstep -> sample the number of MFPs to the next discrete event
+    //             (:TSTEP:);
+    //   kUstep -> inside the :USTEP: loop of one TSTEP.
+    // Each call does the work up to and including one USTEP iteration (or
+    // the discrete interaction that ends a TSTEP).  Random numbers are drawn
+    // in exactly the order of the unsplit routine.
+    enum { kNew = 0, kTstep = 1, kUstep = 2 };
+    int eph = kNew;
+    int e_med = 0, e_irl = -1;
+    R e_rhof = 1, e_demfp = 0, e_sig0 = 0, e_ttstep = 0;
+    bool e_ctstep = true;
+
+    __device__ void electron_step() {
         const R RM = (R)0.5109989461, RMT2 = 2 * RM, RMSQ = RM * RM;
         const R ecut = (R)P.ecut;
-        if (P.kerma) { c_esteps++; ecut_discard(p.iq, p.e); return; }
-        int irl = p.ir;
-        int medium = vmed[irl];
-        R rhof = vrho[irl];
-        for (;;) {   // :NEWELECTRON:
-            int lelec = p.iq;
-            bool pos = lelec > 0;
-            R peie = p.e, eie = peie;
+        if (P.kerma) { c_esteps++; ecut_discard(p.iq, p.e); eph = kNew; return; }
+        const int lelec = p.iq;
+        const bool pos = lelec > 0;
+        R peie = p.e, eie = peie;
+        if (eph == kNew) {   // :NEWELECTRON:
+            e_irl = p.ir;
+            e_med = vmed[e_irl];
+            e_rhof = vrho[e_irl];
             if (eie <= ecut) { ecut_discard(lelec, peie); return; }
             if (p.wt == 0) { pop(); return; }
-            bool interact = false;
-            R elke = 0;
-            int lelke = 1;
-            while (!interact) {   // :TSTEP:
-                int m = medium;
-                const MedScalars<R> &M = T.ms[m];
-                R eke = eie - RM;
-                R rnne1 = rnd();
-                if (rnne1 == 0) rnne1 = (R)1.e-30;
-                R demfp = -dlog(rnne1);
-                demfp = demfp > (R)1.e-8 ? demfp : (R)1.e-8;
-                elke = dlog(eke);
-                lelke = lel(M, elke);
-                R sig0, sigf;
-                bool mono = pos ? M.sig_mono_p : M.sig_mono_e;
-                if (mono) {
-                    sigf = pos ? ev(T.psig0, T.psig1, m, lelke, elke) / ev(T.pdedx0, T.pdedx1, m, lelke, elke)
-                               : ev(T.esig0, T.esig1, m, lelke, elke) / ev(T.ededx0, T.ededx1, m, lelke, elke);
-                    sig0 = sigf;
+            eph = kTstep;
+        }
+        int m = e_med;
+        const MedScalars<R> &M = T.ms[m];
+        R eke = eie - RM;
+        R elke, rhof = e_rhof;
+        int lelke;
+        if (eph == kTstep) {   // :TSTEP:
+            R rnne1 = rnd();
+            if (rnne1 == 0) rnne1 = (R)1.e-30;
+            R demfp = -dlog(rnne1);
+            e_demfp = demfp > (R)1.e-8 ? demfp : (R)1.e-8;
+            elke = dlog(eke);
+            lelke = lel(M, elke);
+            bool mono = pos ? M.sig_mono_p : M.sig_mono_e;
+            if (mono) {
+                e_sig0 = pos ? ev(T.psig0, T.psig1, m, lelke, elke) / ev(T.pdedx0, T.pdedx1, m, lelke, elke)
+                             : ev(T.esig0, T.esig1, m, lelke, elke) / ev(T.ededx0, T.ededx1, m, lelke, elke);
+            } else {
+                e_sig0 = pos ? M.psig_e : M.esig_e;
+            }
+            e_ctstep = true;
+            e_ttstep = 0;
+            eph = kUstep;
+        } else {
+            elke = dlog(eke);
+            lelke = lel(M, elke);
+        }
+        // ---- one :USTEP: iteration
+        if (++hsteps > kMaxStepsPerHistory) { c_ovf++; np = 0; eph = kNew; return; }
+        c_esteps++;
+        R sig = e_sig0, tstep;
+        const R *rep = (pos ? T.range_ep1 : T.range_ep0) + m * kMaxEKE;
+        if (sig <= 0) {
+            tstep = (R)1e8;
+            e_sig0 = (R)1.E-15;
+        } else {
+            if (e_ctstep) {
+                R total_de = e_demfp / sig;
+                R ekef = eke - total_de;
+                if (ekef <= T.e_array[m * kMaxEKE]) {
+                    tstep = (R)1e8;
                 } else {
-                    sig0 = pos ? M.psig_e : M.esig_e;
-                }
-                bool compute_tstep = true;
-                R total_tstep = 0;
-                bool restart = false;
-                do {   // :USTEP:
-                    if (++hsteps > kMaxStepsPerHistory) { c_ovf++; np = 0; return; }
-                    c_esteps++;
-                    R sig = sig0, tstep;
-                    const R *rep = (pos ? T.range_ep1 : T.range_ep0) + m * kMaxEKE;
-                    if (sig <= 0) {
-                        tstep = (R)1e8;
-                        sig0 = (R)1.E-15;
+                    R elkef = dlog(ekef);
+                    int lelkef = lel(M, elkef);
+                    if (lelkef == lelke) {
+                        tstep = drange(m, pos, eke, ekef, lelke, elke, elkef);
                     } else {
-                        if (compute_tstep) {
-                            R total_de = demfp / sig;
-                            R ekef = eke - total_de;
-                            if (ekef <= T.e_array[m * kMaxEKE]) {
-                                tstep = (R)1e8;
-                            } else {
-                                R elkef = dlog(ekef);
-                                int lelkef = lel(M, elkef);
-                                if (lelkef == lelke) {
-                                    tstep = drange(m, pos, eke, ekef, lelke, elke, elkef);
-                                } else {
-                                    R ekei = T.e_array[m * kMaxEKE + lelke - 1];
-                                    R elkei = (R)(lelke - M.eke0) / M.eke1;
-                                    R tuss = drange(m, pos, eke, ekei, lelke, elke, elkei);
-                                    ekei = T.e_array[m * kMaxEKE + lelkef];
-                                    elkei = (R)(lelkef + 1 - M.eke0) / M.eke1;
-                                    tstep = drange(m, pos, ekei, ekef, lelkef, elkei, elkef);
-                                    tstep = tstep + tuss + rep[lelke - 1] - rep[lelkef];
-                                }
-                            }
-                            total_tstep = tstep;
-                            compute_tstep = false;
-                        }
-                        tstep = total_tstep / rhof;
-                    }
-                    R dedx0 = pos ? ev(T.pdedx0, T.pdedx1, m, lelke, elke) : ev(T.ededx0, T.ededx1, m, lelke, elke);
-                    R dedx = rhof * dedx0;
-                    R tmxs = ev(T.tmxs0, T.tmxs1, m, lelke, elke) / rhof;
-                    R range;
-                    {
                         R ekei = T.e_array[m * kMaxEKE + lelke - 1];
                         R elkei = (R)(lelke - M.eke0) / M.eke1;
-                        range = (drange(m, pos, eke, ekei, lelke, elke, elkei) + rep[lelke - 1]) / rhof;
+                        R tuss = drange(m, pos, eke, ekei, lelke, elke, elkei);
+                        ekei = T.e_array[m * kMaxEKE + lelkef];
+                        elkei = (R)(lelkef + 1 - M.eke0) / M.eke1;
+                        tstep = drange(m, pos, ekei, ekef, lelkef, elkei, elkef);
+                        tstep = tstep + tuss + rep[lelke - 1] - rep[lelkef];
                     }
-                    R tustep = tstep < tmxs ? tstep : tmxs;
-                    tustep = tustep < range ? tustep : range;
-                    R tperp = geom_hownear(G, p.ir, p.x, p.y, p.z);
-                    p.dnear = tperp;
-                    R blccl = rhof * M.blcc;
-                    R p2 = eke * (eke + RMT2), beta2 = p2 / (p2 + RMSQ);
-                    R ssmfp = beta2 / blccl;
-                    R skindepth = (R)P.skindepth * ssmfp;
-                    R mx = tperp > skindepth ? tperp : skindepth;
-                    tustep = tustep < mx ? tustep : mx;
-                    bool callhowfar, dosingle = false, callmsdist;
-                    R ustep, de = 0;
-                    R us = 0, vs = 0, ws = 0, xt = 0, yt = 0, zt = 0;
-                    // the CH-versus-boundary decision is taken in FP64
-                    if ((double)tustep <= (double)tperp && (double)tustep > (double)skindepth) {
-                        callhowfar = false;
-                        callmsdist = true;
-                        de = eloss_g(m, M, pos, rhof, range, tustep, eke, elke, lelke);
-                        msdist_pII(M, eke, de, tustep, rhof, us, vs, ws, xt, yt, zt, ustep);
-                    } else {
-                        callmsdist = false;
-                        R rnnoss = rnd();
-                        if (rnnoss < (R)1.e-30) rnnoss = (R)1.e-30;
-                        R lambda = -dlog(1 - rnnoss);
-                        R e1 = eke / RM + 1;
-                        R lambda_max = (R)0.5 * blccl * RM / dedx * e1 * e1 * e1;
-                        if (lambda >= 0 && lambda_max > 0) {
-                            R tuss = lambda < lambda_max ? lambda * ssmfp * (1 - (R)0.5 * lambda / lambda_max)
-                                                         : (R)0.5 * lambda * ssmfp;
-                            if (tuss < tustep) { tustep = tuss; dosingle = true; }
-                        } else {
-                            pop();
-                            return;
-                        }
-                        ustep = tustep;
-                        callhowfar = !(ustep < tperp);
-                    }
-                    int irold = p.ir, irnew = p.ir, idisc = 0;
-                    if (callhowfar || p.wt <= 0) {
-                        int inew = geom_howfar_in(G, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
-                        if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
-                    }
-                    if (ustep <= 0) ustep = 0;
-                    if (ustep == 0) {
-                        if (irnew != irold) {
-                            c_cross++;
-                            p.ir = irnew; irl = irnew;
-                            if (irnew >= 0) { medium = vmed[irnew]; rhof = vrho[irnew]; }
-                        }
-                        if (eie <= ecut) { ecut_discard(lelec, peie); return; }
-                        restart = true;
-                        break;
-                    }
-                    R vstep = ustep, tvstep;
-                    if (callhowfar) {
-                        tvstep = vstep;
-                        if (tvstep != tustep) dosingle = false;
-                        de = eloss_g(m, M, pos, rhof, range, tvstep, eke, elke, lelke);
-                    } else {
-                        tvstep = tustep;
-                    }
-                    R save_de = de;
-                    R edep = de;
-                    R ekef = eke - de;
-                    R xf, yf, zf, uf, vf, wf;
-                    if (callmsdist) {
-                        uf = us; vf = vs; wf = ws; xf = xt; yf = yt; zf = zt;
-                    } else {
-                        xf = p.x + p.u * vstep;
-                        yf = p.y + p.v * vstep;
-                        zf = p.z + p.w * vstep;
-                        if (dosingle) {
-                            R ekems = ekef > ecut - RM ? ekef : ecut - RM;
-                            R p2s = ekems * (ekems + RMT2);
-                            R chia2 = M.xcc / (4 * M.blcc * p2s);
-                            R xi = rnd();
-                            xi = 2 * chia2 * xi / (1 - xi + chia2);
-                            R costhe = 1 - xi, sinthe = dsqrt(xi * (2 - xi));
-                            R ut = p.u, vt = p.v, wt = p.w;
-                            uphi21(costhe, sinthe);
-                            uf = p.u; vf = p.v; wf = p.w;
-                            p.u = ut; p.v = vt; p.w = wt;
-                        } else {
-                            uf = p.u; vf = p.v; wf = p.w;
-                        }
-                    }
-                    score(p.ir, edep, p.wt);
-                    p.x = xf; p.y = yf; p.z = zf;
-                    p.u = uf; p.v = vf; p.w = wf;
-                    p.dnear = p.dnear - vstep;
-                    irold = p.ir;
-                    peie = peie - edep;
-                    eie = peie;
-                    p.e = peie;
-                    if (irnew == irl && eie <= ecut) { ecut_discard(lelec, peie); return; }
-                    int medold = medium;
-                    eke = eie - RM;
-                    elke = dlog(eke);
-                    lelke = lel(M, elke);
-                    if (irnew != irold) {
-                        c_cross++;
-                        p.ir = irnew; irl = irnew;
-                        if (irnew >= 0) { medium = vmed[irnew]; rhof = vrho[irnew]; }
-                    }
-                    if (eie <= ecut) { ecut_discard(lelec, peie); return; }
-                    if (idisc < 0) { pop(); return; }
-                    if (medium != medold) { restart = true; break; }
-                    demfp = demfp - save_de * sig;
-                    total_tstep = total_tstep - tvstep * rhof;
-                    if (total_tstep < (R)1e-9) demfp = 0;
-                } while (!(demfp < (R)1.e-8));
-                if (restart) continue;
-                const R dd = pos ? ev(T.pdedx0, T.pdedx1, m, lelke, elke) : ev(T.ededx0, T.ededx1, m, lelke, elke);
-                sigf = (pos ? ev(T.psig0, T.psig1, m, lelke, elke) : ev(T.esig0, T.esig1, m, lelke, elke)) / dd;
-                R rfict = rnd();
-                if (rfict <= sigf / sig0) interact = true;
-            }
-            // ---- discrete interaction (:1652-1744)
-            int m = medium;
-            const MedScalars<R> &M = T.ms[m];
-            if (!pos) {
-                R ebr1 = ev(T.ebr10, T.ebr11, m, lelke, elke);
-                R rnno24 = rnd();
-                if (rnno24 <= ebr1 || p.e <= M.thmoll) {
-                    if (!(rnno24 <= ebr1) && ebr1 <= 0) continue;
-                    brems(m, M, elke);
-                    if (p.iq == 0) return;
-                    continue;
                 }
-                moller(M);
-                if (p.iq == 0) return;
-                continue;
+                e_ttstep = tstep;
+                e_ctstep = false;
             }
-            R pbr1 = ev(T.pbr10, T.pbr11, m, lelke, elke);
-            R rnno25 = rnd();
-            if (rnno25 < pbr1) {
-                brems(m, M, elke);
-                if (p.iq == 0) return;
-                continue;
+            tstep = e_ttstep / rhof;
+        }
+        R dedx0 = pos ? ev(T.pdedx0, T.pdedx1, m, lelke, elke) : ev(T.ededx0, T.ededx1, m, lelke, elke);
+        R dedx = rhof * dedx0;
+        R tmxs = ev(T.tmxs0, T.tmxs1, m, lelke, elke) / rhof;
+        R range;
+        {
+            R ekei = T.e_array[m * kMaxEKE + lelke - 1];
+            R elkei = (R)(lelke - M.eke0) / M.eke1;
+            range = (drange(m, pos, eke, ekei, lelke, elke, elkei) + rep[lelke - 1]) / rhof;
+        }
+        R tustep = tstep < tmxs ? tstep : tmxs;
+        tustep = tustep < range ? tustep : range;
+        R tperp = geom_hownear(G, p.ir, p.x, p.y, p.z);
+        p.dnear = tperp;
+        R blccl = rhof * M.blcc;
+        R p2 = eke * (eke + RMT2), beta2 = p2 / (p2 + RMSQ);
+        R ssmfp = beta2 / blccl;
+        R skindepth = (R)P.skindepth * ssmfp;
+        R mx = tperp > skindepth ? tperp : skindepth;
+        tustep = tustep < mx ? tustep : mx;
+        bool callhowfar, dosingle = false, callmsdist;
+        R ustep, de = 0;
+        R us = 0, vs = 0, ws = 0, xt = 0, yt = 0, zt = 0;
+        // the CH-versus-boundary decision is taken in FP64
+        if ((double)tustep <= (double)tperp && (double)tustep > (double)skindepth) {
+            callhowfar = false;
+            callmsdist = true;
+            de = eloss_g(m, M, pos, rhof, range, tustep, eke, elke, lelke);
+            msdist_pII(M, eke, de, tustep, rhof, us, vs, ws, xt, yt, zt, ustep);
+        } else {
+            callmsdist = false;
+            R rnnoss = rnd();
+            if (rnnoss < (R)1.e-30) rnnoss = (R)1.e-30;
+            R lambda = -dlog(1 - rnnoss);
+            R e1 = eke / RM + 1;
+            R lambda_max = (R)0.5 * blccl * RM / dedx * e1 * e1 * e1;
+            if (lambda >= 0 && lambda_max > 0) {
+                R tuss = lambda < lambda_max ? lambda * ssmfp * (1 - (R)0.5 * lambda / lambda_max)
+                                             : (R)0.5 * lambda * ssmfp;
+                if (tuss < tustep) { tustep = tuss; dosingle = true; }
+            } else {
+                pop();
+                eph = kNew;
+                return;
             }
-            R pbr2 = ev(T.pbr20, T.pbr21, m, lelke, elke);
-            if (rnno25 < pbr2) {
-                bhabha(M);
-                if (p.iq == 0) return;
-                continue;
+            ustep = tustep;
+            callhowfar = !(ustep < tperp);
+        }
+        int irold = p.ir, irnew = p.ir, idisc = 0;
+        if (callhowfar || p.wt <= 0) {
+            int inew = geom_howfar_in(G, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+            if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
+        }
+        if (ustep <= 0) ustep = 0;
+        if (ustep == 0) {
+            if (irnew != irold) {
+                c_cross++;
+                p.ir = irnew; e_irl = irnew;
+                if (irnew >= 0) { e_med = vmed[irnew]; e_rhof = vrho[irnew]; }
             }
-            annih();
+            if (eie <= ecut) { ecut_discard(lelec, peie); eph = kNew; return; }
+            eph = kTstep;   // restart the TSTEP
             return;
         }
+        R vstep = ustep, tvstep;
+        if (callhowfar) {
+            tvstep = vstep;
+            if (tvstep != tustep) dosingle = false;
+            de = eloss_g(m, M, pos, rhof, range, tvstep, eke, elke, lelke);
+        } else {
+            tvstep = tustep;
+        }
+        R save_de = de;
+        R edep = de;
+        R ekef = eke - de;
+        R xf, yf, zf, uf, vf, wf;
+        if (callmsdist) {
+            uf = us; vf = vs; wf = ws; xf = xt; yf = yt; zf = zt;
+        } else {
+            xf = p.x + p.u * vstep;
+            yf = p.y + p.v * vstep;
+            zf = p.z + p.w * vstep;
+            if (dosingle) {
+                R ekems = ekef > ecut - RM ? ekef : ecut - RM;
+                R p2s = ekems * (ekems + RMT2);
+                R chia2 = M.xcc / (4 * M.blcc * p2s);
+                R xi = rnd();
+                xi = 2 * chia2 * xi / (1 - xi + chia2);
+                R costhe = 1 - xi, sinthe = dsqrt(xi * (2 - xi));
+                R ut = p.u, vt = p.v, wt = p.w;
+                uphi21(costhe, sinthe);
+                uf = p.u; vf = p.v; wf = p.w;
+                p.u = ut; p.v = vt; p.w = wt;
+            } else {
+                uf = p.u; vf = p.v; wf = p.w;
+            }
+        }
+        score(p.ir, edep, p.wt);
+        p.x = xf; p.y = yf; p.z = zf;
+        p.u = uf; p.v = vf; p.w = wf;
+        p.dnear = p.dnear - vstep;
+        irold = p.ir;
+        peie = peie - edep;
+        eie = peie;
+        p.e = peie;
+        if (irnew == e_irl && eie <= ecut) { ecut_discard(lelec, peie); eph = kNew; return; }
+        eke = eie - RM;
+        elke = dlog(eke);
+        lelke = lel(M, elke);
+        if (irnew != irold) {
+            c_cross++;
+            p.ir = irnew; e_irl = irnew;
+            if (irnew >= 0) { e_med = vmed[irnew]; e_rhof = vrho[irnew]; }
+        }
+        if (eie <= ecut) { ecut_discard(lelec, peie); eph = kNew; return; }
+        if (idisc < 0) { pop(); eph = kNew; return; }
+        if (e_med != m) { eph = kTstep; return; }
+        e_demfp = e_demfp - save_de * sig;
+        e_ttstep = e_ttstep - tvstep * rhof;
+        if (e_ttstep < (R)1e-9) e_demfp = 0;
+        if (!(e_demfp < (R)1.e-8)) return;   // next USTEP
+        // ---- end of the TSTEP: real or fictitious event
+        {
+            const R dd = pos ? ev(T.pdedx0, T.pdedx1, m, lelke, elke) : ev(T.ededx0, T.ededx1, m, lelke, elke);
+            R sigf = (pos ? ev(T.psig0, T.psig1, m, lelke, elke) : ev(T.esig0, T.esig1, m, lelke, elke)) / dd;
+            R rfict = rnd();
+            if (!(rfict <= sigf / e_sig0)) { eph = kTstep; return; }
+        }
+        // ---- discrete interaction (:1652-1744); the top may change
+        eph = kNew;
+        if (!pos) {
+            R ebr1 = ev(T.ebr10, T.ebr11, m, lelke, elke);
+            R rnno24 = rnd();
+            if (rnno24 <= ebr1 || p.e <= M.thmoll) {
+                if (!(rnno24 <= ebr1) && ebr1 <= 0) return;
+                brems(m, M, elke);
+                return;
+            }
+            moller(M);
+            return;
+        }
+        R pbr1 = ev(T.pbr10, T.pbr11, m, lelke, elke);
+        R rnno25 = rnd();
+        if (rnno25 < pbr1) {
+            brems(m, M, elke);
+            return;
+        }
+        R pbr2 = ev(T.pbr20, T.pbr21, m, lelke, elke);
+        if (rnno25 < pbr2) {
+            bhabha(M);
+            return;
+        }
+        annih();
     }
 
     // msdist_pII (egsnrc.mortran:4050-4308), spin effects off

@@ -49,7 +49,7 @@ def make_source(charge=0, energy=1.0, spectrum=None, shape="rectangle",
 
 class Simulation:
     def __init__(self, tables, xb, yb, zb, medium, rhor, source, transport,
-                 nstreams=0, block=0):
+                 nstreams=0, block=0, schedule=None):
         self.tables = tables
         self.geom, self._gk = _lib.make_geometry(xb, yb, zb, medium, rhor)
         self.src, self._sk = source
@@ -61,6 +61,9 @@ class Simulation:
                               C.byref(self.tp), C.byref(self.h)))
         if nstreams or block:
             check(L.egsgpu_set_streams(self.h, nstreams, block))
+        if schedule is not None:
+            check(L.egsgpu_set_schedule(self.h, {"history": _lib.SCHED_HISTORY,
+                                                 "event": _lib.SCHED_EVENT}[schedule]))
 
     def seed(self, ixx, jxx):
         check(_lib.lib().egsgpu_seed(self.h, ixx, jxx))
@@ -95,6 +98,19 @@ class Simulation:
         ms = C.c_double(0)
         n = C.c_int32(0)
         check(_lib.lib().egsgpu_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def kernel_counters(self, kernel):
+        c = np.zeros(8, dtype=np.int64)
+        check(_lib.lib().egsgpu_kernel_counters(self.h, kernel, ptr(c)))
+        return c
+
+    def kernel_time(self, kernel):
+        """(ms, launches) of one kernel in the last run: 0 history transport,
+        1 event source, 2 event photon, 3 event charged particle."""
+        ms = C.c_double(0)
+        n = C.c_int32(0)
+        check(_lib.lib().egsgpu_kernel_time(self.h, kernel, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
     def close(self):

@@ -141,6 +141,25 @@ int egsgpu_destroy(egsgpu_handle *h);
 int egsgpu_set_streams(egsgpu_handle *h, int32_t nstreams,
                        int32_t block_size);
 
+/* Scheduling of the transport work.  No counterpart in the reference,
+ * whose shower loop runs one history at a time on one CPU core
+ * (egsShower, egs_interface2.c:92-99).
+ *  EGSGPU_SCHED_HISTORY: stream s simulates its share of the histories one
+ *    after the other in one kernel, like egs-parallel job s; the result is
+ *    fixed by the seeds and the number of streams.
+ *  EGSGPU_SCHED_EVENT: the particles of all histories in flight wait in
+ *    two banks (photons, charged particles) that a photon kernel and a
+ *    condensed-history kernel drain, each lane taking the next particle of
+ *    its type when its own one ends; every history in flight keeps its
+ *    per-voxel sums in a slot of its own, so the uncertainty is still
+ *    history-by-history.  Same physics and statistics; which stream's
+ *    numbers a history uses depends on timing.
+ * Must be chosen before the first run; the default is EGSGPU_SCHED_EVENT
+ * for coupled transport and EGSGPU_SCHED_HISTORY in kerma mode. */
+#define EGSGPU_SCHED_HISTORY 0
+#define EGSGPU_SCHED_EVENT   1
+int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule);
+
 /* Seed all streams: ranmar initial seeds (ixx, jxx).  Re-seeding resets
  * the streams; tallies are kept. */
 int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx);
@@ -176,9 +195,21 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
  * [5] stack overflows, [6] tally-table overflows, [7] reserved. */
 int egsgpu_get_counters(egsgpu_handle *h, int64_t counters[8]);
 
-/* Average duration (ms) of the transport kernel launches of the last
- * egsgpu_run, measured with HIP events on the launch stream. */
+/* The same counters split by the kernel that counted them (kernel numbers
+ * as for egsgpu_kernel_time); egsgpu_get_counters is their sum. */
+int egsgpu_kernel_counters(egsgpu_handle *h, int32_t kernel,
+                           int64_t counters[8]);
+
+/* Device time (ms) of the last egsgpu_run, from HIP events on the launch
+ * stream, and the number of transport kernel launches. */
 int egsgpu_last_kernel_ms(egsgpu_handle *h, double *ms, int32_t *launches);
+
+/* Time of one kernel of the last egsgpu_run: kernel 0 = history-schedule
+ * transport, 1 = event source, 2 = event photon, 3 = event charged
+ * particle.  Sum of its launch durations (ms, HIP events around each
+ * launch) and the number of launches. */
+int egsgpu_kernel_time(egsgpu_handle *h, int32_t kernel, double *ms,
+                       int32_t *launches);
 
 /* ---- test hooks -------------------------------------------------------- */
 /* Fill out[s*n + i] with the first n numbers of stream s of ranmar seeded

@@ -59,15 +59,16 @@ def test_oracle_kerma_uniform_slab_attenuation():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("fp64", [0, 1])
-def test_kerma_dose_parity(engine, fp64):
+@pytest.mark.parametrize("fp64,schedule", [(0, "history"), (1, "history"), (0, "event")])
+def test_kerma_dose_parity(engine, fp64, schedule):
     xb, yb, zb, med, rhor, src, tp = kerma_case()
     tp.fp64 = fp64
     g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
     o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
     o.run(400000)
     a = o.tallies()
-    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=16384)
+    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=16384,
+                     schedule=schedule)
     sim.run(400000)
     b = sim.tallies()
     assert b[2] == 400000
@@ -115,22 +116,64 @@ def test_oracle_electron_depth_dose_20mev():
     assert 7.9 < r50 < 8.8, r50
 
 
+def check_parity(a, b, c, min_voxels, label):
+    """Statistical dose parity of two runs (per-voxel z-scores over voxels
+    above 10% of the maximum) and no overflow of any engine structure."""
+    chi2, out2, nv = workloads.compare_doses(a, b)
+    print("%s: chi2/dof %.3f  >2sigma %.3f  voxels %d  overflows %d/%d"
+          % (label, chi2, out2, nv, c[5], c[6]))
+    assert c[5] == 0, "stack or bank overflow"
+    assert c[6] == 0, "history tally table overflow"
+    assert nv > min_voxels
+    assert chi2 < 1.3
+    assert out2 < 0.09
+    ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
+    assert abs(ta - tb) < 0.01 * ta
+
+
 @pytest.mark.gpu
-def test_electron_dose_parity(engine):
+@pytest.mark.parametrize("schedule", ["history", "event"])
+def test_electron_dose_parity(engine, schedule):
     xb, yb, zb, med, rhor, src, tp = electron_case()
     g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
     o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
     o.run(40000)
     a = o.tallies()
-    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192)
-    sim.run(40000)
+    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192,
+                     schedule=schedule)
+    sim.run(20000)
+    sim.run(20000)
     b = sim.tallies()
-    c = sim.counters()
-    assert c[5] == 0, "stack overflow"
-    chi2, out2, nv = workloads.compare_doses(a, b)
-    print("electrons: chi2/dof %.3f  >2sigma %.3f  voxels %d  tally overflows %d" % (chi2, out2, nv, c[6]))
-    assert nv > 30
-    assert chi2 < 1.3
-    assert out2 < 0.09
-    ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
-    assert abs(ta - tb) < 0.01 * ta
+    assert b[2] == 40000
+    check_parity(a, b, sim.counters(), 30, "electrons/" + schedule)
+
+
+def photon_coupled_case(n=16, dx=1.0):
+    """Photon spectrum (0.1-6 MeV, falling) on water/lung/bone/water slabs
+    with coupled electron transport: every particle type and interaction
+    of the slab workload at a size the oracle runs in seconds."""
+    xb = np.linspace(-n * dx / 2, n * dx / 2, n + 1)
+    zb = np.linspace(0.0, n * dx, n + 1)
+    zc = 0.5 * (zb[1:] + zb[:-1])
+    mz = np.select([zc < 4, zc < 8, zc < 10], [0, 1, 2], 0)
+    med = np.broadcast_to(mz[:, None, None], (n, n, n)).reshape(-1).astype(np.int32)
+    e, w = workloads.falling_spectrum()
+    src = make_source(0, 0.0, spectrum=(e, w), x0=-3, x1=3, y0=-3, y1=3, z=-1.0)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01, kerma=0)
+    return xb, xb.copy(), zb, med, None, src, tp
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", ["history", "event"])
+def test_photon_coupled_dose_parity(engine, schedule):
+    xb, yb, zb, med, rhor, src, tp = photon_coupled_case()
+    g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
+    o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
+    o.run(400000)
+    a = o.tallies()
+    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192,
+                     schedule=schedule)
+    sim.run(400000)
+    b = sim.tallies()
+    assert b[2] == 400000
+    check_parity(a, b, sim.counters(), 100, "photons coupled/" + schedule)
