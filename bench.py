@@ -110,12 +110,32 @@ def main():
     ap.add_argument("--hist", type=int, default=0, help="histories per step; 0 = workload default")
     ap.add_argument("--schedule", choices=["history", "event"], default=None,
                     help="transport schedule; default = the library's choice")
+    ap.add_argument("--waves", default="",
+                    help="event kernels' resident waves per SIMD as P,E (2-4 each)")
+    ap.add_argument("--slots", type=int, default=0, help="event schedule history slots")
+    ap.add_argument("--batching", default="",
+                    help="event kernels: photon steps per turn, interaction batch, max wait")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (the reference "
+                         "kernel for scaling HBM traffic counters)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 << 28, "copy_bytes_written": 4 << 28,
+                          "copies": 3}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(histories would shard across GPUs with one all-reduce of the tallies)",
               file=sys.stderr)
         sys.exit(2)
+    from egsnrc_amd import _lib
     from egsnrc_amd.engine import Simulation, Tables
     name = a.workload
     xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
@@ -124,9 +144,14 @@ def main():
     sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=a.lanes, block=a.block,
                      schedule=a.schedule)
     hps = a.hist or W["hist_per_step"]
+    if a.waves or a.slots:
+        wp, we = (int(x) for x in a.waves.split(",")) if a.waves else (0, 0)
+        _lib.check(_lib.lib().egsgpu_set_event_shape(sim.h, wp, we, a.slots))
+    if a.batching:
+        ks, bt, mw = (int(x) for x in a.batching.split(","))
+        _lib.check(_lib.lib().egsgpu_set_event_batching(sim.h, ks, bt, mw))
     for _ in range(a.warmup):
         sim.run(hps)
-    from egsnrc_amd import _lib
     _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
     kms = np.zeros(4)
     klaunch = np.zeros(4, dtype=np.int64)

@@ -23,7 +23,9 @@ constexpr int kMaxNE = 16;        // elements per medium
 constexpr int kMaxShMed = 200;    // Compton shells per medium ($MXMDSH)
 constexpr int kGE = 2000;         // $MXGE
 constexpr int kMaxEKE = 500;      // $MXEKE
-constexpr int kRNB = 16;          // ranmar outputs per refill (LDS ring)
+constexpr int kRNB = 16;          // ranmar outputs per refill
+constexpr int kRing = 32;         // LDS ring of random numbers per lane
+constexpr int kRingLow = 12;      // top the ring up below this many
 
 // Per-medium scalars.
 template <typename R>

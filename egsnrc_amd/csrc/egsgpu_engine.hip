@@ -62,7 +62,7 @@ __device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R, EV> 
 // (egs_parallel_beam.cpp:40-117): position on the source shape, entry into
 // the grid along the beam direction.  Returns false if it misses the grid.
 template <typename R, bool EV>
-__device__ bool sample_primary(const DevSource<R> &S, const GeomView<R> &G,
+__device__ __forceinline__ bool sample_primary(const DevSource<R> &S, const GeomView<R> &G,
                                Shower<R, EV> &sh, Part<R> &q) {
     const R RM = (R)0.5109989461;
     int ireg = -1;
@@ -118,9 +118,12 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
         sh.np = 1;
         sh.eph = Shower<R>::kNew;
         while (sh.np > 0) {
+            sh.rtopup();
             if (sh.p.iq == 0) {
                 sh.photon();
                 sh.eph = Shower<R>::kNew;
+            } else if (sh.eph >= Shower<R>::kBrems) {
+                sh.electron_interact();
             } else {
                 sh.electron_step();
             }
@@ -140,6 +143,8 @@ struct EvArgs {
     EvCtl *ctl;
     unsigned long long free_head;   // first free-ring entry for new histories
     int ninject;                    // new histories of this source launch
+    int ksteps;                     // photon voxel steps per turn
+    int batch, max_wait;            // interaction batching (wave_choose)
 };
 
 template <typename R>
@@ -218,18 +223,46 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
     add_counters(a.k.tally, sh, 1);
 }
 
-// Photon kernel: every lane takes photons from bank P until it is empty
-// and runs PHOTON on each; charged secondaries go to bank E.
-template <typename R>
-__global__ void __launch_bounds__(256) k_ev_photon(EvArgs<R> a) {
+// Wave-level choice of the next action.  Lanes want either the common
+// action `common` (photon flight / condensed-history step) or one of the
+// rare, expensive ones (interactions, codes lo..hi).  A rare action runs
+// when it has at least `batch` lanes waiting, when nobody wants the common
+// one, or when a lane has waited `max_wait` turns; otherwise the common
+// action runs.  The result is the same for every lane of the wave.
+__device__ __forceinline__ int wave_choose(int act, int wait, int common, int lo, int hi,
+                                           int batch, int max_wait) {
+    unsigned long long starving = __ballot(act != common && wait >= max_wait);
+    if (starving) {
+        int src = __ffsll(starving) - 1;
+        return __builtin_amdgcn_readfirstlane(__shfl(act, src));
+    }
+    int ncommon = __popcll(__ballot(act == common));
+    int best = common, bestn = 0;
+    for (int t = lo; t <= hi; t++) {
+        int c = __popcll(__ballot(act == t));
+        if (c > bestn) { bestn = c; best = t; }
+    }
+    if (bestn > 0 && (ncommon == 0 || bestn >= batch)) return best;
+    return common;
+}
+
+// Photon kernel: every lane takes photons from bank P until it is empty.
+// Flights run kSteps voxel steps per turn; the interactions are batched
+// by type over the wave; charged secondaries go to bank E.
+template <typename R, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
+k_ev_photon(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.k.L.nlanes) return;
-    Shower<R, true> sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
-                       rbuf_all + threadIdx.x);
+    using S = Shower<R, true>;
+    S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
+         rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const int n = min(a.ctl->nP, a.P.cap);
+    int wait = 0;
     for (;;) {
+        sh.rtopup();
         if (sh.np == 0) {
             if (sh.slot >= 0) sh.slot_release();
             int i = wave_claim(&a.ctl->hP);
@@ -237,32 +270,51 @@ __global__ void __launch_bounds__(256) k_ev_photon(EvArgs<R> a) {
             sh.slot = bank_load(a.P, i, sh.p);
             sh.np = 1;
             sh.hsteps = 0;
-            continue;
+            sh.pph = S::kPEnter;
         }
         if (sh.p.iq != 0) {
             bank_push(sh, a.E, &a.ctl->nE);
             sh.pop();
+            sh.pph = S::kPEnter;
             continue;
         }
-        sh.photon();
+        if (sh.pph == S::kPEnter) {
+            sh.photon_enter();
+            if (sh.pph == S::kPEnter) continue;
+        }
+        if (sh.pph == S::kPNewE) {
+            sh.photon_newe();
+            if (sh.pph != S::kPTrans) continue;
+        }
+        int act = sh.pph;
+        int chosen = wave_choose(act, wait, S::kPTrans, S::kPRayl, S::kPPair, a.batch, a.max_wait);
+        if (act != chosen) { wait++; continue; }
+        wait = 0;
+        if (chosen == S::kPTrans) sh.photon_trans(a.ksteps);
+        else sh.photon_interact();
     }
     add_counters(a.k.tally, sh, 2);
 }
 
 // Charged-particle kernel: every lane takes electrons/positrons from bank
-// E and advances them one condensed-history step per loop iteration, so
-// that lanes whose particle ends pick up the next one while the others
-// keep stepping.  Photons created on the way go to bank P.
-template <typename R>
-__global__ void __launch_bounds__(256) k_ev_electron(EvArgs<R> a) {
+// E and advances them one condensed-history step per turn, so that lanes
+// whose particle ends pick up the next one while the others keep
+// stepping; discrete interactions are batched by type over the wave.
+// Photons created on the way go to bank P.
+template <typename R, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
+k_ev_electron(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.k.L.nlanes) return;
-    Shower<R, true> sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
-                       rbuf_all + threadIdx.x);
+    using S = Shower<R, true>;
+    S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
+         rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const int n = min(a.ctl->nE, a.E.cap);
+    int wait = 0;
     for (;;) {
+        sh.rtopup();
         if (sh.np == 0) {
             if (sh.slot >= 0) sh.slot_release();
             int i = wave_claim(&a.ctl->hE);
@@ -270,16 +322,20 @@ __global__ void __launch_bounds__(256) k_ev_electron(EvArgs<R> a) {
             sh.slot = bank_load(a.E, i, sh.p);
             sh.np = 1;
             sh.hsteps = 0;
-            sh.eph = Shower<R, true>::kNew;
-            continue;
+            sh.eph = S::kNew;
         }
         if (sh.p.iq == 0) {
             bank_push(sh, a.P, &a.ctl->nP);
             sh.pop();
-            sh.eph = Shower<R, true>::kNew;
+            sh.eph = S::kNew;
             continue;
         }
-        sh.electron_step();
+        int act = sh.eph < S::kBrems ? 0 : sh.eph;
+        int chosen = wave_choose(act, wait, 0, S::kBrems, S::kAnnih, a.batch, a.max_wait);
+        if (act != chosen) { wait++; continue; }
+        wait = 0;
+        if (chosen == 0) sh.electron_step();
+        else sh.electron_interact();
     }
     add_counters(a.k.tally, sh, 3);
 }
@@ -336,6 +392,9 @@ struct egsgpu_handle {
     DevTally tally{};
     RanmarRec *rng = nullptr;
     int nlanes = 0, block = 256, mxstk = 48, hcap = 256;
+    int ncu = 1;                        // compute units of the device
+    int ev_wp = 3, ev_we = 3;           // waves/SIMD variants of the event kernels
+    int ev_ksteps = 8, ev_batch = 16, ev_max_wait = 16;
     int ixx = 1802, jxx = 9373;
     bool lanes_ready = false;
     int64_t nhist = 0;
@@ -619,6 +678,9 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
 
 int ensure_lanes(egsgpu_handle *h) {
     if (h->lanes_ready) return EGSGPU_OK;
+    // default streams: what the kernels can keep resident (event kernels
+    // up to 4 waves per SIMD, the history kernel 2)
+    if (!h->nlanes) h->nlanes = h->ncu * (h->schedule == EGSGPU_SCHED_EVENT ? 1024 : 512);
     CK(alloc(h, (size_t)h->nlanes, &h->rng));
     if (h->fp64) CK(build_lanes(h, h->pd));
     else CK(build_lanes(h, h->pf));
@@ -664,7 +726,7 @@ int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = n
     a.base = n / h->nlanes;
     a.rem = (int)(n % h->nlanes);
     a.grad = grad;
-    size_t lds = (size_t)kRNB * h->block * sizeof(float);
+    size_t lds = (size_t)kRing * h->block * sizeof(float);
     EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
     hipLaunchKernelGGL(k_transport<R>, dim3((h->nlanes + h->block - 1) / h->block),
                        dim3(h->block), lds, h->stream, a);
@@ -688,7 +750,7 @@ template <typename R>
 int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     if (h->ev_ready) return EGSGPU_OK;
     DevSlots &S = h->slots;
-    S.nslots = h->nslots > 0 ? h->nslots : 4 * h->nlanes;
+    S.nslots = h->nslots > 0 ? h->nslots : 2 * h->nlanes;
     S.hcap = h->slot_hcap;
     size_t ne = (size_t)S.nslots * S.hcap;
     CK(alloc(h, ne, &S.key));
@@ -712,6 +774,27 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     return EGSGPU_OK;
 }
 
+template <typename R>
+using EvKernel = void (*)(EvArgs<R>);
+
+template <typename R>
+EvKernel<R> photon_kernel(int w) {
+    switch (w) {
+    case 3: return k_ev_photon<R, 3>;
+    case 4: return k_ev_photon<R, 4>;
+    default: return k_ev_photon<R, 2>;
+    }
+}
+
+template <typename R>
+EvKernel<R> electron_kernel(int w) {
+    switch (w) {
+    case 3: return k_ev_electron<R, 3>;
+    case 4: return k_ev_electron<R, 4>;
+    default: return k_ev_electron<R, 2>;
+    }
+}
+
 int read_ctl(egsgpu_handle *h, EvCtl &c) {
     EGS_HIP(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(EvCtl), hipMemcpyDeviceToHost, h->stream));
     EGS_HIP(hipStreamSynchronize(h->stream));
@@ -731,8 +814,19 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
     a.E = P.BE;
     a.slots = h->slots;
     a.ctl = h->ctl;
+    a.ksteps = h->ev_ksteps;
+    a.batch = h->ev_batch;
+    a.max_wait = h->ev_max_wait;
     const dim3 grid((h->nlanes + h->block - 1) / h->block), block(h->block);
-    const size_t lds = (size_t)kRNB * h->block * sizeof(float);
+    // persistent kernels: launch what stays resident at the chosen variant
+    auto resident = [&](int w) {
+        int n = std::min(h->nlanes, h->ncu * 4 * 64 * w);
+        return dim3((n + h->block - 1) / h->block);
+    };
+    const dim3 grid_p = resident(h->ev_wp), grid_e = resident(h->ev_we);
+    auto kp = photon_kernel<R>(h->ev_wp);
+    auto ke = electron_kernel<R>(h->ev_we);
+    const size_t lds = (size_t)kRing * h->block * sizeof(float);
     const bool photon_src = P.S.charge == 0;
     EvCtl c;
     CK(read_ctl(h, c));
@@ -760,12 +854,12 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
             remaining -= inject;
         }
         EGS_HIP(hipEventRecord(h->kev[2][0], h->stream));
-        hipLaunchKernelGGL(k_ev_photon<R>, grid, block, lds, h->stream, a);
+        hipLaunchKernelGGL(kp, grid_p, block, lds, h->stream, a);
         EGS_HIP(hipGetLastError());
         EGS_HIP(hipEventRecord(h->kev[2][1], h->stream));
         hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, 0);
         EGS_HIP(hipEventRecord(h->kev[3][0], h->stream));
-        hipLaunchKernelGGL(k_ev_electron<R>, grid, block, lds, h->stream, a);
+        hipLaunchKernelGGL(ke, grid_e, block, lds, h->stream, a);
         EGS_HIP(hipGetLastError());
         EGS_HIP(hipEventRecord(h->kev[3][1], h->stream));
         hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, 1);
@@ -887,7 +981,8 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     int dev = 0;
     (void)hipGetDevice(&dev);
     (void)hipGetDeviceProperties(&prop, dev);
-    h->nlanes = std::max(1, prop.multiProcessorCount) * 512;
+    h->ncu = std::max(1, prop.multiProcessorCount);
+    h->nlanes = 0;   // chosen at the first run
     // per-history tally hash: a kerma history touches a few voxels, a
     // coupled electron shower up to several hundred
     h->hcap = tp->kerma ? 256 : 2048;
@@ -938,6 +1033,33 @@ int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule) {
         return EGSGPU_E_ARG;
     }
     h->schedule = schedule;
+    return EGSGPU_OK;
+}
+
+int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon, int32_t waves_electron,
+                           int32_t nslots) {
+    EGS_ARG(h, "handle");
+    EGS_ARG((waves_photon == 0 || (waves_photon >= 2 && waves_photon <= 4)) &&
+            (waves_electron == 0 || (waves_electron >= 2 && waves_electron <= 4)) && nslots >= 0,
+            "waves per SIMD 2-4 (0 = keep), nslots >= 0");
+    if (nslots && h->ev_ready) {
+        set_error("the number of history slots must be set before the first run");
+        return EGSGPU_E_ARG;
+    }
+    if (waves_photon) h->ev_wp = waves_photon;
+    if (waves_electron) h->ev_we = waves_electron;
+    if (nslots) h->nslots = nslots;
+    return EGSGPU_OK;
+}
+
+int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps, int32_t batch,
+                              int32_t max_wait) {
+    EGS_ARG(h, "handle");
+    EGS_ARG(photon_steps >= 0 && batch >= 0 && batch <= 64 && max_wait >= 0,
+            "photon steps >= 0, batch 0-64, max_wait >= 0 (0 = keep)");
+    if (photon_steps) h->ev_ksteps = photon_steps;
+    if (batch) h->ev_batch = batch;
+    if (max_wait) h->ev_max_wait = max_wait;
     return EGSGPU_OK;
 }
 

@@ -67,9 +67,11 @@ __device__ inline void ranmar_seed(RanmarRec *r, int ixx, int jxx) {
 // disjoint sets for NB <= 33.  So all 2*NB loads are issued before any
 // arithmetic (one memory round trip per refill instead of NB dependent
 // ones), and the updated words are stored at the end.
+// The outputs go to buf[((start + k) & mask) * stride] (a ring in LDS).
 template <int NB>
 __device__ inline void ranmar_refill(RanmarRec *__restrict__ r,
-                                     float *__restrict__ buf, int stride) {
+                                     float *__restrict__ buf, int stride,
+                                     int start = 0, int mask = NB - 1) {
     static_assert(NB >= 1 && NB <= 33, "ranmar refill runs must not overlap");
     int c = r->c, ix = r->ix, jx = r->jx;
     const float twom24 = 1.0f / 16777216.0f;
@@ -91,7 +93,7 @@ __device__ inline void ranmar_refill(RanmarRec *__restrict__ r,
         c += (c < 0) ? 16777213 : 0;
         v -= c;
         v += (v < 0) ? 16777216 : 0;
-        buf[k * stride] = (float)v * twom24;
+        buf[((start + k) & mask) * stride] = (float)v * twom24;
     }
 #pragma unroll
     for (int k = 0; k < NB; k++) {

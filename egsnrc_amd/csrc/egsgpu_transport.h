@@ -24,6 +24,13 @@ struct Part {
     int iq, ir;
 };
 
+// An empty ring inside an action (long rejection loops) refills here, out
+// of line because rnd() is inlined at ~100 places.  A free function with
+// value arguments, so that the caller's state can stay in registers.
+__device__ __attribute__((noinline)) inline void ring_refill(RanmarRec *r, float *buf, int start) {
+    ranmar_refill<kRNB>(r, buf, blockDim.x, start, kRing - 1);
+}
+
 template <typename R>
 __device__ __forceinline__ R dlog(R x) { return log(x); }
 template <typename R>
@@ -43,10 +50,10 @@ struct Shower {
     const DevParams &P;
     const DevLanes<R> &L;
     int lane;
-    // RNG
+    // RNG: ring of kRing numbers per lane in LDS, read at rpos
     RanmarRec *rrec;
     float *rbuf;
-    int rip;
+    int rpos, ravail;
     // particle stack: top in registers
     Part<R> p;
     int np;
@@ -80,7 +87,7 @@ struct Shower {
     // reverse sweep over the history's events: every log-likelihood term
     // is multiplied by the weighted deposits downstream of it (later events
     // of the same particle and its descendants: stack depth >= its own)
-    __device__ void grad_sweep() {
+    __device__ __forceinline__ void grad_sweep() {
         if (log_ovf) { c_hovf++; nlog = 0; log_ovf = false; return; }
         double *acc = grad->acc + (size_t)lane * kGradDepth;
         int dmax = 0;
@@ -114,7 +121,8 @@ struct Shower {
         : T(t), G(g), vmed(vm), vrho(vr), P(pp), L(l), lane(ln), rbuf(rb),
           tally(ta) {
         rrec = L.rng + lane;
-        rip = kRNB;
+        rpos = 0;
+        ravail = 0;
         np = 0;
         A = B = C = 0; cosphi = 1; sinphi = 0;
         hused = 0;
@@ -126,14 +134,24 @@ struct Shower {
         hsteps = 0;
     }
 
-    // the refill is kept out of line: rnd() is called from ~100 places
-    __device__ __attribute__((noinline)) void refill() {
-        ranmar_refill<kRNB>(rrec, rbuf, blockDim.x);
-        rip = 0;
+    // Called once per loop iteration of a kernel: when any lane of the
+    // wave runs low, the lanes with room generate kRNB more numbers
+    // together, so that the refill runs with the wave converged.
+    __device__ __forceinline__ void rtopup() {
+        if (__any(ravail < kRingLow)) {
+            if (ravail <= kRing - kRNB) {
+                ranmar_refill<kRNB>(rrec, rbuf, blockDim.x, rpos + ravail, kRing - 1);
+                ravail += kRNB;
+            }
+        }
     }
     __device__ __forceinline__ R rnd() {
-        if (rip >= kRNB) refill();
-        return (R)rbuf[(rip++) * blockDim.x];
+        if (ravail == 0) {
+            ring_refill(rrec, rbuf, rpos);
+            ravail = kRNB;
+        }
+        ravail--;
+        return (R)rbuf[((rpos++) & (kRing - 1)) * blockDim.x];
     }
 
     // ---------------------------------------------------------- scoring
@@ -242,7 +260,7 @@ struct Shower {
         }
         slot = -1;
     }
-    __device__ void end_history() {
+    __device__ __forceinline__ void end_history() {
         hsteps = 0;
         if (cvox >= 0) hash_add(cvox, cval);
         cvox = -1;
@@ -318,88 +336,122 @@ struct Shower {
     __device__ __forceinline__ void uphi32(R costhe, R sinthe) { rotate(costhe, sinthe); }
 
     // ----------------------------------------------------------- PHOTON
-    __device__ void photon() {
-        const R pcut = (R)P.pcut;
-        R peig = p.e, eig = peig;
-        int medium = vmed[p.ir];
-        R rhof = vrho[p.ir];
-        if (eig <= pcut) { score(p.ir, peig, p.wt); pop(); return; }
-        for (;;) {   // :PNEWENERGY:
-            if (p.wt == 0) { pop(); return; }
-            R gle = dlog(eig);
-            R rnno35 = rnd();
-            if (rnno35 == 0) rnno35 = (R)1.e-30;
-            R dpmfp = -dlog(rnno35);
-            R cohfac = 1;
-            int lgle = 0;
-            bool interact = false;
-            while (!interact) {   // :PNEWMEDIUM:
-                const MedScalars<R> &M = T.ms[medium];
-                lgle = (int)(M.ge1 * gle + M.ge0);
-                lgle = lgle < 1 ? 1 : (lgle > kGE ? kGE : lgle);
-                int li = medium * kGE + lgle - 1;
-                R gmfpr0 = T.gmfp1[li] * gle + T.gmfp0[li];
-                if (P.rayleigh) cohfac = T.cohe1[li] * gle + T.cohe0[li];
-                for (;;) {   // :PTRANS:
-                    c_psteps++;
-                    if (++hsteps > kMaxStepsPerHistory) { c_ovf++; np = 0; return; }
-                    R gmfp = gmfpr0 / rhof;
-                    if (P.rayleigh) gmfp = gmfp * cohfac;
-                    R ustep = gmfp * dpmfp;
-                    int irnew = p.ir, idisc = 0;
-                    if (ustep > p.dnear || p.wt <= 0) {
-                        int inew = geom_howfar_in(G, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
-                        if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
-                    }
-                    if (grad) log_event(1, p.ir, -ustep / (gmfp * rhof));
-                    p.x = p.x + p.u * ustep;
-                    p.y = p.y + p.v * ustep;
-                    p.z = p.z + p.w * ustep;
-                    p.dnear = p.dnear - ustep;
-                    R d = dpmfp - ustep / gmfp;
-                    dpmfp = d > 0 ? d : (R)0;
-                    int medold = medium;
-                    if (irnew != p.ir) {
-                        c_cross++;
-                        p.ir = irnew;
-                        if (irnew >= 0) { medium = vmed[irnew]; rhof = vrho[irnew]; }
-                        else medium = -1;
-                    }
-                    if (idisc < 0) { pop(); return; }   // left the grid: ausgab(3), not scored
-                    if (medium != medold) break;
-                    if (dpmfp <= (R)1.e-8) { interact = true; break; }
-                }
+    // PHOTON (egsnrc.mortran:6439-6643) cut into phases so that a wave can
+    // run one phase for all lanes that need it:
+    //   kPEnter  the photon just came to the top of the stack (PCUT test);
+    //   kPNewE   :PNEWENERGY: (number of MFPs to the next interaction);
+    //   kPTrans  :PTRANS: voxel-by-voxel flight;
+    //   kPRayl, kPCompt, kPPhoto, kPPair: the interaction chosen at the end
+    //            of the flight (:PNEWMEDIUM: choice of :6560-6600).
+    // Random numbers are drawn in exactly the order of the unsplit routine.
+    enum { kPEnter = 0, kPNewE = 1, kPTrans = 2, kPRayl = 3, kPCompt = 4, kPPhoto = 5,
+           kPPair = 6 };
+    int pph = kPEnter;
+    int p_med = 0, p_lgle = 1;
+    R p_rhof = 1, p_gle = 0, p_dpmfp = 0, p_gmfpr0 = 0, p_cohfac = 1;
+
+    __device__ __forceinline__ void photon_medium() {   // :PNEWMEDIUM:
+        const MedScalars<R> &M = T.ms[p_med];
+        int lgle = (int)(M.ge1 * p_gle + M.ge0);
+        p_lgle = lgle < 1 ? 1 : (lgle > kGE ? kGE : lgle);
+        int li = p_med * kGE + p_lgle - 1;
+        p_gmfpr0 = T.gmfp1[li] * p_gle + T.gmfp0[li];
+        if (P.rayleigh) p_cohfac = T.cohe1[li] * p_gle + T.cohe0[li];
+    }
+    // kPEnter -> kPNewE, or the photon is absorbed below PCUT
+    __device__ void photon_enter() {
+        p_med = vmed[p.ir];
+        p_rhof = vrho[p.ir];
+        if (p.e <= (R)P.pcut) { score(p.ir, p.e, p.wt); pop(); return; }
+        pph = kPNewE;
+    }
+    // kPNewE -> kPTrans
+    __device__ void photon_newe() {
+        if (p.wt == 0) { pop(); pph = kPEnter; return; }
+        p_gle = dlog(p.e);
+        R rnno35 = rnd();
+        if (rnno35 == 0) rnno35 = (R)1.e-30;
+        p_dpmfp = -dlog(rnno35);
+        p_cohfac = 1;
+        photon_medium();
+        pph = kPTrans;
+    }
+    // kPTrans: up to kmax voxel steps; at the interaction point the type is
+    // sampled (kPRayl..kPPair); a photon leaving the grid is discarded
+    __device__ void photon_trans(int kmax) {
+        for (int k = 0; k < kmax; k++) {
+            c_psteps++;
+            if (++hsteps > kMaxStepsPerHistory) { c_ovf++; np = 0; pph = kPEnter; return; }
+            R gmfp = p_gmfpr0 / p_rhof;
+            if (P.rayleigh) gmfp = gmfp * p_cohfac;
+            R ustep = gmfp * p_dpmfp;
+            int irnew = p.ir, idisc = 0;
+            if (ustep > p.dnear || p.wt <= 0) {
+                int inew = geom_howfar_in(G, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+                if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
             }
-            const MedScalars<R> &M = T.ms[medium];
-            int li = medium * kGE + lgle - 1;
-            if (grad) log_event(1, p.ir, 1 / rhof);
-            if (P.rayleigh) {
-                R rnno37 = rnd();
-                if (rnno37 <= (R)1.0 - cohfac) {
-                    R costhe, sinthe;
-                    rayleigh_sample(medium, p.e, gle, li, costhe, sinthe);
-                    uphi21(costhe, sinthe);
-                    continue;
-                }
+            if (grad) log_event(1, p.ir, -ustep / (gmfp * p_rhof));
+            p.x = p.x + p.u * ustep;
+            p.y = p.y + p.v * ustep;
+            p.z = p.z + p.w * ustep;
+            p.dnear = p.dnear - ustep;
+            R d = p_dpmfp - ustep / gmfp;
+            p_dpmfp = d > 0 ? d : (R)0;
+            int medold = p_med;
+            if (irnew != p.ir) {
+                c_cross++;
+                p.ir = irnew;
+                if (irnew >= 0) { p_med = vmed[irnew]; p_rhof = vrho[irnew]; }
+                else p_med = -1;
             }
-            R rnno36 = rnd();
-            R gbr1 = T.gbr11[li] * gle + T.gbr10[li];
-            if (rnno36 <= gbr1 && p.e > (R)(2 * 0.5109989461)) {
-                pair(medium, M);
-                if (p.iq != 0) return;
-            } else {
-                R gbr2 = T.gbr21[li] * gle + T.gbr20[li];
-                if (rnno36 < gbr2) {
-                    compt(medium, M);
-                    if (p.iq != 0) return;
-                } else {
-                    photo(medium, M, gle);
-                    if (p.iq != 0) return;
+            // left the grid: ausgab(3), not scored
+            if (idisc < 0) { pop(); pph = kPEnter; return; }
+            if (p_med != medold) { photon_medium(); continue; }
+            if (p_dpmfp <= (R)1.e-8) {   // interaction: choose its type
+                int li = p_med * kGE + p_lgle - 1;
+                if (grad) log_event(1, p.ir, 1 / p_rhof);
+                if (P.rayleigh) {
+                    R rnno37 = rnd();
+                    if (rnno37 <= (R)1.0 - p_cohfac) { pph = kPRayl; return; }
                 }
+                R rnno36 = rnd();
+                R gbr1 = T.gbr11[li] * p_gle + T.gbr10[li];
+                if (rnno36 <= gbr1 && p.e > (R)(2 * 0.5109989461)) { pph = kPPair; return; }
+                R gbr2 = T.gbr21[li] * p_gle + T.gbr20[li];
+                pph = rnno36 < gbr2 ? kPCompt : kPPhoto;
+                return;
             }
-            peig = p.e;
-            eig = peig;
-            if (eig < pcut) { score(p.ir, peig, p.wt); pop(); return; }
+        }
+    }
+    // kPRayl..kPPair: the interaction; then the photon continues
+    // (:PNEWENERGY:) or the stack top changed (kPEnter)
+    __device__ void photon_interact() {
+        const int m = p_med;
+        const MedScalars<R> &M = T.ms[m];
+        if (pph == kPRayl) {
+            int li = m * kGE + p_lgle - 1;
+            R costhe, sinthe;
+            rayleigh_sample(m, p.e, p_gle, li, costhe, sinthe);
+            uphi21(costhe, sinthe);
+            pph = kPNewE;
+            return;
+        }
+        if (pph == kPPair) pair(m, M);
+        else if (pph == kPCompt) compt(m, M);
+        else photo(m, M, p_gle);
+        if (p.iq != 0) { pph = kPEnter; return; }
+        if (p.e < (R)P.pcut) { score(p.ir, p.e, p.wt); pop(); pph = kPEnter; return; }
+        pph = kPNewE;
+    }
+    // one PHOTON call of the history schedule: runs until the photon is
+    // gone or a charged particle is on top
+    __device__ __forceinline__ void photon() {
+        pph = kPEnter;
+        photon_enter();
+        while (pph != kPEnter) {
+            if (pph == kPNewE) photon_newe();
+            else if (pph == kPTrans) photon_trans(1 << 30);
+            else photon_interact();
         }
     }
 
@@ -869,7 +921,7 @@ struct Shower {
                              (1 - (R)0.333333 * fedep * (aux - 1 - (R)0.25 * fedep * (2 - aux * (4 - aux)))));
     }
     // $COMPUTE-ELOSS-G (egsnrc.macros:3315-3352)
-    __device__ R eloss_g(int m, const MedScalars<R> &M, bool pos, R rhof, R range, R step,
+    __device__ __forceinline__ R eloss_g(int m, const MedScalars<R> &M, bool pos, R rhof, R range, R step,
                          R eke, R elke, int lelke) const {
         const R *rep = (pos ? T.range_ep1 : T.range_ep0) + m * kMaxEKE;
         R tuss = range - rep[lelke - 1] / rhof;
@@ -884,7 +936,7 @@ struct Shower {
         return eloss(m, pos, rhof, tuss, eketmp, elktmp, lt) + eke - eketmp;
     }
 
-    __device__ void ecut_discard(int lelec, R peie) {
+    __device__ __forceinline__ void ecut_discard(int lelec, R peie) {
         const R RM = (R)0.5109989461;
         R edep = peie - RM;
         score(p.ir, edep, p.wt);
@@ -900,16 +952,18 @@ struct Shower {
     //   kTstep -> sample the number of MFPs to the next discrete event
     //             (:TSTEP:);
     //   kUstep -> inside the :USTEP: loop of one TSTEP.
-    // Each call does the work up to and including one USTEP iteration (or
-    // the discrete interaction that ends a TSTEP).  Random numbers are drawn
-    // in exactly the order of the unsplit routine.
-    enum { kNew = 0, kTstep = 1, kUstep = 2 };
+    //   kBrems..kAnnih -> a discrete interaction of that type is due.
+    // Each call of electron_step does the work up to and including one
+    // USTEP iteration (and the choice of the discrete interaction that
+    // ends a TSTEP).  Random numbers are drawn in exactly the order of the
+    // unsplit routine.
+    enum { kNew = 0, kTstep = 1, kUstep = 2, kBrems = 3, kMoller = 4, kBhabha = 5, kAnnih = 6 };
     int eph = kNew;
     int e_med = 0, e_irl = -1;
-    R e_rhof = 1, e_demfp = 0, e_sig0 = 0, e_ttstep = 0;
+    R e_rhof = 1, e_demfp = 0, e_sig0 = 0, e_ttstep = 0, e_elke = 0;
     bool e_ctstep = true;
 
-    __device__ void electron_step() {
+    __device__ __forceinline__ void electron_step() {
         const R RM = (R)0.5109989461, RMT2 = 2 * RM, RMSQ = RM * RM;
         const R ecut = (R)P.ecut;
         if (P.kerma) { c_esteps++; ecut_discard(p.iq, p.e); eph = kNew; return; }
@@ -1111,31 +1165,41 @@ struct Shower {
             R rfict = rnd();
             if (!(rfict <= sigf / e_sig0)) { eph = kTstep; return; }
         }
-        // ---- discrete interaction (:1652-1744); the top may change
+        // ---- discrete interaction (:1652-1744): its type is chosen here,
+        // the interaction itself is done by electron_interact()
         eph = kNew;
+        e_elke = elke;
         if (!pos) {
             R ebr1 = ev(T.ebr10, T.ebr11, m, lelke, elke);
             R rnno24 = rnd();
             if (rnno24 <= ebr1 || p.e <= M.thmoll) {
                 if (!(rnno24 <= ebr1) && ebr1 <= 0) return;
-                brems(m, M, elke);
+                eph = kBrems;
                 return;
             }
-            moller(M);
+            eph = kMoller;
             return;
         }
         R pbr1 = ev(T.pbr10, T.pbr11, m, lelke, elke);
         R rnno25 = rnd();
         if (rnno25 < pbr1) {
-            brems(m, M, elke);
+            eph = kBrems;
             return;
         }
         R pbr2 = ev(T.pbr20, T.pbr21, m, lelke, elke);
-        if (rnno25 < pbr2) {
-            bhabha(M);
-            return;
-        }
-        annih();
+        eph = rnno25 < pbr2 ? kBhabha : kAnnih;
+    }
+
+    // kBrems..kAnnih: the discrete interaction; afterwards the top of the
+    // stack is a new particle (kNew)
+    __device__ __forceinline__ void electron_interact() {
+        const int m = e_med;
+        const MedScalars<R> &M = T.ms[m];
+        if (eph == kBrems) brems(m, M, e_elke);
+        else if (eph == kMoller) moller(M);
+        else if (eph == kBhabha) bhabha(M);
+        else annih();
+        eph = kNew;
     }
 
     // msdist_pII (egsnrc.mortran:4050-4308), spin effects off

@@ -160,6 +160,21 @@ int egsgpu_set_streams(egsgpu_handle *h, int32_t nstreams,
 #define EGSGPU_SCHED_EVENT   1
 int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule);
 
+/* Launch shape of the event schedule: register budget of the photon and
+ * charged-particle kernels as resident waves per SIMD (2, 3 or 4; fewer
+ * waves = more registers per lane, 0 = keep), and the number of history
+ * slots (histories in flight; 0 = keep, default twice the streams; only
+ * before the first run).  No counterpart in the reference. */
+int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon,
+                           int32_t waves_electron, int32_t nslots);
+
+/* Work batching inside the event kernels (0 = keep): voxel steps of a
+ * photon flight per turn (default 8); an interaction type runs when
+ * `batch` lanes of a wave wait for it (default 16) or one lane has waited
+ * `max_wait` turns (default 16).  No counterpart in the reference. */
+int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps,
+                              int32_t batch, int32_t max_wait);
+
 /* Seed all streams: ranmar initial seeds (ixx, jxx).  Re-seeding resets
  * the streams; tallies are kept. */
 int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx);
