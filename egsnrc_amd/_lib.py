@@ -97,6 +97,8 @@ def lib():
         "egsgpu_set_event_batching": [vp, i32, i32, i32],
         "egsgpu_kernel_time": [vp, i32, vp, vp],
         "egsgpu_kernel_counters": [vp, i32, vp],
+        "egsgpu_checkpoint": [vp, vp, vp],
+        "egsgpu_restore": [vp, vp, C.c_size_t],
     }
     for name, args in sig.items():
         f = getattr(L, name, None)

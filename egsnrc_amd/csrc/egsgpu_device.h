@@ -27,6 +27,15 @@ constexpr int kRNB = 16;          // ranmar outputs per refill
 constexpr int kRing = 32;         // LDS ring of random numbers per lane
 constexpr int kRingLow = 12;      // top the ring up below this many
 
+// The PWLF tables of one medium and energy bin are interleaved into one
+// row, so that the lookups of a step touch one or two cache lines.
+// Photon rows (a0, a1 pairs of $EVALUATE, egsnrc.macros:1226-1260):
+constexpr int kGRow = 8, kGmfp = 0, kGbr1 = 2, kGbr2 = 4, kCohe = 6;
+// Electron rows: the fields a step needs in the first 16 values, the
+// discrete-interaction branching ratios after them.
+constexpr int kERow = 32, kESig = 0, kPSig = 2, kEDedx = 4, kPDedx = 6, kTmxs = 8,
+              kRepE = 10, kRepP = 11, kEArr = 12, kEBr1 = 16, kPBr1 = 18, kPBr2 = 20;
+
 // Per-medium scalars.
 template <typename R>
 struct MedScalars {
@@ -42,8 +51,8 @@ template <typename R>
 struct DevTables {
     int nmed;
     const MedScalars<R> *ms;          // [nmed]
-    // photon PWLF pairs [m*kGE + L-1]
-    const R *gmfp0, *gmfp1, *gbr10, *gbr11, *gbr20, *gbr21, *cohe0, *cohe1;
+    // photon PWLF rows [(m*kGE + L-1)*kGRow + field]
+    const R *grow;
     const R *pmax0, *pmax1;
     // Rayleigh sampling [m*100 + i]
     const double *ray_x, *ray_fcum, *ray_b, *ray_c;
@@ -59,10 +68,8 @@ struct DevTables {
     const int *enumb;
     // brems/pair fits [m*48 + k*8 + l]
     const R *dl;
-    // electron PWLF pairs [m*kMaxEKE + L-1]
-    const R *esig0, *esig1, *psig0, *psig1, *ededx0, *ededx1, *pdedx0,
-        *pdedx1, *ebr10, *ebr11, *pbr10, *pbr11, *pbr20, *pbr21, *tmxs0,
-        *tmxs1, *range_ep0, *range_ep1, *e_array;
+    // electron PWLF rows [(m*kMaxEKE + L-1)*kERow + field]
+    const R *erow;
     // screened Rutherford q^(2+) surface (msnew.data), [(i*8 + j)*32 + k]
     const float *ms_u, *ms_f, *ms_w;
     const int *ms_i;

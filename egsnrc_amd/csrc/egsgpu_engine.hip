@@ -87,17 +87,29 @@ __device__ __forceinline__ bool sample_primary(const DevSource<R> &S, const Geom
     return ireg >= 0;
 }
 
+// Sum over the 64 lanes of the wave (all lanes must be active).
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned lo = __shfl_xor((unsigned)v, off);
+        unsigned hi = __shfl_xor((unsigned)(v >> 32), off);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+
+// Add this wave's event counters to the kernel's totals: one atomic per
+// counter and wave.  Called by all lanes of every wave.
 template <typename R, bool EV>
 __device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, EV> &sh,
                                              int kernel) {
     unsigned long long *c = t.counters + 8 * kernel;
-    if (sh.c_psteps) atomicAdd(c + 0, sh.c_psteps);
-    if (sh.c_cross) atomicAdd(c + 1, sh.c_cross);
-    if (sh.c_dep) atomicAdd(c + 2, sh.c_dep);
-    if (sh.c_flush) atomicAdd(c + 3, sh.c_flush);
-    if (sh.c_esteps) atomicAdd(c + 4, sh.c_esteps);
-    if (sh.c_ovf) atomicAdd(c + 5, sh.c_ovf);
-    if (sh.c_hovf) atomicAdd(c + 6, sh.c_hovf);
+    const unsigned v[7] = {sh.c_psteps, sh.c_cross, sh.c_dep, sh.c_flush,
+                           sh.c_esteps, sh.c_ovf, sh.c_hovf};
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        unsigned long long w = wave_sum(v[k]);
+        if (__lane_id() == 0 && w) atomicAdd(c + k, w);
+    }
 }
 
 // History schedule: lane s runs its share of the histories one after the
@@ -106,11 +118,11 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= a.L.nlanes) return;
-    Shower<R> sh(a.T, a.G, a.vmed, a.vrho, a.P, a.L, a.tally, lane,
+    const bool live = lane < a.L.nlanes;   // lanes past the last stream idle
+    Shower<R> sh(a.T, a.G, a.vmed, a.vrho, a.P, a.L, a.tally, live ? lane : 0,
                  rbuf_all + threadIdx.x);
     sh.grad = a.grad;
-    int64_t nh = a.base + (lane < a.rem ? 1 : 0);
+    int64_t nh = live ? a.base + (lane < a.rem ? 1 : 0) : 0;
     for (int64_t k = 0; k < nh; k++) {
         Part<R> q;
         if (!sample_primary(a.S, a.G, sh, q)) continue;
@@ -195,12 +207,12 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= a.k.L.nlanes) return;
-    Shower<R, true> sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
-                       rbuf_all + threadIdx.x);
+    const bool live = lane < a.k.L.nlanes;
+    Shower<R, true> sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally,
+                       live ? lane : 0, rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const unsigned long long ns = (unsigned long long)a.slots.nslots;
-    for (int i = lane; i < a.ninject; i += a.k.L.nlanes) {
+    for (int i = lane; live && i < a.ninject; i += a.k.L.nlanes) {
         int slot = a.slots.ring[(a.free_head + (unsigned long long)i) % ns];
         Part<R> q;
         if (!sample_primary(a.k.S, a.k.G, sh, q)) {
@@ -254,14 +266,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_ev_photon(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= a.k.L.nlanes) return;
+    const bool live = lane < a.k.L.nlanes;
     using S = Shower<R, true>;
-    S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
+    S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const int n = min(a.ctl->nP, a.P.cap);
     int wait = 0;
-    for (;;) {
+    for (; live;) {
         sh.rtopup();
         if (sh.np == 0) {
             if (sh.slot >= 0) sh.slot_release();
@@ -306,14 +318,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_ev_electron(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= a.k.L.nlanes) return;
+    const bool live = lane < a.k.L.nlanes;
     using S = Shower<R, true>;
-    S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, lane,
+    S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const int n = min(a.ctl->nE, a.E.cap);
     int wait = 0;
-    for (;;) {
+    for (; live;) {
         sh.rtopup();
         if (sh.np == 0) {
             if (sh.slot >= 0) sh.slot_release();
@@ -393,7 +405,7 @@ struct egsgpu_handle {
     RanmarRec *rng = nullptr;
     int nlanes = 0, block = 256, mxstk = 48, hcap = 256;
     int ncu = 1;                        // compute units of the device
-    int ev_wp = 3, ev_we = 3;           // waves/SIMD variants of the event kernels
+    int ev_wp = 2, ev_we = 2;           // waves/SIMD variants of the event kernels
     int ev_ksteps = 8, ev_batch = 16, ev_max_wait = 16;
     int ixx = 1802, jxx = 9373;
     bool lanes_ready = false;
@@ -466,6 +478,21 @@ int pack_med(egsgpu_handle *h, const Blob &b, int nmed, const char *field,
     }
     return upload(h, v.data(), v.size(), dst);
 }
+// Interleave per-medium tables into rows: dst[((m*nrows + i)*width + off].
+template <typename R>
+int pack_rows(egsgpu_handle *h, const Blob &b, int nmed, int nrows, int width,
+              const std::vector<std::pair<const char *, int>> &fields, const R **dst) {
+    std::vector<R> v((size_t)nmed * nrows * width, (R)0);
+    for (int m = 0; m < nmed; m++) {
+        for (const auto &f : fields) {
+            const auto &a = b.F(Blob::mk(m, f.first));
+            if ((int)a.size() > nrows) { set_error("table %s too long", f.first); return EGSGPU_E_FORMAT; }
+            for (size_t i = 0; i < a.size(); i++)
+                v[((size_t)m * nrows + i) * width + f.second] = (R)a[i];
+        }
+    }
+    return upload(h, v.data(), v.size(), dst);
+}
 int pack_med_i(egsgpu_handle *h, const Blob &b, int nmed, const char *field,
                int stride, const int **dst) {
     std::vector<int> v((size_t)nmed * stride, 0);
@@ -521,14 +548,12 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
         }
     }
     CK(upload(h, ms.data(), ms.size(), &T.ms));
-    CK(pack_med<R>(h, b, nmed, "gmfp0", kGE, &T.gmfp0));
-    CK(pack_med<R>(h, b, nmed, "gmfp1", kGE, &T.gmfp1));
-    CK(pack_med<R>(h, b, nmed, "gbr10", kGE, &T.gbr10));
-    CK(pack_med<R>(h, b, nmed, "gbr11", kGE, &T.gbr11));
-    CK(pack_med<R>(h, b, nmed, "gbr20", kGE, &T.gbr20));
-    CK(pack_med<R>(h, b, nmed, "gbr21", kGE, &T.gbr21));
-    CK(pack_med<R>(h, b, nmed, "cohe0", kGE, &T.cohe0));
-    CK(pack_med<R>(h, b, nmed, "cohe1", kGE, &T.cohe1));
+    {
+        const std::vector<std::pair<const char *, int>> gf = {
+            {"gmfp0", kGmfp}, {"gmfp1", kGmfp + 1}, {"gbr10", kGbr1}, {"gbr11", kGbr1 + 1},
+            {"gbr20", kGbr2}, {"gbr21", kGbr2 + 1}, {"cohe0", kCohe}, {"cohe1", kCohe + 1}};
+        CK(pack_rows<R>(h, b, nmed, kGE, kGRow, gf, &T.grow));
+    }
     bool ray = b.has(Blob::mk(0, "ray_x"));
     if (ray) {
         CK(pack_med<R>(h, b, nmed, "ray_pmax0", kGE, &T.pmax0));
@@ -563,15 +588,16 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
         CK(upload(h, v.data(), v.size(), &T.enumb));
     }
     CK(pack_med<R>(h, b, nmed, "dl", 48, &T.dl));
-    const char *ef[] = {"esig0", "esig1", "psig0", "psig1", "ededx0", "ededx1",
-                        "pdedx0", "pdedx1", "ebr10", "ebr11", "pbr10", "pbr11",
-                        "pbr20", "pbr21", "tmxs0", "tmxs1", "range_ep0",
-                        "range_ep1", "e_array"};
-    const R **ed[] = {&T.esig0, &T.esig1, &T.psig0, &T.psig1, &T.ededx0, &T.ededx1,
-                      &T.pdedx0, &T.pdedx1, &T.ebr10, &T.ebr11, &T.pbr10, &T.pbr11,
-                      &T.pbr20, &T.pbr21, &T.tmxs0, &T.tmxs1, &T.range_ep0,
-                      &T.range_ep1, &T.e_array};
-    for (int k = 0; k < 19; k++) CK(pack_med<R>(h, b, nmed, ef[k], kMaxEKE, ed[k]));
+    {
+        const std::vector<std::pair<const char *, int>> ef = {
+            {"esig0", kESig}, {"esig1", kESig + 1}, {"psig0", kPSig}, {"psig1", kPSig + 1},
+            {"ededx0", kEDedx}, {"ededx1", kEDedx + 1}, {"pdedx0", kPDedx},
+            {"pdedx1", kPDedx + 1}, {"tmxs0", kTmxs}, {"tmxs1", kTmxs + 1},
+            {"range_ep0", kRepE}, {"range_ep1", kRepP}, {"e_array", kEArr},
+            {"ebr10", kEBr1}, {"ebr11", kEBr1 + 1}, {"pbr10", kPBr1}, {"pbr11", kPBr1 + 1},
+            {"pbr20", kPBr2}, {"pbr21", kPBr2 + 1}};
+        CK(pack_rows<R>(h, b, nmed, kMaxEKE, kERow, ef, &T.erow));
+    }
     {
         const char *mf[] = {"ms_u", "ms_f", "ms_w"};
         const float **md[] = {&T.ms_u, &T.ms_f, &T.ms_w};
@@ -750,7 +776,7 @@ template <typename R>
 int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     if (h->ev_ready) return EGSGPU_OK;
     DevSlots &S = h->slots;
-    S.nslots = h->nslots > 0 ? h->nslots : 2 * h->nlanes;
+    S.nslots = h->nslots > 0 ? h->nslots : 4 * h->nlanes;
     S.hcap = h->slot_hcap;
     size_t ne = (size_t)S.nslots * S.hcap;
     CK(alloc(h, ne, &S.key));
@@ -1084,6 +1110,82 @@ int egsgpu_run(egsgpu_handle *h, int64_t nhist) {
     h->last_ms = ms;
     h->last_launches = h->klaunch[0] + h->klaunch[1] + h->klaunch[2] + h->klaunch[3];
     h->nhist += nhist;
+    return EGSGPU_OK;
+}
+
+// ---- checkpoint / restart ----------------------------------------------
+// Layout: "EGSGPUCK", u32 version, u32 nvox, u32 nlanes, u32 pad,
+// i64 nhist, i32 ixx, i32 jxx, f64 sum[nvox], f64 sum2[nvox],
+// RanmarRec rng[nlanes] (streams that have not run yet are stored seeded).
+namespace {
+constexpr uint32_t kCkVersion = 1;
+size_t ck_size(const egsgpu_handle *h, int nlanes) {
+    return 8 + 4 * 4 + 8 + 2 * 4 + 2 * sizeof(double) * (size_t)h->nvox +
+           sizeof(RanmarRec) * (size_t)nlanes;
+}
+}  // namespace
+
+int egsgpu_checkpoint(egsgpu_handle *h, void *buf, size_t *size) {
+    EGS_ARG(h && size, "handle, size");
+    int rc = ensure_lanes(h);
+    if (rc) return rc;
+    size_t need = ck_size(h, h->nlanes);
+    if (!buf) { *size = need; return EGSGPU_OK; }
+    if (*size < need) {
+        set_error("checkpoint buffer holds %zu bytes, %zu needed", *size, need);
+        *size = need;
+        return EGSGPU_E_ARG;
+    }
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    char *p = (char *)buf;
+    memcpy(p, "EGSGPUCK", 8); p += 8;
+    uint32_t hd[4] = {kCkVersion, (uint32_t)h->nvox, (uint32_t)h->nlanes, 0};
+    memcpy(p, hd, sizeof(hd)); p += sizeof(hd);
+    memcpy(p, &h->nhist, 8); p += 8;
+    int32_t seeds[2] = {h->ixx, h->jxx};
+    memcpy(p, seeds, 8); p += 8;
+    EGS_HIP(hipMemcpy(p, h->tally.sum, sizeof(double) * h->nvox, hipMemcpyDeviceToHost));
+    p += sizeof(double) * h->nvox;
+    EGS_HIP(hipMemcpy(p, h->tally.sum2, sizeof(double) * h->nvox, hipMemcpyDeviceToHost));
+    p += sizeof(double) * h->nvox;
+    EGS_HIP(hipMemcpy(p, h->rng, sizeof(RanmarRec) * h->nlanes, hipMemcpyDeviceToHost));
+    *size = need;
+    return EGSGPU_OK;
+}
+
+int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size) {
+    EGS_ARG(h && buf, "handle, buffer");
+    const char *p = (const char *)buf;
+    if (size < 40 || memcmp(p, "EGSGPUCK", 8) != 0) {
+        set_error("not an engine checkpoint");
+        return EGSGPU_E_FORMAT;
+    }
+    uint32_t hd[4];
+    memcpy(hd, p + 8, sizeof(hd));
+    if (hd[0] != kCkVersion) { set_error("checkpoint version %u", hd[0]); return EGSGPU_E_FORMAT; }
+    if ((int)hd[1] != h->nvox) {
+        set_error("checkpoint has %u voxels, the simulation %d", hd[1], h->nvox);
+        return EGSGPU_E_ARG;
+    }
+    if (h->lanes_ready && (int)hd[2] != h->nlanes) {
+        set_error("checkpoint has %u streams, the simulation %d", hd[2], h->nlanes);
+        return EGSGPU_E_ARG;
+    }
+    if (!h->lanes_ready) h->nlanes = (int)hd[2];
+    if (size < ck_size(h, (int)hd[2])) { set_error("checkpoint truncated"); return EGSGPU_E_FORMAT; }
+    int rc = ensure_lanes(h);
+    if (rc) return rc;
+    p += 8 + sizeof(hd);
+    memcpy(&h->nhist, p, 8); p += 8;
+    int32_t seeds[2];
+    memcpy(seeds, p, 8); p += 8;
+    h->ixx = seeds[0];
+    h->jxx = seeds[1];
+    EGS_HIP(hipMemcpy(h->tally.sum, p, sizeof(double) * h->nvox, hipMemcpyHostToDevice));
+    p += sizeof(double) * h->nvox;
+    EGS_HIP(hipMemcpy(h->tally.sum2, p, sizeof(double) * h->nvox, hipMemcpyHostToDevice));
+    p += sizeof(double) * h->nvox;
+    EGS_HIP(hipMemcpy(h->rng, p, sizeof(RanmarRec) * h->nlanes, hipMemcpyHostToDevice));
     return EGSGPU_OK;
 }
 

@@ -106,6 +106,22 @@ __device__ inline void ranmar_refill(RanmarRec *__restrict__ r,
     r->c = c; r->ix = ix; r->jx = jx;
 }
 
+// The next single output of the stream, straight from its record.
+__device__ __forceinline__ float ranmar_next(RanmarRec *__restrict__ r) {
+    int c = r->c, ix = r->ix, jx = r->jx;
+    int v = r->u[ix] - r->u[jx];
+    v += (v < 0) ? 16777216 : 0;
+    r->u[ix] = v;
+    r->ix = (ix == 0) ? 96 : ix - 1;
+    r->jx = (jx == 0) ? 96 : jx - 1;
+    c -= 7654321;
+    c += (c < 0) ? 16777213 : 0;
+    r->c = c;
+    v -= c;
+    v += (v < 0) ? 16777216 : 0;
+    return (float)v * (1.0f / 16777216.0f);
+}
+
 // ---- ranlux ------------------------------------------------------------
 // Stream record: w[k] = seeds((i24-1-k) mod 24 + 1) (Fortran indices), so
 // that the lag-14 partner of word k is word (k+14) mod 24 and the word

@@ -24,13 +24,6 @@ struct Part {
     int iq, ir;
 };
 
-// An empty ring inside an action (long rejection loops) refills here, out
-// of line because rnd() is inlined at ~100 places.  A free function with
-// value arguments, so that the caller's state can stay in registers.
-__device__ __attribute__((noinline)) inline void ring_refill(RanmarRec *r, float *buf, int start) {
-    ranmar_refill<kRNB>(r, buf, blockDim.x, start, kRing - 1);
-}
-
 template <typename R>
 __device__ __forceinline__ R dlog(R x) { return log(x); }
 template <typename R>
@@ -51,7 +44,6 @@ struct Shower {
     const DevLanes<R> &L;
     int lane;
     // RNG: ring of kRing numbers per lane in LDS, read at rpos
-    RanmarRec *rrec;
     float *rbuf;
     int rpos, ravail;
     // particle stack: top in registers
@@ -61,9 +53,6 @@ struct Shower {
     R A, B, C, cosphi, sinphi;
     // history tally hash
     int hused;
-    int *hkey;
-    double *hval;
-    int *hlist;
     int cvox;          // run-length cache: voxel and energy
     double cval;
     const DevTally &tally;
@@ -72,7 +61,8 @@ struct Shower {
     EvCtl *ev_ctl = nullptr;
     int slot = -1;
     // counters
-    unsigned long long c_psteps, c_cross, c_dep, c_flush, c_esteps, c_ovf, c_hovf;
+    // (32-bit per lane and launch; added into 64-bit totals at the end)
+    unsigned c_psteps, c_cross, c_dep, c_flush, c_esteps, c_ovf, c_hovf;
     unsigned hsteps;   // steps of the current history (runaway guard)
     // density-gradient replay log (null when not replaying)
     const DevGrad *grad = nullptr;
@@ -120,15 +110,11 @@ struct Shower {
                       float *rb)
         : T(t), G(g), vmed(vm), vrho(vr), P(pp), L(l), lane(ln), rbuf(rb),
           tally(ta) {
-        rrec = L.rng + lane;
         rpos = 0;
         ravail = 0;
         np = 0;
         A = B = C = 0; cosphi = 1; sinphi = 0;
         hused = 0;
-        hkey = L.hkey + (size_t)lane * L.hcap;
-        hval = L.hval + (size_t)lane * L.hcap;
-        hlist = L.hlist + (size_t)lane * L.hcap;
         cvox = -1; cval = 0;
         c_psteps = c_cross = c_dep = c_flush = c_esteps = c_ovf = c_hovf = 0;
         hsteps = 0;
@@ -140,22 +126,29 @@ struct Shower {
     __device__ __forceinline__ void rtopup() {
         if (__any(ravail < kRingLow)) {
             if (ravail <= kRing - kRNB) {
-                ranmar_refill<kRNB>(rrec, rbuf, blockDim.x, rpos + ravail, kRing - 1);
+                ranmar_refill<kRNB>(L.rng + lane, rbuf, blockDim.x, rpos + ravail, kRing - 1);
                 ravail += kRNB;
             }
         }
     }
+    // The ring holds the stream's next ravail numbers; when a long
+    // rejection loop empties it, numbers come one at a time from the
+    // stream record (inline and call-free, so no registers are saved).
     __device__ __forceinline__ R rnd() {
-        if (ravail == 0) {
-            ring_refill(rrec, rbuf, rpos);
-            ravail = kRNB;
-        }
+        if (ravail == 0) return (R)ranmar_next(L.rng + lane);
         ravail--;
         return (R)rbuf[((rpos++) & (kRing - 1)) * blockDim.x];
     }
 
     // ---------------------------------------------------------- scoring
+    // this lane's history table (history schedule)
+    __device__ __forceinline__ int *hkey() const { return L.hkey + (size_t)lane * L.hcap; }
+    __device__ __forceinline__ double *hval() const { return L.hval + (size_t)lane * L.hcap; }
+    __device__ __forceinline__ int *hlist() const { return L.hlist + (size_t)lane * L.hcap; }
     __device__ void hash_add(int vox, double e) {
+        int *hkey = this->hkey();
+        double *hval = this->hval();
+        int *hlist = this->hlist();
         unsigned key = (unsigned)vox + 1u;
         unsigned mask = (unsigned)L.hcap - 1u;
         for (int attempt = 0; attempt < 2; attempt++) {
@@ -178,6 +171,9 @@ struct Shower {
         }
     }
     __device__ void flush_table() {
+        int *hkey = this->hkey();
+        double *hval = this->hval();
+        int *hlist = this->hlist();
         for (int k = 0; k < hused; k++) {
             int s = hlist[k];
             int vox = hkey[s] - 1;
@@ -355,8 +351,9 @@ struct Shower {
         int lgle = (int)(M.ge1 * p_gle + M.ge0);
         p_lgle = lgle < 1 ? 1 : (lgle > kGE ? kGE : lgle);
         int li = p_med * kGE + p_lgle - 1;
-        p_gmfpr0 = T.gmfp1[li] * p_gle + T.gmfp0[li];
-        if (P.rayleigh) p_cohfac = T.cohe1[li] * p_gle + T.cohe0[li];
+        const R *g = T.grow + (size_t)li * kGRow;
+        p_gmfpr0 = g[kGmfp + 1] * p_gle + g[kGmfp];
+        if (P.rayleigh) p_cohfac = g[kCohe + 1] * p_gle + g[kCohe];
     }
     // kPEnter -> kPNewE, or the photon is absorbed below PCUT
     __device__ void photon_enter() {
@@ -415,9 +412,10 @@ struct Shower {
                     if (rnno37 <= (R)1.0 - p_cohfac) { pph = kPRayl; return; }
                 }
                 R rnno36 = rnd();
-                R gbr1 = T.gbr11[li] * p_gle + T.gbr10[li];
+                const R *g = T.grow + (size_t)li * kGRow;
+                R gbr1 = g[kGbr1 + 1] * p_gle + g[kGbr1];
                 if (rnno36 <= gbr1 && p.e > (R)(2 * 0.5109989461)) { pph = kPPair; return; }
-                R gbr2 = T.gbr21[li] * p_gle + T.gbr20[li];
+                R gbr2 = g[kGbr2 + 1] * p_gle + g[kGbr2];
                 pph = rnno36 < gbr2 ? kPCompt : kPPhoto;
                 return;
             }
@@ -885,21 +883,31 @@ struct Shower {
         int L = (int)(M.eke1 * elke + M.eke0);
         return L < 1 ? 1 : (L > M.meke ? M.meke : L);
     }
-    __device__ __forceinline__ R ev(const R *a0, const R *a1, int m, int L, R x) const {
-        int i = m * kMaxEKE + L - 1;
-        return a1[i] * x + a0[i];
+    // row of the electron tables for medium m, energy bin L (1-based)
+    __device__ __forceinline__ const R *erow(int m, int L) const {
+        return T.erow + (size_t)(m * kMaxEKE + L - 1) * kERow;
     }
+    // PWLF value of field f at bin L
+    __device__ __forceinline__ R ev(int f, int m, int L, R x) const {
+        const R *r = erow(m, L);
+        return r[f + 1] * x + r[f];
+    }
+    // range_ep(j) and E_array(j) of the Fortran tables (0-based j)
+    __device__ __forceinline__ R repv(int m, bool pos, int j) const {
+        return erow(m, j + 1)[pos ? kRepP : kRepE];
+    }
+    __device__ __forceinline__ R earr(int m, int j) const { return erow(m, j + 1)[kEArr]; }
     // $COMPUTE-DRANGE (egsnrc.macros:3214-3243)
     __device__ R drange(int m, bool pos, R e1, R e2, int l1, R el1, R el2) const {
         R fedep = 1 - e2 / e1;
         R elktmp = (R)0.5 * (el1 + el2 + (R)0.25 * fedep * fedep * (1 + fedep * (1 + (R)0.875 * fedep)));
         R dedxmid, aux;
         if (!pos) {
-            dedxmid = 1 / ev(T.ededx0, T.ededx1, m, l1, elktmp);
-            aux = T.ededx1[m * kMaxEKE + l1 - 1] * dedxmid;
+            dedxmid = 1 / ev(kEDedx, m, l1, elktmp);
+            aux = erow(m, l1)[kEDedx + 1] * dedxmid;
         } else {
-            dedxmid = 1 / ev(T.pdedx0, T.pdedx1, m, l1, elktmp);
-            aux = T.pdedx1[m * kMaxEKE + l1 - 1] * dedxmid;
+            dedxmid = 1 / ev(kPDedx, m, l1, elktmp);
+            aux = erow(m, l1)[kPDedx + 1] * dedxmid;
         }
         R f = fedep / (2 - fedep);
         aux = aux * (1 + 2 * aux) * f * f / 6;
@@ -909,11 +917,11 @@ struct Shower {
     __device__ R eloss(int m, bool pos, R rhof, R step, R e, R el, int L) const {
         R dedxmid, aux;
         if (!pos) {
-            dedxmid = ev(T.ededx0, T.ededx1, m, L, el);
-            aux = T.ededx1[m * kMaxEKE + L - 1] / dedxmid;
+            dedxmid = ev(kEDedx, m, L, el);
+            aux = erow(m, L)[kEDedx + 1] / dedxmid;
         } else {
-            dedxmid = ev(T.pdedx0, T.pdedx1, m, L, el);
-            aux = T.pdedx1[m * kMaxEKE + L - 1] / dedxmid;
+            dedxmid = ev(kPDedx, m, L, el);
+            aux = erow(m, L)[kPDedx + 1] / dedxmid;
         }
         R de = dedxmid * step * rhof;
         R fedep = de / e;
@@ -923,16 +931,15 @@ struct Shower {
     // $COMPUTE-ELOSS-G (egsnrc.macros:3315-3352)
     __device__ __forceinline__ R eloss_g(int m, const MedScalars<R> &M, bool pos, R rhof, R range, R step,
                          R eke, R elke, int lelke) const {
-        const R *rep = (pos ? T.range_ep1 : T.range_ep0) + m * kMaxEKE;
-        R tuss = range - rep[lelke - 1] / rhof;
+        R tuss = range - repv(m, pos, lelke - 1) / rhof;
         if (tuss >= step) return eloss(m, pos, rhof, step, eke, elke, lelke);
         int lt = lelke;
         tuss = (range - step) * rhof;
         if (tuss <= 0) return eke - M.te * (R)0.99;
-        while (lt > 1 && tuss < rep[lt - 1]) lt--;
+        while (lt > 1 && tuss < repv(m, pos, lt - 1)) lt--;
         R elktmp = (R)(lt + 1 - M.eke0) / M.eke1;
-        R eketmp = T.e_array[m * kMaxEKE + lt];
-        tuss = (rep[lt] - tuss) / rhof;
+        R eketmp = earr(m, lt);
+        tuss = (repv(m, pos, lt) - tuss) / rhof;
         return eloss(m, pos, rhof, tuss, eketmp, elktmp, lt) + eke - eketmp;
     }
 
@@ -992,8 +999,8 @@ struct Shower {
             lelke = lel(M, elke);
             bool mono = pos ? M.sig_mono_p : M.sig_mono_e;
             if (mono) {
-                e_sig0 = pos ? ev(T.psig0, T.psig1, m, lelke, elke) / ev(T.pdedx0, T.pdedx1, m, lelke, elke)
-                             : ev(T.esig0, T.esig1, m, lelke, elke) / ev(T.ededx0, T.ededx1, m, lelke, elke);
+                e_sig0 = ev(pos ? kPSig : kESig, m, lelke, elke) /
+                         ev(pos ? kPDedx : kEDedx, m, lelke, elke);
             } else {
                 e_sig0 = pos ? M.psig_e : M.esig_e;
             }
@@ -1008,7 +1015,6 @@ struct Shower {
         if (++hsteps > kMaxStepsPerHistory) { c_ovf++; np = 0; eph = kNew; return; }
         c_esteps++;
         R sig = e_sig0, tstep;
-        const R *rep = (pos ? T.range_ep1 : T.range_ep0) + m * kMaxEKE;
         if (sig <= 0) {
             tstep = (R)1e8;
             e_sig0 = (R)1.E-15;
@@ -1016,7 +1022,7 @@ struct Shower {
             if (e_ctstep) {
                 R total_de = e_demfp / sig;
                 R ekef = eke - total_de;
-                if (ekef <= T.e_array[m * kMaxEKE]) {
+                if (ekef <= earr(m, 0)) {
                     tstep = (R)1e8;
                 } else {
                     R elkef = dlog(ekef);
@@ -1024,13 +1030,13 @@ struct Shower {
                     if (lelkef == lelke) {
                         tstep = drange(m, pos, eke, ekef, lelke, elke, elkef);
                     } else {
-                        R ekei = T.e_array[m * kMaxEKE + lelke - 1];
+                        R ekei = earr(m, lelke - 1);
                         R elkei = (R)(lelke - M.eke0) / M.eke1;
                         R tuss = drange(m, pos, eke, ekei, lelke, elke, elkei);
-                        ekei = T.e_array[m * kMaxEKE + lelkef];
+                        ekei = earr(m, lelkef);
                         elkei = (R)(lelkef + 1 - M.eke0) / M.eke1;
                         tstep = drange(m, pos, ekei, ekef, lelkef, elkei, elkef);
-                        tstep = tstep + tuss + rep[lelke - 1] - rep[lelkef];
+                        tstep = tstep + tuss + repv(m, pos, lelke - 1) - repv(m, pos, lelkef);
                     }
                 }
                 e_ttstep = tstep;
@@ -1038,14 +1044,14 @@ struct Shower {
             }
             tstep = e_ttstep / rhof;
         }
-        R dedx0 = pos ? ev(T.pdedx0, T.pdedx1, m, lelke, elke) : ev(T.ededx0, T.ededx1, m, lelke, elke);
+        R dedx0 = ev(pos ? kPDedx : kEDedx, m, lelke, elke);
         R dedx = rhof * dedx0;
-        R tmxs = ev(T.tmxs0, T.tmxs1, m, lelke, elke) / rhof;
+        R tmxs = ev(kTmxs, m, lelke, elke) / rhof;
         R range;
         {
-            R ekei = T.e_array[m * kMaxEKE + lelke - 1];
+            R ekei = earr(m, lelke - 1);
             R elkei = (R)(lelke - M.eke0) / M.eke1;
-            range = (drange(m, pos, eke, ekei, lelke, elke, elkei) + rep[lelke - 1]) / rhof;
+            range = (drange(m, pos, eke, ekei, lelke, elke, elkei) + repv(m, pos, lelke - 1)) / rhof;
         }
         R tustep = tstep < tmxs ? tstep : tmxs;
         tustep = tustep < range ? tustep : range;
@@ -1160,8 +1166,8 @@ struct Shower {
         if (!(e_demfp < (R)1.e-8)) return;   // next USTEP
         // ---- end of the TSTEP: real or fictitious event
         {
-            const R dd = pos ? ev(T.pdedx0, T.pdedx1, m, lelke, elke) : ev(T.ededx0, T.ededx1, m, lelke, elke);
-            R sigf = (pos ? ev(T.psig0, T.psig1, m, lelke, elke) : ev(T.esig0, T.esig1, m, lelke, elke)) / dd;
+            const R dd = ev(pos ? kPDedx : kEDedx, m, lelke, elke);
+            R sigf = ev(pos ? kPSig : kESig, m, lelke, elke) / dd;
             R rfict = rnd();
             if (!(rfict <= sigf / e_sig0)) { eph = kTstep; return; }
         }
@@ -1170,7 +1176,7 @@ struct Shower {
         eph = kNew;
         e_elke = elke;
         if (!pos) {
-            R ebr1 = ev(T.ebr10, T.ebr11, m, lelke, elke);
+            R ebr1 = ev(kEBr1, m, lelke, elke);
             R rnno24 = rnd();
             if (rnno24 <= ebr1 || p.e <= M.thmoll) {
                 if (!(rnno24 <= ebr1) && ebr1 <= 0) return;
@@ -1180,13 +1186,13 @@ struct Shower {
             eph = kMoller;
             return;
         }
-        R pbr1 = ev(T.pbr10, T.pbr11, m, lelke, elke);
+        R pbr1 = ev(kPBr1, m, lelke, elke);
         R rnno25 = rnd();
         if (rnno25 < pbr1) {
             eph = kBrems;
             return;
         }
-        R pbr2 = ev(T.pbr20, T.pbr21, m, lelke, elke);
+        R pbr2 = ev(kPBr2, m, lelke, elke);
         eph = rnno25 < pbr2 ? kBhabha : kAnnih;
     }
 

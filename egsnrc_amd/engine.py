@@ -100,6 +100,18 @@ class Simulation:
         check(_lib.lib().egsgpu_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def checkpoint(self):
+        """The run state (tallies, histories, RNG streams) as bytes."""
+        n = C.c_size_t(0)
+        check(_lib.lib().egsgpu_checkpoint(self.h, None, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(_lib.lib().egsgpu_checkpoint(self.h, buf, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def restore(self, data):
+        buf = C.create_string_buffer(data, len(data))
+        check(_lib.lib().egsgpu_restore(self.h, buf, len(data)))
+
     def kernel_counters(self, kernel):
         c = np.zeros(8, dtype=np.int64)
         check(_lib.lib().egsgpu_kernel_counters(self.h, kernel, ptr(c)))

@@ -163,7 +163,7 @@ int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule);
 /* Launch shape of the event schedule: register budget of the photon and
  * charged-particle kernels as resident waves per SIMD (2, 3 or 4; fewer
  * waves = more registers per lane, 0 = keep), and the number of history
- * slots (histories in flight; 0 = keep, default twice the streams; only
+ * slots (histories in flight; 0 = keep, default four times the streams; only
  * before the first run).  No counterpart in the reference. */
 int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon,
                            int32_t waves_electron, int32_t nslots);
@@ -190,6 +190,17 @@ int egsgpu_run(egsgpu_handle *h, int64_t nhist);
 int egsgpu_get_tallies(egsgpu_handle *h, double *sum, double *sum2,
                        int64_t *nhist);
 int egsgpu_reset_tallies(egsgpu_handle *h);
+
+/* Checkpoint and restart (EGS_Application::outputData / readData,
+ * egs_application.cpp:482-560, which store the run control, RNG, source
+ * and scoring states): the tallies, the number of histories and the state
+ * of every RNG stream, in an engine-private binary layout.  With
+ * buf == NULL, egsgpu_checkpoint only returns the size needed.  A restore
+ * needs a simulation of the same number of voxels and streams; continuing
+ * after a restore gives the same random numbers as continuing the
+ * checkpointed run. */
+int egsgpu_checkpoint(egsgpu_handle *h, void *buf, size_t *size);
+int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size);
 
 /* Density gradient (photon kerma mode): runs nhist histories forward from
  * the current stream states to get the dose D_v = E_v/(N m_v), then replays

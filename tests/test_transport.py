@@ -177,3 +177,30 @@ def test_photon_coupled_dose_parity(engine, schedule):
     b = sim.tallies()
     assert b[2] == 400000
     check_parity(a, b, sim.counters(), 100, "photons coupled/" + schedule)
+
+
+@pytest.mark.gpu
+def test_checkpoint_restart_continues_the_run(engine):
+    """A run restored from a checkpoint continues with the same random
+    numbers: checkpoint after N1 histories, then N2 more, equals N1 + N2
+    in one go (history schedule; FP64 atomics make the sums exact only up
+    to their order)."""
+    xb, yb, zb, med, rhor, src, tp = kerma_case()
+    tab = Tables(BLOB521)
+    a = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=4096, schedule="history")
+    a.run(20000)
+    ck = a.checkpoint()
+    a.run(30000)
+    sa = a.tallies()
+    b = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=4096, schedule="history")
+    b.restore(ck)
+    b.run(30000)
+    sb = b.tallies()
+    assert sa[2] == sb[2] == 50000
+    np.testing.assert_allclose(sb[0], sa[0], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(sb[1], sa[1], rtol=1e-12, atol=1e-12)
+    # a checkpoint does not fit a simulation with other streams
+    c = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=2048, schedule="history")
+    c.run(10)
+    with pytest.raises(_lib.EngineError):
+        c.restore(ck)
