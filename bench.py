@@ -114,7 +114,11 @@ def main():
                     help="event kernels' resident waves per SIMD as P,E (2-4 each)")
     ap.add_argument("--slots", type=int, default=0, help="event schedule history slots")
     ap.add_argument("--batching", default="",
-                    help="event kernels: photon steps per turn, interaction batch, max wait")
+                    help="event kernels: photon steps per turn, interaction batch, max wait, "
+                         "defer below")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="only run the workload with FP32 and FP64 transport (same seeds) "
+                         "and print the per-voxel difference summary")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (the reference "
                          "kernel for scaling HBM traffic counters)")
@@ -141,6 +145,16 @@ def main():
     xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
     W = WORKLOADS[name]
     tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", W["blob"]))
+    if a.precision_report:
+        from egsnrc_amd.precision_report import precision_report
+        n = a.hist or 1_000_000
+        t0 = time.perf_counter()
+        rep, _ = precision_report(tab, xb, yb, zb, med, rhor, src, tp, n)
+        rep.update({"workload": cfg["workload"], "seconds": time.perf_counter() - t0,
+                    "comparison": "FP32 vs FP64 transport, same seeds and streams, "
+                                  "voxels above 10% of Dmax; z = diff / combined sigma"})
+        print(json.dumps(rep))
+        return
     sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=a.lanes, block=a.block,
                      schedule=a.schedule)
     hps = a.hist or W["hist_per_step"]
@@ -148,8 +162,8 @@ def main():
         wp, we = (int(x) for x in a.waves.split(",")) if a.waves else (0, 0)
         _lib.check(_lib.lib().egsgpu_set_event_shape(sim.h, wp, we, a.slots))
     if a.batching:
-        ks, bt, mw = (int(x) for x in a.batching.split(","))
-        _lib.check(_lib.lib().egsgpu_set_event_batching(sim.h, ks, bt, mw))
+        ks, bt, mw, db = (int(x) for x in a.batching.split(","))
+        _lib.check(_lib.lib().egsgpu_set_event_batching(sim.h, ks, bt, mw, db))
     for _ in range(a.warmup):
         sim.run(hps)
     _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))

@@ -94,7 +94,7 @@ def lib():
         "egsgpu_tables_medium_info": [vp, i32, vp, C.c_size_t, vp],
         "egsgpu_set_schedule": [vp, i32],
         "egsgpu_set_event_shape": [vp, i32, i32, i32],
-        "egsgpu_set_event_batching": [vp, i32, i32, i32],
+        "egsgpu_set_event_batching": [vp, i32, i32, i32, i32],
         "egsgpu_kernel_time": [vp, i32, vp, vp],
         "egsgpu_kernel_counters": [vp, i32, vp],
         "egsgpu_checkpoint": [vp, vp, vp],

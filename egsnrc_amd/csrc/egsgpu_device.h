@@ -111,11 +111,15 @@ struct DevTally {
 // ---- event schedule -----------------------------------------------------
 // Particles waiting for a transport kernel (SoA, index fastest).  tag =
 // slot*4 + (iq+1): the history slot the particle belongs to and its charge.
+// A particle deferred in the middle of its transport also carries the
+// loop state of PHOTON / ELECTR (s0..s3, ph); fresh particles have ph = 0.
 template <typename R>
 struct DevBank {
     int cap;
     R *e, *x, *y, *z, *u, *v, *w, *wt, *dn;
     int *ir, *tag;
+    R *s0, *s1, *s2, *s3;
+    int *ph;
 };
 
 // History slots of the event schedule.  Every history in flight owns one
@@ -137,12 +141,13 @@ struct DevSlots {
     int *ring;                // free slots, ring of nslots entries
 };
 
-// Control words of the event schedule (device memory).
+// Control words of the event schedule (device memory).  Banks 0, 1 hold
+// photons, 2, 3 charged particles (two of each: one drained by a kernel
+// while particles deferred by it go to the other).
 struct EvCtl {
     unsigned long long free_tail;   // ring append position
-    int nP, nE;                     // bank fill counts
-    int hP, hE;                     // bank consume heads
-    int pad[2];
+    int n[4];                       // bank fill counts
+    int head[4];                    // bank consume heads
 };
 
 // All active lanes of the wave claim consecutive indices from *counter

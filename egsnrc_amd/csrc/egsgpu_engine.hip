@@ -150,13 +150,17 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
 template <typename R>
 struct EvArgs {
     KArgs<R> k;
-    DevBank<R> P, E;        // photon and charged-particle banks
     DevSlots slots;
     EvCtl *ctl;
+    // the bank this launch drains, the bank particles of the other type go
+    // to, and the bank particles deferred at the end of the launch go to
+    DevBank<R> cons, push, defr;
+    int ic, ip, id;                 // their indices in EvCtl
     unsigned long long free_head;   // first free-ring entry for new histories
     int ninject;                    // new histories of this source launch
     int ksteps;                     // photon voxel steps per turn
     int batch, max_wait;            // interaction batching (wave_choose)
+    int defer_below;                // defer when fewer lanes of a wave are busy
 };
 
 template <typename R>
@@ -176,15 +180,19 @@ __device__ __forceinline__ int bank_load(const DevBank<R> &b, int i, Part<R> &q)
     return tag >> 2;
 }
 
-// Move the particle on top of the lane's stack into bank b.  The history's
-// pending count goes up before the particle becomes visible.  A full bank
-// absorbs the particle where it is (counted as an overflow).
+// Move the particle on top of the lane's stack into bank b (count *count)
+// with loop state ph, s[].  The history's pending count goes up before the
+// particle becomes visible.  A full bank absorbs the particle where it is
+// (counted as an overflow).
 template <typename R>
-__device__ void bank_push(Shower<R, true> &sh, const DevBank<R> &b, int *count) {
+__device__ void bank_push(Shower<R, true> &sh, const DevBank<R> &b, int *count, int ph = 0,
+                          R s0 = 0, R s1 = 0, R s2 = 0, R s3 = 0) {
     atomicAdd(sh.slots->pending + sh.slot, 1);
     int i = wave_claim(count);
     if (i < b.cap) {
         bank_store(b, i, sh.p, sh.slot);
+        b.ph[i] = ph;
+        if (ph) { b.s0[i] = s0; b.s1[i] = s1; b.s2[i] = s2; b.s3[i] = s3; }
         return;
     }
     atomicSub(sh.slots->pending + sh.slot, 1);
@@ -203,6 +211,7 @@ __device__ __forceinline__ void ev_setup(Shower<R, true> &sh, const EvArgs<R> &a
 }
 
 // New histories into free slots: thread i samples primaries i, i+nlanes, ...
+// into bank `push` (photon or charged, by the source).
 template <typename R>
 __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
@@ -221,10 +230,10 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
             continue;
         }
         atomicExch(a.slots.pending + slot, 1);
-        const DevBank<R> &b = q.iq == 0 ? a.P : a.E;
-        int j = wave_claim(q.iq == 0 ? &a.ctl->nP : &a.ctl->nE);
-        if (j < b.cap) {
-            bank_store(b, j, q, slot);
+        int j = wave_claim(&a.ctl->n[a.ip]);
+        if (j < a.push.cap) {
+            bank_store(a.push, j, q, slot);
+            a.push.ph[j] = 0;
         } else {   // the host leaves room for every new primary
             sh.c_ovf++;
             atomicExch(a.slots.pending + slot, 0);
@@ -258,9 +267,12 @@ __device__ __forceinline__ int wave_choose(int act, int wait, int common, int lo
     return common;
 }
 
-// Photon kernel: every lane takes photons from bank P until it is empty.
-// Flights run kSteps voxel steps per turn; the interactions are batched
-// by type over the wave; charged secondaries go to bank E.
+// Photon kernel: every lane takes photons from bank `cons` until it is
+// empty.  Flights run ksteps voxel steps per turn; the interactions are
+// batched by type over the wave; charged secondaries go to bank `push`.
+// Once the bank is empty and fewer than defer_below lanes of a wave are
+// still busy, they hand their photons (with the loop state) to bank `defr`
+// for the next launch instead of holding the wave.
 template <typename R, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_ev_photon(EvArgs<R> a) {
@@ -271,21 +283,51 @@ k_ev_photon(EvArgs<R> a) {
     S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
-    const int n = min(a.ctl->nP, a.P.cap);
+    const int n = min(a.ctl->n[a.ic], a.cons.cap);
+    // deferral starts only after a lane of this wave found the bank empty,
+    // and only in launches with work for at least half of the lanes (small
+    // launches run every particle to its end, so everything progresses)
+    const int defer_below = n >= (int)(gridDim.x * blockDim.x / 2)
+                                ? min(a.defer_below, __popcll(__ballot(live))) : 0;
     int wait = 0;
     for (; live;) {
+        if (__popcll(__ballot(1)) < defer_below) {
+            // the bank is drained and this wave is nearly idle
+            bool top = true;
+            while (sh.np > 0) {
+                if (sh.p.iq != 0) {
+                    bank_push(sh, a.push, &a.ctl->n[a.ip]);
+                } else if (top && sh.pph != S::kPEnter) {
+                    bank_push(sh, a.defr, &a.ctl->n[a.id], sh.pph, sh.p_dpmfp, sh.p_gle);
+                } else {
+                    bank_push(sh, a.defr, &a.ctl->n[a.id]);
+                }
+                sh.pop();
+                top = false;
+            }
+            if (sh.slot >= 0) sh.slot_release();
+            break;
+        }
         sh.rtopup();
         if (sh.np == 0) {
             if (sh.slot >= 0) sh.slot_release();
-            int i = wave_claim(&a.ctl->hP);
+            int i = wave_claim(&a.ctl->head[a.ic]);
             if (i >= n) break;
-            sh.slot = bank_load(a.P, i, sh.p);
+            sh.slot = bank_load(a.cons, i, sh.p);
             sh.np = 1;
             sh.hsteps = 0;
-            sh.pph = S::kPEnter;
+            sh.pph = a.cons.ph[i];
+            if (sh.pph != S::kPEnter) {   // resume a deferred flight
+                sh.p_dpmfp = a.cons.s0[i];
+                sh.p_gle = a.cons.s1[i];
+                sh.p_med = a.k.vmed[sh.p.ir];
+                sh.p_rhof = a.k.vrho[sh.p.ir];
+                sh.p_cohfac = 1;
+                sh.photon_medium();
+            }
         }
         if (sh.p.iq != 0) {
-            bank_push(sh, a.E, &a.ctl->nE);
+            bank_push(sh, a.push, &a.ctl->n[a.ip]);
             sh.pop();
             sh.pph = S::kPEnter;
             continue;
@@ -309,10 +351,11 @@ k_ev_photon(EvArgs<R> a) {
 }
 
 // Charged-particle kernel: every lane takes electrons/positrons from bank
-// E and advances them one condensed-history step per turn, so that lanes
-// whose particle ends pick up the next one while the others keep
+// `cons` and advances them one condensed-history step per turn, so that
+// lanes whose particle ends pick up the next one while the others keep
 // stepping; discrete interactions are batched by type over the wave.
-// Photons created on the way go to bank P.
+// Photons created on the way go to bank `push`; a nearly idle wave after
+// the bank is drained defers its particles to bank `defr`.
 template <typename R, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_ev_electron(EvArgs<R> a) {
@@ -323,21 +366,54 @@ k_ev_electron(EvArgs<R> a) {
     S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
-    const int n = min(a.ctl->nE, a.E.cap);
+    const int n = min(a.ctl->n[a.ic], a.cons.cap);
+    // deferral starts only after a lane of this wave found the bank empty,
+    // and only in launches with work for at least half of the lanes (small
+    // launches run every particle to its end, so everything progresses)
+    const int defer_below = n >= (int)(gridDim.x * blockDim.x / 2)
+                                ? min(a.defer_below, __popcll(__ballot(live))) : 0;
     int wait = 0;
     for (; live;) {
+        if (__popcll(__ballot(1)) < defer_below) {
+            bool top = true;
+            while (sh.np > 0) {
+                if (sh.p.iq == 0) {
+                    bank_push(sh, a.push, &a.ctl->n[a.ip]);
+                } else if (top && sh.eph != S::kNew) {
+                    bank_push(sh, a.defr, &a.ctl->n[a.id], sh.eph | (sh.e_ctstep ? 8 : 0),
+                              sh.e_demfp, sh.e_ttstep, sh.e_sig0, sh.e_elke);
+                } else {
+                    bank_push(sh, a.defr, &a.ctl->n[a.id]);
+                }
+                sh.pop();
+                top = false;
+            }
+            if (sh.slot >= 0) sh.slot_release();
+            break;
+        }
         sh.rtopup();
         if (sh.np == 0) {
             if (sh.slot >= 0) sh.slot_release();
-            int i = wave_claim(&a.ctl->hE);
+            int i = wave_claim(&a.ctl->head[a.ic]);
             if (i >= n) break;
-            sh.slot = bank_load(a.E, i, sh.p);
+            sh.slot = bank_load(a.cons, i, sh.p);
             sh.np = 1;
             sh.hsteps = 0;
-            sh.eph = S::kNew;
+            int ph = a.cons.ph[i];
+            sh.eph = ph & 7;
+            if (sh.eph != S::kNew) {   // resume a deferred particle
+                sh.e_ctstep = (ph & 8) != 0;
+                sh.e_demfp = a.cons.s0[i];
+                sh.e_ttstep = a.cons.s1[i];
+                sh.e_sig0 = a.cons.s2[i];
+                sh.e_elke = a.cons.s3[i];
+                sh.e_irl = sh.p.ir;
+                sh.e_med = a.k.vmed[sh.p.ir];
+                sh.e_rhof = a.k.vrho[sh.p.ir];
+            }
         }
         if (sh.p.iq == 0) {
-            bank_push(sh, a.P, &a.ctl->nP);
+            bank_push(sh, a.push, &a.ctl->n[a.ip]);
             sh.pop();
             sh.eph = S::kNew;
             continue;
@@ -352,11 +428,11 @@ k_ev_electron(EvArgs<R> a) {
     add_counters(a.k.tally, sh, 3);
 }
 
-// Bank bookkeeping between kernels: which = 0 empties P, 1 empties E.
-__global__ void k_ev_ctl(EvCtl *c, int which) {
+// Bank bookkeeping between kernels: bank b is drained.
+__global__ void k_ev_ctl(EvCtl *c, int b) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (which == 0) { c->nP = 0; c->hP = 0; }
-        else { c->nE = 0; c->hE = 0; }
+        c->n[b] = 0;
+        c->head[b] = 0;
     }
 }
 
@@ -369,7 +445,7 @@ __global__ void k_ev_init(DevSlots S, EvCtl *c) {
     }
     if (i == 0) {
         c->free_tail = (unsigned long long)S.nslots;
-        c->nP = c->nE = c->hP = c->hE = 0;
+        for (int k = 0; k < 4; k++) c->n[k] = c->head[k] = 0;
     }
 }
 
@@ -386,7 +462,7 @@ struct Precision {
     const R *vrho = nullptr;
     DevSource<R> S;
     DevLanes<R> L;
-    DevBank<R> BP{}, BE{};   // event-schedule banks
+    DevBank<R> bank[4]{};    // event-schedule banks: 0, 1 photons; 2, 3 charged
 };
 
 }  // namespace egsgpu
@@ -406,7 +482,8 @@ struct egsgpu_handle {
     int nlanes = 0, block = 256, mxstk = 48, hcap = 256;
     int ncu = 1;                        // compute units of the device
     int ev_wp = 2, ev_we = 2;           // waves/SIMD variants of the event kernels
-    int ev_ksteps = 8, ev_batch = 16, ev_max_wait = 16;
+    int ev_ksteps = 8, ev_batch = 16, ev_max_wait = 16, ev_defer = 16;
+    int pc = 0, ec = 2;                 // banks to drain next (photon, charged)
     int ixx = 1802, jxx = 9373;
     bool lanes_ready = false;
     int64_t nhist = 0;
@@ -768,6 +845,9 @@ int alloc_bank(egsgpu_handle *h, int cap, DevBank<R> &b) {
     for (R **x : f) CK(alloc(h, (size_t)cap, x));
     CK(alloc(h, (size_t)cap, &b.ir));
     CK(alloc(h, (size_t)cap, &b.tag));
+    R **g[] = {&b.s0, &b.s1, &b.s2, &b.s3};
+    for (R **x : g) CK(alloc(h, (size_t)cap, x));
+    CK(alloc(h, (size_t)cap, &b.ph));
     return EGSGPU_OK;
 }
 
@@ -789,8 +869,10 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     EGS_HIP(hipMemset(S.val, 0, ne * sizeof(double)));
     CK(alloc(h, 1, &h->ctl));
     EGS_HIP(hipHostMalloc((void **)&h->ctl_host, sizeof(EvCtl), hipHostMallocDefault));
-    CK(alloc_bank(h, 2 * S.nslots, P.BP));
-    CK(alloc_bank(h, 8 * S.nslots, P.BE));
+    for (int k = 0; k < 2; k++) CK(alloc_bank(h, 2 * S.nslots, P.bank[k]));
+    for (int k = 2; k < 4; k++) CK(alloc_bank(h, 8 * S.nslots, P.bank[k]));
+    h->pc = 0;
+    h->ec = 2;
     hipLaunchKernelGGL(k_ev_init, dim3((S.nslots + 255) / 256), dim3(256), 0, h->stream,
                        S, h->ctl);
     EGS_HIP(hipGetLastError());
@@ -830,19 +912,20 @@ int read_ctl(egsgpu_handle *h, EvCtl &c) {
 
 // Event schedule: inject new histories into free slots while banks drain.
 // One iteration = source, photon kernel, charged-particle kernel; the host
-// reads the bank counts after each iteration.
+// reads the bank counts after each iteration.  Each kernel drains one bank
+// of its type (pc, ec) and defers leftovers to the other one, which is
+// drained in the next iteration.
 template <typename R>
 int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
     CK(ensure_event(h, P));
     EvArgs<R> a;
     a.k = kargs(h, P);
-    a.P = P.BP;
-    a.E = P.BE;
     a.slots = h->slots;
     a.ctl = h->ctl;
     a.ksteps = h->ev_ksteps;
     a.batch = h->ev_batch;
     a.max_wait = h->ev_max_wait;
+    a.defer_below = h->ev_defer;
     const dim3 grid((h->nlanes + h->block - 1) / h->block), block(h->block);
     // persistent kernels: launch what stays resident at the chosen variant
     auto resident = [&](int w) {
@@ -854,16 +937,25 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
     auto ke = electron_kernel<R>(h->ev_we);
     const size_t lds = (size_t)kRing * h->block * sizeof(float);
     const bool photon_src = P.S.charge == 0;
+    auto setb = [&](int ic, int ip, int id) {
+        a.ic = ic; a.ip = ip; a.id = id;
+        a.cons = P.bank[ic]; a.push = P.bank[ip]; a.defr = P.bank[id];
+    };
+    auto banks_empty = [](const EvCtl &c) {
+        return c.n[0] == 0 && c.n[1] == 0 && c.n[2] == 0 && c.n[3] == 0;
+    };
     EvCtl c;
     CK(read_ctl(h, c));
     int64_t remaining = nhist;
     const unsigned long long ns = (unsigned long long)h->slots.nslots;
     for (int64_t iter = 0;; iter++) {
+        const int pc = h->pc, ec = h->ec;
         unsigned long long freeslots = c.free_tail - h->free_head;
         int64_t inject = std::min<int64_t>((int64_t)freeslots, remaining);
-        int64_t room = photon_src ? (int64_t)a.P.cap - c.nP : (int64_t)a.E.cap - c.nE;
+        int64_t room = photon_src ? (int64_t)P.bank[pc].cap - c.n[pc]
+                                  : (int64_t)P.bank[ec].cap - c.n[ec];
         inject = std::max<int64_t>(0, std::min(inject, room));
-        if (inject == 0 && c.nP == 0 && c.nE == 0) {
+        if (inject == 0 && banks_empty(c)) {
             set_error("event schedule: no free history slot with empty banks");
             return EGSGPU_E_OVERFLOW;
         }
@@ -871,6 +963,7 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
         if (inject > 0) {
             a.free_head = h->free_head;
             a.ninject = (int)inject;
+            setb(photon_src ? pc : ec, photon_src ? pc : ec, photon_src ? pc : ec);
             EGS_HIP(hipEventRecord(h->kev[1][0], h->stream));
             hipLaunchKernelGGL(k_ev_source<R>, grid, block, lds, h->stream, a);
             EGS_HIP(hipGetLastError());
@@ -879,22 +972,28 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
             h->free_head += (unsigned long long)inject;
             remaining -= inject;
         }
+        // photons: drain pc, electrons to ec, deferred photons to the other
+        setb(pc, ec, 1 - pc);
         EGS_HIP(hipEventRecord(h->kev[2][0], h->stream));
         hipLaunchKernelGGL(kp, grid_p, block, lds, h->stream, a);
         EGS_HIP(hipGetLastError());
         EGS_HIP(hipEventRecord(h->kev[2][1], h->stream));
-        hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, 0);
+        hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, pc);
+        h->pc = 1 - pc;
+        // charged: drain ec, photons to the new pc, deferred to the other
+        setb(ec, h->pc, 5 - ec);
         EGS_HIP(hipEventRecord(h->kev[3][0], h->stream));
         hipLaunchKernelGGL(ke, grid_e, block, lds, h->stream, a);
         EGS_HIP(hipGetLastError());
         EGS_HIP(hipEventRecord(h->kev[3][1], h->stream));
-        hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, 1);
+        hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, ec);
         EGS_HIP(hipGetLastError());
+        h->ec = 5 - ec;
         CK(read_ctl(h, c));
         if (timed_src) CK(take_time(h, 1));
         CK(take_time(h, 2));
         CK(take_time(h, 3));
-        if (remaining == 0 && c.nP == 0 && c.nE == 0) break;
+        if (remaining == 0 && banks_empty(c)) break;
     }
     if (c.free_tail - h->free_head != ns) {
         set_error("event schedule ended with %llu of %llu history slots in use",
@@ -1079,13 +1178,15 @@ int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon, int32_t waves
 }
 
 int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps, int32_t batch,
-                              int32_t max_wait) {
+                              int32_t max_wait, int32_t defer_below) {
     EGS_ARG(h, "handle");
-    EGS_ARG(photon_steps >= 0 && batch >= 0 && batch <= 64 && max_wait >= 0,
-            "photon steps >= 0, batch 0-64, max_wait >= 0 (0 = keep)");
+    EGS_ARG(photon_steps >= 0 && batch >= 0 && batch <= 64 && max_wait >= 0 &&
+            defer_below >= -1 && defer_below <= 64,
+            "photon steps >= 0, batch 0-64, max_wait >= 0, defer_below -1..64 (0 = keep)");
     if (photon_steps) h->ev_ksteps = photon_steps;
     if (batch) h->ev_batch = batch;
     if (max_wait) h->ev_max_wait = max_wait;
+    if (defer_below) h->ev_defer = defer_below < 0 ? 0 : defer_below;
     return EGSGPU_OK;
 }
 

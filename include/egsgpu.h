@@ -171,9 +171,13 @@ int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon,
 /* Work batching inside the event kernels (0 = keep): voxel steps of a
  * photon flight per turn (default 8); an interaction type runs when
  * `batch` lanes of a wave wait for it (default 16) or one lane has waited
- * `max_wait` turns (default 16).  No counterpart in the reference. */
+ * `max_wait` turns (default 16); once its bank is drained, a wave with
+ * fewer than `defer_below` busy lanes (default 16, -1 = never) hands its
+ * particles, mid-flight state included, to the next launch.  No
+ * counterpart in the reference. */
 int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps,
-                              int32_t batch, int32_t max_wait);
+                              int32_t batch, int32_t max_wait,
+                              int32_t defer_below);
 
 /* Seed all streams: ranmar initial seeds (ixx, jxx).  Re-seeding resets
  * the streams; tallies are kept. */

@@ -204,3 +204,19 @@ def test_checkpoint_restart_continues_the_run(engine):
     c.run(10)
     with pytest.raises(_lib.EngineError):
         c.restore(ck)
+
+
+@pytest.mark.gpu
+def test_fp32_vs_fp64_precision_report(engine):
+    """FP32 transport against FP64 transport with the same random numbers:
+    per-voxel differences stay well inside the statistical uncertainty and
+    the total deposit agrees to 1e-3."""
+    from egsnrc_amd.precision_report import precision_report
+    xb, yb, zb, med, rhor, src, tp = photon_coupled_case()
+    rep, arrays = precision_report(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, 100000,
+                                   nstreams=4096)
+    print(rep)
+    assert rep["voxels"] > 100
+    assert rep["chi2_per_dof"] < 1.0
+    assert rep["frac_beyond_2sigma"] < 0.05
+    assert abs(rep["total_rel_diff"]) < 1e-3
