@@ -5,8 +5,8 @@
 //    log-energy grids, egsnrc.macros:1226-1248) in precision R; the
 //    cumulative-probability tables (Rayleigh CDF, source alias table) in
 //    FP64;
-//  * the voxel grid: medium id (uint8) + relative density (R) per voxel,
-//    plane positions in R;
+//  * the voxel grid: relative density (R) + medium index per voxel in one
+//    record, plane positions in R;
 //  * per-lane state: ranmar record (400 B), particle stack (SoA, lane
 //    fastest), history tally hash (keys int32 + values FP64).
 #pragma once
@@ -36,6 +36,14 @@ constexpr int kGRow = 8, kGmfp = 0, kGbr1 = 2, kGbr2 = 4, kCohe = 6;
 constexpr int kERow = 32, kESig = 0, kPSig = 2, kEDedx = 4, kPDedx = 6, kTmxs = 8,
               kRepE = 10, kRepP = 11, kEArr = 12, kEBr1 = 16, kPBr1 = 18, kPBr2 = 20;
 
+// One voxel: relative density and medium index, read with one load per
+// voxel crossing.
+template <typename R>
+struct alignas(2 * sizeof(R)) VoxData {
+    R rho;
+    int med;
+};
+
 // Per-medium scalars.
 template <typename R>
 struct MedScalars {
@@ -49,6 +57,7 @@ struct MedScalars {
 
 template <typename R>
 struct DevTables {
+    using value_type = R;
     int nmed;
     const MedScalars<R> *ms;          // [nmed]
     // photon PWLF rows [(m*kGE + L-1)*kGRow + field]
@@ -89,14 +98,16 @@ struct DevParams {
     double ecut, pcut, skindepth;
 };
 
+// Fields of a stacked particle.
+constexpr int kStkF = 11;
+
 // Per-lane state arrays (lane-fastest SoA for the stack).
 template <typename R>
 struct DevLanes {
     int nlanes;
     RanmarRec *rng;                 // [nlanes]
     int mxstk;
-    R *se, *sx, *sy, *sz, *su, *sv, *sw, *swt, *sdn;   // [k*nlanes + lane]
-    int *siq, *sir;
+    R *stk;   // [(k*kStkF + field)*nlanes + lane]; iq and ir stored bitwise
     int hcap;                       // hash capacity per lane (power of 2)
     int *hkey;                      // [lane*hcap + slot], voxel+1, 0 = empty
     double *hval;                   // [lane*hcap + slot]
@@ -148,6 +159,16 @@ struct EvCtl {
     unsigned long long free_tail;   // ring append position
     int n[4];                       // bank fill counts
     int head[4];                    // bank consume heads
+};
+
+// One simulation of a batch (egsgpu_run_batched): its geometry, voxels,
+// dose sums and source.
+template <typename R>
+struct JobDesc {
+    GeomView<R> G;
+    const VoxData<R> *vox;
+    double *sum, *sum2;
+    DevSource<R> S;
 };
 
 // All active lanes of the wave claim consecutive indices from *counter

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/egsgpu.h"
@@ -31,8 +32,7 @@ template <typename R>
 struct KArgs {
     DevTables<R> T;
     GeomView<R> G;
-    const uint8_t *vmed;
-    const R *vrho;
+    const VoxData<R> *vox;
     DevParams P;
     DevLanes<R> L;
     DevTally tally;
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.L.nlanes;   // lanes past the last stream idle
-    Shower<R> sh(a.T, a.G, a.vmed, a.vrho, a.P, a.L, a.tally, live ? lane : 0,
+    Shower<R> sh(a.T, a.G, a.vox, a.P, a.L, a.tally, live ? lane : 0,
                  rbuf_all + threadIdx.x);
     sh.grad = a.grad;
     int64_t nh = live ? a.base + (lane < a.rem ? 1 : 0) : 0;
@@ -161,7 +161,24 @@ struct EvArgs {
     int ksteps;                     // photon voxel steps per turn
     int batch, max_wait;            // interaction batching (wave_choose)
     int defer_below;                // defer when fewer lanes of a wave are busy
+    // batch of simulations (null jobs: the one simulation of k)
+    const JobDesc<R> *jobs;
+    int *slot_job;                  // [nslots] job of the history in each slot
+    const long long *job_first;     // [njobs+1] first history index of each job
+    int njobs;
+    long long hist_base;            // index of this source launch's first history
 };
+
+// Point the shower at the simulation of the history in `slot`.
+template <typename R>
+__device__ __forceinline__ void ev_job(Shower<R, true> &sh, const EvArgs<R> &a, int slot) {
+    if (!a.jobs) return;
+    const JobDesc<R> &J = a.jobs[a.slot_job[slot]];
+    sh.Gp = &J.G;
+    sh.vox = J.vox;
+    sh.tsum = J.sum;
+    sh.tsum2 = J.sum2;
+}
 
 template <typename R>
 __device__ __forceinline__ void bank_store(const DevBank<R> &b, int i, const Part<R> &q, int slot) {
@@ -217,14 +234,27 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
-    Shower<R, true> sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally,
+    Shower<R, true> sh(a.k.T, a.k.G, a.k.vox, a.k.P, a.k.L, a.k.tally,
                        live ? lane : 0, rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const unsigned long long ns = (unsigned long long)a.slots.nslots;
     for (int i = lane; live && i < a.ninject; i += a.k.L.nlanes) {
         int slot = a.slots.ring[(a.free_head + (unsigned long long)i) % ns];
         Part<R> q;
-        if (!sample_primary(a.k.S, a.k.G, sh, q)) {
+        const DevSource<R> *src = &a.k.S;
+        const GeomView<R> *geo = &a.k.G;
+        if (a.jobs) {   // the job of history hist_base + i
+            long long hidx = a.hist_base + i;
+            int lo = 0, hi = a.njobs;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (a.job_first[mid] <= hidx) lo = mid; else hi = mid;
+            }
+            a.slot_job[slot] = lo;
+            src = &a.jobs[lo].S;
+            geo = &a.jobs[lo].G;
+        }
+        if (!sample_primary(*src, *geo, sh, q)) {
             unsigned long long at = atomicAdd(&a.ctl->free_tail, 1ull);
             a.slots.ring[at % ns] = slot;
             continue;
@@ -280,7 +310,7 @@ k_ev_photon(EvArgs<R> a) {
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
     using S = Shower<R, true>;
-    S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
+    S sh(a.k.T, a.k.G, a.k.vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const int n = min(a.ctl->n[a.ic], a.cons.cap);
@@ -314,14 +344,16 @@ k_ev_photon(EvArgs<R> a) {
             int i = wave_claim(&a.ctl->head[a.ic]);
             if (i >= n) break;
             sh.slot = bank_load(a.cons, i, sh.p);
+            ev_job(sh, a, sh.slot);
             sh.np = 1;
             sh.hsteps = 0;
             sh.pph = a.cons.ph[i];
             if (sh.pph != S::kPEnter) {   // resume a deferred flight
                 sh.p_dpmfp = a.cons.s0[i];
                 sh.p_gle = a.cons.s1[i];
-                sh.p_med = a.k.vmed[sh.p.ir];
-                sh.p_rhof = a.k.vrho[sh.p.ir];
+                VoxData<R> d = sh.vox[sh.p.ir];
+                sh.p_med = d.med;
+                sh.p_rhof = d.rho;
                 sh.p_cohfac = 1;
                 sh.photon_medium();
             }
@@ -363,7 +395,7 @@ k_ev_electron(EvArgs<R> a) {
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
     using S = Shower<R, true>;
-    S sh(a.k.T, a.k.G, a.k.vmed, a.k.vrho, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
+    S sh(a.k.T, a.k.G, a.k.vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const int n = min(a.ctl->n[a.ic], a.cons.cap);
@@ -397,6 +429,7 @@ k_ev_electron(EvArgs<R> a) {
             int i = wave_claim(&a.ctl->head[a.ic]);
             if (i >= n) break;
             sh.slot = bank_load(a.cons, i, sh.p);
+            ev_job(sh, a, sh.slot);
             sh.np = 1;
             sh.hsteps = 0;
             int ph = a.cons.ph[i];
@@ -408,8 +441,9 @@ k_ev_electron(EvArgs<R> a) {
                 sh.e_sig0 = a.cons.s2[i];
                 sh.e_elke = a.cons.s3[i];
                 sh.e_irl = sh.p.ir;
-                sh.e_med = a.k.vmed[sh.p.ir];
-                sh.e_rhof = a.k.vrho[sh.p.ir];
+                VoxData<R> d = sh.vox[sh.p.ir];
+                sh.e_med = d.med;
+                sh.e_rhof = d.rho;
             }
         }
         if (sh.p.iq == 0) {
@@ -459,7 +493,7 @@ template <typename R>
 struct Precision {
     DevTables<R> T;
     GeomView<R> G;
-    const R *vrho = nullptr;
+    const VoxData<R> *vox = nullptr;
     DevSource<R> S;
     DevLanes<R> L;
     DevBank<R> bank[4]{};    // event-schedule banks: 0, 1 photons; 2, 3 charged
@@ -475,15 +509,16 @@ struct egsgpu_handle {
     std::vector<void *> allocs;      // everything to hipFree
     Precision<float> pf;
     Precision<double> pd;
-    const uint8_t *vmed = nullptr;
     DevParams P{};
     DevTally tally{};
     RanmarRec *rng = nullptr;
     int nlanes = 0, block = 256, mxstk = 48, hcap = 256;
     int ncu = 1;                        // compute units of the device
     int ev_wp = 2, ev_we = 2;           // waves/SIMD variants of the event kernels
-    int ev_ksteps = 8, ev_batch = 16, ev_max_wait = 16, ev_defer = 16;
+    int ev_ksteps = 8, ev_batch = 16, ev_max_wait = 16, ev_defer = 32;
     int pc = 0, ec = 2;                 // banks to drain next (photon, charged)
+    int *slot_job = nullptr;            // [nslots] job of each slot (batched runs)
+    const egsgpu_tables *tab = nullptr; // the tables the simulation was made with
     int ixx = 1802, jxx = 9373;
     bool lanes_ready = false;
     int64_t nhist = 0;
@@ -693,7 +728,8 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
 }
 
 template <typename R>
-int build_geom(egsgpu_handle *h, const EgsXYZGeometry *g, Precision<R> &P) {
+int build_geom(egsgpu_handle *h, const EgsXYZGeometry *g, const std::vector<uint8_t> &med,
+               Precision<R> &P) {
     int n = (g->nx + 1) + (g->ny + 1) + (g->nz + 1);
     std::vector<R> pl(n);
     int k = 0;
@@ -709,10 +745,12 @@ int build_geom(egsgpu_handle *h, const EgsXYZGeometry *g, Precision<R> &P) {
     if (tol <= 0) tol = sizeof(R) == 4 ? 1.0 / (1 << 21) : 1.0 / 4294967296.0;
     v.btol = (R)tol;
     v.hbtol = v.btol / (R)2;
-    std::vector<R> rho((size_t)h->nvox, (R)1);
-    if (g->rhor)
-        for (int i = 0; i < h->nvox; i++) rho[i] = (R)g->rhor[i];
-    CK(upload(h, rho.data(), rho.size(), &P.vrho));
+    std::vector<VoxData<R>> vox((size_t)h->nvox);
+    for (int i = 0; i < h->nvox; i++) {
+        vox[i].rho = g->rhor ? (R)g->rhor[i] : (R)1;
+        vox[i].med = med[i];
+    }
+    CK(upload(h, vox.data(), vox.size(), &P.vox));
     return EGSGPU_OK;
 }
 
@@ -764,11 +802,11 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
     L.nlanes = h->nlanes;
     L.rng = h->rng;
     L.mxstk = h->mxstk;
-    size_t ns = (size_t)h->mxstk * h->nlanes;
-    R **rf[] = {&L.se, &L.sx, &L.sy, &L.sz, &L.su, &L.sv, &L.sw, &L.swt, &L.sdn};
-    for (R **f : rf) CK(alloc(h, ns, f));
-    CK(alloc(h, ns, &L.siq));
-    CK(alloc(h, ns, &L.sir));
+    if ((double)h->mxstk * kStkF * h->nlanes * sizeof(R) >= 4294967296.0) {
+        set_error("particle stacks of %d streams exceed 4 GB", h->nlanes);
+        return EGSGPU_E_ARG;
+    }
+    CK(alloc(h, (size_t)h->mxstk * kStkF * h->nlanes, &L.stk));
     // per-lane history tables: history schedule only
     L.hcap = h->schedule == EGSGPU_SCHED_HISTORY ? h->hcap : 0;
     size_t nh = (size_t)L.hcap * h->nlanes;
@@ -798,7 +836,7 @@ int ensure_lanes(egsgpu_handle *h) {
 template <typename R>
 KArgs<R> kargs(egsgpu_handle *h, Precision<R> &P) {
     KArgs<R> a;
-    a.T = P.T; a.G = P.G; a.vmed = h->vmed; a.vrho = P.vrho; a.P = h->P;
+    a.T = P.T; a.G = P.G; a.vox = P.vox; a.P = h->P;
     a.L = P.L; a.tally = h->tally; a.S = P.S;
     a.base = 0;
     a.rem = 0;
@@ -869,6 +907,7 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     EGS_HIP(hipMemset(S.val, 0, ne * sizeof(double)));
     CK(alloc(h, 1, &h->ctl));
     EGS_HIP(hipHostMalloc((void **)&h->ctl_host, sizeof(EvCtl), hipHostMallocDefault));
+    CK(alloc(h, (size_t)S.nslots, &h->slot_job));
     for (int k = 0; k < 2; k++) CK(alloc_bank(h, 2 * S.nslots, P.bank[k]));
     for (int k = 2; k < 4; k++) CK(alloc_bank(h, 8 * S.nslots, P.bank[k]));
     h->pc = 0;
@@ -915,11 +954,25 @@ int read_ctl(egsgpu_handle *h, EvCtl &c) {
 // reads the bank counts after each iteration.  Each kernel drains one bank
 // of its type (pc, ec) and defers leftovers to the other one, which is
 // drained in the next iteration.
+// A batch of simulations run together (egsgpu_run_batched).
 template <typename R>
-int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
+struct BatchRun {
+    const JobDesc<R> *jobs = nullptr;   // device
+    const long long *job_first = nullptr;
+    int njobs = 0;
+};
+
+template <typename R>
+int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
+              const BatchRun<R> *batch = nullptr) {
     CK(ensure_event(h, P));
     EvArgs<R> a;
     a.k = kargs(h, P);
+    a.jobs = batch ? batch->jobs : nullptr;
+    a.job_first = batch ? batch->job_first : nullptr;
+    a.njobs = batch ? batch->njobs : 0;
+    a.slot_job = h->slot_job;
+    a.hist_base = 0;
     a.slots = h->slots;
     a.ctl = h->ctl;
     a.ksteps = h->ev_ksteps;
@@ -971,6 +1024,7 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist) {
             timed_src = true;
             h->free_head += (unsigned long long)inject;
             remaining -= inject;
+            a.hist_base += inject;
         }
         // photons: drain pc, electrons to ec, deferred photons to the other
         setb(pc, ec, 1 - pc);
@@ -1031,6 +1085,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     auto *h = new egsgpu_handle();
     auto fail = [&](int code) { delete h; return code; };
     h->fp64 = tp->fp64 ? 1 : 0;
+    h->tab = tab;
     h->nvox = geom->nx * geom->ny * geom->nz;
     int nmed = b.i1("nmed");
     if (nmed > kMaxMed) { set_error("too many media"); return fail(EGSGPU_E_UNSUPPORTED); }
@@ -1053,7 +1108,6 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         set_error("stream/event creation failed");
         return fail(EGSGPU_E_HIP);
     }
-    if ((rc = upload(h, med.data(), med.size(), &h->vmed))) return fail(rc);
     h->hx.assign(geom->xb, geom->xb + geom->nx + 1);
     h->hy.assign(geom->yb, geom->yb + geom->ny + 1);
     h->hz.assign(geom->zb, geom->zb + geom->nz + 1);
@@ -1082,11 +1136,11 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     try {
         if (h->fp64) {
             if ((rc = build_tables(h, b, h->pd.T))) return fail(rc);
-            if ((rc = build_geom(h, geom, h->pd))) return fail(rc);
+            if ((rc = build_geom(h, geom, med, h->pd))) return fail(rc);
             if ((rc = build_source(h, src, h->pd))) return fail(rc);
         } else {
             if ((rc = build_tables(h, b, h->pf.T))) return fail(rc);
-            if ((rc = build_geom(h, geom, h->pf))) return fail(rc);
+            if ((rc = build_geom(h, geom, med, h->pf))) return fail(rc);
             if ((rc = build_source(h, src, h->pf))) return fail(rc);
         }
     } catch (const std::exception &e) {
@@ -1147,6 +1201,87 @@ int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx) {
         EGS_HIP(hipGetLastError());
         EGS_HIP(hipStreamSynchronize(h->stream));
     }
+    return EGSGPU_OK;
+}
+
+int egsgpu_run_batched(egsgpu_handle *const *jobs, int32_t njobs, const int64_t *nhist) {
+    EGS_ARG(jobs && njobs > 0 && nhist, "jobs, njobs > 0, nhist");
+    egsgpu_handle *h = jobs[0];
+    EGS_ARG(h, "jobs[0]");
+    int64_t total = 0;
+    for (int j = 0; j < njobs; j++) {
+        egsgpu_handle *g = jobs[j];
+        EGS_ARG(g && nhist[j] >= 0, "every job handle, nhist >= 0");
+        if (g->tab != h->tab || g->fp64 != h->fp64 ||
+            memcmp(&g->P, &h->P, sizeof(DevParams)) != 0) {
+            set_error("job %d differs from job 0 in tables, precision or transport "
+                      "parameters; a batch shares them", j);
+            return EGSGPU_E_ARG;
+        }
+        for (int k = 0; k < j; k++)
+            if (jobs[k] == g) { set_error("job %d appears twice in the batch", j); return EGSGPU_E_ARG; }
+        total += nhist[j];
+    }
+    if (h->schedule != EGSGPU_SCHED_EVENT) {
+        set_error("a batch runs on the event schedule of job 0");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    int rc = ensure_lanes(h);
+    if (rc) return rc;
+    CK(timing_events(h));
+    std::vector<long long> first(njobs + 1, 0);
+    for (int j = 0; j < njobs; j++) first[j + 1] = first[j] + nhist[j];
+    auto go = [&](auto &P0, auto pick) -> int {
+        using R = typename std::remove_reference<decltype(P0.T)>::type::value_type;
+        std::vector<JobDesc<R>> jd(njobs);
+        for (int j = 0; j < njobs; j++) {
+            auto &Pj = pick(jobs[j]);
+            jd[j].G = Pj.G;
+            jd[j].vox = Pj.vox;
+            jd[j].sum = jobs[j]->tally.sum;
+            jd[j].sum2 = jobs[j]->tally.sum2;
+            jd[j].S = Pj.S;
+        }
+        JobDesc<R> *djobs = nullptr;
+        long long *dfirst = nullptr;
+        EGS_HIP(hipMalloc(&djobs, sizeof(JobDesc<R>) * njobs));
+        if (hipMalloc(&dfirst, sizeof(long long) * (njobs + 1)) != hipSuccess) {
+            (void)hipFree(djobs);
+            set_error("hipMalloc failed");
+            return EGSGPU_E_HIP;
+        }
+        int r = EGSGPU_OK;
+        if (hipMemcpy(djobs, jd.data(), sizeof(JobDesc<R>) * njobs, hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(dfirst, first.data(), sizeof(long long) * (njobs + 1),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("job table upload failed");
+            r = EGSGPU_E_HIP;
+        }
+        if (!r) {
+            BatchRun<R> b;
+            b.jobs = djobs;
+            b.job_first = dfirst;
+            b.njobs = njobs;
+            EGS_HIP(hipEventRecord(h->ev0, h->stream));
+            r = run_event(h, P0, total, &b);
+            if (!r) {
+                EGS_HIP(hipEventRecord(h->ev1, h->stream));
+                EGS_HIP(hipStreamSynchronize(h->stream));
+            }
+        }
+        (void)hipFree(djobs);
+        (void)hipFree(dfirst);
+        return r;
+    };
+    rc = h->fp64 ? go(h->pd, [](egsgpu_handle *g) -> Precision<double> & { return g->pd; })
+                 : go(h->pf, [](egsgpu_handle *g) -> Precision<float> & { return g->pf; });
+    if (rc) return rc;
+    float ms = 0;
+    EGS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    h->last_launches = h->klaunch[1] + h->klaunch[2] + h->klaunch[3];
+    for (int j = 0; j < njobs; j++) jobs[j]->nhist += nhist[j];
     return EGSGPU_OK;
 }
 

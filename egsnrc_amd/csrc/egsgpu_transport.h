@@ -37,9 +37,8 @@ __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 template <typename R, bool EV = false>
 struct Shower {
     const DevTables<R> &T;
-    const GeomView<R> &G;
-    const uint8_t *__restrict__ vmed;
-    const R *__restrict__ vrho;
+    const GeomView<R> *Gp;   // geometry of the current particle's simulation
+    const VoxData<R> *__restrict__ vox;
     const DevParams &P;
     const DevLanes<R> &L;
     int lane;
@@ -56,6 +55,7 @@ struct Shower {
     int cvox;          // run-length cache: voxel and energy
     double cval;
     const DevTally &tally;
+    double *tsum, *tsum2;    // dose sums of the current particle's simulation
     // event schedule: the history slot of the particles on this lane
     const DevSlots *slots = nullptr;
     EvCtl *ev_ctl = nullptr;
@@ -105,11 +105,11 @@ struct Shower {
     }
 
     __device__ Shower(const DevTables<R> &t, const GeomView<R> &g,
-                      const uint8_t *vm, const R *vr, const DevParams &pp,
+                      const VoxData<R> *vx, const DevParams &pp,
                       const DevLanes<R> &l, const DevTally &ta, int ln,
                       float *rb)
-        : T(t), G(g), vmed(vm), vrho(vr), P(pp), L(l), lane(ln), rbuf(rb),
-          tally(ta) {
+        : T(t), Gp(&g), vox(vx), P(pp), L(l), lane(ln), rbuf(rb),
+          tally(ta), tsum(ta.sum), tsum2(ta.sum2) {
         rpos = 0;
         ravail = 0;
         np = 0;
@@ -178,8 +178,8 @@ struct Shower {
             int s = hlist[k];
             int vox = hkey[s] - 1;
             double e = hval[s];
-            atomicAdd(tally.sum + vox, e);
-            atomicAdd(tally.sum2 + vox, e * e);
+            atomicAdd(tsum + vox, e);
+            atomicAdd(tsum2 + vox, e * e);
             hkey[s] = 0;
         }
         c_flush += hused;
@@ -226,8 +226,8 @@ struct Shower {
         }
         // table full: score this part outside the history grouping (counted)
         c_hovf++;
-        atomicAdd(tally.sum + vox, e);
-        atomicAdd(tally.sum2 + vox, e * e);
+        atomicAdd(tsum + vox, e);
+        atomicAdd(tsum2 + vox, e * e);
     }
     // event schedule: this lane is done with its particles of the slot
     __device__ void slot_release() {
@@ -246,8 +246,8 @@ struct Shower {
                                           __HIP_MEMORY_SCOPE_AGENT);
                 int key = atomicExch(S.key + base + h, 0);
                 double e = atomicExch(S.val + base + h, 0.0);
-                atomicAdd(tally.sum + key - 1, e);
-                atomicAdd(tally.sum2 + key - 1, e * e);
+                atomicAdd(tsum + key - 1, e);
+                atomicAdd(tsum2 + key - 1, e * e);
             }
             c_flush += n;
             unsigned long long at = atomicAdd(&ev_ctl->free_tail, 1ull);
@@ -265,17 +265,30 @@ struct Shower {
     }
 
     // ------------------------------------------------------------ stack
+    // element (k, f) of this lane's stack: a 32-bit byte offset from the
+    // (uniform) base, so that the access is base register + lane offset
+    __device__ __forceinline__ R &sk(int k, int f) const {
+        unsigned off = ((unsigned)(k * kStkF + f) * (unsigned)L.nlanes + (unsigned)lane) *
+                       (unsigned)sizeof(R);
+        return *(R *)((char *)L.stk + off);
+    }
+    __device__ __forceinline__ static R ibits(int v) {
+        if constexpr (sizeof(R) == 4) return __int_as_float(v);
+        else return (R)v;
+    }
+    __device__ __forceinline__ static int bitsi(R v) {
+        if constexpr (sizeof(R) == 4) return __float_as_int(v);
+        else return (int)v;
+    }
     __device__ __forceinline__ void store(int k, const Part<R> &q) {
-        size_t i = (size_t)k * L.nlanes + lane;
-        L.se[i] = q.e; L.sx[i] = q.x; L.sy[i] = q.y; L.sz[i] = q.z;
-        L.su[i] = q.u; L.sv[i] = q.v; L.sw[i] = q.w; L.swt[i] = q.wt;
-        L.sdn[i] = q.dnear; L.siq[i] = q.iq; L.sir[i] = q.ir;
+        sk(k, 0) = q.e; sk(k, 1) = q.x; sk(k, 2) = q.y; sk(k, 3) = q.z;
+        sk(k, 4) = q.u; sk(k, 5) = q.v; sk(k, 6) = q.w; sk(k, 7) = q.wt;
+        sk(k, 8) = q.dnear; sk(k, 9) = ibits(q.iq); sk(k, 10) = ibits(q.ir);
     }
     __device__ __forceinline__ void load(int k, Part<R> &q) {
-        size_t i = (size_t)k * L.nlanes + lane;
-        q.e = L.se[i]; q.x = L.sx[i]; q.y = L.sy[i]; q.z = L.sz[i];
-        q.u = L.su[i]; q.v = L.sv[i]; q.w = L.sw[i]; q.wt = L.swt[i];
-        q.dnear = L.sdn[i]; q.iq = L.siq[i]; q.ir = L.sir[i];
+        q.e = sk(k, 0); q.x = sk(k, 1); q.y = sk(k, 2); q.z = sk(k, 3);
+        q.u = sk(k, 4); q.v = sk(k, 5); q.w = sk(k, 6); q.wt = sk(k, 7);
+        q.dnear = sk(k, 8); q.iq = bitsi(sk(k, 9)); q.ir = bitsi(sk(k, 10));
     }
     // push a copy of the top ($TRANSFER PROPERTIES); the copy is the new top
     __device__ __forceinline__ bool push_copy() {
@@ -291,7 +304,7 @@ struct Shower {
     // the particle just below the top
     __device__ __forceinline__ void below(Part<R> &q) { load(np - 2, q); }
     __device__ __forceinline__ void set_below_iq(int iq) {
-        L.siq[(size_t)(np - 2) * L.nlanes + lane] = iq;
+        sk(np - 2, 9) = ibits(iq);
     }
 
     // ------------------------------------------------------------- UPHI
@@ -357,8 +370,7 @@ struct Shower {
     }
     // kPEnter -> kPNewE, or the photon is absorbed below PCUT
     __device__ void photon_enter() {
-        p_med = vmed[p.ir];
-        p_rhof = vrho[p.ir];
+        { VoxData<R> d = vox[p.ir]; p_med = d.med; p_rhof = d.rho; }
         if (p.e <= (R)P.pcut) { score(p.ir, p.e, p.wt); pop(); return; }
         pph = kPNewE;
     }
@@ -384,7 +396,7 @@ struct Shower {
             R ustep = gmfp * p_dpmfp;
             int irnew = p.ir, idisc = 0;
             if (ustep > p.dnear || p.wt <= 0) {
-                int inew = geom_howfar_in(G, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+                int inew = geom_howfar_in(*Gp, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
                 if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
             }
             if (grad) log_event(1, p.ir, -ustep / (gmfp * p_rhof));
@@ -398,7 +410,7 @@ struct Shower {
             if (irnew != p.ir) {
                 c_cross++;
                 p.ir = irnew;
-                if (irnew >= 0) { p_med = vmed[irnew]; p_rhof = vrho[irnew]; }
+                if (irnew >= 0) { VoxData<R> d = vox[irnew]; p_med = d.med; p_rhof = d.rho; }
                 else p_med = -1;
             }
             // left the grid: ausgab(3), not scored
@@ -979,8 +991,9 @@ struct Shower {
         R peie = p.e, eie = peie;
         if (eph == kNew) {   // :NEWELECTRON:
             e_irl = p.ir;
-            e_med = vmed[e_irl];
-            e_rhof = vrho[e_irl];
+            VoxData<R> d = vox[e_irl];
+            e_med = d.med;
+            e_rhof = d.rho;
             if (eie <= ecut) { ecut_discard(lelec, peie); return; }
             if (p.wt == 0) { pop(); return; }
             eph = kTstep;
@@ -1055,7 +1068,7 @@ struct Shower {
         }
         R tustep = tstep < tmxs ? tstep : tmxs;
         tustep = tustep < range ? tustep : range;
-        R tperp = geom_hownear(G, p.ir, p.x, p.y, p.z);
+        R tperp = geom_hownear(*Gp, p.ir, p.x, p.y, p.z);
         p.dnear = tperp;
         R blccl = rhof * M.blcc;
         R p2 = eke * (eke + RMT2), beta2 = p2 / (p2 + RMSQ);
@@ -1093,7 +1106,7 @@ struct Shower {
         }
         int irold = p.ir, irnew = p.ir, idisc = 0;
         if (callhowfar || p.wt <= 0) {
-            int inew = geom_howfar_in(G, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+            int inew = geom_howfar_in(*Gp, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
             if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
         }
         if (ustep <= 0) ustep = 0;
@@ -1101,7 +1114,7 @@ struct Shower {
             if (irnew != irold) {
                 c_cross++;
                 p.ir = irnew; e_irl = irnew;
-                if (irnew >= 0) { e_med = vmed[irnew]; e_rhof = vrho[irnew]; }
+                if (irnew >= 0) { VoxData<R> d = vox[irnew]; e_med = d.med; e_rhof = d.rho; }
             }
             if (eie <= ecut) { ecut_discard(lelec, peie); eph = kNew; return; }
             eph = kTstep;   // restart the TSTEP
@@ -1155,7 +1168,7 @@ struct Shower {
         if (irnew != irold) {
             c_cross++;
             p.ir = irnew; e_irl = irnew;
-            if (irnew >= 0) { e_med = vmed[irnew]; e_rhof = vrho[irnew]; }
+            if (irnew >= 0) { VoxData<R> d = vox[irnew]; e_med = d.med; e_rhof = d.rho; }
         }
         if (eie <= ecut) { ecut_discard(lelec, peie); eph = kNew; return; }
         if (idisc < 0) { pop(); eph = kNew; return; }

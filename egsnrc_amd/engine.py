@@ -137,6 +137,14 @@ class Simulation:
             pass
 
 
+def run_batched(sims, nhist):
+    """egsgpu_run_batched: nhist[j] histories of each Simulation sims[j] in
+    one event schedule (streams and slots of sims[0])."""
+    hs = (C.c_void_p * len(sims))(*[s.h.value for s in sims])
+    n = np.ascontiguousarray(nhist, dtype=np.int64)
+    check(_lib.lib().egsgpu_run_batched(hs, len(sims), ptr(n)))
+
+
 def dose_stats(sum_, sum2, n):
     """Per-history mean and its standard error (EGS_ScoringSingle::
     currentResult, egs_scoring.h:115-125)."""

@@ -172,7 +172,7 @@ int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon,
  * photon flight per turn (default 8); an interaction type runs when
  * `batch` lanes of a wave wait for it (default 16) or one lane has waited
  * `max_wait` turns (default 16); once its bank is drained, a wave with
- * fewer than `defer_below` busy lanes (default 16, -1 = never) hands its
+ * fewer than `defer_below` busy lanes (default 32, -1 = never) hands its
  * particles, mid-flight state included, to the next launch.  No
  * counterpart in the reference. */
 int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps,
@@ -187,6 +187,16 @@ int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx);
  * egs_application.cpp:839-917).  Histories are dealt round-robin to the
  * streams; tallies accumulate history-by-history (egs_scoring.h:63-125). */
 int egsgpu_run(egsgpu_handle *h, int64_t nhist);
+
+/* Run a batch of simulations together (many small jobs): nhist[j] further
+ * histories of simulation jobs[j], j < njobs, in one event schedule on the
+ * streams and history slots of jobs[0].  The jobs share tables, precision
+ * and transport parameters (checked); geometries, media maps, densities and
+ * sources differ.  Each job's tallies and history count accumulate as with
+ * egsgpu_run.  The reference runs such a set as separate processes
+ * (egs-parallel, scripts/bin/egs-parallel); no single-call counterpart. */
+int egsgpu_run_batched(egsgpu_handle *const *jobs, int32_t njobs,
+                       const int64_t *nhist);
 
 /* Per-voxel FP64 sums of energy deposited (MeV, weight included) and of
  * its per-history squares, plus the number of histories so far.  Arrays

@@ -220,3 +220,44 @@ def test_fp32_vs_fp64_precision_report(engine):
     assert rep["chi2_per_dof"] < 1.0
     assert rep["frac_beyond_2sigma"] < 0.05
     assert abs(rep["total_rel_diff"]) < 1e-3
+
+
+def water_point_case(n=10, dx=1.5, energy=2.0):
+    """A second geometry for the batch: water cube, 2 MeV point beam."""
+    xb = np.linspace(-n * dx / 2, n * dx / 2, n + 1)
+    zb = np.linspace(0.0, n * dx, n + 1)
+    med = np.zeros(n ** 3, dtype=np.int32)
+    src = make_source(0, energy, shape="point", x0=0.0, y0=0.0, z=-1.0)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01, kerma=0)
+    return xb, xb.copy(), zb, med, None, src, tp
+
+
+@pytest.mark.gpu
+def test_batched_jobs_match_oracle(engine):
+    """Three simulations of different geometry and source run as one batch:
+    each job's dose agrees with the oracle run of that job alone."""
+    from egsnrc_amd.engine import run_batched
+    cases = [electron_case(), photon_coupled_case(), water_point_case()]
+    nh = [30000, 200000, 100000]
+    tab = Tables(BLOB521)
+    sims = []
+    for c in cases:
+        xb, yb, zb, med, rhor, src, tp = c
+        tp.ecut, tp.pcut = 0.521, 0.01
+        sims.append(Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=8192,
+                               schedule="event"))
+    run_batched(sims, nh)
+    for c, sim, n, label in zip(cases, sims, nh, ["electron", "photon slabs", "point"]):
+        xb, yb, zb, med, rhor, src, tp = c
+        g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
+        o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
+        o.run(n)
+        b = sim.tallies()
+        assert b[2] == n
+        check_parity(o.tallies(), b, sims[0].counters(), 30, "batch/" + label)
+    # jobs must share transport parameters
+    xb, yb, zb, med, rhor, src, tp = water_point_case()
+    tp.ecut = 0.7
+    other = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=8192, schedule="event")
+    with pytest.raises(_lib.EngineError):
+        run_batched([sims[0], other], [10, 10])
