@@ -34,6 +34,11 @@ WORKLOADS = {
     # photon-only kerma phantom: 64^3 x 4 mm, density field in [0.2, 2.0],
     # 2 MeV 8x8 cm^2 parallel beam, ECUT 20 MeV (electrons deposited locally)
     "kerma64": dict(blob="521icru.egstab", hist_per_step=4_000_000),
+    # many small jobs: 256 phantoms of 32-160^3 voxels of 1-5 mm (random
+    # air/lung/water/bone fields), photon and 4-20 MeV electron beams,
+    # histories log-uniform in [1e5, 1e7] per job scaled by --hist-scale,
+    # all run as one batch (egsgpu_run_batched)
+    "batch256": dict(blob="521icru.egstab", hist_per_step=0),
 }
 
 
@@ -97,6 +102,82 @@ def cpu_baseline(name, seconds_target=15.0):
                       % (n2, cores, dt)}
 
 
+def bench_batch(a):
+    """The batch workload: all jobs advanced by one egsgpu_run_batched call
+    per step."""
+    from egsnrc_amd import _lib
+    from egsnrc_amd.engine import Simulation, Tables, make_source, run_batched
+    from tests import workloads
+    tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab"))
+    jobs = workloads.batch_jobs(256, hist_scale=a.hist_scale)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01, kerma=0)
+    sims = []
+    for k, j in enumerate(jobs):
+        s = j["src"]
+        src = make_source(s["charge"], s["energy"], spectrum=s["spectrum"], x0=s["x0"],
+                          x1=s["x1"], y0=s["y0"], y1=s["y1"], z=s["z"])
+        sims.append(Simulation(tab, j["xb"], j["yb"], j["zb"], j["med"], j["rhor"], src, tp,
+                               nstreams=a.lanes if k == 0 else 0, schedule="event"))
+        j["med"] = j["rhor"] = None
+    nh = np.array([j["nhist"] for j in jobs], dtype=np.int64)
+    for _ in range(a.warmup):
+        run_batched(sims, nh)
+    for s in sims:
+        _lib.check(_lib.lib().egsgpu_reset_tallies(s.h))
+    kms = np.zeros(4)
+    klaunch = np.zeros(4, dtype=np.int64)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run_batched(sims, nh)
+        for k in range(4):
+            ms, nl = sims[0].kernel_time(k)
+            kms[k] += ms
+            klaunch[k] += nl
+    T = time.perf_counter() - t0
+    total = int(nh.sum()) * a.steps
+    rel2 = []
+    for s in sims:
+        su, s2, n = s.tallies()
+        r = su / n
+        v = np.where(s2 / n - r * r > 0, (s2 / n - r * r) / (n - 1), 0.0)
+        m = r > 0.5 * r.max()
+        rel2.append(float(np.mean(v[m] / (r[m] * r[m]))))
+    rel2 = float(np.mean(rel2))
+    kdom = int(np.argmax(kms))
+    dc = sims[0].kernel_counters(kdom)
+    alg_bytes = 5.0 * dc[1] + 16.0 * dc[2] + 32.0 * dc[3]
+    achieved = alg_bytes / (kms[kdom] / 1e3) / 1e9
+    knames = ["k_transport<float>", "k_ev_source<float>", "k_ev_photon<float>",
+              "k_ev_electron<float>"]
+    call = sims[0].counters()
+    out = {
+        "metric": METRIC, "value": total / T, "unit": "histories/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * T / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "tally_dtype": "f64", "data": "synthetic phantoms and sources",
+        "config": {"workload": "batch256: 256 jobs, 32-160^3 voxels of 1-5 mm, random "
+                               "air/lung/water/bone fields, photon 0.1-6 MeV and electron "
+                               "4-20 MeV beams, histories log-uniform 1e5-1e7 x %g per step"
+                               % a.hist_scale,
+                   "jobs": len(jobs), "voxels": int(sum(j["n"] ** 3 for j in jobs)),
+                   "histories_per_step": int(nh.sum()), "parallelism": "none",
+                   "schedule": "event"},
+        "efficiency": {"value": 1.0 / (rel2 * T), "unit": "1/(sigma^2 s)", "sigma2": rel2,
+                       "definition": "mean over jobs of the mean relative variance over "
+                                     "D > 0.5 Dmax"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": knames[kdom], "kernel_launches": int(klaunch[kdom]),
+                     "kernel_ms_avg": kms[kdom] / max(1, klaunch[kdom])},
+        "kernels_ms_per_step": {knames[k]: kms[k] / a.steps for k in range(4) if klaunch[k]},
+        "counters": {"photon_steps": int(call[0]), "voxel_crossings": int(call[1]),
+                     "deposits": int(call[2]), "flushes": int(call[3]),
+                     "electron_steps": int(call[4]), "overflows": int(call[5]),
+                     "tally_overflows": int(call[6])},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +197,11 @@ def main():
     ap.add_argument("--batching", default="",
                     help="event kernels: photon steps per turn, interaction batch, max wait, "
                          "defer below")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="run the event-schedule autotuner with this many probe histories "
+                         "before the warm-up")
+    ap.add_argument("--hist-scale", type=float, default=0.01,
+                    help="batch256: fraction of each job's histories run per step")
     ap.add_argument("--precision-report", action="store_true",
                     help="only run the workload with FP32 and FP64 transport (same seeds) "
                          "and print the per-voxel difference summary")
@@ -142,6 +228,8 @@ def main():
     from egsnrc_amd import _lib
     from egsnrc_amd.engine import Simulation, Tables
     name = a.workload
+    if name == "batch256":
+        return bench_batch(a)
     xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
     W = WORKLOADS[name]
     tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", W["blob"]))
@@ -164,6 +252,7 @@ def main():
     if a.batching:
         ks, bt, mw, db = (int(x) for x in a.batching.split(","))
         _lib.check(_lib.lib().egsgpu_set_event_batching(sim.h, ks, bt, mw, db))
+    tuned = sim.autotune(a.autotune) if a.autotune else None
     for _ in range(a.warmup):
         sim.run(hps)
     _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
@@ -208,7 +297,10 @@ def main():
         "tally_dtype": "f64",
         "data": "synthetic phantom and source",
         "config": dict(cfg, histories_per_step=hps, parallelism="none",
-                       schedule=a.schedule or "default"),
+                       schedule=a.schedule or "default",
+                       **({"autotuned": dict(zip(["waves_photon", "waves_electron",
+                                                  "defer_below", "batch", "photon_steps"],
+                                                 tuned))} if tuned else {})),
         "efficiency": {"value": eff, "unit": "1/(sigma^2 s)",
                        "sigma2": rel2, "definition": "mean relative variance over D > 0.5 Dmax"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -99,6 +99,7 @@ def lib():
         "egsgpu_kernel_counters": [vp, i32, vp],
         "egsgpu_checkpoint": [vp, vp, vp],
         "egsgpu_run_batched": [vp, i32, vp],
+        "egsgpu_autotune": [vp, i64, vp],
         "egsgpu_restore": [vp, vp, C.c_size_t],
     }
     for name, args in sig.items():

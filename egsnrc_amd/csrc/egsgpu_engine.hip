@@ -1168,7 +1168,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->mxstk = tp->kerma ? 32 : 64;
     // the event schedule pays off where charged particles are transported
     h->schedule = tp->kerma ? EGSGPU_SCHED_HISTORY : EGSGPU_SCHED_EVENT;
-    h->slot_hcap = tp->kerma ? 64 : 256;
+    h->slot_hcap = tp->kerma ? 64 : 512;
     *out = h;
     return EGSGPU_OK;
 }
@@ -1323,6 +1323,72 @@ int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps, int32_t ba
     if (max_wait) h->ev_max_wait = max_wait;
     if (defer_below) h->ev_defer = defer_below < 0 ? 0 : defer_below;
     return EGSGPU_OK;
+}
+
+int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice) {
+    EGS_ARG(h && nhist_probe > 0, "handle, nhist_probe > 0");
+    if (h->schedule != EGSGPU_SCHED_EVENT) {   // nothing to choose
+        if (choice) { choice[0] = choice[1] = choice[2] = choice[3] = choice[4] = 0; }
+        return EGSGPU_OK;
+    }
+    int rc = ensure_lanes(h);
+    if (rc) return rc;
+    // the probes must not change the tallies: keep them aside
+    size_t nb = sizeof(double) * (size_t)h->nvox;
+    double *keep = nullptr;
+    EGS_HIP(hipMalloc(&keep, 2 * nb));
+    auto done = [&](int r) { (void)hipFree(keep); return r; };
+    if (hipMemcpyAsync(keep, h->tally.sum, nb, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(keep + h->nvox, h->tally.sum2, nb, hipMemcpyDeviceToDevice, h->stream) !=
+            hipSuccess) {
+        set_error("tally save failed");
+        return done(EGSGPU_E_HIP);
+    }
+    int64_t nh0 = h->nhist;
+    unsigned long long cnt[32];
+    if (hipMemcpy(cnt, h->tally.counters, sizeof(cnt), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("counter save failed");
+        return done(EGSGPU_E_HIP);
+    }
+    auto probe = [&](double &ms) -> int {
+        int r = egsgpu_run(h, nhist_probe);
+        ms = h->last_ms;
+        return r;
+    };
+    double best, ms;
+    if ((rc = probe(best))) return done(rc);   // warm-up
+    if ((rc = probe(best))) return done(rc);
+    // coordinate search: register budget, deferral, interaction batch,
+    // photon steps per turn
+    struct Knob { int *v; std::vector<int> opts; };
+    int wpe = h->ev_wp;
+    std::vector<Knob> knobs = {{&wpe, {2, 3}}, {&h->ev_defer, {16, 32, 48}},
+                               {&h->ev_batch, {8, 16, 24}}, {&h->ev_ksteps, {4, 8, 16}}};
+    for (auto &k : knobs) {
+        int cur = *k.v, pick = cur;
+        for (int o : k.opts) {
+            if (o == cur) continue;
+            *k.v = o;
+            h->ev_wp = h->ev_we = wpe;
+            if ((rc = probe(ms))) return done(rc);
+            if (ms < best) { best = ms; pick = o; }
+        }
+        *k.v = pick;
+        h->ev_wp = h->ev_we = wpe;
+    }
+    if (hipMemcpyAsync(h->tally.sum, keep, nb, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(h->tally.sum2, keep + h->nvox, nb, hipMemcpyDeviceToDevice, h->stream) !=
+            hipSuccess ||
+        hipMemcpy(h->tally.counters, cnt, sizeof(cnt), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("tally restore failed");
+        return done(EGSGPU_E_HIP);
+    }
+    h->nhist = nh0;
+    if (choice) {
+        choice[0] = h->ev_wp; choice[1] = h->ev_we; choice[2] = h->ev_defer;
+        choice[3] = h->ev_batch; choice[4] = h->ev_ksteps;
+    }
+    return done(EGSGPU_OK);
 }
 
 int egsgpu_run(egsgpu_handle *h, int64_t nhist) {

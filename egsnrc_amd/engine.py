@@ -59,6 +59,7 @@ class Simulation:
         L = _lib.lib()
         check(L.egsgpu_create(tables.h, C.byref(self.geom), C.byref(self.src),
                               C.byref(self.tp), C.byref(self.h)))
+        self._gk = None   # the device holds its own copy of the geometry
         if nstreams or block:
             check(L.egsgpu_set_streams(self.h, nstreams, block))
         if schedule is not None:
@@ -99,6 +100,13 @@ class Simulation:
         n = C.c_int32(0)
         check(_lib.lib().egsgpu_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def autotune(self, nhist_probe=200000):
+        """egsgpu_autotune; returns the chosen (waves photon, waves electron,
+        defer_below, batch, photon steps)."""
+        c = np.zeros(5, dtype=np.int32)
+        check(_lib.lib().egsgpu_autotune(self.h, int(nhist_probe), ptr(c)))
+        return tuple(int(x) for x in c)
 
     def checkpoint(self):
         """The run state (tallies, histories, RNG streams) as bytes."""

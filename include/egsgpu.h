@@ -179,6 +179,16 @@ int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps,
                               int32_t batch, int32_t max_wait,
                               int32_t defer_below);
 
+/* Autotuner of the event schedule: times short runs of nhist_probe
+ * histories while varying, one after the other, the register budget (2 or
+ * 3 waves per SIMD), the deferral threshold, the interaction batch and the
+ * photon steps per turn, and keeps the fastest setting.  The probes
+ * advance the RNG streams but leave tallies, history count and counters as
+ * they were.  choice (5 ints, may be NULL) receives waves photon, waves
+ * electron, defer_below, batch, photon steps.  A no-op for the history
+ * schedule.  No counterpart in the reference. */
+int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice);
+
 /* Seed all streams: ranmar initial seeds (ixx, jxx).  Re-seeding resets
  * the streams; tallies are kept. */
 int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx);

@@ -58,3 +58,38 @@ def compare_doses(a, b, frac=0.1):
     m = (np.maximum(ra, rb) > frac * dmax) & (va + vb > 0)
     z = (ra[m] - rb[m]) / np.sqrt(va[m] + vb[m])
     return float(np.mean(z * z)), float(np.mean(np.abs(z) > 2)), int(m.sum())
+
+
+def batch_jobs(njobs=256, seed=3, hist_scale=1.0, nmin=32, nmax=160):
+    """The many-small-jobs workload: njobs independent simulations drawn
+    from numpy default_rng(seed).  Per job: cube edge n ~ U{nmin..nmax}
+    voxels of 1-5 mm; a Gaussian random field (sigma 3 voxels) thresholded
+    at fixed quantiles into air / lung / water / bone (5 / 20 / 60 / 15 %)
+    with +-5 % uniform density noise; even jobs a photon beam with the
+    falling 0.1-6 MeV spectrum, odd jobs a 4-20 MeV electron beam; the beam
+    covers the central half of the entrance face; histories log-uniform in
+    [1e5, 1e7] times hist_scale.  Media indices are those of the 521icru
+    blob (0 water, 1 lung, 2 bone, 3 air)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(njobs):
+        n = int(rng.integers(nmin, nmax + 1))
+        dx = float(rng.uniform(0.1, 0.5))
+        f = gaussian_filter(rng.normal(size=(n, n, n)).astype(np.float32), 3.0, mode="wrap")
+        q = np.quantile(f, [0.05, 0.25, 0.85])
+        m = np.select([f < q[0], f < q[1], f < q[2]], [3, 1, 0], 2).astype(np.int32)
+        rho = 1.0 + rng.uniform(-0.05, 0.05, size=f.shape)
+        med = np.ascontiguousarray(np.transpose(m, (2, 1, 0))).reshape(-1)
+        rhor = np.ascontiguousarray(np.transpose(rho, (2, 1, 0))).reshape(-1)
+        half = n * dx / 2
+        xb = np.linspace(-half, half, n + 1)
+        zb = np.linspace(0.0, n * dx, n + 1)
+        if j % 2 == 0:
+            src = dict(charge=0, energy=0.0, spectrum=falling_spectrum())
+        else:
+            src = dict(charge=-1, energy=float(rng.uniform(4.0, 20.0)), spectrum=None)
+        src.update(x0=-half / 2, x1=half / 2, y0=-half / 2, y1=half / 2, z=-1.0)
+        nh = int(round(10 ** rng.uniform(5.0, 7.0) * hist_scale))
+        jobs.append(dict(xb=xb, yb=xb.copy(), zb=zb, med=med, rhor=rhor, src=src,
+                         nhist=max(1, nh), n=n, dx=dx))
+    return jobs
