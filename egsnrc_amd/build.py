@@ -41,9 +41,10 @@ HOST_APP = os.path.join(HERE, "egs_gpu_app")
 
 def build_app():
     """The C++ host application linked against libegsgpu.so."""
-    srcs = [os.path.join(HERE, "host", f) for f in ("egs_input.cpp", "egs_gpu_app.cpp")]
+    srcs = [os.path.join(HERE, "host", f)
+            for f in ("egs_input.cpp", "egs_phantom.cpp", "egs_gpu_app.cpp")]
     cmd = ["g++", "-O2", "-std=c++17", "-o", HOST_APP + ".tmp"] + srcs + [
-        "-I", os.path.join(HERE, "..", "include"), "-L", HERE, "-legsgpu",
+        "-I", os.path.join(HERE, "..", "include"), "-L", HERE, "-legsgpu", "-lz",
         "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

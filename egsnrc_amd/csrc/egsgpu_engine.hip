@@ -1168,7 +1168,8 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->mxstk = tp->kerma ? 32 : 64;
     // the event schedule pays off where charged particles are transported
     h->schedule = tp->kerma ? EGSGPU_SCHED_HISTORY : EGSGPU_SCHED_EVENT;
-    h->slot_hcap = tp->kerma ? 64 : 512;
+    // electron beams touch more voxels per history than photon beams
+    h->slot_hcap = tp->kerma ? 64 : (src->charge ? 512 : 256);
     *out = h;
     return EGSGPU_OK;
 }

@@ -33,6 +33,7 @@
 
 #include "../../include/egsgpu.h"
 #include "egs_input.h"
+#include "egs_phantom.h"
 
 using namespace egshost;
 
@@ -127,6 +128,11 @@ class EGS_GPUApplication {
         g->get("type", type);
         if (!InputItem::same(type, "EGS_XYZGeometry"))
             throw std::runtime_error("only EGS_XYZGeometry is supported, got " + type);
+        std::string phant;
+        if (!g->get("egsphant file", phant)) {
+            init_egsphant(g, phant);
+            return;
+        }
         std::vector<double> v;
         const char *axes[3][2] = {{"x-planes", "x-slabs"}, {"y-planes", "y-slabs"}, {"z-planes", "z-slabs"}};
         Planes *P[3] = {&xp, &yp, &zp};
@@ -178,6 +184,57 @@ class EGS_GPUApplication {
                 end = std::min(end, nreg - 1);
                 for (int j = start; j <= end; j++) rhor[j] = rho;
             }
+        }
+    }
+
+    // A file named in the input: as given, else next to the input file.
+    std::string resolve(const std::string &f) const {
+        if (std::ifstream(f).good()) return f;
+        size_t d = input_file.rfind('/');
+        if (d != std::string::npos) {
+            std::string g = input_file.substr(0, d + 1) + f;
+            if (std::ifstream(g).good()) return g;
+        }
+        return f;
+    }
+
+    // EGS_XYZGeometry from an .egsphant file (egs_nd_geometry.cpp:303-772).
+    // With a CT ramp, media and relative densities follow from the mass
+    // densities (first ramp medium with rho_min <= rho < rho_max, relative
+    // density rho/rho_default unless within epsilon of 1); without one,
+    // the phantom's own medium characters are used, as DOSXYZnrc does, and
+    // the relative density is rho over the medium's nominal density.
+    void init_egsphant(const InputItem *g, const std::string &file) {
+        EgsPhantom ph = read_egsphant(resolve(file));
+        xp.p = ph.xb; yp.p = ph.yb; zp.p = ph.zb;
+        size_t nreg = ph.density.size();
+        medium.assign(nreg, 0);
+        rhor.assign(nreg, 1.0);
+        std::string ramp_file;
+        if (!g->get("ct ramp", ramp_file)) {
+            std::vector<CtRampEntry> ramp = read_ct_ramp(resolve(ramp_file));
+            media.clear();
+            for (auto &e : ramp) media.push_back(e.name);
+            const double eps = 1.0 / (double)(1ULL << 50);
+            for (size_t j = 0; j < nreg; j++) {
+                double rho = ph.density[j];
+                size_t i = 0;
+                while (i < ramp.size() && !(rho >= ramp[i].rho_min && rho < ramp[i].rho_max)) i++;
+                if (i == ramp.size())
+                    throw std::runtime_error("voxel " + std::to_string(j) + " of density " +
+                                             std::to_string(rho) +
+                                             " has no medium in the CT ramp (vacuum voxels are "
+                                             "not supported)");
+                medium[j] = (int32_t)i;
+                if (ramp[i].rho_def > 0) {
+                    double r = rho / ramp[i].rho_def;
+                    if (std::fabs(r - 1) > eps) rhor[j] = r;
+                }
+            }
+        } else {
+            media = ph.media;
+            for (size_t j = 0; j < nreg; j++) medium[j] = ph.medium[j];
+            phantom_density = std::move(ph.density);
         }
     }
 
@@ -343,6 +400,9 @@ class EGS_GPUApplication {
         }
         std::vector<int32_t> tmed(medium.size());
         for (size_t i = 0; i < medium.size(); i++) tmed[i] = map[medium[i]];
+        if (!phantom_density.empty())   // .egsphant without a CT ramp
+            for (size_t i = 0; i < medium.size(); i++)
+                rhor[i] = phantom_density[i] / mdens[medium[i]];
         EgsXYZGeometry g{};
         g.nx = (int)xp.p.size() - 1; g.ny = (int)yp.p.size() - 1; g.nz = (int)zp.p.size() - 1;
         g.xb = xp.p.data(); g.yb = yp.p.data(); g.zb = zp.p.data();
@@ -412,6 +472,7 @@ class EGS_GPUApplication {
     }
 
     std::vector<double> mdens;   // nominal density of each geometry medium
+    std::vector<double> phantom_density;   // .egsphant mass densities (no CT ramp)
     double medium_density(int m) { return m < (int)mdens.size() ? mdens[m] : 1.0; }
 };
 
