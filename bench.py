@@ -102,6 +102,40 @@ def cpu_baseline(name, seconds_target=15.0):
                       % (n2, cores, dt)}
 
 
+def grad_bench(nhist):
+    """Density-gradient evaluations/s on the C4 problem: 64^3 x 0.4 cm
+    water with densities from a smoothed random field in [0.2, 2.0], 2 MeV
+    8x8 cm^2 parallel beam, kerma mode; target dose from a second field
+    with a 10 % perturbation.  One evaluation = forward transport + replay
+    of nhist histories (egsgpu_density_grad)."""
+    from egsnrc_amd import _lib
+    from egsnrc_amd.engine import Simulation, Tables, make_source
+    from tests import workloads
+    n, dx = 64, 0.4
+    xb, yb, zb, med, rhor = workloads.kerma_phantom(n, dx, seed=4)
+    _, _, _, _, rho2 = workloads.kerma_phantom(n, dx, seed=5)
+    trho = np.clip(rhor * (1 + 0.1 * (rho2 - rho2.mean()) / rho2.std()), 0.2, 2.0)
+    src = make_source(0, 2.0, x0=-4, x1=4, y0=-4, y1=4, z=-1.0)
+    tp = _lib.default_transport(ecut=20.0, kerma=1)
+    tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab"))
+    st = Simulation(tab, xb, yb, zb, med, trho, src, tp)
+    st.run(min(nhist, 10_000_000))
+    s_, _, nn = st.tallies()
+    st.close()
+    sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp)
+    vol = np.prod([np.diff(b)[0] for b in (xb, yb, zb)])
+    dstar = s_ / nn / (vol * trho)            # water, nominal density 1
+    sim.density_grad(1_000_000, dstar)        # warm-up
+    t0 = time.perf_counter()
+    g, d, L = sim.density_grad(nhist, dstar)
+    T = time.perf_counter() - t0
+    return {"workload": "C4: 64^3 x 0.4 cm water, density field [0.2, 2.0], 2 MeV 8x8 cm2 "
+                        "beam, kerma mode, target from a 10% perturbed field",
+            "histories": int(nhist), "seconds": T, "evals_per_s": 1.0 / T,
+            "histories_per_s": 2.0 * nhist / T, "loss": L,
+            "note": "one evaluation = forward transport + replay from the saved RNG states"}
+
+
 def bench_batch(a):
     """The batch workload: all jobs advanced by one egsgpu_run_batched call
     per step."""
@@ -175,7 +209,47 @@ def bench_batch(a):
                      "electron_steps": int(call[4]), "overflows": int(call[5]),
                      "tally_overflows": int(call[6])},
     }
+    if a.compare_plain:
+        out["plain"] = plain_jobs(a, jobs, tab, tp, T, rel2)
     print(json.dumps(out))
+
+
+def plain_jobs(a, jobs, tab, tp, t_batch, rel2_batch):
+    """The same jobs one after the other, each a separate simulation run by
+    the plain history-schedule kernel (created and destroyed outside the
+    timed region): time, mean relative variance and the efficiency ratio of
+    the batch at equal per-voxel sigma."""
+    from egsnrc_amd.engine import Simulation, make_source
+    from tests import workloads
+    jobs = workloads.batch_jobs(len(jobs), hist_scale=a.hist_scale)
+    T = 0.0
+    rel2 = []
+    for j in jobs:
+        s = j["src"]
+        src = make_source(s["charge"], s["energy"], spectrum=s["spectrum"], x0=s["x0"],
+                          x1=s["x1"], y0=s["y0"], y1=s["y1"], z=s["z"])
+        sim = Simulation(tab, j["xb"], j["yb"], j["zb"], j["med"], j["rhor"], src, tp,
+                         schedule="history")
+        j["med"] = j["rhor"] = None
+        sim.run(1000)                       # allocation and first launch
+        from egsnrc_amd import _lib
+        _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            sim.run(j["nhist"])
+        T += time.perf_counter() - t0
+        su, s2, n = sim.tallies()
+        sim.close()
+        r = su / n
+        v = np.where(s2 / n - r * r > 0, (s2 / n - r * r) / (n - 1), 0.0)
+        m = r > 0.5 * r.max()
+        rel2.append(float(np.mean(v[m] / (r[m] * r[m]))))
+    rel2 = float(np.mean(rel2))
+    eff_plain = 1.0 / (rel2 * T)
+    eff_batch = 1.0 / (rel2_batch * t_batch)
+    return {"schedule": "history, one job at a time", "seconds": T, "sigma2": rel2,
+            "efficiency": eff_plain, "batch_over_plain_efficiency": eff_batch / eff_plain,
+            "histories_per_s": sum(j["nhist"] for j in jobs) * a.steps / T}
 
 
 def main():
@@ -200,6 +274,11 @@ def main():
     ap.add_argument("--autotune", type=int, default=0,
                     help="run the event-schedule autotuner with this many probe histories "
                          "before the warm-up")
+    ap.add_argument("--compare-plain", action="store_true",
+                    help="batch256: also run the jobs one at a time with the plain kernel")
+    ap.add_argument("--grad-hist", type=int, default=100_000_000,
+                    help="histories of the density-gradient evaluation reported beside "
+                         "the main line (C4 problem); 0 = skip")
     ap.add_argument("--hist-scale", type=float, default=0.01,
                     help="batch256: fraction of each job's histories run per step")
     ap.add_argument("--precision-report", action="store_true",
@@ -316,6 +395,8 @@ def main():
                      "electron_steps": int(call[4]), "overflows": int(call[5]),
                      "tally_overflows": int(call[6])},
     }
+    if a.grad_hist and name == "slab128":
+        out["density_grad"] = grad_bench(a.grad_hist)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(name)
     print(json.dumps(out))

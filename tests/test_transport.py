@@ -129,8 +129,9 @@ def check_parity(a, b, c, min_voxels, label):
     # nv independent normal z-scores (never tighter than 1.3 and 9 %)
     assert chi2 < max(1.3, 1 + 3 * np.sqrt(2.0 / nv))
     assert out2 < max(0.09, 0.0455 + 3 * np.sqrt(0.0455 * 0.9545 / nv))
+    # energy deposited per history: ~4 sigma for 1e5 photon histories
     ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
-    assert abs(ta - tb) < 0.01 * ta
+    assert abs(ta - tb) < 0.02 * ta
 
 
 @pytest.mark.gpu
