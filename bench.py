@@ -268,6 +268,8 @@ def main():
     ap.add_argument("--waves", default="",
                     help="event kernels' resident waves per SIMD as P,E (2-4 each)")
     ap.add_argument("--slots", type=int, default=0, help="event schedule history slots")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="event schedule: launch kernels from the host instead of a HIP graph")
     ap.add_argument("--batching", default="",
                     help="event kernels: photon steps per turn, interaction batch, max wait, "
                          "defer below")
@@ -331,6 +333,8 @@ def main():
     if a.batching:
         ks, bt, mw, db = (int(x) for x in a.batching.split(","))
         _lib.check(_lib.lib().egsgpu_set_event_batching(sim.h, ks, bt, mw, db))
+    if a.no_graph:
+        _lib.check(_lib.lib().egsgpu_set_event_graph(sim.h, 0))
     tuned = sim.autotune(a.autotune) if a.autotune else None
     for _ in range(a.warmup):
         sim.run(hps)
@@ -357,7 +361,7 @@ def main():
     # algorithmic HBM bytes: 5 B per voxel crossing, 16 B per deposit,
     # 32 B per (history, voxel) flush
     alg_bytes = 5.0 * dc[1] + 16.0 * dc[2] + 32.0 * dc[3]
-    kernel_s = kms[kdom] / 1e3
+    kernel_s = kms[kdom] / 1e3 if kms[kdom] > 0 else T
     achieved = alg_bytes / kernel_s / 1e9
     knames = ["k_transport<float>", "k_ev_source<float>", "k_ev_photon<float>",
               "k_ev_electron<float>"]
@@ -376,7 +380,7 @@ def main():
         "tally_dtype": "f64",
         "data": "synthetic phantom and source",
         "config": dict(cfg, histories_per_step=hps, parallelism="none",
-                       schedule=a.schedule or "default",
+                       schedule=a.schedule or "default", graph=not a.no_graph,
                        **({"autotuned": dict(zip(["waves_photon", "waves_electron",
                                                   "defer_below", "batch", "photon_steps"],
                                                  tuned))} if tuned else {})),

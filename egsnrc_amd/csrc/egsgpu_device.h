@@ -159,6 +159,14 @@ struct EvCtl {
     unsigned long long free_tail;   // ring append position
     int n[4];                       // bank fill counts
     int head[4];                    // bank consume heads
+    // graph-driven runs plan the source launches on the device
+    unsigned long long free_head;   // first free-ring entry not yet handed out
+    long long remaining;            // histories still to start
+    long long hist_base;            // index of the next history to start
+    unsigned long long plan_head;   // this iteration's source: ring position,
+    long long plan_base;            // first history index
+    int plan_n;                     // and number of histories
+    int stall;                      // no free slot while all banks are empty
 };
 
 // One simulation of a batch (egsgpu_run_batched): its geometry, voxels,

@@ -238,13 +238,17 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
                        live ? lane : 0, rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const unsigned long long ns = (unsigned long long)a.slots.nslots;
-    for (int i = lane; live && i < a.ninject; i += a.k.L.nlanes) {
-        int slot = a.slots.ring[(a.free_head + (unsigned long long)i) % ns];
+    // a negative count: the plan made on the device by k_ev_plan
+    const int ninject = a.ninject >= 0 ? a.ninject : a.ctl->plan_n;
+    const unsigned long long free_head = a.ninject >= 0 ? a.free_head : a.ctl->plan_head;
+    const long long hist_base = a.ninject >= 0 ? a.hist_base : a.ctl->plan_base;
+    for (int i = lane; live && i < ninject; i += a.k.L.nlanes) {
+        int slot = a.slots.ring[(free_head + (unsigned long long)i) % ns];
         Part<R> q;
         const DevSource<R> *src = &a.k.S;
         const GeomView<R> *geo = &a.k.G;
         if (a.jobs) {   // the job of history hist_base + i
-            long long hidx = a.hist_base + i;
+            long long hidx = hist_base + i;
             int lo = 0, hi = a.njobs;
             while (hi - lo > 1) {
                 int mid = (lo + hi) >> 1;
@@ -470,6 +474,33 @@ __global__ void k_ev_ctl(EvCtl *c, int b) {
     }
 }
 
+// Plan the next source launch of a graph-driven run: as many new histories
+// as there are free slots, histories left and room in the bank they go to.
+__global__ void k_ev_plan(EvCtl *c, int bank, int cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long fr = c->free_tail - c->free_head;
+    long long inj = (long long)fr < c->remaining ? (long long)fr : c->remaining;
+    long long room = (long long)cap - c->n[bank];
+    inj = inj < room ? inj : room;
+    inj = inj > 0 ? inj : 0;
+    bool empty = c->n[0] == 0 && c->n[1] == 0 && c->n[2] == 0 && c->n[3] == 0;
+    if (inj == 0 && empty && c->remaining > 0) c->stall = 1;
+    c->plan_n = (int)inj;
+    c->plan_head = c->free_head;
+    c->plan_base = c->hist_base;
+    c->free_head += (unsigned long long)inj;
+    c->remaining -= inj;
+    c->hist_base += inj;
+}
+
+__global__ void k_ev_setrun(EvCtl *c, long long nhist, unsigned long long free_head) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->remaining = nhist;
+    c->hist_base = 0;
+    c->free_head = free_head;
+    c->stall = 0;
+}
+
 __global__ void k_ev_init(DevSlots S, EvCtl *c) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < S.nslots) {
@@ -518,6 +549,8 @@ struct egsgpu_handle {
     int ev_ksteps = 8, ev_batch = 16, ev_max_wait = 16, ev_defer = 32;
     int pc = 0, ec = 2;                 // banks to drain next (photon, charged)
     int *slot_job = nullptr;            // [nslots] job of each slot (batched runs)
+    bool use_graph = true;              // event schedule: replay captured iterations
+    hipEvent_t gev[2][3][2] = {};       // timing events inside the graph
     const egsgpu_tables *tab = nullptr; // the tables the simulation was made with
     int ixx = 1802, jxx = 9373;
     bool lanes_ready = false;
@@ -548,6 +581,10 @@ struct egsgpu_handle {
         for (auto &e : kev)
             for (hipEvent_t x : e)
                 if (x) (void)hipEventDestroy(x);
+        for (auto &it : gev)
+            for (auto &e : it)
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -963,8 +1000,12 @@ struct BatchRun {
 };
 
 template <typename R>
+int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist, const BatchRun<R> *batch);
+
+template <typename R>
 int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
               const BatchRun<R> *batch = nullptr) {
+    if (h->use_graph) return run_event_graph(h, P, nhist, batch);
     CK(ensure_event(h, P));
     EvArgs<R> a;
     a.k = kargs(h, P);
@@ -1049,6 +1090,130 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
         CK(take_time(h, 3));
         if (remaining == 0 && banks_empty(c)) break;
     }
+    if (c.free_tail - h->free_head != ns) {
+        set_error("event schedule ended with %llu of %llu history slots in use",
+                  (unsigned long long)(ns - (c.free_tail - h->free_head)), ns);
+        return EGSGPU_E_OVERFLOW;
+    }
+    return EGSGPU_OK;
+}
+
+// The same iteration (plan, source, photon kernel, charged-particle
+// kernel, bank bookkeeping) captured twice -- once per bank parity -- into
+// a HIP graph that is replayed until every history has been started and
+// all banks are empty.  The source launches are planned on the device
+// (k_ev_plan), so a replay needs nothing from the host; the host reads the
+// control words once per replay.
+template <typename R>
+int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
+                    const BatchRun<R> *batch) {
+    CK(ensure_event(h, P));
+    for (auto &it : h->gev)
+        for (auto &e : it)
+            for (hipEvent_t &x : e)
+                if (!x) EGS_HIP(hipEventCreate(&x));
+    EvArgs<R> a;
+    a.k = kargs(h, P);
+    a.jobs = batch ? batch->jobs : nullptr;
+    a.job_first = batch ? batch->job_first : nullptr;
+    a.njobs = batch ? batch->njobs : 0;
+    a.slot_job = h->slot_job;
+    a.hist_base = 0;
+    a.free_head = 0;
+    a.ninject = -1;   // planned on the device
+    a.slots = h->slots;
+    a.ctl = h->ctl;
+    a.ksteps = h->ev_ksteps;
+    a.batch = h->ev_batch;
+    a.max_wait = h->ev_max_wait;
+    a.defer_below = h->ev_defer;
+    const dim3 grid((h->nlanes + h->block - 1) / h->block), block(h->block);
+    auto resident = [&](int w) {
+        int n = std::min(h->nlanes, h->ncu * 4 * 64 * w);
+        return dim3((n + h->block - 1) / h->block);
+    };
+    const dim3 grid_p = resident(h->ev_wp), grid_e = resident(h->ev_we);
+    auto kp = photon_kernel<R>(h->ev_wp);
+    auto ke = electron_kernel<R>(h->ev_we);
+    const size_t lds = (size_t)kRing * h->block * sizeof(float);
+    const bool photon_src = P.S.charge == 0;
+    auto setb = [&](int ic, int ip, int id) {
+        a.ic = ic; a.ip = ip; a.id = id;
+        a.cons = P.bank[ic]; a.push = P.bank[ip]; a.defr = P.bank[id];
+    };
+    hipLaunchKernelGGL(k_ev_setrun, dim3(1), dim3(64), 0, h->stream, h->ctl, (long long)nhist,
+                       h->free_head);
+    EGS_HIP(hipGetLastError());
+    // capture two iterations
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    EGS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int pc = h->pc, ec = h->ec;
+    hipError_t cerr = hipSuccess;
+    for (int it = 0; it < 2 && cerr == hipSuccess; it++) {
+        int sb = photon_src ? pc : ec;
+        hipLaunchKernelGGL(k_ev_plan, dim3(1), dim3(64), 0, h->stream, h->ctl, sb,
+                           P.bank[sb].cap);
+        setb(sb, sb, sb);
+        (void)hipEventRecord(h->gev[it][0][0], h->stream);
+        hipLaunchKernelGGL(k_ev_source<R>, grid, block, lds, h->stream, a);
+        (void)hipEventRecord(h->gev[it][0][1], h->stream);
+        setb(pc, ec, 1 - pc);
+        (void)hipEventRecord(h->gev[it][1][0], h->stream);
+        hipLaunchKernelGGL(kp, grid_p, block, lds, h->stream, a);
+        (void)hipEventRecord(h->gev[it][1][1], h->stream);
+        hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, pc);
+        pc = 1 - pc;
+        setb(ec, pc, 5 - ec);
+        (void)hipEventRecord(h->gev[it][2][0], h->stream);
+        hipLaunchKernelGGL(ke, grid_e, block, lds, h->stream, a);
+        (void)hipEventRecord(h->gev[it][2][1], h->stream);
+        hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, ec);
+        ec = 5 - ec;
+        cerr = hipGetLastError();
+    }
+    hipError_t eerr = hipStreamEndCapture(h->stream, &graph);
+    if (cerr != hipSuccess || eerr != hipSuccess ||
+        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        // no graph on this runtime: the host-driven loop does the same work
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        h->use_graph = false;
+        return run_event(h, P, nhist, batch);
+    }
+    auto cleanup = [&]() {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    };
+    EvCtl c;
+    int rc = EGSGPU_OK;
+    const unsigned long long ns = (unsigned long long)h->slots.nslots;
+    for (int64_t rep = 0;; rep++) {
+        if (hipGraphLaunch(exec, h->stream) != hipSuccess) {
+            set_error("graph launch failed");
+            rc = EGSGPU_E_HIP;
+            break;
+        }
+        if ((rc = read_ctl(h, c))) break;
+        for (int it = 0; it < 2; it++)
+            for (int k = 0; k < 3; k++) {
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, h->gev[it][k][0], h->gev[it][k][1]) == hipSuccess) {
+                    h->kms[k + 1] += ms;
+                    h->klaunch[k + 1]++;
+                }
+            }
+        if (c.stall) {
+            set_error("event schedule: no free history slot with empty banks");
+            rc = EGSGPU_E_OVERFLOW;
+            break;
+        }
+        bool empty = c.n[0] == 0 && c.n[1] == 0 && c.n[2] == 0 && c.n[3] == 0;
+        if (c.remaining == 0 && empty) break;
+    }
+    cleanup();
+    if (rc) return rc;
+    h->free_head = c.free_head;
     if (c.free_tail - h->free_head != ns) {
         set_error("event schedule ended with %llu of %llu history slots in use",
                   (unsigned long long)(ns - (c.free_tail - h->free_head)), ns);
@@ -1310,6 +1475,12 @@ int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon, int32_t waves
     if (waves_photon) h->ev_wp = waves_photon;
     if (waves_electron) h->ev_we = waves_electron;
     if (nslots) h->nslots = nslots;
+    return EGSGPU_OK;
+}
+
+int egsgpu_set_event_graph(egsgpu_handle *h, int32_t on) {
+    EGS_ARG(h, "handle");
+    h->use_graph = on != 0;
     return EGSGPU_OK;
 }
 

@@ -179,6 +179,14 @@ int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps,
                               int32_t batch, int32_t max_wait,
                               int32_t defer_below);
 
+/* Event schedule driven by a HIP graph (on = 1, the default): one
+ * iteration (source planned on the device, photon kernel, charged-particle
+ * kernel) is captured per bank parity and the graph is replayed until the
+ * run is complete, with one read of the control words per replay; on = 0
+ * launches the kernels one by one from the host.  Same results either way.
+ * No counterpart in the reference. */
+int egsgpu_set_event_graph(egsgpu_handle *h, int32_t on);
+
 /* Autotuner of the event schedule: times short runs of nhist_probe
  * histories while varying, one after the other, the register budget (2 or
  * 3 waves per SIMD), the deferral threshold, the interaction batch and the

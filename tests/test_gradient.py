@@ -186,8 +186,9 @@ def test_density_gradient_autograd_matches_finite_differences(engine):
     so the loss gradient is that of the linear dose functional a.D, which
     finite differences resolve: with common random numbers a deposit jumps
     when an interaction point crosses a voxel face, so the differences use
-    h = 2e-2 rho (at h = 1e-3 rho those jumps leave them noise-dominated
-    unless ~50x more histories are run) and two independent seeds."""
+    h = 5e-2 rho (at h = 1e-3 rho those jumps leave them noise-dominated
+    unless ~50x more histories are run) and four independent seeds; the
+    O(h^2) bias of the central difference is far below the tolerance."""
     import torch
     from egsnrc_amd.engine import Simulation, Tables
     n, dx = 32, 0.8
@@ -204,7 +205,7 @@ def test_density_gradient_autograd_matches_finite_differences(engine):
         f = workloads.grf_density(n, 3.0, 100 + k)
         dirs.append(rhor * np.transpose((f - f.mean()) / f.std(), (2, 1, 0)).reshape(-1))
     sf, fd = [], []
-    for seed in (9373, 19373):
+    for seed in (9373, 29373, 49373, 69373):
         def build(r):
             s = Simulation(tab, xb, yb, zb, med, r, src, tp, nstreams=16384)
             s.seed(1802, seed)
@@ -229,7 +230,7 @@ def test_density_gradient_autograd_matches_finite_differences(engine):
             s.close()
             return a @ d
 
-        h = 2e-2
+        h = 5e-2
         fd.append([(functional(rhor + h * dl) - functional(rhor - h * dl)) / (2 * h)
                    for dl in dirs])
     sf, fd = np.mean(sf, 0), np.mean(fd, 0)
