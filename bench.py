@@ -34,6 +34,12 @@ WORKLOADS = {
     # photon-only kerma phantom: 64^3 x 4 mm, density field in [0.2, 2.0],
     # 2 MeV 8x8 cm^2 parallel beam, ECUT 20 MeV (electrons deposited locally)
     "kerma64": dict(blob="521icru.egstab", hist_per_step=4_000_000),
+    # C0: 1.25 MeV photon pencil beam on 30^3 x 1 cm water, ECUT 0.7,
+    # PCUT 0.01, Rayleigh off (the reference's own CPU-runnable case)
+    "c0": dict(blob="700icru.egstab", hist_per_step=4_000_000),
+    # C2: 20 MeV electrons, 10x10 cm2, on 256^3 x 0.2 cm water with a 3 cm
+    # aluminium cube at 3 cm depth; ECUT 0.521, PCUT 0.01, spin off
+    "c2": dict(blob="521icru.egstab", hist_per_step=500_000),
     # many small jobs: 256 phantoms of 32-160^3 voxels of 1-5 mm (random
     # air/lung/water/bone fields), photon and 4-20 MeV electron beams,
     # histories log-uniform in [1e5, 1e7] per job scaled by --hist-scale,
@@ -56,6 +62,22 @@ def build_workload(name):
                            "0.1-6 MeV falling linearly, ECUT 0.521 MeV, PCUT 0.01 MeV, coupled "
                            "photon/electron transport (PRESTA-II, exact BCA, spin off)",
                "voxels": 128 ** 3}
+        return xb, yb, zb, med, rhor, src, tp, cfg
+    if name == "c0":
+        xb, yb, zb, med, rhor = workloads.water_cube(30, 1.0)
+        src = make_source(0, 1.25, shape="point", x0=0.0, y0=0.0, z=-1.0)
+        tp = _lib.default_transport(ecut=0.7, pcut=0.01, rayleigh=0)
+        cfg = {"workload": "c0: 1.25 MeV photon pencil beam, 30^3 x 1 cm water, ECUT 0.7 MeV, "
+                           "PCUT 0.01 MeV, Rayleigh off, bound Compton, spin off",
+               "voxels": 30 ** 3}
+        return xb, yb, zb, med, rhor, src, tp, cfg
+    if name == "c2":
+        xb, yb, zb, med, rhor = workloads.al_cube_phantom(256, 0.2)
+        src = make_source(-1, 20.0, x0=-5, x1=5, y0=-5, y1=5, z=-1.0)
+        tp = _lib.default_transport(ecut=0.521, pcut=0.01)
+        cfg = {"workload": "c2: 20 MeV electrons 10x10 cm2 on 256^3 x 0.2 cm water with a 3 cm "
+                           "Al cube at 3 cm depth, ECUT 0.521, PCUT 0.01, ESTEPE 0.25, spin off",
+               "voxels": 256 ** 3}
         return xb, yb, zb, med, rhor, src, tp, cfg
     if name == "kerma64":
         xb, yb, zb, med, rhor = workloads.kerma_phantom(64, 0.4)
@@ -268,8 +290,11 @@ def main():
     ap.add_argument("--waves", default="",
                     help="event kernels' resident waves per SIMD as P,E (2-4 each)")
     ap.add_argument("--slots", type=int, default=0, help="event schedule history slots")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="event schedule: launch kernels from the host instead of a HIP graph")
+    ap.add_argument("--graph", action="store_true",
+                    help="time the event schedule in HIP-graph mode (no per-kernel times); "
+                         "by default the timed steps launch kernels from the host with HIP "
+                         "events around each launch, and a short graph-mode pass is "
+                         "reported beside them")
     ap.add_argument("--batching", default="",
                     help="event kernels: photon steps per turn, interaction batch, max wait, "
                          "defer below")
@@ -333,7 +358,7 @@ def main():
     if a.batching:
         ks, bt, mw, db = (int(x) for x in a.batching.split(","))
         _lib.check(_lib.lib().egsgpu_set_event_batching(sim.h, ks, bt, mw, db))
-    if a.no_graph:
+    if not a.graph:
         _lib.check(_lib.lib().egsgpu_set_event_graph(sim.h, 0))
     tuned = sim.autotune(a.autotune) if a.autotune else None
     for _ in range(a.warmup):
@@ -380,7 +405,7 @@ def main():
         "tally_dtype": "f64",
         "data": "synthetic phantom and source",
         "config": dict(cfg, histories_per_step=hps, parallelism="none",
-                       schedule=a.schedule or "default", graph=not a.no_graph,
+                       schedule=a.schedule or "default", graph=a.graph,
                        **({"autotuned": dict(zip(["waves_photon", "waves_electron",
                                                   "defer_below", "batch", "photon_steps"],
                                                  tuned))} if tuned else {})),
@@ -399,6 +424,13 @@ def main():
                      "electron_steps": int(call[4]), "overflows": int(call[5]),
                      "tally_overflows": int(call[6])},
     }
+    if not a.graph and klaunch[1:].sum() > 0:   # event schedule: graph-mode throughput
+        _lib.check(_lib.lib().egsgpu_set_event_graph(sim.h, 1))
+        t0 = time.perf_counter()
+        for _ in range(2):
+            sim.run(hps)
+        out["graph_mode"] = {"histories_per_s": 2 * hps / (time.perf_counter() - t0),
+                             "steps": 2, "note": "same steps replayed from a HIP graph"}
     if a.grad_hist and name == "slab128":
         out["density_grad"] = grad_bench(a.grad_hist)
     if not a.no_cpu:

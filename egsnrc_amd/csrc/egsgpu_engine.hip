@@ -161,7 +161,9 @@ struct EvArgs {
     int ksteps;                     // photon voxel steps per turn
     int batch, max_wait;            // interaction batching (wave_choose)
     int defer_below;                // defer when fewer lanes of a wave are busy
-    // batch of simulations (null jobs: the one simulation of k)
+    // the simulations of this run (one, or a batch); always in device
+    // memory, so that a pointer to the current job's geometry is never a
+    // pointer into the kernel arguments (which would move them to scratch)
     const JobDesc<R> *jobs;
     int *slot_job;                  // [nslots] job of the history in each slot
     const long long *job_first;     // [njobs+1] first history index of each job
@@ -172,8 +174,7 @@ struct EvArgs {
 // Point the shower at the simulation of the history in `slot`.
 template <typename R>
 __device__ __forceinline__ void ev_job(Shower<R, true> &sh, const EvArgs<R> &a, int slot) {
-    if (!a.jobs) return;
-    const JobDesc<R> &J = a.jobs[a.slot_job[slot]];
+    const JobDesc<R> &J = a.jobs[a.njobs > 1 ? a.slot_job[slot] : 0];
     sh.Gp = &J.G;
     sh.vox = J.vox;
     sh.tsum = J.sum;
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
-    Shower<R, true> sh(a.k.T, a.k.G, a.k.vox, a.k.P, a.k.L, a.k.tally,
+    Shower<R, true> sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally,
                        live ? lane : 0, rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const unsigned long long ns = (unsigned long long)a.slots.nslots;
@@ -245,19 +246,18 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
     for (int i = lane; live && i < ninject; i += a.k.L.nlanes) {
         int slot = a.slots.ring[(free_head + (unsigned long long)i) % ns];
         Part<R> q;
-        const DevSource<R> *src = &a.k.S;
-        const GeomView<R> *geo = &a.k.G;
-        if (a.jobs) {   // the job of history hist_base + i
+        int lo = 0;
+        if (a.njobs > 1) {   // the job of history hist_base + i
             long long hidx = hist_base + i;
-            int lo = 0, hi = a.njobs;
+            int hi = a.njobs;
             while (hi - lo > 1) {
                 int mid = (lo + hi) >> 1;
                 if (a.job_first[mid] <= hidx) lo = mid; else hi = mid;
             }
             a.slot_job[slot] = lo;
-            src = &a.jobs[lo].S;
-            geo = &a.jobs[lo].G;
         }
+        const DevSource<R> *src = &a.jobs[lo].S;
+        const GeomView<R> *geo = &a.jobs[lo].G;
         if (!sample_primary(*src, *geo, sh, q)) {
             unsigned long long at = atomicAdd(&a.ctl->free_tail, 1ull);
             a.slots.ring[at % ns] = slot;
@@ -314,7 +314,7 @@ k_ev_photon(EvArgs<R> a) {
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
     using S = Shower<R, true>;
-    S sh(a.k.T, a.k.G, a.k.vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
+    S sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const int n = min(a.ctl->n[a.ic], a.cons.cap);
@@ -399,7 +399,7 @@ k_ev_electron(EvArgs<R> a) {
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
     using S = Shower<R, true>;
-    S sh(a.k.T, a.k.G, a.k.vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
+    S sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const int n = min(a.ctl->n[a.ic], a.cons.cap);
@@ -550,7 +550,7 @@ struct egsgpu_handle {
     int pc = 0, ec = 2;                 // banks to drain next (photon, charged)
     int *slot_job = nullptr;            // [nslots] job of each slot (batched runs)
     bool use_graph = true;              // event schedule: replay captured iterations
-    hipEvent_t gev[2][3][2] = {};       // timing events inside the graph
+    void *job_self = nullptr;           // device JobDesc of this simulation alone
     const egsgpu_tables *tab = nullptr; // the tables the simulation was made with
     int ixx = 1802, jxx = 9373;
     bool lanes_ready = false;
@@ -581,10 +581,6 @@ struct egsgpu_handle {
         for (auto &e : kev)
             for (hipEvent_t x : e)
                 if (x) (void)hipEventDestroy(x);
-        for (auto &it : gev)
-            for (auto &e : it)
-                for (hipEvent_t x : e)
-                    if (x) (void)hipEventDestroy(x);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -945,6 +941,17 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     CK(alloc(h, 1, &h->ctl));
     EGS_HIP(hipHostMalloc((void **)&h->ctl_host, sizeof(EvCtl), hipHostMallocDefault));
     CK(alloc(h, (size_t)S.nslots, &h->slot_job));
+    {
+        JobDesc<R> jd;
+        jd.G = P.G;
+        jd.vox = P.vox;
+        jd.sum = h->tally.sum;
+        jd.sum2 = h->tally.sum2;
+        jd.S = P.S;
+        const JobDesc<R> *dj = nullptr;
+        CK(upload(h, &jd, 1, &dj));
+        h->job_self = (void *)dj;
+    }
     for (int k = 0; k < 2; k++) CK(alloc_bank(h, 2 * S.nslots, P.bank[k]));
     for (int k = 2; k < 4; k++) CK(alloc_bank(h, 8 * S.nslots, P.bank[k]));
     h->pc = 0;
@@ -1009,9 +1016,9 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
     CK(ensure_event(h, P));
     EvArgs<R> a;
     a.k = kargs(h, P);
-    a.jobs = batch ? batch->jobs : nullptr;
+    a.jobs = batch ? batch->jobs : (const JobDesc<R> *)h->job_self;
     a.job_first = batch ? batch->job_first : nullptr;
-    a.njobs = batch ? batch->njobs : 0;
+    a.njobs = batch ? batch->njobs : 1;
     a.slot_job = h->slot_job;
     a.hist_base = 0;
     a.slots = h->slots;
@@ -1101,22 +1108,19 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
 // The same iteration (plan, source, photon kernel, charged-particle
 // kernel, bank bookkeeping) captured twice -- once per bank parity -- into
 // a HIP graph that is replayed until every history has been started and
-// all banks are empty.  The source launches are planned on the device
+// all banks are empty.  Per-kernel times are not split in this mode (the
+// whole run is timed); the host-driven loop gives them.  The source launches are planned on the device
 // (k_ev_plan), so a replay needs nothing from the host; the host reads the
 // control words once per replay.
 template <typename R>
 int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
                     const BatchRun<R> *batch) {
     CK(ensure_event(h, P));
-    for (auto &it : h->gev)
-        for (auto &e : it)
-            for (hipEvent_t &x : e)
-                if (!x) EGS_HIP(hipEventCreate(&x));
     EvArgs<R> a;
     a.k = kargs(h, P);
-    a.jobs = batch ? batch->jobs : nullptr;
+    a.jobs = batch ? batch->jobs : (const JobDesc<R> *)h->job_self;
     a.job_first = batch ? batch->job_first : nullptr;
-    a.njobs = batch ? batch->njobs : 0;
+    a.njobs = batch ? batch->njobs : 1;
     a.slot_job = h->slot_job;
     a.hist_base = 0;
     a.free_head = 0;
@@ -1155,19 +1159,13 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
         hipLaunchKernelGGL(k_ev_plan, dim3(1), dim3(64), 0, h->stream, h->ctl, sb,
                            P.bank[sb].cap);
         setb(sb, sb, sb);
-        (void)hipEventRecord(h->gev[it][0][0], h->stream);
         hipLaunchKernelGGL(k_ev_source<R>, grid, block, lds, h->stream, a);
-        (void)hipEventRecord(h->gev[it][0][1], h->stream);
         setb(pc, ec, 1 - pc);
-        (void)hipEventRecord(h->gev[it][1][0], h->stream);
         hipLaunchKernelGGL(kp, grid_p, block, lds, h->stream, a);
-        (void)hipEventRecord(h->gev[it][1][1], h->stream);
         hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, pc);
         pc = 1 - pc;
         setb(ec, pc, 5 - ec);
-        (void)hipEventRecord(h->gev[it][2][0], h->stream);
         hipLaunchKernelGGL(ke, grid_e, block, lds, h->stream, a);
-        (void)hipEventRecord(h->gev[it][2][1], h->stream);
         hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, ec);
         ec = 5 - ec;
         cerr = hipGetLastError();
@@ -1195,14 +1193,6 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
             break;
         }
         if ((rc = read_ctl(h, c))) break;
-        for (int it = 0; it < 2; it++)
-            for (int k = 0; k < 3; k++) {
-                float ms = 0;
-                if (hipEventElapsedTime(&ms, h->gev[it][k][0], h->gev[it][k][1]) == hipSuccess) {
-                    h->kms[k + 1] += ms;
-                    h->klaunch[k + 1]++;
-                }
-            }
         if (c.stall) {
             set_error("event schedule: no free history slot with empty banks");
             rc = EGSGPU_E_OVERFLOW;

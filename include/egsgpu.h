@@ -183,7 +183,9 @@ int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps,
  * iteration (source planned on the device, photon kernel, charged-particle
  * kernel) is captured per bank parity and the graph is replayed until the
  * run is complete, with one read of the control words per replay; on = 0
- * launches the kernels one by one from the host.  Same results either way.
+ * launches the kernels one by one from the host, with HIP events around
+ * each launch for egsgpu_kernel_time (in graph mode only the whole run is
+ * timed, egsgpu_last_kernel_ms).  Same results either way.
  * No counterpart in the reference. */
 int egsgpu_set_event_graph(egsgpu_handle *h, int32_t on);
 

@@ -205,10 +205,10 @@ def test_density_gradient_autograd_matches_finite_differences(engine):
         f = workloads.grf_density(n, 3.0, 100 + k)
         dirs.append(rhor * np.transpose((f - f.mean()) / f.std(), (2, 1, 0)).reshape(-1))
     sf, fd = [], []
-    for seed in (9373, 29373, 49373, 69373):
+    for seed in (1802, 2802, 3802, 4802):   # ixx of the ranmar seed pairs
         def build(r):
             s = Simulation(tab, xb, yb, zb, med, r, src, tp, nstreams=16384)
-            s.seed(1802, seed)
+            s.seed(seed, 9373)
             return s
 
         s0 = build(rhor)

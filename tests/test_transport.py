@@ -287,3 +287,36 @@ def test_autotune_leaves_tallies_alone(engine):
     wp, we, defer, batch, ks = choice
     assert wp in (2, 3) and we == wp and defer in (16, 32, 48) and batch in (8, 16, 24)
     assert ks in (4, 8, 16)
+
+
+BLOB700 = os.path.join(os.path.dirname(BLOB521), "700icru.egstab")
+
+
+@pytest.mark.gpu
+def test_c0_pencil_beam_parity(engine):
+    """C0 exactly: 1.25 MeV photon pencil beam on the centre of 30^3 x 1 cm
+    water, ECUT 0.7 MeV (700icru data), PCUT 0.01 MeV, Rayleigh off."""
+    xb, yb, zb, med, rhor = workloads.water_cube(30, 1.0)
+    src = make_source(0, 1.25, shape="point", x0=0.0, y0=0.0, z=-1.0)
+    tp = _lib.default_transport(ecut=0.7, pcut=0.01, rayleigh=0)
+    g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
+    o = oracle_lib.OracleSimulation(BLOB700, g, src[0], tp, nstreams=256, nthreads=8)
+    o.run(500000)
+    sim = Simulation(Tables(BLOB700), xb, yb, zb, med, rhor, src, tp)
+    sim.run(500000)
+    check_parity(o.tallies(), sim.tallies(), sim.counters(), 30, "c0")
+
+
+@pytest.mark.gpu
+def test_c2_electron_beam_reduced_parity(engine):
+    """C2 at reduced size: 20 MeV electrons, 10x10 cm^2, on 64^3 x 0.4 cm
+    water with a 3.2 cm aluminium cube at 3.2 cm depth."""
+    xb, yb, zb, med, rhor = workloads.al_cube_phantom(64, 0.4, side=3.2, depth=3.2)
+    src = make_source(-1, 20.0, x0=-5, x1=5, y0=-5, y1=5, z=-1.0)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01)
+    g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
+    o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
+    o.run(20000)
+    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=16384)
+    sim.run(20000)
+    check_parity(o.tallies(), sim.tallies(), sim.counters(), 100, "c2 reduced")

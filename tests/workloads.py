@@ -35,6 +35,31 @@ def slab_phantom(n=128, dx=0.25):
     return xb, xb.copy(), zb, med, None
 
 
+def water_cube(n=30, dx=1.0):
+    """C0 phantom: n^3 voxels of water, x/y centred, z from 0 (medium 0)."""
+    xb = np.linspace(-n * dx / 2, n * dx / 2, n + 1)
+    zb = np.linspace(0.0, n * dx, n + 1)
+    return xb, xb.copy(), zb, np.zeros(n ** 3, dtype=np.int32), None
+
+
+def al_cube_phantom(n=256, dx=0.2, side=3.0, depth=3.0, al=4):
+    """C2 phantom: n^3 voxels of water with an aluminium cube of
+    round(side/dx) voxels per edge (15 at 2 mm), laterally the voxels
+    n/2 - k/2 - 1 ... (centred to within half a voxel), from z = depth
+    (media of the 521icru blob: 0 water, 4 aluminium)."""
+    xb = np.linspace(-n * dx / 2, n * dx / 2, n + 1)
+    zb = np.linspace(0.0, n * dx, n + 1)
+    k = int(round(side / dx))
+    lo = n // 2 - (k + 1) // 2
+    inx = np.zeros(n, bool)
+    inx[lo:lo + k] = True
+    iz0 = int(round(depth / dx))
+    inz = np.zeros(n, bool)
+    inz[iz0:iz0 + k] = True
+    m = (inz[:, None, None] & inx[None, :, None] & inx[None, None, :]).astype(np.int32) * al
+    return xb, xb.copy(), zb, m.reshape(-1), None
+
+
 def falling_spectrum(emin=0.1, emax=6.0, nbins=60):
     """60-bin histogram with fluence falling linearly from emin to emax."""
     e = np.linspace(emin, emax, nbins + 1)
