@@ -124,7 +124,7 @@ def cpu_baseline(name, seconds_target=15.0):
                       % (n2, cores, dt)}
 
 
-def grad_bench(nhist):
+def grad_bench(nhist, lanes=0):
     """Density-gradient evaluations/s on the C4 problem: 64^3 x 0.4 cm
     water with densities from a smoothed random field in [0.2, 2.0], 2 MeV
     8x8 cm^2 parallel beam, kerma mode; target dose from a second field
@@ -140,11 +140,11 @@ def grad_bench(nhist):
     src = make_source(0, 2.0, x0=-4, x1=4, y0=-4, y1=4, z=-1.0)
     tp = _lib.default_transport(ecut=20.0, kerma=1)
     tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab"))
-    st = Simulation(tab, xb, yb, zb, med, trho, src, tp)
+    st = Simulation(tab, xb, yb, zb, med, trho, src, tp, nstreams=lanes)
     st.run(min(nhist, 10_000_000))
     s_, _, nn = st.tallies()
     st.close()
-    sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp)
+    sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=lanes)
     vol = np.prod([np.diff(b)[0] for b in (xb, yb, zb)])
     dstar = s_ / nn / (vol * trho)            # water, nominal density 1
     sim.density_grad(1_000_000, dstar)        # warm-up
@@ -199,13 +199,20 @@ def bench_batch(a):
         m = r > 0.5 * r.max()
         rel2.append(float(np.mean(v[m] / (r[m] * r[m]))))
     rel2 = float(np.mean(rel2))
-    kdom = int(np.argmax(kms))
-    dc = sims[0].kernel_counters(kdom)
-    alg_bytes = 5.0 * dc[1] + 16.0 * dc[2] + 32.0 * dc[3]
-    achieved = alg_bytes / (kms[kdom] / 1e3) / 1e9
     knames = ["k_transport<float>", "k_ev_source<float>", "k_ev_photon<float>",
               "k_ev_electron<float>"]
     call = sims[0].counters()
+    if klaunch.sum() > 0:
+        kdom = int(np.argmax(kms))
+        dc = sims[0].kernel_counters(kdom)
+        kname, kt, nl = knames[kdom], kms[kdom] / 1e3, int(klaunch[kdom])
+    else:
+        # the batch ran from a HIP graph, which has no per-kernel times:
+        # all kernels' events over the whole timed region
+        dc = call
+        kname, kt, nl = "all kernels (HIP graph)", T, 0
+    alg_bytes = 5.0 * dc[1] + 16.0 * dc[2] + 32.0 * dc[3]
+    achieved = alg_bytes / kt / 1e9
     out = {
         "metric": METRIC, "value": total / T, "unit": "histories/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * T / a.steps,
@@ -223,8 +230,8 @@ def bench_batch(a):
                                      "D > 0.5 Dmax"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": knames[kdom], "kernel_launches": int(klaunch[kdom]),
-                     "kernel_ms_avg": kms[kdom] / max(1, klaunch[kdom])},
+                     "kernel": kname, "kernel_launches": nl,
+                     "kernel_ms_avg": 1e3 * kt / max(1, nl)},
         "kernels_ms_per_step": {knames[k]: kms[k] / a.steps for k in range(4) if klaunch[k]},
         "counters": {"photon_steps": int(call[0]), "voxel_crossings": int(call[1]),
                      "deposits": int(call[2]), "flushes": int(call[3]),
@@ -274,6 +281,118 @@ def plain_jobs(a, jobs, tab, tp, t_batch, rel2_batch):
             "histories_per_s": sum(j["nhist"] for j in jobs) * a.steps / T}
 
 
+def _kernel_key(name):
+    """Kernel name without return type, namespace and argument list."""
+    return name.split("(")[0].replace("void ", "").replace("egsgpu::", "").strip()
+
+
+def _pmc_per_dispatch(d, counter):
+    """Per-kernel (dispatches, mean counter value) from one rocprofv3 --pmc
+    output directory written with `-o run --output-format csv`."""
+    import csv
+    tot, disp = {}, {}
+    with open(os.path.join(d, "run_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            k = _kernel_key(r["Kernel_Name"])
+            tot[k] = tot.get(k, 0.0) + float(r["Counter_Value"])
+            disp.setdefault(k, set()).add(r["Dispatch_Id"])
+    return {k: (len(disp[k]), tot[k] / len(disp[k])) for k in tot}
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, command):
+    """Markdown summary of HBM traffic per dispatch.
+
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch, one rocprofv3 pass each)
+    are scaled by what the same counters read for the device-to-device
+    copy of `--copy-calibration` (3 copies of 1 GiB: 3 GiB read, 3 GiB
+    written).  The line `traffic_bytes_per_launch <kernel>: <bytes>` is
+    what the bench reads back as roofline.traffic."""
+    cf = _pmc_per_dispatch(copy_fetch_dir, "FETCH_SIZE")
+    cw = _pmc_per_dispatch(copy_write_dir, "WRITE_SIZE")
+    ck = [k for k in cf if "copyBuffer" in k][0]
+    copy_bytes = float(1 << 30)
+    f_fac = copy_bytes / (cf[ck][1] * 1024.0)
+    w_fac = copy_bytes / (cw[ck][1] * 1024.0)
+    fe = _pmc_per_dispatch(fetch_dir, "FETCH_SIZE")
+    wr = _pmc_per_dispatch(write_dir, "WRITE_SIZE")
+    out = ["Commands (from the repository root, `TMPDIR=/tmp`):", ""]
+    for c, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        out.append("    rocprofv3 --pmc %s -d %s -o run --output-format csv -- %s"
+                   % (c, d, command))
+    for c, d in (("FETCH_SIZE", copy_fetch_dir), ("WRITE_SIZE", copy_write_dir)):
+        out.append("    rocprofv3 --pmc %s -d %s -o run --output-format csv -- "
+                   "python3 bench.py --copy-calibration" % (c, d))
+    out += ["", "Copy calibration (`%s`, 1 GiB read + 1 GiB written per dispatch): "
+            "FETCH_SIZE reads %.1f KiB, WRITE_SIZE %.1f KiB per dispatch; scale factors "
+            "FETCH x%.3f, WRITE x%.3f." % (ck, cf[ck][1], cw[ck][1], f_fac, w_fac), "",
+            "| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch "
+            "| HBM read B/dispatch | HBM written B/dispatch | HBM total B/dispatch |",
+            "|---|---|---|---|---|---|---|"]
+    lines = []
+    for k in sorted(fe, key=lambda k: -fe[k][1]):
+        if k not in wr:
+            continue
+        rb = fe[k][1] * 1024.0 * f_fac
+        wb = wr[k][1] * 1024.0 * w_fac
+        out.append("| `%s` | %d | %.4g | %.4g | %.4g | %.4g | %.4g |"
+                   % (k, fe[k][0], fe[k][1], wr[k][1], rb, wb, rb + wb))
+        if k.startswith("k_"):
+            lines.append("traffic_bytes_per_launch %s: %.6g" % (k, rb + wb))
+    return "\n".join(out + [""] + lines) + "\n"
+
+
+def trace_summary(trace_dir, bench_log, command):
+    """Markdown summary of a `rocprofv3 --kernel-trace --stats` run
+    (`-o run --output-format csv`) next to the bench line printed by the
+    same run, comparing the trace's average duration of the dominant kernel
+    with the HIP-event average the bench measured."""
+    import csv
+    rows = list(csv.DictReader(open(os.path.join(trace_dir, "run_kernel_stats.csv"))))
+    line = json.loads([l for l in open(bench_log) if l.startswith("{")][-1])
+    rf = line["roofline"]
+    out = ["Command (from the repository root, `TMPDIR=/tmp`):", "",
+           "    rocprofv3 --kernel-trace --stats -d %s -o run --output-format csv -- %s"
+           % (trace_dir, command), "",
+           "| kernel | calls | total (us) | average (us) | min (us) | max (us) | % |",
+           "|---|---|---|---|---|---|---|"]
+    dom = None
+    for r in rows:
+        k = _kernel_key(r["Name"])
+        out.append("| `%s` | %s | %.1f | %.1f | %.1f | %.1f | %.3f |"
+                   % (k, r["Calls"], float(r["TotalDurationNs"]) / 1e3,
+                      float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
+                      float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+        if k == _kernel_key(rf["kernel"]):
+            dom = float(r["AverageNs"]) / 1e6
+    out += ["", "Bench line of the same run: %.0f histories/s, %.1f ms per step; "
+            "dominant kernel `%s`, %d launches, HIP-event average %.3f ms%s; "
+            "roofline achieved %.1f GB/s (%.2f %% of %.0f GB/s)."
+            % (line["value"], line["ms_per_step"], rf["kernel"], rf["kernel_launches"],
+               rf["kernel_ms_avg"],
+               "" if dom is None else ", trace average %.3f ms (ratio %.3f)"
+               % (dom, rf["kernel_ms_avg"] / dom),
+               rf["achieved"], 100 * rf["frac"], rf["peak"])]
+    return "\n".join(out) + "\n"
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` (matched by name, template
+    arguments aside) recorded by pmc_summary for this workload under
+    profiles/, with the file it came from; (None, None) if there is none."""
+    import re
+    p = os.path.join(ROOT, "profiles", "hbm_traffic_%s.md" % workload)
+    if not os.path.exists(p):
+        return None, None
+    base = _kernel_key(kernel).split("<")[0]
+    for l in open(p):
+        m = re.match(r"traffic_bytes_per_launch (.+): (\S+)\s*$", l)
+        if m and m.group(1).split("<")[0] == base:
+            return float(m.group(2)), os.path.relpath(p, ROOT)
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,7 +404,7 @@ def main():
                     help="transport lanes (ranmar streams); 0 = library default")
     ap.add_argument("--block", type=int, default=0, help="workgroup size; 0 = default")
     ap.add_argument("--hist", type=int, default=0, help="histories per step; 0 = workload default")
-    ap.add_argument("--schedule", choices=["history", "event"], default=None,
+    ap.add_argument("--schedule", choices=["history", "event", "history-plain"], default=None,
                     help="transport schedule; default = the library's choice")
     ap.add_argument("--waves", default="",
                     help="event kernels' resident waves per SIMD as P,E (2-4 each)")
@@ -314,7 +433,20 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (the reference "
                          "kernel for scaling HBM traffic counters)")
+    ap.add_argument("--pmc-summary", nargs=4, default=None,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="only print the HBM traffic summary of four rocprofv3 --pmc runs")
+    ap.add_argument("--trace-summary", nargs=2, default=None, metavar=("TRACE_DIR", "BENCH_LOG"),
+                    help="only print the summary of a rocprofv3 --kernel-trace --stats run")
+    ap.add_argument("--profiled-command", default="python3 bench.py",
+                    help="command line quoted in --pmc-summary / --trace-summary output")
     a = ap.parse_args()
+    if a.pmc_summary:
+        sys.stdout.write(pmc_summary(*a.pmc_summary, a.profiled_command))
+        return
+    if a.trace_summary:
+        sys.stdout.write(trace_summary(*a.trace_summary, a.profiled_command))
+        return
     if a.copy_calibration:
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -390,6 +522,8 @@ def main():
     achieved = alg_bytes / kernel_s / 1e9
     knames = ["k_transport<float>", "k_ev_source<float>", "k_ev_photon<float>",
               "k_ev_electron<float>"]
+    nl = max(1, int(klaunch[kdom]))
+    traffic, traffic_src = read_traffic(knames[kdom], name)
     out = {
         "metric": METRIC,
         "value": hist_s,
@@ -412,7 +546,9 @@ def main():
         "efficiency": {"value": eff, "unit": "1/(sigma^2 s)",
                        "sigma2": rel2, "definition": "mean relative variance over D > 0.5 Dmax"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": alg_bytes / nl,
                      "kernel": knames[kdom],
                      "kernel_launches": int(klaunch[kdom]),
                      "kernel_ms_avg": kms[kdom] / max(1, klaunch[kdom]),
@@ -431,8 +567,8 @@ def main():
             sim.run(hps)
         out["graph_mode"] = {"histories_per_s": 2 * hps / (time.perf_counter() - t0),
                              "steps": 2, "note": "same steps replayed from a HIP graph"}
-    if a.grad_hist and name == "slab128":
-        out["density_grad"] = grad_bench(a.grad_hist)
+    if a.grad_hist and name in ("slab128", "kerma64"):
+        out["density_grad"] = grad_bench(a.grad_hist, a.lanes)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(name)
     print(json.dumps(out))

@@ -56,6 +56,7 @@ def default_transport(**kw):
 
 SCHED_HISTORY = 0
 SCHED_EVENT = 1
+SCHED_HISTORY_PLAIN = 2
 
 
 class EngineError(RuntimeError):

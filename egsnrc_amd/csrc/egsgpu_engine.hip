@@ -146,6 +146,110 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     add_counters(a.tally, sh, 0);
 }
 
+// Wave-level choice of the next action.  Lanes want either the common
+// action `common` (photon flight / condensed-history step) or one of the
+// rare, expensive ones (interactions, codes lo..hi).  A rare action runs
+// when it has at least `batch` lanes waiting, when nobody wants the common
+// one, or when a lane has waited `max_wait` turns; otherwise the common
+// action runs.  The result is the same for every lane of the wave.
+__device__ __forceinline__ int wave_choose(int act, int wait, int common, int lo, int hi,
+                                           int batch, int max_wait) {
+    unsigned long long starving = __ballot(act != common && act >= 0 && wait >= max_wait);
+    if (starving) {
+        int src = __ffsll(starving) - 1;
+        return __builtin_amdgcn_readfirstlane(__shfl(act, src));
+    }
+    int ncommon = __popcll(__ballot(act == common));
+    int best = common, bestn = 0;
+    for (int t = lo; t <= hi; t++) {
+        int c = __popcll(__ballot(act == t));
+        if (c > bestn) { bestn = c; best = t; }
+    }
+    if (bestn > 0 && (ncommon == 0 || bestn >= batch)) return best;
+    return common;
+}
+
+#ifndef EGSGPU_KERMA_WAVES
+#define EGSGPU_KERMA_WAVES 2
+#endif
+// resident waves per SIMD the kerma-mode kernel is compiled for (its
+// register budget); the default stream count fills them
+constexpr int kKermaWaves = EGSGPU_KERMA_WAVES;
+
+// History schedule in kerma mode (ECUT above every electron energy, so a
+// charged particle always takes the ECUT discard of ELECTR and only
+// photons are transported).  Each lane runs exactly the histories and the
+// random-number sequence of k_transport; what changes is how the lanes of
+// a wave share the instruction stream: every turn the wave votes for one
+// action -- ksteps voxel steps of the photon flights (the common action),
+// one interaction type, the start of new histories or the end-of-history
+// flush -- and only the lanes that want it run it, so that flights of
+// different lengths and the rare, long samplers do not serialise the
+// wave.  Cheap phase changes (photon entry, new energy, the charged
+// discard) run at once.
+template <typename R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKermaWaves)))
+k_kerma(KArgs<R> a, int ksteps, int batch, int max_wait) {
+    extern __shared__ float rbuf_all[];
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = lane < a.L.nlanes;
+    using S = Shower<R>;
+    S sh(a.T, a.G, a.vox, a.P, a.L, a.tally, live ? lane : 0, rbuf_all + threadIdx.x);
+    sh.grad = a.grad;
+    const int64_t nh = live ? a.base + (lane < a.rem ? 1 : 0) : 0;
+    enum { kNewHist = S::kPPair + 1, kEndHist = S::kPPair + 2 };
+    int64_t k = 0;          // histories of this lane started
+    bool inhist = false;    // a history's particles are (or were) on the stack
+    int wait = 0;
+    for (;;) {
+        sh.rtopup();
+        int act;
+        for (;;) {   // cheap steps until the lane wants a voted action
+            if (sh.np == 0) {
+                act = inhist ? (int)kEndHist : (k < nh ? (int)kNewHist : -1);
+                break;
+            }
+            if (sh.p.iq != 0) {
+                sh.c_esteps++;
+                sh.ecut_discard(sh.p.iq, sh.p.e);
+                sh.pph = S::kPEnter;
+                continue;
+            }
+            if (sh.pph == S::kPEnter) { sh.photon_enter(); continue; }
+            if (sh.pph == S::kPNewE) { sh.photon_newe(); continue; }
+            act = sh.pph;
+            break;
+        }
+        if (__all(act < 0)) break;
+        int chosen = wave_choose(act, wait, S::kPTrans, S::kPRayl, kEndHist, batch, max_wait);
+        if (act != chosen) {
+            if (act >= 0) wait++;
+            continue;
+        }
+        wait = 0;
+        if (chosen == S::kPTrans) {
+            sh.photon_trans(ksteps);
+        } else if (chosen == kNewHist) {
+            k++;
+            Part<R> q;
+            if (sample_primary(a.S, a.G, sh, q)) {
+                sh.p = q;
+                sh.np = 1;
+                sh.eph = S::kNew;
+                sh.pph = S::kPEnter;
+                inhist = true;
+            }
+        } else if (chosen == kEndHist) {
+            if (a.grad) sh.grad_sweep();
+            sh.end_history();
+            inhist = false;
+        } else {
+            sh.photon_interact();
+        }
+    }
+    add_counters(a.tally, sh, 0);
+}
+
 // ---- event schedule ------------------------------------------------------
 template <typename R>
 struct EvArgs {
@@ -276,29 +380,6 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
         }
     }
     add_counters(a.k.tally, sh, 1);
-}
-
-// Wave-level choice of the next action.  Lanes want either the common
-// action `common` (photon flight / condensed-history step) or one of the
-// rare, expensive ones (interactions, codes lo..hi).  A rare action runs
-// when it has at least `batch` lanes waiting, when nobody wants the common
-// one, or when a lane has waited `max_wait` turns; otherwise the common
-// action runs.  The result is the same for every lane of the wave.
-__device__ __forceinline__ int wave_choose(int act, int wait, int common, int lo, int hi,
-                                           int batch, int max_wait) {
-    unsigned long long starving = __ballot(act != common && wait >= max_wait);
-    if (starving) {
-        int src = __ffsll(starving) - 1;
-        return __builtin_amdgcn_readfirstlane(__shfl(act, src));
-    }
-    int ncommon = __popcll(__ballot(act == common));
-    int best = common, bestn = 0;
-    for (int t = lo; t <= hi; t++) {
-        int c = __popcll(__ballot(act == t));
-        if (c > bestn) { bestn = c; best = t; }
-    }
-    if (bestn > 0 && (ncommon == 0 || bestn >= batch)) return best;
-    return common;
 }
 
 // Photon kernel: every lane takes photons from bank `cons` until it is
@@ -841,7 +922,7 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
     }
     CK(alloc(h, (size_t)h->mxstk * kStkF * h->nlanes, &L.stk));
     // per-lane history tables: history schedule only
-    L.hcap = h->schedule == EGSGPU_SCHED_HISTORY ? h->hcap : 0;
+    L.hcap = h->schedule != EGSGPU_SCHED_EVENT ? h->hcap : 0;
     size_t nh = (size_t)L.hcap * h->nlanes;
     CK(alloc(h, nh, &L.hkey));
     CK(alloc(h, nh, &L.hval));
@@ -854,7 +935,10 @@ int ensure_lanes(egsgpu_handle *h) {
     if (h->lanes_ready) return EGSGPU_OK;
     // default streams: what the kernels can keep resident (event kernels
     // up to 4 waves per SIMD, the history kernel 2)
-    if (!h->nlanes) h->nlanes = h->ncu * (h->schedule == EGSGPU_SCHED_EVENT ? 1024 : 512);
+    if (!h->nlanes)
+        h->nlanes = h->ncu * (h->schedule == EGSGPU_SCHED_EVENT ? 1024
+                              : h->P.kerma && h->schedule == EGSGPU_SCHED_HISTORY
+                                    ? 256 * kKermaWaves : 512);
     CK(alloc(h, (size_t)h->nlanes, &h->rng));
     if (h->fp64) CK(build_lanes(h, h->pd));
     else CK(build_lanes(h, h->pf));
@@ -902,8 +986,12 @@ int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = n
     a.grad = grad;
     size_t lds = (size_t)kRing * h->block * sizeof(float);
     EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
-    hipLaunchKernelGGL(k_transport<R>, dim3((h->nlanes + h->block - 1) / h->block),
-                       dim3(h->block), lds, h->stream, a);
+    dim3 grid((h->nlanes + h->block - 1) / h->block);
+    if (h->P.kerma && h->schedule == EGSGPU_SCHED_HISTORY)
+        hipLaunchKernelGGL(k_kerma<R>, grid, dim3(h->block), lds, h->stream, a,
+                           h->ev_ksteps, h->ev_batch, h->ev_max_wait);
+    else
+        hipLaunchKernelGGL(k_transport<R>, grid, dim3(h->block), lds, h->stream, a);
     EGS_HIP(hipGetLastError());
     EGS_HIP(hipEventRecord(h->kev[0][1], h->stream));
     return EGSGPU_OK;
@@ -1443,7 +1531,8 @@ int egsgpu_run_batched(egsgpu_handle *const *jobs, int32_t njobs, const int64_t 
 
 int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule) {
     EGS_ARG(h, "handle");
-    EGS_ARG(schedule == EGSGPU_SCHED_HISTORY || schedule == EGSGPU_SCHED_EVENT, "schedule");
+    EGS_ARG(schedule == EGSGPU_SCHED_HISTORY || schedule == EGSGPU_SCHED_EVENT ||
+            schedule == EGSGPU_SCHED_HISTORY_PLAIN, "schedule");
     if (h->lanes_ready) {
         set_error("the schedule must be chosen before the first run");
         return EGSGPU_E_ARG;
@@ -1667,7 +1756,7 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
         set_error("the density gradient is defined for the photon kerma mode (kerma = 1)");
         return EGSGPU_E_UNSUPPORTED;
     }
-    if (h->schedule != EGSGPU_SCHED_HISTORY) {
+    if (h->schedule == EGSGPU_SCHED_EVENT) {
         set_error("the density gradient replays histories: it needs the history schedule");
         return EGSGPU_E_UNSUPPORTED;
     }

@@ -86,6 +86,59 @@ __host__ __device__ inline int geom_howfar_in(const GeomView<R> &g, int ireg,
     return inew;
 }
 
+// geom_howfar_in for a caller that carries the voxel's plane indices
+// (ix, iy, iz) beside the region index: the same comparisons and the same
+// divisions in the same order, so t and the returned region are those of
+// geom_howfar_in, without the two integer divisions that recover the
+// indices from ireg.  The indices are advanced only along the axis whose
+// plane is crossed (the last axis to pass the test, as in geom_howfar_in).
+template <typename R>
+__host__ __device__ inline int geom_howfar_idx(const GeomView<R> &g, int ireg,
+                                               int &ix, int &iy, int &iz,
+                                               R x, R y, R z, R u, R v, R w,
+                                               R &t) {
+#pragma clang fp contract(off)
+    int axis = -1, step = 0;
+    if (u > 0) {
+        R d = (g.xp[ix + 1] - x) / u;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 0; step = 1; }
+    } else if (u < 0) {
+        R d = (g.xp[ix] - x) / u;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 0; step = -1; }
+    }
+    if (v > 0) {
+        R d = (g.yp[iy + 1] - y) / v;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 1; step = 1; }
+    } else if (v < 0) {
+        R d = (g.yp[iy] - y) / v;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 1; step = -1; }
+    }
+    if (w > 0) {
+        R d = (g.zp[iz + 1] - z) / w;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 2; step = 1; }
+    } else if (w < 0) {
+        R d = (g.zp[iz] - z) / w;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 2; step = -1; }
+    }
+    if (axis < 0) return ireg;
+    if (axis == 0) {
+        int n = ix + step;
+        if (n < 0 || n >= g.nx) return -1;
+        ix = n;
+        return ireg + step;
+    }
+    if (axis == 1) {
+        int n = iy + step;
+        if (n < 0 || n >= g.ny) return -1;
+        iy = n;
+        return ireg + step * g.nx;
+    }
+    int n = iz + step;
+    if (n < 0 || n >= g.nz) return -1;
+    iz = n;
+    return ireg + step * g.nxy;
+}
+
 // howfarFromOut: entry into the grid from outside (t overwritten only on a
 // hit; the tolerance re-check overwrites t1 through howfar before t = t1,
 // as the reference does).

@@ -103,9 +103,20 @@ __global__ void k_trace(GeomView<R> g, int nrays, const double *x,
         steps[base + k] = 0;
         k++;
     }
+    // inside the grid the ray steps like a photon flight: with the plane
+    // indices carried (geom_howfar_idx); every step is also taken by
+    // geom_howfar_in, and a difference ends the ray with nsteps = -1
+    int iz = 0, iy = 0, ix = 0;
+    if (ireg >= 0) {
+        iz = ireg / g.nxy;
+        iy = (ireg - iz * g.nxy) / g.nx;
+        ix = ireg - iz * g.nxy - iy * g.nx;
+    }
     while (ireg >= 0 && k < max_steps) {
-        R t = (R)1e30;
-        int inew = geom_howfar(g, ireg, px, py, pz, ux, uy, uz, t);
+        R t = (R)1e30, t_in = (R)1e30;
+        int inew = geom_howfar_idx(g, ireg, ix, iy, iz, px, py, pz, ux, uy, uz, t);
+        int inew_in = geom_howfar_in(g, ireg, px, py, pz, ux, uy, uz, t_in);
+        if (inew != inew_in || t != t_in) { nsteps[r] = -1; return; }
         px = px + ux * t; py = py + uy * t; pz = pz + uz * t;
         regions[base + k] = inew;
         steps[base + k] = (double)t;

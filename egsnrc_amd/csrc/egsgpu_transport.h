@@ -388,6 +388,11 @@ struct Shower {
     // kPTrans: up to kmax voxel steps; at the interaction point the type is
     // sampled (kPRayl..kPPair); a photon leaving the grid is discarded
     __device__ void photon_trans(int kmax) {
+        const GeomView<R> &G = *Gp;
+        int iz = p.ir / G.nxy, iy = p.ir - iz * G.nxy;
+        int ix = iy;
+        iy = iy / G.nx;
+        ix -= iy * G.nx;
         for (int k = 0; k < kmax; k++) {
             c_psteps++;
             if (++hsteps > kMaxStepsPerHistory) { c_ovf++; np = 0; pph = kPEnter; return; }
@@ -396,7 +401,8 @@ struct Shower {
             R ustep = gmfp * p_dpmfp;
             int irnew = p.ir, idisc = 0;
             if (ustep > p.dnear || p.wt <= 0) {
-                int inew = geom_howfar_in(*Gp, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+                int inew = geom_howfar_idx(G, p.ir, ix, iy, iz, p.x, p.y, p.z, p.u, p.v, p.w,
+                                           ustep);
                 if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
             }
             if (grad) log_event(1, p.ir, -ustep / (gmfp * p_rhof));

@@ -64,7 +64,9 @@ class Simulation:
             check(L.egsgpu_set_streams(self.h, nstreams, block))
         if schedule is not None:
             check(L.egsgpu_set_schedule(self.h, {"history": _lib.SCHED_HISTORY,
-                                                 "event": _lib.SCHED_EVENT}[schedule]))
+                                                 "event": _lib.SCHED_EVENT,
+                                                 "history-plain": _lib.SCHED_HISTORY_PLAIN}
+                                        [schedule]))
 
     def seed(self, ixx, jxx):
         check(_lib.lib().egsgpu_seed(self.h, ixx, jxx))

@@ -154,10 +154,20 @@ int egsgpu_set_streams(egsgpu_handle *h, int32_t nstreams,
  *    per-voxel sums in a slot of its own, so the uncertainty is still
  *    history-by-history.  Same physics and statistics; which stream's
  *    numbers a history uses depends on timing.
+ *    In kerma mode the lanes of a wave vote every turn for one action
+ *    (voxel steps of the flights, one interaction type, new histories,
+ *    end-of-history flushes), so that each lane still runs its own
+ *    histories with its own numbers but flights of different lengths do
+ *    not serialise the wave.
+ *  EGSGPU_SCHED_HISTORY_PLAIN: the history schedule without voting: each
+ *    lane runs its histories in one loop.  Same histories, numbers and
+ *    counters as EGSGPU_SCHED_HISTORY (the dose sums differ only by the
+ *    order of the FP64 additions); kept as the plain baseline.
  * Must be chosen before the first run; the default is EGSGPU_SCHED_EVENT
  * for coupled transport and EGSGPU_SCHED_HISTORY in kerma mode. */
-#define EGSGPU_SCHED_HISTORY 0
-#define EGSGPU_SCHED_EVENT   1
+#define EGSGPU_SCHED_HISTORY       0
+#define EGSGPU_SCHED_EVENT         1
+#define EGSGPU_SCHED_HISTORY_PLAIN 2
 int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule);
 
 /* Launch shape of the event schedule: register budget of the photon and
@@ -168,13 +178,14 @@ int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule);
 int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon,
                            int32_t waves_electron, int32_t nslots);
 
-/* Work batching inside the event kernels (0 = keep): voxel steps of a
- * photon flight per turn (default 8); an interaction type runs when
- * `batch` lanes of a wave wait for it (default 16) or one lane has waited
- * `max_wait` turns (default 16); once its bank is drained, a wave with
- * fewer than `defer_below` busy lanes (default 32, -1 = never) hands its
- * particles, mid-flight state included, to the next launch.  No
- * counterpart in the reference. */
+/* Work batching inside the event kernels and the kerma-mode history
+ * kernel (0 = keep): voxel steps of a photon flight per turn (default 8);
+ * an interaction type runs when `batch` lanes of a wave wait for it
+ * (default 16) or one lane has waited `max_wait` turns (default 16); once
+ * its bank is drained, a wave of the event kernels with fewer than
+ * `defer_below` busy lanes (default 32, -1 = never) hands its particles,
+ * mid-flight state included, to the next launch.  No counterpart in the
+ * reference. */
 int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps,
                               int32_t batch, int32_t max_wait,
                               int32_t defer_below);

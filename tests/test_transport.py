@@ -84,6 +84,32 @@ def test_kerma_dose_parity(engine, fp64, schedule):
     assert abs(ta - tb) < 0.01 * ta
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("fp64", [0, 1])
+def test_kerma_voting_kernel_runs_the_plain_histories(engine, fp64):
+    """Within one launch the kerma-mode history kernel with wave voting runs
+    every lane's histories with the same numbers as the plain per-lane
+    loop: identical event counters, and dose sums equal up to the order of
+    the FP64 additions.  (Across launches the two may differ: numbers left
+    in a lane's ring when a launch ends are dropped, and how many depends
+    on when the wave refilled.)"""
+    xb, yb, zb, med, rhor, src, tp = kerma_case()
+    tp.fp64 = fp64
+    out = []
+    for schedule in ("history", "history-plain"):
+        sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192,
+                         schedule=schedule)
+        sim.run(200001)
+        out.append((sim.tallies(), sim.counters()))
+        sim.close()
+    (a, ca), (b, cb) = out
+    assert a[2] == b[2] == 200001
+    assert list(ca) == list(cb)
+    assert ca[2] > 0 and ca[1] > 0
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(a[1], b[1], rtol=1e-12, atol=0)
+
+
 def electron_case(n=12, dx=1.0, energy=6.0):
     """Electron beam on a water phantom with a bone slab (coupled transport,
     ECUT 0.521 MeV total, PCUT 0.01 MeV)."""
