@@ -197,12 +197,13 @@ __device__ __forceinline__ int wave_claim(int *counter) {
 // (value = energy * weight), type 1 = d log p / d rho of the voxel.
 struct DevGrad {
     int cap;                        // events per lane
-    int2 *meta;                     // [lane*cap + i]
-    float *val;
-    double *acc;                    // [lane*kGradDepth + d] sweep accumulators
+    // [lane*cap + i]: x = voxel << 7 | stack depth << 1 | type, y = value bits
+    uint2 *ev;
+    double *acc;                    // [lane*kGradDepth + d] sweep accumulators (deep stacks)
     const double *w;                // per-voxel loss weights 2(D-D*)/(N m)
     double *g;                      // gradient accumulator [nvox]
 };
 constexpr int kGradDepth = 32;
+constexpr int kGradVoxBits = 25;    // voxels addressable by the event log
 
 }  // namespace egsgpu

@@ -1760,6 +1760,10 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
         set_error("the density gradient replays histories: it needs the history schedule");
         return EGSGPU_E_UNSUPPORTED;
     }
+    if (h->nvox >= (1 << kGradVoxBits)) {
+        set_error("the density gradient supports up to %d voxels", (1 << kGradVoxBits) - 1);
+        return EGSGPU_E_UNSUPPORTED;
+    }
     int rc = ensure_lanes(h);
     if (rc) return rc;
     size_t nv = (size_t)h->nvox;
@@ -1767,8 +1771,7 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
         CK(alloc(h, (size_t)h->nlanes, &h->rng_saved));
         DevGrad &G = h->dgrad;
         G.cap = 2048;
-        CK(alloc(h, (size_t)G.cap * h->nlanes, &G.meta));
-        CK(alloc(h, (size_t)G.cap * h->nlanes, &G.val));
+        CK(alloc(h, (size_t)G.cap * h->nlanes, &G.ev));
         CK(alloc(h, (size_t)kGradDepth * h->nlanes, &G.acc));
         double *w = nullptr;
         CK(alloc(h, nv, &w));

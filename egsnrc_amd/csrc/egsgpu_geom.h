@@ -87,11 +87,13 @@ __host__ __device__ inline int geom_howfar_in(const GeomView<R> &g, int ireg,
 }
 
 // geom_howfar_in for a caller that carries the voxel's plane indices
-// (ix, iy, iz) beside the region index: the same comparisons and the same
-// divisions in the same order, so t and the returned region are those of
+// (ix, iy, iz) beside the region index: the same operands, divisions and
+// comparisons in the same order, so t and the returned region are those of
 // geom_howfar_in, without the two integer divisions that recover the
-// indices from ireg.  The indices are advanced only along the axis whose
-// plane is crossed (the last axis to pass the test, as in geom_howfar_in).
+// indices from ireg, and with one plane load per axis whatever the sign of
+// the direction cosine (lanes of a wave moving either way share the
+// instructions).  The indices are advanced only along the axis whose plane
+// is crossed (the last axis to pass the test, as in geom_howfar_in).
 template <typename R>
 __host__ __device__ inline int geom_howfar_idx(const GeomView<R> &g, int ireg,
                                                int &ix, int &iy, int &iz,
@@ -99,26 +101,20 @@ __host__ __device__ inline int geom_howfar_idx(const GeomView<R> &g, int ireg,
                                                R &t) {
 #pragma clang fp contract(off)
     int axis = -1, step = 0;
-    if (u > 0) {
-        R d = (g.xp[ix + 1] - x) / u;
-        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 0; step = 1; }
-    } else if (u < 0) {
-        R d = (g.xp[ix] - x) / u;
-        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 0; step = -1; }
+    if (u != 0) {
+        const bool up = u > 0;
+        R d = (g.xp[ix + (up ? 1 : 0)] - x) / u;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 0; step = up ? 1 : -1; }
     }
-    if (v > 0) {
-        R d = (g.yp[iy + 1] - y) / v;
-        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 1; step = 1; }
-    } else if (v < 0) {
-        R d = (g.yp[iy] - y) / v;
-        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 1; step = -1; }
+    if (v != 0) {
+        const bool up = v > 0;
+        R d = (g.yp[iy + (up ? 1 : 0)] - y) / v;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 1; step = up ? 1 : -1; }
     }
-    if (w > 0) {
-        R d = (g.zp[iz + 1] - z) / w;
-        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 2; step = 1; }
-    } else if (w < 0) {
-        R d = (g.zp[iz] - z) / w;
-        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 2; step = -1; }
+    if (w != 0) {
+        const bool up = w > 0;
+        R d = (g.zp[iz + (up ? 1 : 0)] - z) / w;
+        if (d <= t) { t = d <= g.btol ? g.hbtol : d; axis = 2; step = up ? 1 : -1; }
     }
     if (axis < 0) return ireg;
     if (axis == 0) {

@@ -64,44 +64,73 @@ struct Shower {
     // (32-bit per lane and launch; added into 64-bit totals at the end)
     unsigned c_psteps, c_cross, c_dep, c_flush, c_esteps, c_ovf, c_hovf;
     unsigned hsteps;   // steps of the current history (runaway guard)
-    // density-gradient replay log (null when not replaying)
+    // density-gradient replay log (null when not replaying): one 8-byte
+    // record per event, written with one store
     const DevGrad *grad = nullptr;
-    int nlog = 0;
+    int nlog = 0, gdmax = 0;
     bool log_ovf = false;
     __device__ __forceinline__ void log_event(int type, int vox, R v) {
         if (nlog >= grad->cap) { log_ovf = true; return; }
         size_t i = (size_t)lane * grad->cap + nlog++;
-        grad->meta[i] = make_int2(vox, np * 2 + type);
-        grad->val[i] = (float)v;
+        int depth = np < 63 ? np : 63;
+        gdmax = depth > gdmax ? depth : gdmax;
+        grad->ev[i] = make_uint2(((unsigned)vox << 7) | ((unsigned)depth << 1) | (unsigned)type,
+                                 __float_as_uint((float)v));
     }
     // reverse sweep over the history's events: every log-likelihood term
     // is multiplied by the weighted deposits downstream of it (later events
-    // of the same particle and its descendants: stack depth >= its own)
+    // of the same particle and its descendants: stack depth >= its own).
+    // acc[d] is the weighted deposit downstream of the last event at depth
+    // d; an event at depth `depth` ends the descendants deeper than it.
+    // Histories with stacks up to 4 deep (all of them in kerma mode but
+    // for pair and relaxation cascades) keep acc in registers.
     __device__ __forceinline__ void grad_sweep() {
-        if (log_ovf) { c_hovf++; nlog = 0; log_ovf = false; return; }
-        double *acc = grad->acc + (size_t)lane * kGradDepth;
-        int dmax = 0;
-        for (int q = 0; q < nlog; q++) {
-            int d = grad->meta[(size_t)lane * grad->cap + q].y >> 1;
-            dmax = d > dmax ? d : dmax;
-        }
-        dmax = dmax < kGradDepth - 1 ? dmax : kGradDepth - 1;
-        for (int d = 0; d <= dmax; d++) acc[d] = 0;
-        for (int q = nlog - 1; q >= 0; q--) {
-            int2 m = grad->meta[(size_t)lane * grad->cap + q];
-            int depth = m.y >> 1;
-            depth = depth < dmax ? depth : dmax;
-            for (int d = depth + 1; d <= dmax; d++) acc[d] = 0;
-            double v = (double)grad->val[(size_t)lane * grad->cap + q];
-            if ((m.y & 1) == 0) {
-                double wd = grad->w[m.x] * v;
-                for (int d = 0; d <= depth; d++) acc[d] += wd;
-            } else {
-                double c = v * acc[depth];
-                if (c != 0) atomicAdd(grad->g + m.x, c);
+        if (log_ovf) { c_hovf++; nlog = 0; gdmax = 0; log_ovf = false; return; }
+        const uint2 *ev = grad->ev + (size_t)lane * grad->cap;
+        const int dmax = gdmax < kGradDepth - 1 ? gdmax : kGradDepth - 1;
+        if (dmax <= 3) {
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            for (int q = nlog - 1; q >= 0; q--) {
+                uint2 e = ev[q];
+                int depth = (int)((e.x >> 1) & 63u);
+                int vox = (int)(e.x >> 7);
+                double v = (double)__uint_as_float(e.y);
+                if (depth < 1) a1 = 0;
+                if (depth < 2) a2 = 0;
+                if (depth < 3) a3 = 0;
+                if ((e.x & 1u) == 0) {
+                    double wd = grad->w[vox] * v;
+                    a0 += wd;
+                    if (depth >= 1) a1 += wd;
+                    if (depth >= 2) a2 += wd;
+                    if (depth >= 3) a3 += wd;
+                } else {
+                    double A = depth == 0 ? a0 : depth == 1 ? a1 : depth == 2 ? a2 : a3;
+                    double c = v * A;
+                    if (c != 0) atomicAdd(grad->g + vox, c);
+                }
+            }
+        } else {
+            double *acc = grad->acc + (size_t)lane * kGradDepth;
+            for (int d = 0; d <= dmax; d++) acc[d] = 0;
+            for (int q = nlog - 1; q >= 0; q--) {
+                uint2 e = ev[q];
+                int depth = (int)((e.x >> 1) & 63u);
+                depth = depth < dmax ? depth : dmax;
+                int vox = (int)(e.x >> 7);
+                double v = (double)__uint_as_float(e.y);
+                for (int d = depth + 1; d <= dmax; d++) acc[d] = 0;
+                if ((e.x & 1u) == 0) {
+                    double wd = grad->w[vox] * v;
+                    for (int d = 0; d <= depth; d++) acc[d] += wd;
+                } else {
+                    double c = v * acc[depth];
+                    if (c != 0) atomicAdd(grad->g + vox, c);
+                }
             }
         }
         nlog = 0;
+        gdmax = 0;
     }
 
     __device__ Shower(const DevTables<R> &t, const GeomView<R> &g,

@@ -254,8 +254,9 @@ int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size);
  *   L = sum_v (D_v - target_v)^2
  * with respect to the relative voxel densities (FP64 tallies).  Writes
  * dL/drho to grad (nvox), the dose to dose (nvox, may be NULL) and L to
- * loss.  The explicit 1/m_v dependence of the dose is included.  No
- * counterpart in the reference (egs_chamber's correlated sampling with
+ * loss.  The explicit 1/m_v dependence of the dose is included.  Needs
+ * the history schedule and fewer than 2^25 voxels.  No counterpart in the
+ * reference (egs_chamber's correlated sampling with
  * RNG save/restore, user_codes/egs_chamber/egs_chamber.cpp:2424-2446, is
  * the precedent for the replay). */
 int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
