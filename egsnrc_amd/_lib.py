@@ -29,7 +29,9 @@ class EgsSource(C.Structure):
                 ("shape", C.c_int32),
                 ("x0", C.c_double), ("x1", C.c_double), ("y0", C.c_double),
                 ("y1", C.c_double), ("z", C.c_double),
-                ("ux", C.c_double), ("uy", C.c_double), ("uz", C.c_double)]
+                ("ux", C.c_double), ("uy", C.c_double), ("uz", C.c_double),
+                ("kind", C.c_int32), ("rot", C.c_double * 9), ("iso", C.c_double * 3),
+                ("xcol", C.c_double), ("ycol", C.c_double), ("dsource", C.c_double)]
 
 
 class EgsTransport(C.Structure):
@@ -57,6 +59,8 @@ def default_transport(**kw):
 SCHED_HISTORY = 0
 SCHED_EVENT = 1
 SCHED_HISTORY_PLAIN = 2
+RNG_RANMAR = 0
+RNG_RANLUX = 1
 
 
 class EngineError(RuntimeError):
@@ -86,6 +90,7 @@ def lib():
         "egsgpu_destroy": [vp],
         "egsgpu_set_streams": [vp, i32, i32],
         "egsgpu_seed": [vp, i32, i32],
+        "egsgpu_set_rng": [vp, i32, i32, i32],
         "egsgpu_run": [vp, i64],
         "egsgpu_get_tallies": [vp, vp, vp, vp],
         "egsgpu_reset_tallies": [vp],

@@ -23,9 +23,6 @@ constexpr int kMaxNE = 16;        // elements per medium
 constexpr int kMaxShMed = 200;    // Compton shells per medium ($MXMDSH)
 constexpr int kGE = 2000;         // $MXGE
 constexpr int kMaxEKE = 500;      // $MXEKE
-constexpr int kRNB = 16;          // ranmar outputs per refill
-constexpr int kRing = 32;         // LDS ring of random numbers per lane
-constexpr int kRingLow = 12;      // top the ring up below this many
 
 // The PWLF tables of one medium and energy bin are interleaved into one
 // row, so that the lookups of a step touch one or two cache lines.
@@ -89,8 +86,11 @@ template <typename R>
 struct DevSource {
     int charge, spectrum, shape, np, type;
     R energy, x0, x1, y0, y1, z, ux, uy, uz;
-    const double *xi, *wi;   // alias table (FP64 cumulative data)
+    const double *xi, *wi;   // alias table (FP64 cumulative data); CDFINV columns
     const int *bin;
+    int kind;                // EGSGPU_SOURCE_*
+    double rot[9];           // DOSXYZnrc isource 1: r_11..r_33
+    double iso[3], xcol, ycol, dsource;
 };
 
 struct DevParams {
@@ -105,7 +105,8 @@ constexpr int kStkF = 11;
 template <typename R>
 struct DevLanes {
     int nlanes;
-    RanmarRec *rng;                 // [nlanes]
+    RanmarRec *rng;                 // [nlanes] ranmar streams (ranmar runs)
+    RanluxRec *rlx;                 // [nlanes] ranlux streams (ranlux runs)
     int mxstk;
     R *stk;   // [(k*kStkF + field)*nlanes + lane]; iq and ir stored bitwise
     int hcap;                       // hash capacity per lane (power of 2)

@@ -28,6 +28,14 @@ __global__ void k_seed(RanmarRec *recs, int nlanes, int ixx, int jxx) {
     ranmar_seed(recs + s, is, js);
 }
 
+// ranlux stream s: init_ranlux(luxury, seed + s), the seed of parallel job
+// s + 1 of DOSXYZnrc (dosxyznrc.mortran:1916, ranlux.macros:96-99)
+__global__ void k_seed_lux(RanluxRec *recs, int nlanes, int lux, int seed) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nlanes) return;
+    ranlux_seed(recs + s, lux, seed + s);
+}
+
 template <typename R>
 struct KArgs {
     DevTables<R> T;
@@ -43,9 +51,18 @@ struct KArgs {
 };
 
 // EGS_AliasTable::sample (egs_alias_table.cpp:268-300), types 0 and 1
-template <typename R, bool EV>
-__device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R, EV> &sh) {
+template <typename R, bool EV, bool LUX>
+__device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R, EV, LUX> &sh) {
     if (S.spectrum == 0) return S.energy;
+    if (S.spectrum == EGSGPU_SPEC_CDFINV) {
+        // DOSXYZnrc srchst (srcxyznrc.mortran:3440-3446): grid point
+        // K = min(DELTAK*r1 + 1, DELTAK), E = CDFINV(K,1) + r2*CDFINV(K,2)
+        double r1 = (double)sh.rnd();
+        double r2 = (double)sh.rnd();
+        double kk = (double)S.np * r1 + 1.0;
+        int k = (int)(kk < (double)S.np ? kk : (double)S.np);
+        return (R)(S.xi[k - 1] + r2 * S.wi[k - 1]);
+    }
     double r1 = (double)sh.rnd();
     double aj = r1 * S.np;
     int j = (int)aj;
@@ -57,13 +74,124 @@ __device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R, EV> 
     return (R)(x + dx * r2);
 }
 
+// Plane index i with p[i] <= x < p[i+1] (the region search of DOSXYZnrc's
+// srchst, srcxyznrc.mortran:3427-3437), clamped to the grid.
+template <typename R>
+__device__ __forceinline__ int planes_lower(double x, int n, const R *p) {
+    int lo = 0, hi = n;   // p[lo] <= x < p[hi] is kept
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if ((double)p[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Region index of the first plane j with p[j] <= x <= p[j+1] (the entry
+// search of srcxyznrc.mortran:4255-4392, both ends inclusive).
+template <typename R>
+__device__ __forceinline__ int planes_first(double x, int n, const R *p) {
+    int lo = 0, hi = n;   // the first j with x <= p[j+1]
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (x <= (double)p[mid + 1]) hi = mid; else lo = mid + 1;
+    }
+    return lo < n ? lo : n - 1;
+}
+
+// DOSXYZnrc isource 0, srchst (srcxyznrc.mortran:3416-3448): position on
+// the front face z = zbound(1) (x drawn, then y, each only if its range
+// is not empty), fixed direction, then the energy.
+template <typename R, bool EV, bool LUX>
+__device__ __forceinline__ bool sample_dosxyz_front(const DevSource<R> &S, const GeomView<R> &G,
+                                                    Shower<R, EV, LUX> &sh, Part<R> &q) {
+    double x = (double)S.x0, y = (double)S.y0;
+    double dx = (double)S.x1 - (double)S.x0, dy = (double)S.y1 - (double)S.y0;
+    if (dx != 0) x = (double)S.x0 + (double)sh.rnd() * dx;
+    if (dy != 0) y = (double)S.y0 + (double)sh.rnd() * dy;
+    R E = source_energy(S, sh);
+    int ix = planes_lower(x, G.nx, G.xp), iy = planes_lower(y, G.ny, G.yp);
+    q.e = S.charge ? E + (R)0.5109989461 : E;
+    q.x = (R)x; q.y = (R)y; q.z = G.zp[0];
+    q.u = S.ux; q.v = S.uy; q.w = S.uz;
+    q.wt = 1; q.dnear = 0; q.iq = S.charge; q.ir = ix + iy * G.nx;
+    return true;
+}
+
+// DOSXYZnrc isource 1, srchst (srcxyznrc.mortran:3551-3571, 4157-4165,
+// 4255-4392): a point of the collimator rectangle and the energy, rotated
+// into position at distance dsource from the isocentre, then carried to
+// the first face of the grid it meets; a beam particle that misses the
+// grid is counted (sh.c_miss) and sampled again.
+template <typename R, bool EV, bool LUX>
+__device__ __forceinline__ bool sample_dosxyz_beam(const DevSource<R> &S, const GeomView<R> &G,
+                                                   Shower<R, EV, LUX> &sh, Part<R> &q) {
+    const double *r = S.rot;
+    const double u = r[2], v = r[5], w = r[8];
+    const double x0 = G.xp[0], x1 = G.xp[G.nx], y0 = G.yp[0], y1 = G.yp[G.ny];
+    const double z0 = G.zp[0], z1 = G.zp[G.nz];
+    for (int ntry = 0; ntry < 100000; ntry++) {
+        double xs = ((double)sh.rnd() - 0.5) * S.xcol;
+        double ys = ((double)sh.rnd() - 0.5) * S.ycol;
+        R E = source_energy(S, sh);
+        double xp = r[0] * xs + r[1] * ys + r[2] * S.dsource + S.iso[0];
+        double yp = r[3] * xs + r[4] * ys + r[5] * S.dsource + S.iso[1];
+        double zp = r[6] * xs + r[7] * ys + r[8] * S.dsource + S.iso[2];
+        double xi = 0, yi = 0, zi = 0;
+        int ix = -1, iy = -1, iz = -1;
+        if (u != 0) {
+            double xf = u > 0 ? x0 : x1;
+            double d = (xf - xp) / u;
+            double yy = yp + d * v, zz = zp + d * w;
+            if (yy >= y0 && yy <= y1 && zz >= z0 && zz <= z1) {
+                xi = xf; yi = yy; zi = zz;
+                ix = u > 0 ? 0 : G.nx - 1;
+                iy = planes_first(yy, G.ny, G.yp);
+                iz = planes_first(zz, G.nz, G.zp);
+            }
+        }
+        if (ix < 0 && v != 0) {
+            double yf = v > 0 ? y0 : y1;
+            double d = (yf - yp) / v;
+            double xx = xp + d * u, zz = zp + d * w;
+            if (xx >= x0 && xx <= x1 && zz >= z0 && zz <= z1) {
+                xi = xx; yi = yf; zi = zz;
+                iy = v > 0 ? 0 : G.ny - 1;
+                ix = planes_first(xx, G.nx, G.xp);
+                iz = planes_first(zz, G.nz, G.zp);
+            }
+        }
+        if (ix < 0 && w != 0) {
+            double zf = w > 0 ? z0 : z1;
+            double d = (zf - zp) / w;
+            double xx = xp + d * u, yy = yp + d * v;
+            if (xx >= x0 && xx <= x1 && yy >= y0 && yy <= y1) {
+                xi = xx; yi = yy; zi = zf;
+                iz = w > 0 ? 0 : G.nz - 1;
+                ix = planes_first(xx, G.nx, G.xp);
+                iy = planes_first(yy, G.ny, G.yp);
+            }
+        }
+        if (ix < 0) { sh.c_miss++; continue; }
+        q.e = S.charge ? E + (R)0.5109989461 : E;
+        q.x = (R)xi; q.y = (R)yi; q.z = (R)zi;
+        q.u = (R)u; q.v = (R)v; q.w = (R)w;
+        q.wt = 1; q.dnear = 0; q.iq = S.charge;
+        q.ir = ix + iy * G.nx + iz * G.nxy;
+        return true;
+    }
+    return false;
+}
+
 // One primary of EGS_Application::simulateSingleShower
 // (egs_application.cpp:919-959) from the parallel-beam source
 // (egs_parallel_beam.cpp:40-117): position on the source shape, entry into
 // the grid along the beam direction.  Returns false if it misses the grid.
-template <typename R, bool EV>
+// The DOSXYZnrc sources (isource 0 and 1) have their own sampling.
+template <typename R, bool EV, bool LUX>
 __device__ __forceinline__ bool sample_primary(const DevSource<R> &S, const GeomView<R> &G,
-                               Shower<R, EV> &sh, Part<R> &q) {
+                               Shower<R, EV, LUX> &sh, Part<R> &q) {
+    if (S.kind == EGSGPU_SOURCE_DOSXYZ_FRONT) return sample_dosxyz_front(S, G, sh, q);
+    if (S.kind == EGSGPU_SOURCE_DOSXYZ_BEAM) return sample_dosxyz_beam(S, G, sh, q);
     const R RM = (R)0.5109989461;
     int ireg = -1;
     for (int ntry = 0; ntry < 100000 && ireg < 0; ntry++) {
@@ -99,14 +227,14 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // Add this wave's event counters to the kernel's totals: one atomic per
 // counter and wave.  Called by all lanes of every wave.
-template <typename R, bool EV>
-__device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, EV> &sh,
+template <typename R, bool EV, bool LUX>
+__device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, EV, LUX> &sh,
                                              int kernel) {
     unsigned long long *c = t.counters + 8 * kernel;
-    const unsigned v[7] = {sh.c_psteps, sh.c_cross, sh.c_dep, sh.c_flush,
-                           sh.c_esteps, sh.c_ovf, sh.c_hovf};
+    const unsigned v[8] = {sh.c_psteps, sh.c_cross, sh.c_dep, sh.c_flush,
+                           sh.c_esteps, sh.c_ovf, sh.c_hovf, sh.c_miss};
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
+    for (int k = 0; k < 8; k++) {
         unsigned long long w = wave_sum(v[k]);
         if (__lane_id() == 0 && w) atomicAdd(c + k, w);
     }
@@ -114,13 +242,13 @@ __device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, 
 
 // History schedule: lane s runs its share of the histories one after the
 // other with ranmar stream s, like egs-parallel job s.
-template <typename R>
+template <typename R, bool LUX>
 __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.L.nlanes;   // lanes past the last stream idle
-    Shower<R> sh(a.T, a.G, a.vox, a.P, a.L, a.tally, live ? lane : 0,
-                 rbuf_all + threadIdx.x);
+    using S = Shower<R, false, LUX>;
+    S sh(a.T, a.G, a.vox, a.P, a.L, a.tally, live ? lane : 0, rbuf_all + threadIdx.x);
     sh.grad = a.grad;
     int64_t nh = live ? a.base + (lane < a.rem ? 1 : 0) : 0;
     for (int64_t k = 0; k < nh; k++) {
@@ -128,13 +256,13 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
         if (!sample_primary(a.S, a.G, sh, q)) continue;
         sh.p = q;
         sh.np = 1;
-        sh.eph = Shower<R>::kNew;
+        sh.eph = S::kNew;
         while (sh.np > 0) {
             sh.rtopup();
             if (sh.p.iq == 0) {
                 sh.photon();
-                sh.eph = Shower<R>::kNew;
-            } else if (sh.eph >= Shower<R>::kBrems) {
+                sh.eph = S::kNew;
+            } else if (sh.eph >= S::kBrems) {
                 sh.electron_interact();
             } else {
                 sh.electron_step();
@@ -187,13 +315,13 @@ constexpr int kKermaWaves = EGSGPU_KERMA_WAVES;
 // different lengths and the rare, long samplers do not serialise the
 // wave.  Cheap phase changes (photon entry, new energy, the charged
 // discard) run at once.
-template <typename R>
+template <typename R, bool LUX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kKermaWaves)))
 k_kerma(KArgs<R> a, int ksteps, int batch, int max_wait) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.L.nlanes;
-    using S = Shower<R>;
+    using S = Shower<R, false, LUX>;
     S sh(a.T, a.G, a.vox, a.P, a.L, a.tally, live ? lane : 0, rbuf_all + threadIdx.x);
     sh.grad = a.grad;
     const int64_t nh = live ? a.base + (lane < a.rem ? 1 : 0) : 0;
@@ -276,8 +404,8 @@ struct EvArgs {
 };
 
 // Point the shower at the simulation of the history in `slot`.
-template <typename R>
-__device__ __forceinline__ void ev_job(Shower<R, true> &sh, const EvArgs<R> &a, int slot) {
+template <typename R, bool LUX>
+__device__ __forceinline__ void ev_job(Shower<R, true, LUX> &sh, const EvArgs<R> &a, int slot) {
     const JobDesc<R> &J = a.jobs[a.njobs > 1 ? a.slot_job[slot] : 0];
     sh.Gp = &J.G;
     sh.vox = J.vox;
@@ -306,8 +434,8 @@ __device__ __forceinline__ int bank_load(const DevBank<R> &b, int i, Part<R> &q)
 // with loop state ph, s[].  The history's pending count goes up before the
 // particle becomes visible.  A full bank absorbs the particle where it is
 // (counted as an overflow).
-template <typename R>
-__device__ void bank_push(Shower<R, true> &sh, const DevBank<R> &b, int *count, int ph = 0,
+template <typename R, bool LUX>
+__device__ void bank_push(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *count, int ph = 0,
                           R s0 = 0, R s1 = 0, R s2 = 0, R s3 = 0) {
     atomicAdd(sh.slots->pending + sh.slot, 1);
     int i = wave_claim(count);
@@ -324,8 +452,8 @@ __device__ void bank_push(Shower<R, true> &sh, const DevBank<R> &b, int *count, 
     sh.score(sh.p.ir, edep, sh.p.wt);
 }
 
-template <typename R>
-__device__ __forceinline__ void ev_setup(Shower<R, true> &sh, const EvArgs<R> &a) {
+template <typename R, bool LUX>
+__device__ __forceinline__ void ev_setup(Shower<R, true, LUX> &sh, const EvArgs<R> &a) {
     sh.slots = &a.slots;
     sh.ev_ctl = a.ctl;
     sh.np = 0;
@@ -334,13 +462,13 @@ __device__ __forceinline__ void ev_setup(Shower<R, true> &sh, const EvArgs<R> &a
 
 // New histories into free slots: thread i samples primaries i, i+nlanes, ...
 // into bank `push` (photon or charged, by the source).
-template <typename R>
+template <typename R, bool LUX>
 __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
-    Shower<R, true> sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally,
-                       live ? lane : 0, rbuf_all + threadIdx.x);
+    Shower<R, true, LUX> sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally,
+                            live ? lane : 0, rbuf_all + threadIdx.x);
     ev_setup(sh, a);
     const unsigned long long ns = (unsigned long long)a.slots.nslots;
     // a negative count: the plan made on the device by k_ev_plan
@@ -388,13 +516,13 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
 // Once the bank is empty and fewer than defer_below lanes of a wave are
 // still busy, they hand their photons (with the loop state) to bank `defr`
 // for the next launch instead of holding the wave.
-template <typename R, int W>
+template <typename R, int W, bool LUX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_ev_photon(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
-    using S = Shower<R, true>;
+    using S = Shower<R, true, LUX>;
     S sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
@@ -473,13 +601,13 @@ k_ev_photon(EvArgs<R> a) {
 // stepping; discrete interactions are batched by type over the wave.
 // Photons created on the way go to bank `push`; a nearly idle wave after
 // the bank is drained defers its particles to bank `defr`.
-template <typename R, int W>
+template <typename R, int W, bool LUX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_ev_electron(EvArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.k.L.nlanes;
-    using S = Shower<R, true>;
+    using S = Shower<R, true, LUX>;
     S sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
     ev_setup(sh, a);
@@ -624,6 +752,9 @@ struct egsgpu_handle {
     DevParams P{};
     DevTally tally{};
     RanmarRec *rng = nullptr;
+    RanluxRec *rlx = nullptr;
+    int rng_kind = EGSGPU_RNG_RANMAR;   // which streams drive the transport
+    int lux = 1, lux_seed = 1;          // ranlux luxury level and first seed
     int nlanes = 0, block = 256, mxstk = 48, hcap = 256;
     int ncu = 1;                        // compute units of the device
     int ev_wp = 2, ev_we = 2;           // waves/SIMD variants of the event kernels
@@ -643,6 +774,7 @@ struct egsgpu_handle {
     // host copies for dose normalisation and the gradient
     std::vector<double> hx, hy, hz, hrhor, hmass;
     RanmarRec *rng_saved = nullptr;
+    RanluxRec *rlx_saved = nullptr;
     DevGrad dgrad{};
     DevGrad *dgrad_dev = nullptr;
     // event schedule
@@ -878,7 +1010,17 @@ int build_source(egsgpu_handle *h, const EgsSource *s, Precision<R> &P) {
     S.np = 0; S.type = 0;
     S.xi = S.wi = nullptr;
     S.bin = nullptr;
+    S.kind = s->kind;
+    for (int k = 0; k < 9; k++) S.rot[k] = s->rot[k];
+    for (int k = 0; k < 3; k++) S.iso[k] = s->iso[k];
+    S.xcol = s->xcol; S.ycol = s->ycol; S.dsource = s->dsource;
     if (s->spectrum == EGSGPU_SPEC_MONO) return EGSGPU_OK;
+    if (s->spectrum == EGSGPU_SPEC_CDFINV) {
+        S.np = s->nbins;
+        CK(upload(h, s->spec_e, (size_t)s->nbins, &S.xi));
+        CK(upload(h, s->spec_w, (size_t)s->nbins, &S.wi));
+        return EGSGPU_OK;
+    }
     // EGS_AliasTable::make (egs_alias_table.cpp:102-171)
     int type = s->spectrum == EGSGPU_SPEC_HISTOGRAM ? 1 : 0;
     int n = type ? s->nbins + 1 : s->nbins;
@@ -915,6 +1057,7 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
     DevLanes<R> &L = P.L;
     L.nlanes = h->nlanes;
     L.rng = h->rng;
+    L.rlx = h->rlx;
     L.mxstk = h->mxstk;
     if ((double)h->mxstk * kStkF * h->nlanes * sizeof(R) >= 4294967296.0) {
         set_error("particle stacks of %d streams exceed 4 GB", h->nlanes);
@@ -931,6 +1074,27 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
     return EGSGPU_OK;
 }
 
+// (re)seed every stream of the kind in use
+int seed_streams(egsgpu_handle *h) {
+    dim3 grid((h->nlanes + 255) / 256);
+    if (h->rng_kind == EGSGPU_RNG_RANLUX)
+        hipLaunchKernelGGL(k_seed_lux, grid, dim3(256), 0, h->stream, h->rlx, h->nlanes,
+                           h->lux, h->lux_seed);
+    else
+        hipLaunchKernelGGL(k_seed, grid, dim3(256), 0, h->stream, h->rng, h->nlanes,
+                           h->ixx, h->jxx);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    return EGSGPU_OK;
+}
+
+// the stream records in use and their size
+void *rng_records(egsgpu_handle *h, size_t *bytes) {
+    bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
+    *bytes = (size_t)h->nlanes * (lux ? sizeof(RanluxRec) : sizeof(RanmarRec));
+    return lux ? (void *)h->rlx : (void *)h->rng;
+}
+
 int ensure_lanes(egsgpu_handle *h) {
     if (h->lanes_ready) return EGSGPU_OK;
     // default streams: what the kernels can keep resident (event kernels
@@ -940,12 +1104,10 @@ int ensure_lanes(egsgpu_handle *h) {
                               : h->P.kerma && h->schedule == EGSGPU_SCHED_HISTORY
                                     ? 256 * kKermaWaves : 512);
     CK(alloc(h, (size_t)h->nlanes, &h->rng));
+    CK(alloc(h, (size_t)h->nlanes, &h->rlx));
     if (h->fp64) CK(build_lanes(h, h->pd));
     else CK(build_lanes(h, h->pf));
-    hipLaunchKernelGGL(k_seed, dim3((h->nlanes + 255) / 256), dim3(256), 0, h->stream,
-                       h->rng, h->nlanes, h->ixx, h->jxx);
-    EGS_HIP(hipGetLastError());
-    EGS_HIP(hipStreamSynchronize(h->stream));
+    CK(seed_streams(h));
     h->lanes_ready = true;
     return EGSGPU_OK;
 }
@@ -987,11 +1149,19 @@ int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = n
     size_t lds = (size_t)kRing * h->block * sizeof(float);
     EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
     dim3 grid((h->nlanes + h->block - 1) / h->block);
-    if (h->P.kerma && h->schedule == EGSGPU_SCHED_HISTORY)
-        hipLaunchKernelGGL(k_kerma<R>, grid, dim3(h->block), lds, h->stream, a,
-                           h->ev_ksteps, h->ev_batch, h->ev_max_wait);
-    else
-        hipLaunchKernelGGL(k_transport<R>, grid, dim3(h->block), lds, h->stream, a);
+    const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
+    if (h->P.kerma && h->schedule == EGSGPU_SCHED_HISTORY) {
+        if (lux)
+            hipLaunchKernelGGL((k_kerma<R, true>), grid, dim3(h->block), lds, h->stream, a,
+                               h->ev_ksteps, h->ev_batch, h->ev_max_wait);
+        else
+            hipLaunchKernelGGL((k_kerma<R, false>), grid, dim3(h->block), lds, h->stream, a,
+                               h->ev_ksteps, h->ev_batch, h->ev_max_wait);
+    } else if (lux) {
+        hipLaunchKernelGGL((k_transport<R, true>), grid, dim3(h->block), lds, h->stream, a);
+    } else {
+        hipLaunchKernelGGL((k_transport<R, false>), grid, dim3(h->block), lds, h->stream, a);
+    }
     EGS_HIP(hipGetLastError());
     EGS_HIP(hipEventRecord(h->kev[0][1], h->stream));
     return EGSGPU_OK;
@@ -1056,21 +1226,24 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
 template <typename R>
 using EvKernel = void (*)(EvArgs<R>);
 
+// ranlux runs use the 2-waves-per-SIMD build only
 template <typename R>
-EvKernel<R> photon_kernel(int w) {
+EvKernel<R> photon_kernel(int w, bool lux) {
+    if (lux) return k_ev_photon<R, 2, true>;
     switch (w) {
-    case 3: return k_ev_photon<R, 3>;
-    case 4: return k_ev_photon<R, 4>;
-    default: return k_ev_photon<R, 2>;
+    case 3: return k_ev_photon<R, 3, false>;
+    case 4: return k_ev_photon<R, 4, false>;
+    default: return k_ev_photon<R, 2, false>;
     }
 }
 
 template <typename R>
-EvKernel<R> electron_kernel(int w) {
+EvKernel<R> electron_kernel(int w, bool lux) {
+    if (lux) return k_ev_electron<R, 2, true>;
     switch (w) {
-    case 3: return k_ev_electron<R, 3>;
-    case 4: return k_ev_electron<R, 4>;
-    default: return k_ev_electron<R, 2>;
+    case 3: return k_ev_electron<R, 3, false>;
+    case 4: return k_ev_electron<R, 4, false>;
+    default: return k_ev_electron<R, 2, false>;
     }
 }
 
@@ -1122,8 +1295,10 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
         return dim3((n + h->block - 1) / h->block);
     };
     const dim3 grid_p = resident(h->ev_wp), grid_e = resident(h->ev_we);
-    auto kp = photon_kernel<R>(h->ev_wp);
-    auto ke = electron_kernel<R>(h->ev_we);
+    const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
+    auto kp = photon_kernel<R>(h->ev_wp, lux);
+    auto ke = electron_kernel<R>(h->ev_we, lux);
+    auto ks = lux ? k_ev_source<R, true> : k_ev_source<R, false>;
     const size_t lds = (size_t)kRing * h->block * sizeof(float);
     const bool photon_src = P.S.charge == 0;
     auto setb = [&](int ic, int ip, int id) {
@@ -1154,7 +1329,7 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
             a.ninject = (int)inject;
             setb(photon_src ? pc : ec, photon_src ? pc : ec, photon_src ? pc : ec);
             EGS_HIP(hipEventRecord(h->kev[1][0], h->stream));
-            hipLaunchKernelGGL(k_ev_source<R>, grid, block, lds, h->stream, a);
+            hipLaunchKernelGGL(ks, grid, block, lds, h->stream, a);
             EGS_HIP(hipGetLastError());
             EGS_HIP(hipEventRecord(h->kev[1][1], h->stream));
             timed_src = true;
@@ -1225,8 +1400,10 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
         return dim3((n + h->block - 1) / h->block);
     };
     const dim3 grid_p = resident(h->ev_wp), grid_e = resident(h->ev_we);
-    auto kp = photon_kernel<R>(h->ev_wp);
-    auto ke = electron_kernel<R>(h->ev_we);
+    const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
+    auto kp = photon_kernel<R>(h->ev_wp, lux);
+    auto ke = electron_kernel<R>(h->ev_we, lux);
+    auto ks = lux ? k_ev_source<R, true> : k_ev_source<R, false>;
     const size_t lds = (size_t)kRing * h->block * sizeof(float);
     const bool photon_src = P.S.charge == 0;
     auto setb = [&](int ic, int ip, int id) {
@@ -1247,7 +1424,7 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
         hipLaunchKernelGGL(k_ev_plan, dim3(1), dim3(64), 0, h->stream, h->ctl, sb,
                            P.bank[sb].cap);
         setb(sb, sb, sb);
-        hipLaunchKernelGGL(k_ev_source<R>, grid, block, lds, h->stream, a);
+        hipLaunchKernelGGL(ks, grid, block, lds, h->stream, a);
         setb(pc, ec, 1 - pc);
         hipLaunchKernelGGL(kp, grid_p, block, lds, h->stream, a);
         hipLaunchKernelGGL(k_ev_ctl, dim3(1), dim3(64), 0, h->stream, h->ctl, pc);
@@ -1312,8 +1489,13 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
             "geometry planes");
     EGS_ARG(src->charge >= -1 && src->charge <= 1, "source charge");
     EGS_ARG(src->spectrum == EGSGPU_SPEC_MONO || ((src->spectrum == EGSGPU_SPEC_HISTOGRAM ||
-            src->spectrum == EGSGPU_SPEC_LINE) && src->nbins > 0 && src->spec_e && src->spec_w),
+            src->spectrum == EGSGPU_SPEC_LINE || src->spectrum == EGSGPU_SPEC_CDFINV) &&
+            src->nbins > 0 && src->spec_e && src->spec_w),
             "source spectrum");
+    EGS_ARG(src->kind >= EGSGPU_SOURCE_PARALLEL_BEAM && src->kind <= EGSGPU_SOURCE_DOSXYZ_BEAM,
+            "source kind");
+    EGS_ARG(src->kind != EGSGPU_SOURCE_DOSXYZ_FRONT || src->uz > 0,
+            "a front-face source must point into the grid (uz > 0)");
     int rc = require_device();
     if (rc) return rc;
     if (tp->spin) {
@@ -1436,15 +1618,21 @@ int egsgpu_set_streams(egsgpu_handle *h, int32_t nstreams, int32_t block_size) {
 }
 
 int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx) {
+    return egsgpu_set_rng(h, EGSGPU_RNG_RANMAR, ixx, jxx);
+}
+
+int egsgpu_set_rng(egsgpu_handle *h, int32_t kind, int32_t a, int32_t b) {
     EGS_ARG(h, "handle");
-    h->ixx = ixx;
-    h->jxx = jxx;
-    if (h->lanes_ready) {
-        hipLaunchKernelGGL(k_seed, dim3((h->nlanes + 255) / 256), dim3(256), 0, h->stream,
-                           h->rng, h->nlanes, h->ixx, h->jxx);
-        EGS_HIP(hipGetLastError());
-        EGS_HIP(hipStreamSynchronize(h->stream));
+    EGS_ARG(kind == EGSGPU_RNG_RANMAR || kind == EGSGPU_RNG_RANLUX, "rng kind");
+    if (kind == EGSGPU_RNG_RANMAR) {
+        h->ixx = a;
+        h->jxx = b;
+    } else {
+        h->lux = a >= 0 && a <= 4 ? a : 1;   // init_ranlux: out of range -> 1
+        h->lux_seed = b;
     }
+    h->rng_kind = kind;
+    if (h->lanes_ready) CK(seed_streams(h));
     return EGSGPU_OK;
 }
 
@@ -1667,14 +1855,16 @@ int egsgpu_run(egsgpu_handle *h, int64_t nhist) {
 }
 
 // ---- checkpoint / restart ----------------------------------------------
-// Layout: "EGSGPUCK", u32 version, u32 nvox, u32 nlanes, u32 pad,
-// i64 nhist, i32 ixx, i32 jxx, f64 sum[nvox], f64 sum2[nvox],
-// RanmarRec rng[nlanes] (streams that have not run yet are stored seeded).
+// Layout: "EGSGPUCK", u32 version, u32 nvox, u32 nlanes, u32 rng kind,
+// i64 nhist, i32 seeds[2] (ixx, jxx or luxury, seed), f64 sum[nvox],
+// f64 sum2[nvox], then the stream records: RanmarRec[nlanes] or
+// RanluxRec[nlanes] (streams that have not run yet are stored seeded).
+// Version 1 (ranmar only, kind word 0) is still read.
 namespace {
-constexpr uint32_t kCkVersion = 1;
-size_t ck_size(const egsgpu_handle *h, int nlanes) {
+constexpr uint32_t kCkVersion = 2;
+size_t ck_size(const egsgpu_handle *h, int nlanes, int kind) {
     return 8 + 4 * 4 + 8 + 2 * 4 + 2 * sizeof(double) * (size_t)h->nvox +
-           sizeof(RanmarRec) * (size_t)nlanes;
+           (kind == EGSGPU_RNG_RANLUX ? sizeof(RanluxRec) : sizeof(RanmarRec)) * (size_t)nlanes;
 }
 }  // namespace
 
@@ -1682,7 +1872,7 @@ int egsgpu_checkpoint(egsgpu_handle *h, void *buf, size_t *size) {
     EGS_ARG(h && size, "handle, size");
     int rc = ensure_lanes(h);
     if (rc) return rc;
-    size_t need = ck_size(h, h->nlanes);
+    size_t need = ck_size(h, h->nlanes, h->rng_kind);
     if (!buf) { *size = need; return EGSGPU_OK; }
     if (*size < need) {
         set_error("checkpoint buffer holds %zu bytes, %zu needed", *size, need);
@@ -1692,16 +1882,19 @@ int egsgpu_checkpoint(egsgpu_handle *h, void *buf, size_t *size) {
     EGS_HIP(hipStreamSynchronize(h->stream));
     char *p = (char *)buf;
     memcpy(p, "EGSGPUCK", 8); p += 8;
-    uint32_t hd[4] = {kCkVersion, (uint32_t)h->nvox, (uint32_t)h->nlanes, 0};
+    const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
+    uint32_t hd[4] = {kCkVersion, (uint32_t)h->nvox, (uint32_t)h->nlanes, (uint32_t)h->rng_kind};
     memcpy(p, hd, sizeof(hd)); p += sizeof(hd);
     memcpy(p, &h->nhist, 8); p += 8;
-    int32_t seeds[2] = {h->ixx, h->jxx};
+    int32_t seeds[2] = {lux ? h->lux : h->ixx, lux ? h->lux_seed : h->jxx};
     memcpy(p, seeds, 8); p += 8;
     EGS_HIP(hipMemcpy(p, h->tally.sum, sizeof(double) * h->nvox, hipMemcpyDeviceToHost));
     p += sizeof(double) * h->nvox;
     EGS_HIP(hipMemcpy(p, h->tally.sum2, sizeof(double) * h->nvox, hipMemcpyDeviceToHost));
     p += sizeof(double) * h->nvox;
-    EGS_HIP(hipMemcpy(p, h->rng, sizeof(RanmarRec) * h->nlanes, hipMemcpyDeviceToHost));
+    size_t rb = 0;
+    void *recs = rng_records(h, &rb);
+    EGS_HIP(hipMemcpy(p, recs, rb, hipMemcpyDeviceToHost));
     *size = need;
     return EGSGPU_OK;
 }
@@ -1715,7 +1908,15 @@ int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size) {
     }
     uint32_t hd[4];
     memcpy(hd, p + 8, sizeof(hd));
-    if (hd[0] != kCkVersion) { set_error("checkpoint version %u", hd[0]); return EGSGPU_E_FORMAT; }
+    if (hd[0] != kCkVersion && hd[0] != 1) {
+        set_error("checkpoint version %u", hd[0]);
+        return EGSGPU_E_FORMAT;
+    }
+    const int kind = hd[0] == 1 ? EGSGPU_RNG_RANMAR : (int)hd[3];
+    if (kind != EGSGPU_RNG_RANMAR && kind != EGSGPU_RNG_RANLUX) {
+        set_error("checkpoint rng kind %d", kind);
+        return EGSGPU_E_FORMAT;
+    }
     if ((int)hd[1] != h->nvox) {
         set_error("checkpoint has %u voxels, the simulation %d", hd[1], h->nvox);
         return EGSGPU_E_ARG;
@@ -1725,20 +1926,31 @@ int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size) {
         return EGSGPU_E_ARG;
     }
     if (!h->lanes_ready) h->nlanes = (int)hd[2];
-    if (size < ck_size(h, (int)hd[2])) { set_error("checkpoint truncated"); return EGSGPU_E_FORMAT; }
+    if (size < ck_size(h, (int)hd[2], kind)) {
+        set_error("checkpoint truncated");
+        return EGSGPU_E_FORMAT;
+    }
     int rc = ensure_lanes(h);
     if (rc) return rc;
     p += 8 + sizeof(hd);
     memcpy(&h->nhist, p, 8); p += 8;
     int32_t seeds[2];
     memcpy(seeds, p, 8); p += 8;
-    h->ixx = seeds[0];
-    h->jxx = seeds[1];
+    h->rng_kind = kind;
+    if (kind == EGSGPU_RNG_RANLUX) {
+        h->lux = seeds[0];
+        h->lux_seed = seeds[1];
+    } else {
+        h->ixx = seeds[0];
+        h->jxx = seeds[1];
+    }
     EGS_HIP(hipMemcpy(h->tally.sum, p, sizeof(double) * h->nvox, hipMemcpyHostToDevice));
     p += sizeof(double) * h->nvox;
     EGS_HIP(hipMemcpy(h->tally.sum2, p, sizeof(double) * h->nvox, hipMemcpyHostToDevice));
     p += sizeof(double) * h->nvox;
-    EGS_HIP(hipMemcpy(h->rng, p, sizeof(RanmarRec) * h->nlanes, hipMemcpyHostToDevice));
+    size_t rb = 0;
+    void *recs = rng_records(h, &rb);
+    EGS_HIP(hipMemcpy(recs, p, rb, hipMemcpyHostToDevice));
     return EGSGPU_OK;
 }
 
@@ -1769,6 +1981,7 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
     size_t nv = (size_t)h->nvox;
     if (!h->rng_saved) {
         CK(alloc(h, (size_t)h->nlanes, &h->rng_saved));
+        CK(alloc(h, (size_t)h->nlanes, &h->rlx_saved));
         DevGrad &G = h->dgrad;
         G.cap = 2048;
         CK(alloc(h, (size_t)G.cap * h->nlanes, &G.ev));
@@ -1780,8 +1993,10 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
         CK(alloc(h, 1, &h->dgrad_dev));
     }
     // forward pass from the current stream states
-    EGS_HIP(hipMemcpyAsync(h->rng_saved, h->rng, sizeof(RanmarRec) * h->nlanes,
-                           hipMemcpyDeviceToDevice, h->stream));
+    size_t rb = 0;
+    void *recs = rng_records(h, &rb);
+    void *saved = h->rng_kind == EGSGPU_RNG_RANLUX ? (void *)h->rlx_saved : (void *)h->rng_saved;
+    EGS_HIP(hipMemcpyAsync(saved, recs, rb, hipMemcpyDeviceToDevice, h->stream));
     CK(egsgpu_reset_tallies(h));
     CK(egsgpu_run(h, nhist));
     std::vector<double> s(nv), w(nv);
@@ -1798,8 +2013,7 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
     EGS_HIP(hipMemcpyAsync((void *)h->dgrad.w, w.data(), sizeof(double) * nv, hipMemcpyHostToDevice, h->stream));
     EGS_HIP(hipMemsetAsync(h->dgrad.g, 0, sizeof(double) * nv, h->stream));
     EGS_HIP(hipMemcpyAsync(h->dgrad_dev, &h->dgrad, sizeof(DevGrad), hipMemcpyHostToDevice, h->stream));
-    EGS_HIP(hipMemcpyAsync(h->rng, h->rng_saved, sizeof(RanmarRec) * h->nlanes,
-                           hipMemcpyDeviceToDevice, h->stream));
+    EGS_HIP(hipMemcpyAsync(recs, saved, rb, hipMemcpyDeviceToDevice, h->stream));
     EGS_HIP(hipEventRecord(h->ev0, h->stream));
     rc = h->fp64 ? launch(h, h->pd, nhist, h->dgrad_dev) : launch(h, h->pf, nhist, h->dgrad_dev);
     if (rc) return rc;

@@ -37,42 +37,45 @@ int require_device() {
 }
 
 // ---- RNG fill ----------------------------------------------------------
+// Streams drawn through the per-lane ring exactly as the transport draws
+// them (ring_topup / ring_next).  The ring is topped up on 16 draws out of
+// every 64 and left alone for the other 48, so that some blocks come from
+// the top-up of a ring running low and others from a ring that ran dry.
 constexpr int kFillBlock = 64;
-constexpr int kRanmarNB = 16;
+
+__device__ __forceinline__ bool fill_topup_turn(int64_t i) { return (i & 63) < 16; }
 
 __global__ void __launch_bounds__(kFillBlock)
 k_ranmar_fill(int ixx, int jxx, int nstreams, int64_t n, RanmarRec *recs,
               double *out) {
-    __shared__ float buf[kRanmarNB * kFillBlock];
+    __shared__ float ring[kRing * kFillBlock];
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nstreams) return;
     int is, js;
     stream_seeds(ixx, jxx, s, is, js);
     RanmarRec *r = recs + s;
     ranmar_seed(r, is, js);
-    float *b = buf + threadIdx.x;
-    int ip = kRanmarNB;
+    float *b = ring + threadIdx.x;
+    int pos = 0, avail = 0;
     for (int64_t i = 0; i < n; i++) {
-        if (ip >= kRanmarNB) { ranmar_refill<kRanmarNB>(r, b, kFillBlock); ip = 0; }
-        out[(int64_t)s * n + i] = (double)b[ip * kFillBlock];
-        ip++;
+        if (fill_topup_turn(i)) ring_topup<false>(r, b, kFillBlock, pos, avail);
+        out[(int64_t)s * n + i] = (double)ring_next<false>(r, b, kFillBlock, pos, avail);
     }
 }
 
 __global__ void __launch_bounds__(kFillBlock)
 k_ranlux_fill(int lux, int seed, int nstreams, int64_t n, RanluxRec *recs,
               double *out) {
-    __shared__ float buf[24 * kFillBlock];
+    __shared__ float ring[kRing * kFillBlock];
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nstreams) return;
     RanluxRec *r = recs + s;
     ranlux_seed(r, lux, seed + s);
-    float *b = buf + threadIdx.x;
-    int ip = 24;
+    float *b = ring + threadIdx.x;
+    int pos = 0, avail = 0;
     for (int64_t i = 0; i < n; i++) {
-        if (ip >= 24) { ranlux_refill(r, b, kFillBlock); ip = 0; }
-        out[(int64_t)s * n + i] = (double)b[ip * kFillBlock];
-        ip++;
+        if (fill_topup_turn(i)) ring_topup<true>(r, b, kFillBlock, pos, avail);
+        out[(int64_t)s * n + i] = (double)ring_next<true>(r, b, kFillBlock, pos, avail);
     }
 }
 

@@ -16,6 +16,10 @@
 
 namespace egsgpu {
 
+constexpr int kRNB = 16;          // ranmar outputs per refill
+constexpr int kRing = 32;         // LDS ring of random numbers per lane
+constexpr int kRingLow = 12;      // top the ring up below this many
+
 // Layout of one ranmar stream record (int32 words).
 struct RanmarRec {
     int u[97];
@@ -151,10 +155,12 @@ __device__ inline void ranlux_seed(RanluxRec *r, int luxury, int seed) {
 }
 
 // Steps k0..k0+23 of the subtract-with-borrow recurrence on the rotated
-// registers s[]; OUT selects whether outputs are written.
+// registers s[]; OUT selects whether outputs are written, output k going
+// to buf[((start + k) & mask) * stride].
 template <bool OUT>
 __device__ inline void ranlux_block(int (&s)[24], int &carry,
-                                    float *__restrict__ buf, int stride) {
+                                    float *__restrict__ buf, int stride,
+                                    int start = 0, int mask = -1) {
     const float twom24 = 1.0f / 16777216.0f;
 #pragma unroll
     for (int k = 0; k < 24; k++) {
@@ -171,7 +177,7 @@ __device__ inline void ranlux_block(int (&s)[24], int &carry,
                 float b = (float)s[(k + 15) % 24] * twom24 * twom24;
                 v = __fadd_rn(a, b);
             }
-            buf[k * stride] = v;
+            buf[((start + k) & mask) * stride] = v;
         }
     }
 }
@@ -194,14 +200,16 @@ __device__ inline void ranlux_tail(int (&s)[24], int &carry) {
 }
 
 // ranlux(rng_array): 24 outputs then nskip = {0,24,73,199,365}[lux]
-// further steps (ranlux.mortran).  nskip = 24*q + rem.
+// further steps (ranlux.mortran).  nskip = 24*q + rem.  Output k goes to
+// buf[((start + k) & mask) * stride].
 __device__ inline void ranlux_refill(RanluxRec *__restrict__ r,
-                                     float *__restrict__ buf, int stride) {
+                                     float *__restrict__ buf, int stride,
+                                     int start = 0, int mask = -1) {
     int s[24];
 #pragma unroll
     for (int k = 0; k < 24; k++) s[k] = r->w[k];
     int carry = r->carry;
-    ranlux_block<true>(s, carry, buf, stride);
+    ranlux_block<true>(s, carry, buf, stride, start, mask);
     switch (r->lux) {
         case 0: break;
         case 1: ranlux_block<false>(s, carry, buf, stride); break;
@@ -221,6 +229,65 @@ __device__ inline void ranlux_refill(RanluxRec *__restrict__ r,
 #pragma unroll
     for (int k = 0; k < 24; k++) r->w[k] = s[k];
     r->carry = carry;
+}
+
+// ranlux_refill as a called function, for the rare refill from inside a
+// sampling loop (a ring that ran dry): one copy of the unrolled block code
+// per kernel instead of one per draw site.  Only the ranlux builds of the
+// kernels contain the call.
+__device__ __noinline__ void ranlux_refill_call(RanluxRec *r, float *buf, int stride, int start,
+                                                int mask) {
+    ranlux_refill(r, buf, stride, start, mask);
+}
+
+// ---- per-lane ring -------------------------------------------------------
+// A stream's next numbers wait in a ring of kRing floats per lane in LDS
+// (ring[i * stride], positions pos .. pos + avail - 1 mod kRing), the
+// device analogue of rng_array.  ring_topup is called once per loop turn
+// with the wave converged: when any lane runs low, the lanes with room add
+// a block (16 ranmar or 24 ranlux outputs).  ring_next takes the next
+// number; a lane whose ring ran dry inside a sampling loop takes it
+// straight from its record (ranmar: one output, inline; ranlux: a whole
+// block, through ranlux_refill_call).  Either way the lane consumes its stream in
+// order, whenever the refills happen.
+template <bool LUX>
+struct RingRec;
+template <>
+struct RingRec<false> { using type = RanmarRec; };
+template <>
+struct RingRec<true> { using type = RanluxRec; };
+
+template <bool LUX>
+__device__ __forceinline__ void ring_topup(typename RingRec<LUX>::type *rec, float *ring,
+                                           int stride, int pos, int &avail) {
+    if (__any(avail < kRingLow)) {
+        if constexpr (LUX) {
+            if (avail <= kRing - 24) {
+                ranlux_refill(rec, ring, stride, pos + avail, kRing - 1);
+                avail += 24;
+            }
+        } else {
+            if (avail <= kRing - kRNB) {
+                ranmar_refill<kRNB>(rec, ring, stride, pos + avail, kRing - 1);
+                avail += kRNB;
+            }
+        }
+    }
+}
+
+template <bool LUX>
+__device__ __forceinline__ float ring_next(typename RingRec<LUX>::type *rec, float *ring,
+                                           int stride, int &pos, int &avail) {
+    if (avail == 0) {
+        if constexpr (LUX) {
+            ranlux_refill_call(rec, ring, stride, pos, kRing - 1);
+            avail = 24;
+        } else {
+            return ranmar_next(rec);
+        }
+    }
+    avail--;
+    return ring[((pos++) & (kRing - 1)) * stride];
 }
 
 }  // namespace egsgpu

@@ -34,7 +34,8 @@ __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 // EV = false: the history schedule (a lane owns its histories; per-lane
 // tally hash).  EV = true: the event schedule (particles of one history
 // may be spread over lanes and kernels; per-slot shared tally tables).
-template <typename R, bool EV = false>
+// LUX = false: ranmar streams; true: ranlux streams.
+template <typename R, bool EV = false, bool LUX = false>
 struct Shower {
     const DevTables<R> &T;
     const GeomView<R> *Gp;   // geometry of the current particle's simulation
@@ -62,7 +63,7 @@ struct Shower {
     int slot = -1;
     // counters
     // (32-bit per lane and launch; added into 64-bit totals at the end)
-    unsigned c_psteps, c_cross, c_dep, c_flush, c_esteps, c_ovf, c_hovf;
+    unsigned c_psteps, c_cross, c_dep, c_flush, c_esteps, c_ovf, c_hovf, c_miss;
     unsigned hsteps;   // steps of the current history (runaway guard)
     // density-gradient replay log (null when not replaying): one 8-byte
     // record per event, written with one store
@@ -145,28 +146,21 @@ struct Shower {
         A = B = C = 0; cosphi = 1; sinphi = 0;
         hused = 0;
         cvox = -1; cval = 0;
-        c_psteps = c_cross = c_dep = c_flush = c_esteps = c_ovf = c_hovf = 0;
+        c_psteps = c_cross = c_dep = c_flush = c_esteps = c_ovf = c_hovf = c_miss = 0;
         hsteps = 0;
     }
 
-    // Called once per loop iteration of a kernel: when any lane of the
-    // wave runs low, the lanes with room generate kRNB more numbers
-    // together, so that the refill runs with the wave converged.
-    __device__ __forceinline__ void rtopup() {
-        if (__any(ravail < kRingLow)) {
-            if (ravail <= kRing - kRNB) {
-                ranmar_refill<kRNB>(L.rng + lane, rbuf, blockDim.x, rpos + ravail, kRing - 1);
-                ravail += kRNB;
-            }
-        }
+    // RNG ring (ring_topup / ring_next): rtopup once per loop turn with the
+    // wave converged, rnd for every number
+    __device__ __forceinline__ typename RingRec<LUX>::type *rrec() const {
+        if constexpr (LUX) return L.rlx + lane;
+        else return L.rng + lane;
     }
-    // The ring holds the stream's next ravail numbers; when a long
-    // rejection loop empties it, numbers come one at a time from the
-    // stream record (inline and call-free, so no registers are saved).
+    __device__ __forceinline__ void rtopup() {
+        ring_topup<LUX>(rrec(), rbuf, blockDim.x, rpos, ravail);
+    }
     __device__ __forceinline__ R rnd() {
-        if (ravail == 0) return (R)ranmar_next(L.rng + lane);
-        ravail--;
-        return (R)rbuf[((rpos++) & (kRing - 1)) * blockDim.x];
+        return (R)ring_next<LUX>(rrec(), rbuf, blockDim.x, rpos, ravail);
     }
 
     // ---------------------------------------------------------- scoring

@@ -71,6 +71,11 @@ class Simulation:
     def seed(self, ixx, jxx):
         check(_lib.lib().egsgpu_seed(self.h, ixx, jxx))
 
+    def set_ranlux(self, luxury, seed):
+        """Drive the transport with ranlux streams: stream s is
+        init_ranlux(luxury, seed + s)."""
+        check(_lib.lib().egsgpu_set_rng(self.h, _lib.RNG_RANLUX, luxury, seed))
+
     def run(self, nhist):
         check(_lib.lib().egsgpu_run(self.h, int(nhist)))
 

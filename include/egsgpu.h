@@ -61,6 +61,18 @@ typedef struct EgsXYZGeometry {
 #define EGSGPU_SPEC_MONO       0   /* EGS_MonoEnergy                       */
 #define EGSGPU_SPEC_HISTOGRAM  1   /* tabulated spectrum type 1 (per MeV)  */
 #define EGSGPU_SPEC_LINE       2   /* tabulated spectrum type 0 (lines)    */
+#define EGSGPU_SPEC_CDFINV     3   /* DOSXYZnrc inverse-CDF grid (enflag 1):
+                                      nbins grid points, spec_e = CDFINV(K,1),
+                                      spec_w = CDFINV(K,2) (srcxyznrc.mortran
+                                      :1874-1899, sampled as :3440-3446)   */
+#define EGSGPU_SOURCE_PARALLEL_BEAM 0 /* egs_parallel_beam                   */
+#define EGSGPU_SOURCE_DOSXYZ_FRONT  1 /* DOSXYZnrc isource 0: x0..x1, y0..y1
+                                         on the front face (clamped to it),
+                                         direction ux, uy, uz               */
+#define EGSGPU_SOURCE_DOSXYZ_BEAM   2 /* DOSXYZnrc isource 1: collimator
+                                         xcol x ycol at dsource (< 0) from
+                                         iso, rotation rot; misses are
+                                         resampled and counted             */
 #define EGSGPU_SHAPE_POINT     0   /* EGS_PointShape                       */
 #define EGSGPU_SHAPE_RECTANGLE 1   /* EGS_RectangleShape (z = shape_z)     */
 typedef struct EgsSource {
@@ -73,6 +85,10 @@ typedef struct EgsSource {
     int32_t shape;               /* EGSGPU_SHAPE_*                         */
     double x0, x1, y0, y1, z;    /* point = (x0,y0,z); rect [x0,x1]x[y0,y1]*/
     double ux, uy, uz;           /* beam direction (normalised)            */
+    int32_t kind;                /* EGSGPU_SOURCE_* (0 = egs_parallel_beam) */
+    double rot[9];               /* DOSXYZ_BEAM: r_11, r_12, ..., r_33     */
+    double iso[3];               /* DOSXYZ_BEAM: isocentre                 */
+    double xcol, ycol, dsource;  /* DOSXYZ_BEAM: widths, source distance   */
 } EgsSource;
 
 /* ---- transport parameters (":start MC transport parameter:",
@@ -211,8 +227,23 @@ int egsgpu_set_event_graph(egsgpu_handle *h, int32_t on);
 int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice);
 
 /* Seed all streams: ranmar initial seeds (ixx, jxx).  Re-seeding resets
- * the streams; tallies are kept. */
+ * the streams; tallies are kept.  Same as
+ * egsgpu_set_rng(h, EGSGPU_RNG_RANMAR, ixx, jxx). */
 int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx);
+
+/* Choose the generator that drives the transport and seed all streams
+ * (any time; tallies are kept):
+ *  EGSGPU_RNG_RANMAR (a = ixx, b = jxx): stream s is EGS_Ranmar seeded as
+ *    createRNG(input, s) (egs++/egs_rndm.cpp:409-463), the default;
+ *  EGSGPU_RNG_RANLUX (a = luxury level 0..4, out of range -> 1, b = seed):
+ *    stream s is init_ranlux(luxury, seed + s) (src/ranlux.mortran), the
+ *    seeding of parallel job s + 1 of DOSXYZnrc (dosxyznrc.mortran:1916),
+ *    consumed in blocks of 24 as ranlux(rng_array) / $RANDOMSET do
+ *    (src/ranlux.macros:86-120).
+ * Every stream is consumed in order by the histories it runs. */
+#define EGSGPU_RNG_RANMAR 0
+#define EGSGPU_RNG_RANLUX 1
+int egsgpu_set_rng(egsgpu_handle *h, int32_t kind, int32_t a, int32_t b);
 
 /* Simulate nhist further primary histories (runSimulation,
  * egs_application.cpp:839-917).  Histories are dealt round-robin to the
@@ -264,7 +295,8 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
 
 /* Engine counters: [0] photon steps, [1] voxel crossings, [2] deposits,
  * [3] history flushes (voxel,history pairs), [4] electron steps,
- * [5] stack overflows, [6] tally-table overflows, [7] reserved. */
+ * [5] stack overflows, [6] tally-table overflows, [7] primaries that
+ * missed the grid and were sampled again (DOSXYZnrc isource 1, nmissm). */
 int egsgpu_get_counters(egsgpu_handle *h, int64_t counters[8]);
 
 /* The same counters split by the kernel that counted them (kernel numbers
@@ -286,7 +318,9 @@ int egsgpu_kernel_time(egsgpu_handle *h, int32_t kernel, double *ms,
 /* ---- test hooks -------------------------------------------------------- */
 /* Fill out[s*n + i] with the first n numbers of stream s of ranmar seeded
  * (ixx, jxx + s), s < nstreams, computed on the device
- * (EGS_Ranmar::setState/fillArray, egs_rndm.cpp:303-398). */
+ * (EGS_Ranmar::setState/fillArray, egs_rndm.cpp:303-398), drawn through
+ * the per-lane ring the transport uses, with its top-ups and its path for
+ * a ring that ran dry. */
 int egsgpu_ranmar_fill(int32_t ixx, int32_t jxx, int32_t nstreams,
                        int64_t n, double *out);
 

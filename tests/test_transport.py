@@ -85,8 +85,8 @@ def test_kerma_dose_parity(engine, fp64, schedule):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("fp64", [0, 1])
-def test_kerma_voting_kernel_runs_the_plain_histories(engine, fp64):
+@pytest.mark.parametrize("fp64,rng", [(0, "ranmar"), (1, "ranmar"), (0, "ranlux")])
+def test_kerma_voting_kernel_runs_the_plain_histories(engine, fp64, rng):
     """Within one launch the kerma-mode history kernel with wave voting runs
     every lane's histories with the same numbers as the plain per-lane
     loop: identical event counters, and dose sums equal up to the order of
@@ -99,6 +99,8 @@ def test_kerma_voting_kernel_runs_the_plain_histories(engine, fp64):
     for schedule in ("history", "history-plain"):
         sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192,
                          schedule=schedule)
+        if rng == "ranlux":
+            sim.set_ranlux(4, 1)
         sim.run(200001)
         out.append((sim.tallies(), sim.counters()))
         sim.close()
@@ -192,16 +194,27 @@ def photon_coupled_case(n=16, dx=1.0):
     return xb, xb.copy(), zb, med, None, src, tp
 
 
+_ORACLE_RUNS = {}
+
+
+def _oracle_photon_coupled(n):
+    """The oracle's tallies for photon_coupled_case (one run per n)."""
+    if n not in _ORACLE_RUNS:
+        xb, yb, zb, med, rhor, src, tp = photon_coupled_case()
+        g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
+        o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
+        o.run(n)
+        _ORACLE_RUNS[n] = o.tallies()
+    return _ORACLE_RUNS[n]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("schedule", ["history", "event", "event-host"])
 def test_photon_coupled_dose_parity(engine, schedule):
     """event: iterations replayed from a HIP graph (the default);
     event-host: the same kernels launched one by one from the host."""
     xb, yb, zb, med, rhor, src, tp = photon_coupled_case()
-    g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
-    o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
-    o.run(400000)
-    a = o.tallies()
+    a = _oracle_photon_coupled(400000)
     sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192,
                      schedule=schedule.split("-")[0])
     if schedule == "event-host":
@@ -213,7 +226,24 @@ def test_photon_coupled_dose_parity(engine, schedule):
 
 
 @pytest.mark.gpu
-def test_checkpoint_restart_continues_the_run(engine):
+@pytest.mark.parametrize("schedule,lux", [("history", 1), ("event", 1), ("event", 4)])
+def test_ranlux_driven_transport_parity(engine, schedule, lux):
+    """Coupled photon transport driven by ranlux streams (DOSXYZnrc's
+    generator) agrees with the oracle's ranmar run."""
+    xb, yb, zb, med, rhor, src, tp = photon_coupled_case()
+    a = _oracle_photon_coupled(400000)
+    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192,
+                     schedule=schedule)
+    sim.set_ranlux(lux, 33)
+    sim.run(400000)
+    b = sim.tallies()
+    assert b[2] == 400000
+    check_parity(a, b, sim.counters(), 100, "ranlux %d %s" % (lux, schedule))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rng", ["ranmar", "ranlux"])
+def test_checkpoint_restart_continues_the_run(engine, rng):
     """A run restored from a checkpoint continues with the same random
     numbers: checkpoint after N1 histories, then N2 more, equals N1 + N2
     in one go (history schedule; FP64 atomics make the sums exact only up
@@ -221,6 +251,8 @@ def test_checkpoint_restart_continues_the_run(engine):
     xb, yb, zb, med, rhor, src, tp = kerma_case()
     tab = Tables(BLOB521)
     a = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=4096, schedule="history")
+    if rng == "ranlux":
+        a.set_ranlux(2, 77)
     a.run(20000)
     ck = a.checkpoint()
     a.run(30000)
