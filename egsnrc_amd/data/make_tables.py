@@ -18,7 +18,8 @@ BLOBS = {
     "521icru.egstab": ("521icru.pegs4dat",
                        ["H2O521ICRU", "LUNG521ICRU", "ICRPBONE521ICRU",
                         "AIR521ICRU", "AL521ICRU"]),
-    "700icru.egstab": ("700icru.pegs4dat", ["H2O700ICRU"]),
+    "700icru.egstab": ("700icru.pegs4dat",
+                       ["H2O700ICRU", "ICRPBONE700ICRU", "AIR700ICRU", "LUNG700ICRU"]),
 }
 
 

@@ -16,9 +16,15 @@
 //   rng definition        ranmar initial seeds (egs_rndm.cpp:409-463)
 //   ausgab object         egs_dose_scoring .3ddose output
 //                         (egs_dose_scoring.cpp:416-479)
+// or, for a DOSXYZnrc input file (recognised by its records), the
+// DOSXYZnrc front end: phantom records or an .egsphant CT phantom,
+// isource 0 and 1 with monoenergetic or .spectrum energies, NCASE and the
+// ranmar seeds of record 13, the EGSnrc transport parameter block, and the
+// .3ddose of DOSXYZnrc normalised per incident fluence
+// (dosxyznrc.mortran:2997-3002, 3100-3130).
 // Transport runs through the C ABI of include/egsgpu.h.
 //
-// usage: egs_gpu_app -i input.egsinp -p tables.egstab [-o output]
+// usage: egs_gpu_app -i input.egsinp -p tables.egstab [-o output] [--parse-only]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -32,6 +38,7 @@
 #include <vector>
 
 #include "../../include/egsgpu.h"
+#include "dosxyz_input.h"
 #include "egs_input.h"
 #include "egs_phantom.h"
 
@@ -84,6 +91,11 @@ class EGS_GPUApplication {
     int ixx = 1802, jxx = 9373;
     double area = 0;
     std::string dose_file;
+    // DOSXYZnrc front end
+    bool dosxyz = false;
+    bool parse_only = false;
+    int zeroairdose = 0;
+    DosxyzInput dz;
 
     ~EGS_GPUApplication() {
         if (h) egsgpu_destroy(h);
@@ -100,10 +112,12 @@ class EGS_GPUApplication {
             if (a == "-i") input_file = next();
             else if (a == "-p") tables_file = next();
             else if (a == "-o") output = next();
+            else if (a == "--parse-only") parse_only = true;
             else throw std::runtime_error("unknown argument " + a);
         }
-        if (input_file.empty() || tables_file.empty())
-            throw std::runtime_error("usage: egs_gpu_app -i input.egsinp -p tables.egstab [-o output]");
+        if (input_file.empty() || (tables_file.empty() && !parse_only))
+            throw std::runtime_error(
+                "usage: egs_gpu_app -i input.egsinp -p tables.egstab [-o output] [--parse-only]");
         if (output.empty()) {
             output = input_file;
             size_t d = output.rfind(".egsinp");
@@ -344,8 +358,12 @@ class EGS_GPUApplication {
             else if (InputItem::same(v, "norej")) tp.bound_compton = 3;
             else throw std::runtime_error("Bound Compton scattering = " + v + " not supported");
         }
-        if (!t->get("Boundary crossing algorithm", v) && !InputItem::same(v, "Exact"))
-            throw std::runtime_error("only the Exact boundary crossing algorithm is supported");
+        if (!t->get("Boundary crossing algorithm", v) && !InputItem::same(v, "Exact")) {
+            if (!dosxyz)
+                throw std::runtime_error("only the Exact boundary crossing algorithm is supported");
+            fprintf(stderr, "note: Boundary crossing algorithm = %s requested; the engine "
+                            "implements EXACT only and runs with it\n", v.c_str());
+        }
         if (!t->get("Electron-step algorithm", v) && !InputItem::same(v, "PRESTA-II"))
             throw std::runtime_error("only PRESTA-II is supported");
         if (!t->get("Brems angular sampling", v)) tp.brems_angle = InputItem::same(v, "Simple") ? 0 : 1;
@@ -358,6 +376,163 @@ class EGS_GPUApplication {
         int kerma = 0;
         if (!t->get("kerma approximation", v)) kerma = InputItem::same(v, "yes") || InputItem::same(v, "on");
         tp.kerma = kerma;
+    }
+
+    // ---- DOSXYZnrc front end ---------------------------------------------
+    void init_dosxyz() {
+        dosxyz = true;
+        dz = read_dosxyz_input(input_file);
+        if (dz.nmed == 0) {   // CT phantom (dosxyznrc.mortran:1533-1640)
+            EgsPhantom ph = read_egsphant(resolve(dz.phantom_file));
+            xp.p = ph.xb; yp.p = ph.yb; zp.p = ph.zb;
+            media = ph.media;
+            medium.assign(ph.medium.begin(), ph.medium.end());
+            rhor.assign(medium.size(), 1.0);
+            phantom_density = std::move(ph.density);
+            zeroairdose = dz.zeroairdose;
+        } else {
+            xp.p = dz.xb; yp.p = dz.yb; zp.p = dz.zb;
+            media = dz.media;
+            medium.resize(dz.med.size());
+            rhor.assign(dz.med.size(), 1.0);
+            phantom_density.assign(dz.med.size(), 0.0);
+            for (size_t i = 0; i < dz.med.size(); i++) {
+                if (dz.med[i] < 1)
+                    throw std::runtime_error("vacuum voxels (medium 0) are not supported");
+                medium[i] = dz.med[i] - 1;
+                phantom_density[i] = dz.rho[i];   // 0: the medium's default density
+            }
+        }
+        init_dosxyz_source();
+        input = parse_input(dz.transport_text);
+        init_transport();
+        // ECUTIN / PCUTIN win when above the EGSnrc parameters (record 4)
+        tp.ecut = std::max(tp.ecut, dz.ecutin);
+        tp.pcut = std::max(tp.pcut, dz.pcutin);
+        ncase = dz.ncase;
+        if (ncase <= 0) throw std::runtime_error("NCASE must be > 0");
+        nbatch = 10;   // $NBATCH
+        // seeds (dosxyznrc.mortran:1905-1916; DOSXYZnrc is built with ranmar)
+        ixx = dz.inseed1;
+        jxx = dz.inseed2;
+        if (ixx <= 0 || ixx > 31328) ixx = 1802;
+        if (jxx <= 0 || jxx > 30081) jxx = 9373;
+        if (dz.ihowfarless)
+            fprintf(stderr, "note: HOWFARLESS requested; every voxel boundary is tracked\n");
+        if (dz.ireject)
+            fprintf(stderr, "note: charged-particle range rejection requested; not applied\n");
+        if (dz.n_split > 1)
+            fprintf(stderr, "note: photon splitting requested; histories are not split\n");
+        if (dz.i_phsp_out)
+            fprintf(stderr, "note: phase-space output requested; not written\n");
+        if (dz.irestart)
+            fprintf(stderr, "note: IRESTART = %d requested; the run starts afresh\n", dz.irestart);
+        dose_file = output + ".3ddose";
+    }
+
+    // srcinit for isource 0 and 1 (srcxyznrc.mortran:2442-2539, 2771-2789)
+    void init_dosxyz_source() {
+        const double deg = 3.141593 / 180.;   // as in srcinit
+        src = EgsSource{};
+        src.charge = dz.iqin;
+        const int nx = (int)xp.p.size() - 1, ny = (int)yp.p.size() - 1, nz = (int)zp.p.size() - 1;
+        const std::vector<double> &s = dz.src;
+        if (dz.isource == 0) {
+            double xinl = s[0], xinu = s[1], yinl = s[2], yinu = s[3];
+            double thetax = s[4], thetay = s[5], thetaz = s[6];
+            if (xinl < xp.p[0]) xinl = xp.p[0];
+            if (xinu <= xinl) xinu = xinl;
+            if (xinu > xp.p[nx]) xinu = xp.p[nx];
+            if (xinl > xp.p[nx]) xinl = xp.p[nx];
+            if (yinl < yp.p[0]) yinl = yp.p[0];
+            if (yinu <= yinl) yinu = yinl;
+            if (yinu > yp.p[ny]) yinu = yp.p[ny];
+            if (yinl > yp.p[ny]) yinl = yp.p[ny];
+            double winc = std::cos(thetaz * deg), uinc = std::cos(thetax * deg),
+                   vinc = std::cos(thetay * deg);
+            if (thetaz > 90. || thetaz < -90. || thetax > 180. || thetax < 0. || thetay > 180. ||
+                thetay < 0.)
+                throw std::runtime_error("source 0: the angles do not point into the front face");
+            double ss = uinc * uinc + vinc * vinc + winc * winc;
+            if (ss > 1.001 || ss < 0.009)
+                throw std::runtime_error("source 0: direction cosines must have unit sum of squares");
+            winc = std::sqrt(1 - uinc * uinc - vinc * vinc);
+            src.kind = EGSGPU_SOURCE_DOSXYZ_FRONT;
+            src.x0 = xinl; src.x1 = xinu; src.y0 = yinl; src.y1 = yinu; src.z = zp.p[0];
+            src.ux = uinc; src.uy = vinc; src.uz = winc;
+            area = (xinu - xinl) * (yinu - yinl);
+        } else {
+            double xiso = s[0], yiso = s[1], ziso = s[2], theta = s[3], phi = s[4];
+            double xcol = s[5], ycol = s[6], phicol = s[7];
+            double ct = std::cos(theta * deg), st = std::sin(theta * deg);
+            double cc = std::cos(phicol * deg), sc = std::sin(phicol * deg);
+            double cp = std::cos(phi * deg), sp = std::sin(phi * deg);
+            double r[9] = { ct * cp * cc + sp * sc, -ct * cp * sc + sp * cc, -st * cp,
+                            ct * sp * cc - cp * sc, -ct * sp * sc - cp * cc, -st * sp,
+                           -st * cc,                st * sc,                -ct};
+            double d = 0;
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++)
+                    for (int c = 0; c < 2; c++) {
+                        double dx = xiso - xp.p[a ? nx : 0], dy = yiso - yp.p[b ? ny : 0],
+                               dzz = ziso - zp.p[c ? nz : 0];
+                        d = std::max(d, dx * dx + dy * dy + dzz * dzz);
+                    }
+            src.kind = EGSGPU_SOURCE_DOSXYZ_BEAM;
+            for (int k = 0; k < 9; k++) src.rot[k] = r[k];
+            src.iso[0] = xiso; src.iso[1] = yiso; src.iso[2] = ziso;
+            src.xcol = xcol; src.ycol = ycol;
+            src.dsource = -std::sqrt(d + xcol * xcol + ycol * ycol);
+            src.ux = r[2]; src.uy = r[5]; src.uz = r[8];
+            area = xcol * ycol;
+        }
+        if (dz.enflag == 0) {
+            src.spectrum = EGSGPU_SPEC_MONO;
+            src.energy = dz.ein;
+        } else {
+            dosxyz_spectrum_cdfinv(resolve(dz.spectrum_file), 1000, spec_e, spec_w, nullptr);
+            src.spectrum = EGSGPU_SPEC_CDFINV;
+            src.nbins = (int)spec_e.size();
+            src.spec_e = spec_e.data();
+            src.spec_w = spec_w.data();
+        }
+    }
+
+    void print_parsed() const {
+        int nx = (int)xp.p.size() - 1, ny = (int)yp.p.size() - 1, nz = (int)zp.p.size() - 1;
+        printf("format: %s\n", dosxyz ? "dosxyznrc" : "egs++");
+        printf("grid: %d %d %d\n", nx, ny, nz);
+        printf("x: %.6g %.6g\ny: %.6g %.6g\nz: %.6g %.6g\n", xp.p[0], xp.p[nx], yp.p[0],
+               yp.p[ny], zp.p[0], zp.p[nz]);
+        printf("media:");
+        for (auto &m : media) printf(" %s", m.c_str());
+        printf("\n");
+        std::vector<long> count(media.size(), 0);
+        for (int m : medium) if (m >= 0 && m < (int)count.size()) count[m]++;
+        printf("voxels per medium:");
+        for (long c : count) printf(" %ld", c);
+        printf("\n");
+        long ndens = 0;
+        for (double d : phantom_density) ndens += d > 0;
+        printf("voxels with a density: %ld\n", ndens);
+        printf("source: kind %d charge %d spectrum %d energy %.6g area %.6g\n", src.kind,
+               src.charge, src.spectrum, src.energy, area);
+        printf("source rectangle: %.6g %.6g %.6g %.6g z %.6g\n", src.x0, src.x1, src.y0, src.y1,
+               src.z);
+        printf("direction: %.9g %.9g %.9g\n", src.ux, src.uy, src.uz);
+        printf("dsource: %.9g\n", src.dsource);
+        if (src.spectrum == EGSGPU_SPEC_CDFINV && !spec_e.empty()) {
+            size_t n = spec_e.size(), first = 0;
+            while (first < n && spec_e[first] == spec_e[0]) first++;
+            printf("cdfinv: %zu %zu %.9g %.9g %.9g %.9g\n", n, first, spec_e[0], spec_w[0],
+                   spec_e[n - 1], spec_w[n - 1]);
+        }
+        printf("ncase: %lld\nseeds: %d %d\n", (long long)ncase, ixx, jxx);
+        printf("ecut: %.6g\npcut: %.6g\n", tp.ecut, tp.pcut);
+        printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
+               "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g\n",
+               tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
+               tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth);
     }
 
     void init_run() {
@@ -378,11 +553,19 @@ class EGS_GPUApplication {
 
     // EGS_AdvancedApplication::initSimulation analogue
     int initSimulation() {
-        input = parse_input_file(input_file);
-        init_geometry();
-        init_source();
-        init_transport();
-        init_run();
+        if (is_dosxyz_input(input_file)) {
+            init_dosxyz();
+        } else {
+            input = parse_input_file(input_file);
+            init_geometry();
+            init_source();
+            init_transport();
+            init_run();
+        }
+        if (parse_only) {
+            print_parsed();
+            return 1;
+        }
         check(egsgpu_tables_load(tables_file.c_str(), &tab), "loading tables");
         // map geometry media names to table indices (egsAddMedium)
         int32_t ntab = 0;
@@ -400,9 +583,9 @@ class EGS_GPUApplication {
         }
         std::vector<int32_t> tmed(medium.size());
         for (size_t i = 0; i < medium.size(); i++) tmed[i] = map[medium[i]];
-        if (!phantom_density.empty())   // .egsphant without a CT ramp
+        if (!phantom_density.empty())   // mass densities (.egsphant, DOSXYZnrc records)
             for (size_t i = 0; i < medium.size(); i++)
-                rhor[i] = phantom_density[i] / mdens[medium[i]];
+                if (phantom_density[i] > 0) rhor[i] = phantom_density[i] / mdens[medium[i]];
         EgsXYZGeometry g{};
         g.nx = (int)xp.p.size() - 1; g.ny = (int)yp.p.size() - 1; g.nz = (int)zp.p.size() - 1;
         g.xb = xp.p.data(); g.yb = yp.p.data(); g.zb = zp.p.data();
@@ -411,6 +594,11 @@ class EGS_GPUApplication {
         g.boundary_tolerance = 0;
         if (tp.spin) {
             fprintf(stderr, "note: spin effects requested but the spin data are not available; running with Spin effects = Off\n");
+            tp.spin = 0;
+        }
+        if (dosxyz && tp.spin) {
+            fprintf(stderr, "note: Spin effects = On requested; the spin data are not available, "
+                            "running with Spin effects = Off\n");
             tp.spin = 0;
         }
         check(egsgpu_create(tab, &g, &src, &tp, &h), "creating the simulation");
@@ -443,6 +631,17 @@ class EGS_GPUApplication {
         check(egsgpu_get_tallies(h, s.data(), s2.data(), &n), "tallies");
         double normE = area > 0 ? area : 1.0;   // per fluence for beams with an area
         double normD = 1.602e-10 * normE;
+        // DOSXYZnrc: incident fluence counts the beam particles that missed
+        // the phantom (nmissm, dosxyznrc.mortran:2998-3001); they are
+        // histories with nothing scored
+        int64_t nmiss = 0;
+        if (dosxyz) {
+            int64_t c[8];
+            check(egsgpu_get_counters(h, c), "counters");
+            nmiss = c[7];
+        }
+        const int64_t nhist = n;
+        n += nmiss;
         std::ofstream f(dose_file);
         f << nx << " " << ny << " " << nz << "\n";
         for (auto *P : {&xp, &yp, &zp}) {
@@ -460,13 +659,19 @@ class EGS_GPUApplication {
             double mass = vol * rhor[i] * medium_density(medium[i]);
             dose[i] = (float)(r * normD / mass);
             unc[i] = (float)(r > 0 ? dr / r : 1.0);
+            if (zeroairdose && rhor[i] * medium_density(medium[i]) <= 0.044) {
+                dose[i] = 0;   // zeroairdose (dosxyznrc.mortran:3395-3405)
+                unc[i] = 1.0;
+            }
             etot += r;
         }
         for (float d : dose) f << d << " ";
         f << "\n";
         for (float u : unc) f << u << " ";
         f << "\n";
-        printf("\n  histories: %lld   energy deposited per history: %.6g MeV\n", (long long)n, etot);
+        printf("\n  histories: %lld   energy deposited per history: %.6g MeV\n", (long long)nhist,
+               etot);
+        if (nmiss) printf("  beam particles that missed the phantom: %lld\n", (long long)nmiss);
         printf("  dose written to %s\n", dose_file.c_str());
         return 0;
     }
@@ -480,7 +685,7 @@ int main(int argc, char **argv) {
     try {
         EGS_GPUApplication app;
         app.parse_args(argc, argv);
-        app.initSimulation();
+        if (app.initSimulation()) return 0;   // --parse-only
         app.runSimulation();
         app.finishSimulation();
     } catch (const std::exception &e) {
