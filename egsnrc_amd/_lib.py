@@ -42,7 +42,7 @@ class EgsTransport(C.Structure):
                 ("brems_angle", C.c_int32), ("spin", C.c_int32),
                 ("exact_bca", C.c_int32), ("estepe", C.c_double),
                 ("ximax", C.c_double), ("skin_depth", C.c_double),
-                ("kerma", C.c_int32), ("fp64", C.c_int32)]
+                ("kerma", C.c_int32), ("fp64", C.c_int32), ("howfarless", C.c_int32)]
 
 
 def default_transport(**kw):
@@ -50,7 +50,7 @@ def default_transport(**kw):
     t = EgsTransport(ecut=0.0, pcut=0.0, rayleigh=1, bound_compton=3,
                      photoelectron_angle=1, atomic_relaxation=1, pair_angle=1,
                      brems_angle=1, spin=0, exact_bca=1, estepe=0.25,
-                     ximax=0.5, skin_depth=3.0, kerma=0, fp64=0)
+                     ximax=0.5, skin_depth=3.0, kerma=0, fp64=0, howfarless=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t
@@ -61,6 +61,10 @@ SCHED_EVENT = 1
 SCHED_HISTORY_PLAIN = 2
 RNG_RANMAR = 0
 RNG_RANLUX = 1
+SOURCE_PARALLEL_BEAM = 0
+SOURCE_DOSXYZ_FRONT = 1
+SOURCE_DOSXYZ_BEAM = 2
+SOURCE_PHSP = 3
 
 
 class EngineError(RuntimeError):
@@ -91,6 +95,7 @@ def lib():
         "egsgpu_set_streams": [vp, i32, i32],
         "egsgpu_seed": [vp, i32, i32],
         "egsgpu_set_rng": [vp, i32, i32, i32],
+        "egsgpu_set_phsp": [vp, i64, vp, vp, vp, vp, vp, vp, vp],
         "egsgpu_run": [vp, i64],
         "egsgpu_get_tallies": [vp, vp, vp, vp],
         "egsgpu_reset_tallies": [vp],

@@ -91,6 +91,10 @@ struct DevSource {
     int kind;                // EGSGPU_SOURCE_*
     double rot[9];           // DOSXYZnrc isource 1: r_11..r_33
     double iso[3], xcol, ycol, dsource;
+    // phase-space particles (kind EGSGPU_SOURCE_PHSP)
+    long long nphsp;
+    const float *px, *py, *pu, *pv, *pe, *pw;
+    const signed char *pq;
 };
 
 struct DevParams {
@@ -101,6 +105,14 @@ struct DevParams {
 // Fields of a stacked particle.
 constexpr int kStkF = 11;
 
+// One entry of a history slot's table: the voxel's key and its energy in
+// one 16-byte record, so that a deposit touches one cache line.
+struct alignas(16) SlotEntry {
+    int key;                  // voxel+1, 0 = empty
+    int pad;
+    double val;
+};
+
 // Per-lane state arrays (lane-fastest SoA for the stack).
 template <typename R>
 struct DevLanes {
@@ -110,8 +122,7 @@ struct DevLanes {
     int mxstk;
     R *stk;   // [(k*kStkF + field)*nlanes + lane]; iq and ir stored bitwise
     int hcap;                       // hash capacity per lane (power of 2)
-    int *hkey;                      // [lane*hcap + slot], voxel+1, 0 = empty
-    double *hval;                   // [lane*hcap + slot]
+    SlotEntry *hent;                // [lane*hcap + slot]: voxel+1 (0 = empty), energy
     int *hlist;                     // touched slots [lane*hcap + k]
 };
 
@@ -145,8 +156,7 @@ struct DevBank {
 struct DevSlots {
     int nslots;
     int hcap;                 // entries per slot (power of 2)
-    int *key;                 // [slot*hcap + i]: voxel+1, 0 = empty
-    double *val;              // [slot*hcap + i]
+    SlotEntry *ent;           // [slot*hcap + i]
     int *list;                // [slot*hcap + k]: claimed entry indices
     int *used;                // [slot]: number of claimed entries
     int *pending;             // [slot]
@@ -178,6 +188,7 @@ struct JobDesc {
     const VoxData<R> *vox;
     double *sum, *sum2;
     DevSource<R> S;
+    long long hist0;          // batched runs: histories the job had before this run
 };
 
 // All active lanes of the wave claim consecutive indices from *counter

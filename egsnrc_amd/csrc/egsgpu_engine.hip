@@ -48,11 +48,12 @@ struct KArgs {
     int64_t base;      // histories per lane
     int rem;           // lanes < rem do one more
     const DevGrad *grad;   // non-null: density-gradient replay
+    long long hist0;   // index of the launch's first history in the simulation
 };
 
 // EGS_AliasTable::sample (egs_alias_table.cpp:268-300), types 0 and 1
-template <typename R, bool EV, bool LUX>
-__device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R, EV, LUX> &sh) {
+template <typename R, bool EV, bool LUX, bool HFL>
+__device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R, EV, LUX, HFL> &sh) {
     if (S.spectrum == 0) return S.energy;
     if (S.spectrum == EGSGPU_SPEC_CDFINV) {
         // DOSXYZnrc srchst (srcxyznrc.mortran:3440-3446): grid point
@@ -101,9 +102,9 @@ __device__ __forceinline__ int planes_first(double x, int n, const R *p) {
 // DOSXYZnrc isource 0, srchst (srcxyznrc.mortran:3416-3448): position on
 // the front face z = zbound(1) (x drawn, then y, each only if its range
 // is not empty), fixed direction, then the energy.
-template <typename R, bool EV, bool LUX>
+template <typename R, bool EV, bool LUX, bool HFL>
 __device__ __forceinline__ bool sample_dosxyz_front(const DevSource<R> &S, const GeomView<R> &G,
-                                                    Shower<R, EV, LUX> &sh, Part<R> &q) {
+                                                    Shower<R, EV, LUX, HFL> &sh, Part<R> &q) {
     double x = (double)S.x0, y = (double)S.y0;
     double dx = (double)S.x1 - (double)S.x0, dy = (double)S.y1 - (double)S.y0;
     if (dx != 0) x = (double)S.x0 + (double)sh.rnd() * dx;
@@ -122,9 +123,9 @@ __device__ __forceinline__ bool sample_dosxyz_front(const DevSource<R> &S, const
 // into position at distance dsource from the isocentre, then carried to
 // the first face of the grid it meets; a beam particle that misses the
 // grid is counted (sh.c_miss) and sampled again.
-template <typename R, bool EV, bool LUX>
+template <typename R, bool EV, bool LUX, bool HFL>
 __device__ __forceinline__ bool sample_dosxyz_beam(const DevSource<R> &S, const GeomView<R> &G,
-                                                   Shower<R, EV, LUX> &sh, Part<R> &q) {
+                                                   Shower<R, EV, LUX, HFL> &sh, Part<R> &q) {
     const double *r = S.rot;
     const double u = r[2], v = r[5], w = r[8];
     const double x0 = G.xp[0], x1 = G.xp[G.nx], y0 = G.yp[0], y1 = G.yp[G.ny];
@@ -182,16 +183,46 @@ __device__ __forceinline__ bool sample_dosxyz_beam(const DevSource<R> &S, const 
     return false;
 }
 
+// Phase-space particle hidx mod n (egs_phsp_source.cpp:285-323): no random
+// numbers; entry into the grid along the particle's direction.
+template <typename R>
+__device__ __forceinline__ bool sample_phsp(const DevSource<R> &S, const GeomView<R> &G,
+                                            long long hidx, Part<R> &q) {
+    const long long k = hidx % S.nphsp;
+    R x = (R)S.px[k] + S.x0, y = (R)S.py[k] + S.y0, z = S.z;
+    R u = (R)S.pu[k], v = (R)S.pv[k];
+    R aux = u * u + v * v;
+    aux = aux < (R)1 ? dsqrt((R)1 - aux) : (R)0;
+    const float wts = S.pw[k];
+    R w = wts > 0 ? aux : -aux;
+    int ireg = geom_is_where(G, x, y, z);
+    if (ireg < 0) {
+        R t = (R)1e30;
+        ireg = geom_howfar_out(G, x, y, z, u, v, w, t);
+        if (ireg >= 0) { x += u * t; y += v * t; z += w * t; }
+    }
+    const int iq = S.pq[k];
+    const R E = (R)S.pe[k];
+    q.e = iq ? E + (R)0.5109989461 : E;
+    q.x = x; q.y = y; q.z = z;
+    q.u = u; q.v = v; q.w = w;
+    q.wt = (R)fabsf(wts); q.dnear = 0; q.iq = iq; q.ir = ireg;
+    return ireg >= 0 && q.wt > 0;
+}
+
 // One primary of EGS_Application::simulateSingleShower
 // (egs_application.cpp:919-959) from the parallel-beam source
 // (egs_parallel_beam.cpp:40-117): position on the source shape, entry into
 // the grid along the beam direction.  Returns false if it misses the grid.
-// The DOSXYZnrc sources (isource 0 and 1) have their own sampling.
-template <typename R, bool EV, bool LUX>
+// The DOSXYZnrc sources (isource 0 and 1) have their own sampling; a
+// phase-space source takes particle hidx (the history's index in the
+// simulation).
+template <typename R, bool EV, bool LUX, bool HFL>
 __device__ __forceinline__ bool sample_primary(const DevSource<R> &S, const GeomView<R> &G,
-                               Shower<R, EV, LUX> &sh, Part<R> &q) {
+                               Shower<R, EV, LUX, HFL> &sh, Part<R> &q, long long hidx) {
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_FRONT) return sample_dosxyz_front(S, G, sh, q);
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_BEAM) return sample_dosxyz_beam(S, G, sh, q);
+    if (S.kind == EGSGPU_SOURCE_PHSP) return sample_phsp(S, G, hidx, q);
     const R RM = (R)0.5109989461;
     int ireg = -1;
     for (int ntry = 0; ntry < 100000 && ireg < 0; ntry++) {
@@ -227,8 +258,8 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // Add this wave's event counters to the kernel's totals: one atomic per
 // counter and wave.  Called by all lanes of every wave.
-template <typename R, bool EV, bool LUX>
-__device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, EV, LUX> &sh,
+template <typename R, bool EV, bool LUX, bool HFL>
+__device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, EV, LUX, HFL> &sh,
                                              int kernel) {
     unsigned long long *c = t.counters + 8 * kernel;
     const unsigned v[8] = {sh.c_psteps, sh.c_cross, sh.c_dep, sh.c_flush,
@@ -242,18 +273,18 @@ __device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, 
 
 // History schedule: lane s runs its share of the histories one after the
 // other with ranmar stream s, like egs-parallel job s.
-template <typename R, bool LUX>
+template <typename R, bool LUX, bool HFL = false>
 __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.L.nlanes;   // lanes past the last stream idle
-    using S = Shower<R, false, LUX>;
+    using S = Shower<R, false, LUX, HFL>;
     S sh(a.T, a.G, a.vox, a.P, a.L, a.tally, live ? lane : 0, rbuf_all + threadIdx.x);
     sh.grad = a.grad;
     int64_t nh = live ? a.base + (lane < a.rem ? 1 : 0) : 0;
     for (int64_t k = 0; k < nh; k++) {
         Part<R> q;
-        if (!sample_primary(a.S, a.G, sh, q)) continue;
+        if (!sample_primary(a.S, a.G, sh, q, a.hist0 + k * a.L.nlanes + lane)) continue;
         sh.p = q;
         sh.np = 1;
         sh.eph = S::kNew;
@@ -358,9 +389,10 @@ k_kerma(KArgs<R> a, int ksteps, int batch, int max_wait) {
         if (chosen == S::kPTrans) {
             sh.photon_trans(ksteps);
         } else if (chosen == kNewHist) {
-            k++;
             Part<R> q;
-            if (sample_primary(a.S, a.G, sh, q)) {
+            const bool hit = sample_primary(a.S, a.G, sh, q, a.hist0 + k * a.L.nlanes + lane);
+            k++;
+            if (hit) {
                 sh.p = q;
                 sh.np = 1;
                 sh.eph = S::kNew;
@@ -490,7 +522,10 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
         }
         const DevSource<R> *src = &a.jobs[lo].S;
         const GeomView<R> *geo = &a.jobs[lo].G;
-        if (!sample_primary(*src, *geo, sh, q)) {
+        // the history's index in its own simulation
+        const long long hsim = a.njobs > 1 ? a.jobs[lo].hist0 + hist_base + i - a.job_first[lo]
+                                           : a.k.hist0 + hist_base + i;
+        if (!sample_primary(*src, *geo, sh, q, hsim)) {
             unsigned long long at = atomicAdd(&a.ctl->free_tail, 1ull);
             a.slots.ring[at % ns] = slot;
             continue;
@@ -779,6 +814,7 @@ struct egsgpu_handle {
     DevGrad *dgrad_dev = nullptr;
     // event schedule
     int schedule = EGSGPU_SCHED_EVENT;
+    bool howfarless = false;            // DOSXYZnrc HOWFARLESS (homogeneous phantom)
     bool ev_ready = false;
     int nslots = 0, slot_hcap = 256;
     DevSlots slots{};
@@ -1011,6 +1047,10 @@ int build_source(egsgpu_handle *h, const EgsSource *s, Precision<R> &P) {
     S.xi = S.wi = nullptr;
     S.bin = nullptr;
     S.kind = s->kind;
+    S.nphsp = 0;
+    S.px = S.py = S.pu = S.pv = S.pe = S.pw = nullptr;
+    S.pq = nullptr;
+    if (s->kind == EGSGPU_SOURCE_PHSP) S.charge = 0;   // primaries enter the photon bank
     for (int k = 0; k < 9; k++) S.rot[k] = s->rot[k];
     for (int k = 0; k < 3; k++) S.iso[k] = s->iso[k];
     S.xcol = s->xcol; S.ycol = s->ycol; S.dsource = s->dsource;
@@ -1067,10 +1107,9 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
     // per-lane history tables: history schedule only
     L.hcap = h->schedule != EGSGPU_SCHED_EVENT ? h->hcap : 0;
     size_t nh = (size_t)L.hcap * h->nlanes;
-    CK(alloc(h, nh, &L.hkey));
-    CK(alloc(h, nh, &L.hval));
+    CK(alloc(h, nh, &L.hent));
     CK(alloc(h, nh, &L.hlist));
-    if (nh) EGS_HIP(hipMemset(L.hkey, 0, nh * sizeof(int)));
+    if (nh) EGS_HIP(hipMemset(L.hent, 0, nh * sizeof(SlotEntry)));
     return EGSGPU_OK;
 }
 
@@ -1120,6 +1159,7 @@ KArgs<R> kargs(egsgpu_handle *h, Precision<R> &P) {
     a.base = 0;
     a.rem = 0;
     a.grad = nullptr;
+    a.hist0 = h->nhist;
     return a;
 }
 
@@ -1141,8 +1181,10 @@ int take_time(egsgpu_handle *h, int k) {
 }
 
 template <typename R>
-int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = nullptr) {
+int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = nullptr,
+           long long hist0 = -1) {
     KArgs<R> a = kargs(h, P);
+    if (hist0 >= 0) a.hist0 = hist0;   // a replay of histories already counted
     a.base = n / h->nlanes;
     a.rem = (int)(n % h->nlanes);
     a.grad = grad;
@@ -1150,7 +1192,14 @@ int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = n
     EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
     dim3 grid((h->nlanes + h->block - 1) / h->block);
     const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
-    if (h->P.kerma && h->schedule == EGSGPU_SCHED_HISTORY) {
+    if (h->howfarless) {   // HOWFARLESS: the plain history loop, ranmar or ranlux
+        if (lux)
+            hipLaunchKernelGGL((k_transport<R, true, true>), grid, dim3(h->block), lds, h->stream,
+                               a);
+        else
+            hipLaunchKernelGGL((k_transport<R, false, true>), grid, dim3(h->block), lds, h->stream,
+                               a);
+    } else if (h->P.kerma && h->schedule == EGSGPU_SCHED_HISTORY) {
         if (lux)
             hipLaunchKernelGGL((k_kerma<R, true>), grid, dim3(h->block), lds, h->stream, a,
                                h->ev_ksteps, h->ev_batch, h->ev_max_wait);
@@ -1188,14 +1237,12 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     S.nslots = h->nslots > 0 ? h->nslots : 4 * h->nlanes;
     S.hcap = h->slot_hcap;
     size_t ne = (size_t)S.nslots * S.hcap;
-    CK(alloc(h, ne, &S.key));
-    CK(alloc(h, ne, &S.val));
+    CK(alloc(h, ne, &S.ent));
     CK(alloc(h, ne, &S.list));
     CK(alloc(h, (size_t)S.nslots, &S.used));
     CK(alloc(h, (size_t)S.nslots, &S.pending));
     CK(alloc(h, (size_t)S.nslots, &S.ring));
-    EGS_HIP(hipMemset(S.key, 0, ne * sizeof(int)));
-    EGS_HIP(hipMemset(S.val, 0, ne * sizeof(double)));
+    EGS_HIP(hipMemset(S.ent, 0, ne * sizeof(SlotEntry)));
     CK(alloc(h, 1, &h->ctl));
     EGS_HIP(hipHostMalloc((void **)&h->ctl_host, sizeof(EvCtl), hipHostMallocDefault));
     CK(alloc(h, (size_t)S.nslots, &h->slot_job));
@@ -1206,6 +1253,7 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
         jd.sum = h->tally.sum;
         jd.sum2 = h->tally.sum2;
         jd.S = P.S;
+        jd.hist0 = 0;
         const JobDesc<R> *dj = nullptr;
         CK(upload(h, &jd, 1, &dj));
         h->job_self = (void *)dj;
@@ -1492,7 +1540,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
             src->spectrum == EGSGPU_SPEC_LINE || src->spectrum == EGSGPU_SPEC_CDFINV) &&
             src->nbins > 0 && src->spec_e && src->spec_w),
             "source spectrum");
-    EGS_ARG(src->kind >= EGSGPU_SOURCE_PARALLEL_BEAM && src->kind <= EGSGPU_SOURCE_DOSXYZ_BEAM,
+    EGS_ARG(src->kind >= EGSGPU_SOURCE_PARALLEL_BEAM && src->kind <= EGSGPU_SOURCE_PHSP,
             "source kind");
     EGS_ARG(src->kind != EGSGPU_SOURCE_DOSXYZ_FRONT || src->uz > 0,
             "a front-face source must point into the grid (uz > 0)");
@@ -1505,6 +1553,16 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     if (!tp->kerma && !tp->exact_bca) {
         set_error("only the exact boundary-crossing algorithm is implemented");
         return EGSGPU_E_UNSUPPORTED;
+    }
+    if (tp->howfarless) {   // one medium and one density everywhere
+        const int64_t nv = (int64_t)geom->nx * geom->ny * geom->nz;
+        for (int64_t i = 1; i < nv; i++)
+            if ((geom->medium && geom->medium[i] != geom->medium[0]) ||
+                (geom->rhor && geom->rhor[i] != geom->rhor[0])) {
+                set_error("HOWFARLESS needs a homogeneous phantom (voxel %lld differs)",
+                          (long long)i);
+                return EGSGPU_E_ARG;
+            }
     }
     const Blob &b = tab->blob;
     auto *h = new egsgpu_handle();
@@ -1593,8 +1651,10 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->mxstk = tp->kerma ? 32 : 64;
     // the event schedule pays off where charged particles are transported
     h->schedule = tp->kerma ? EGSGPU_SCHED_HISTORY : EGSGPU_SCHED_EVENT;
+    h->howfarless = tp->howfarless != 0;
+    if (h->howfarless) h->schedule = EGSGPU_SCHED_HISTORY_PLAIN;
     // electron beams touch more voxels per history than photon beams
-    h->slot_hcap = tp->kerma ? 64 : (src->charge ? 512 : 256);
+    h->slot_hcap = tp->kerma ? 64 : (src->charge || src->kind == EGSGPU_SOURCE_PHSP ? 512 : 256);
     *out = h;
     return EGSGPU_OK;
 }
@@ -1614,6 +1674,41 @@ int egsgpu_set_streams(egsgpu_handle *h, int32_t nstreams, int32_t block_size) {
     }
     if (nstreams > 0) h->nlanes = nstreams;
     if (block_size > 0) h->block = block_size;
+    return EGSGPU_OK;
+}
+
+int egsgpu_set_phsp(egsgpu_handle *h, int64_t n, const float *x, const float *y,
+                    const float *u, const float *v, const float *e, const float *wt,
+                    const int8_t *q) {
+    EGS_ARG(h && n > 0 && x && y && u && v && e && wt && q, "handle, n > 0, particle arrays");
+    DevSource<float> &Sf = h->pf.S;
+    DevSource<double> &Sd = h->pd.S;
+    if (Sf.kind != EGSGPU_SOURCE_PHSP) {
+        set_error("the simulation was not created with a phase-space source");
+        return EGSGPU_E_ARG;
+    }
+    if (h->lanes_ready) {
+        set_error("phase-space particles must be given before the first run");
+        return EGSGPU_E_ARG;
+    }
+    for (int64_t k = 0; k < n; k++)
+        if (q[k] < -1 || q[k] > 1 || !(e[k] >= 0)) {
+            set_error("phase-space particle %lld: charge %d, energy %g", (long long)k, (int)q[k],
+                      (double)e[k]);
+            return EGSGPU_E_ARG;
+        }
+    const float *src[6] = {x, y, u, v, e, wt};
+    const float *dst[6] = {};
+    for (int f = 0; f < 6; f++) CK(upload(h, src[f], (size_t)n, &dst[f]));
+    const signed char *dq = nullptr;
+    CK(upload(h, (const signed char *)q, (size_t)n, &dq));
+    auto set = [&](auto &S) {
+        S.nphsp = n;
+        S.px = dst[0]; S.py = dst[1]; S.pu = dst[2]; S.pv = dst[3]; S.pe = dst[4]; S.pw = dst[5];
+        S.pq = dq;
+    };
+    set(Sf);
+    set(Sd);
     return EGSGPU_OK;
 }
 
@@ -1673,6 +1768,7 @@ int egsgpu_run_batched(egsgpu_handle *const *jobs, int32_t njobs, const int64_t 
             jd[j].sum = jobs[j]->tally.sum;
             jd[j].sum2 = jobs[j]->tally.sum2;
             jd[j].S = Pj.S;
+            jd[j].hist0 = jobs[j]->nhist;
         }
         JobDesc<R> *djobs = nullptr;
         long long *dfirst = nullptr;
@@ -1721,6 +1817,10 @@ int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule) {
     EGS_ARG(h, "handle");
     EGS_ARG(schedule == EGSGPU_SCHED_HISTORY || schedule == EGSGPU_SCHED_EVENT ||
             schedule == EGSGPU_SCHED_HISTORY_PLAIN, "schedule");
+    if (h->howfarless && schedule != EGSGPU_SCHED_HISTORY_PLAIN) {
+        set_error("HOWFARLESS runs on the plain history schedule");
+        return EGSGPU_E_UNSUPPORTED;
+    }
     if (h->lanes_ready) {
         set_error("the schedule must be chosen before the first run");
         return EGSGPU_E_ARG;
@@ -2015,7 +2115,8 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
     EGS_HIP(hipMemcpyAsync(h->dgrad_dev, &h->dgrad, sizeof(DevGrad), hipMemcpyHostToDevice, h->stream));
     EGS_HIP(hipMemcpyAsync(recs, saved, rb, hipMemcpyDeviceToDevice, h->stream));
     EGS_HIP(hipEventRecord(h->ev0, h->stream));
-    rc = h->fp64 ? launch(h, h->pd, nhist, h->dgrad_dev) : launch(h, h->pf, nhist, h->dgrad_dev);
+    rc = h->fp64 ? launch(h, h->pd, nhist, h->dgrad_dev, h->nhist - nhist)
+                 : launch(h, h->pf, nhist, h->dgrad_dev, h->nhist - nhist);
     if (rc) return rc;
     EGS_HIP(hipEventRecord(h->ev1, h->stream));
     EGS_HIP(hipStreamSynchronize(h->stream));

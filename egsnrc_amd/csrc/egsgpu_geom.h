@@ -135,6 +135,50 @@ __host__ __device__ inline int geom_howfar_idx(const GeomView<R> &g, int ireg,
     return ireg + step * g.nxy;
 }
 
+// ---- HOWFARLESS (DOSXYZnrc, dosxyznrc.mortran:3900-4360 with
+// ihowfarless): in a homogeneous phantom howfar and hownear see only the
+// outer faces of the grid.
+
+// Distance along (u, v, w) from a point inside the grid to its outer
+// boundary; a distance within the boundary tolerance becomes half of it,
+// as in geom_howfar_in, so that a particle on an outer face moving out
+// always leaves.
+template <typename R>
+__host__ __device__ inline R geom_box_exit(const GeomView<R> &g, R x, R y, R z, R u, R v, R w) {
+    R t = (R)1e30;
+    if (u > 0) { R d = (g.xp[g.nx] - x) / u; t = d < t ? d : t; }
+    else if (u < 0) { R d = (g.xp[0] - x) / u; t = d < t ? d : t; }
+    if (v > 0) { R d = (g.yp[g.ny] - y) / v; t = d < t ? d : t; }
+    else if (v < 0) { R d = (g.yp[0] - y) / v; t = d < t ? d : t; }
+    if (w > 0) { R d = (g.zp[g.nz] - z) / w; t = d < t ? d : t; }
+    else if (w < 0) { R d = (g.zp[0] - z) / w; t = d < t ? d : t; }
+    return t <= g.btol ? g.hbtol : t;
+}
+
+// Perpendicular distance from a point inside the grid to its outer faces.
+template <typename R>
+__host__ __device__ inline R geom_box_near(const GeomView<R> &g, R x, R y, R z) {
+    R t = x - g.xp[0];
+    R d = g.xp[g.nx] - x; t = d < t ? d : t;
+    d = y - g.yp[0]; t = d < t ? d : t;
+    d = g.yp[g.ny] - y; t = d < t ? d : t;
+    d = z - g.zp[0]; t = d < t ? d : t;
+    d = g.zp[g.nz] - z; t = d < t ? d : t;
+    return t > 0 ? t : (R)0;
+}
+
+// Voxel of a point inside the grid (the binary searches of DOSXYZnrc's
+// ibsearch4, clamped to the grid).
+template <typename R>
+__host__ __device__ inline int geom_locate(const GeomView<R> &g, R x, R y, R z) {
+    int ix = planes_find(x, g.nx, g.xp), iy = planes_find(y, g.ny, g.yp),
+        iz = planes_find(z, g.nz, g.zp);
+    ix = ix < 0 ? 0 : (ix >= g.nx ? g.nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= g.ny ? g.ny - 1 : iy);
+    iz = iz < 0 ? 0 : (iz >= g.nz ? g.nz - 1 : iz);
+    return ix + iy * g.nx + iz * g.nxy;
+}
+
 // howfarFromOut: entry into the grid from outside (t overwritten only on a
 // hit; the tolerance re-check overwrites t1 through howfar before t = t1,
 // as the reference does).

@@ -35,7 +35,8 @@ __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 // tally hash).  EV = true: the event schedule (particles of one history
 // may be spread over lanes and kernels; per-slot shared tally tables).
 // LUX = false: ranmar streams; true: ranlux streams.
-template <typename R, bool EV = false, bool LUX = false>
+// HFL = true: DOSXYZnrc's HOWFARLESS transport for a homogeneous phantom.
+template <typename R, bool EV = false, bool LUX = false, bool HFL = false>
 struct Shower {
     const DevTables<R> &T;
     const GeomView<R> *Gp;   // geometry of the current particle's simulation
@@ -165,24 +166,22 @@ struct Shower {
 
     // ---------------------------------------------------------- scoring
     // this lane's history table (history schedule)
-    __device__ __forceinline__ int *hkey() const { return L.hkey + (size_t)lane * L.hcap; }
-    __device__ __forceinline__ double *hval() const { return L.hval + (size_t)lane * L.hcap; }
+    __device__ __forceinline__ SlotEntry *hent() const { return L.hent + (size_t)lane * L.hcap; }
     __device__ __forceinline__ int *hlist() const { return L.hlist + (size_t)lane * L.hcap; }
     __device__ void hash_add(int vox, double e) {
-        int *hkey = this->hkey();
-        double *hval = this->hval();
+        SlotEntry *hent = this->hent();
         int *hlist = this->hlist();
         unsigned key = (unsigned)vox + 1u;
         unsigned mask = (unsigned)L.hcap - 1u;
         for (int attempt = 0; attempt < 2; attempt++) {
             unsigned s = (key * 2654435761u) & mask;
             for (int probe = 0; probe < L.hcap; probe++) {
-                int k = hkey[s];
-                if (k == (int)key) { hval[s] += e; return; }
+                int k = hent[s].key;
+                if (k == (int)key) { hent[s].val += e; return; }
                 if (k == 0) {
                     if (hused * 4 >= L.hcap * 3) break;
-                    hkey[s] = (int)key;
-                    hval[s] = e;
+                    hent[s].key = (int)key;
+                    hent[s].val = e;
                     hlist[hused++] = (int)s;
                     return;
                 }
@@ -194,16 +193,15 @@ struct Shower {
         }
     }
     __device__ void flush_table() {
-        int *hkey = this->hkey();
-        double *hval = this->hval();
+        SlotEntry *hent = this->hent();
         int *hlist = this->hlist();
         for (int k = 0; k < hused; k++) {
             int s = hlist[k];
-            int vox = hkey[s] - 1;
-            double e = hval[s];
+            int vox = hent[s].key - 1;
+            double e = hent[s].val;
             atomicAdd(tsum + vox, e);
             atomicAdd(tsum2 + vox, e * e);
-            hkey[s] = 0;
+            hent[s].key = 0;
         }
         c_flush += hused;
         hused = 0;
@@ -233,16 +231,17 @@ struct Shower {
         unsigned mask = (unsigned)S.hcap - 1u;
         unsigned h = ((unsigned)key * 2654435761u) & mask;
         for (int probe = 0; probe < S.hcap; probe++) {
-            int k = atomicCAS(S.key + base + h, 0, key);
+            SlotEntry *en = S.ent + base + h;
+            int k = atomicCAS(&en->key, 0, key);
             if (k == 0) {
                 int idx = atomicAdd(S.used + slot, 1);
                 __hip_atomic_store(S.list + base + idx, (int)h, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-                atomicAdd(S.val + base + h, e);
+                atomicAdd(&en->val, e);
                 return;
             }
             if (k == key) {
-                atomicAdd(S.val + base + h, e);
+                atomicAdd(&en->val, e);
                 return;
             }
             h = (h + 1) & mask;
@@ -267,8 +266,8 @@ struct Shower {
             for (int k = 0; k < n; k++) {
                 int h = __hip_atomic_load(S.list + base + k, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
-                int key = atomicExch(S.key + base + h, 0);
-                double e = atomicExch(S.val + base + h, 0.0);
+                int key = atomicExch(&S.ent[base + h].key, 0);
+                double e = atomicExch(&S.ent[base + h].val, 0.0);
                 atomicAdd(tsum + key - 1, e);
                 atomicAdd(tsum2 + key - 1, e * e);
             }
@@ -419,6 +418,26 @@ struct Shower {
         for (int k = 0; k < kmax; k++) {
             c_psteps++;
             if (++hsteps > kMaxStepsPerHistory) { c_ovf++; np = 0; pph = kPEnter; return; }
+            if constexpr (HFL) {
+                // HOWFARLESS: one step to the interaction point or out of the
+                // grid; the voxel of the point is found for the scoring
+                // (dosxyznrc.mortran:3546-3562)
+                R gmfp = p_gmfpr0 / p_rhof;
+                if (P.rayleigh) gmfp = gmfp * p_cohfac;
+                R ustep = gmfp * p_dpmfp;
+                R tb = geom_box_exit(G, p.x, p.y, p.z, p.u, p.v, p.w);
+                const bool out = tb <= ustep;
+                if (out) ustep = tb;
+                p.x = p.x + p.u * ustep;
+                p.y = p.y + p.v * ustep;
+                p.z = p.z + p.w * ustep;
+                R d = p_dpmfp - ustep / gmfp;
+                p_dpmfp = d > 0 ? d : (R)0;
+                if (out) { pop(); pph = kPEnter; return; }
+                p.ir = geom_locate(G, p.x, p.y, p.z);
+                if (p_dpmfp <= (R)1.e-8) { photon_choose(); return; }
+                continue;
+            }
             R gmfp = p_gmfpr0 / p_rhof;
             if (P.rayleigh) gmfp = gmfp * p_cohfac;
             R ustep = gmfp * p_dpmfp;
@@ -446,21 +465,25 @@ struct Shower {
             if (idisc < 0) { pop(); pph = kPEnter; return; }
             if (p_med != medold) { photon_medium(); continue; }
             if (p_dpmfp <= (R)1.e-8) {   // interaction: choose its type
-                int li = p_med * kGE + p_lgle - 1;
-                if (grad) log_event(1, p.ir, 1 / p_rhof);
-                if (P.rayleigh) {
-                    R rnno37 = rnd();
-                    if (rnno37 <= (R)1.0 - p_cohfac) { pph = kPRayl; return; }
-                }
-                R rnno36 = rnd();
-                const R *g = T.grow + (size_t)li * kGRow;
-                R gbr1 = g[kGbr1 + 1] * p_gle + g[kGbr1];
-                if (rnno36 <= gbr1 && p.e > (R)(2 * 0.5109989461)) { pph = kPPair; return; }
-                R gbr2 = g[kGbr2 + 1] * p_gle + g[kGbr2];
-                pph = rnno36 < gbr2 ? kPCompt : kPPhoto;
+                photon_choose();
                 return;
             }
         }
+    }
+    // at the interaction point: the interaction type (kPRayl..kPPair)
+    __device__ __forceinline__ void photon_choose() {
+        int li = p_med * kGE + p_lgle - 1;
+        if (grad) log_event(1, p.ir, 1 / p_rhof);
+        if (P.rayleigh) {
+            R rnno37 = rnd();
+            if (rnno37 <= (R)1.0 - p_cohfac) { pph = kPRayl; return; }
+        }
+        R rnno36 = rnd();
+        const R *g = T.grow + (size_t)li * kGRow;
+        R gbr1 = g[kGbr1 + 1] * p_gle + g[kGbr1];
+        if (rnno36 <= gbr1 && p.e > (R)(2 * 0.5109989461)) { pph = kPPair; return; }
+        R gbr2 = g[kGbr2 + 1] * p_gle + g[kGbr2];
+        pph = rnno36 < gbr2 ? kPCompt : kPPhoto;
     }
     // kPRayl..kPPair: the interaction; then the photon continues
     // (:PNEWENERGY:) or the stack top changed (kPEnter)
@@ -1011,6 +1034,134 @@ struct Shower {
     R e_rhof = 1, e_demfp = 0, e_sig0 = 0, e_ttstep = 0, e_elke = 0;
     bool e_ctstep = true;
 
+    // HOWFARLESS deposition of one charged-particle step
+    // (dosxyznrc.mortran:3565-3765 and 3830-3860): a step that stays in its
+    // voxel deposits there; otherwise the curved path is taken as two
+    // straight pieces meeting at a hinge (placed from the initial or, with
+    // probability 1/2, the final direction for a condensed-history step;
+    // one straight piece for a single-scattering or boundary step) and the
+    // energy is shared among the voxels the pieces cross in proportion to
+    // the length in each.  Returns the voxel where the path ends, -1 if it
+    // leaves the grid.
+    __device__ __forceinline__ int hfl_deposit(R edep, R vstep, R tv, bool ch, R xf, R yf, R zf,
+                                               R uf, R vf, R wf) {
+        const GeomView<R> &G = *Gp;
+        const int ir = p.ir;
+        int irz = ir / G.nxy, iry = (ir - irz * G.nxy) / G.nx;
+        int irx = ir - irz * G.nxy - iry * G.nx;
+        R dn = G.xp[irx + 1] - p.x, a;
+        a = p.x - G.xp[irx]; dn = a < dn ? a : dn;
+        a = G.yp[iry + 1] - p.y; dn = a < dn ? a : dn;
+        a = p.y - G.yp[iry]; dn = a < dn ? a : dn;
+        a = G.zp[irz + 1] - p.z; dn = a < dn ? a : dn;
+        a = p.z - G.zp[irz]; dn = a < dn ? a : dn;
+        if (dn >= vstep ||
+            (xf > G.xp[irx] && xf < G.xp[irx + 1] && yf > G.yp[iry] && yf < G.yp[iry + 1] &&
+             zf > G.zp[irz] && zf < G.zp[irz + 1])) {
+            score(ir, edep, p.wt);
+            return ir;
+        }
+        R xi = p.x, yi = p.y, zi = p.z;
+        R u1 = p.u, v1 = p.v, w1 = p.w, u2 = 0, v2 = 0, w2 = 0, t1 = tv, t2 = 0;
+        const R dx = xf - xi, dy = yf - yi, dz = zf - zi;
+        const R d2 = dx * dx + dy * dy + dz * dz;
+        auto straight = [&]() {
+            t1 = dsqrt(d2);
+            if (t1 > 0) { u1 = dx / t1; v1 = dy / t1; w1 = dz / t1; }
+            t2 = 0;
+        };
+        if (ch) {
+            if (rnd() < (R)0.5) {   // hinge from the initial direction
+                R c = 1 - (dx * u1 + dy * v1 + dz * w1) / tv;
+                if (c < (R)1e-3) {
+                    straight();
+                } else {
+                    t1 = (tv * tv - d2) / (2 * tv * c);
+                    t2 = tv - t1;
+                    if (t2 > 0) {
+                        u2 = (dx - t1 * u1) / t2; v2 = (dy - t1 * v1) / t2; w2 = (dz - t1 * w1) / t2;
+                    } else {
+                        straight();
+                    }
+                }
+            } else {                 // hinge from the final direction
+                u2 = uf; v2 = vf; w2 = wf;
+                R c = 1 - (dx * u2 + dy * v2 + dz * w2) / tv;
+                if (c < (R)1e-3) {
+                    straight();
+                } else {
+                    t2 = (tv * tv - d2) / (2 * tv * c);
+                    t1 = tv - t2;
+                    if (t1 > 0) {
+                        u1 = (dx - t2 * u2) / t1; v1 = (dy - t2 * v2) / t1; w1 = (dz - t2 * w2) / t1;
+                    } else {
+                        straight();
+                    }
+                }
+            }
+        }
+        const R ttot = t1 + t2;
+        const R efrac = ttot > 0 ? edep / ttot : (R)0;
+        int irl = ir;
+        bool outside = false;
+        for (int seg = 0; seg < 2 && !outside; seg++) {
+            const R t = seg ? t2 : t1;
+            const R uu = seg ? u2 : u1, vv = seg ? v2 : v1, ww = seg ? w2 : w1;
+            if (!(t > 0)) continue;
+            if (seg) { xi = xi + u1 * t1; yi = yi + v1 * t1; zi = zi + w1 * t1; }
+            int dirx = 0, diry = 0, dirz = 0;
+            R uii = 0, vii = 0, wii = 0, distx = (R)1e20, disty = (R)1e20, distz = (R)1e20;
+            if (uu > 0) { uii = 1 / uu; dirx = 1; distx = (G.xp[irx + 1] - xi) * uii; }
+            else if (uu < 0) { uii = -1 / uu; dirx = -1; distx = (xi - G.xp[irx]) * uii; }
+            if (vv > 0) { vii = 1 / vv; diry = 1; disty = (G.yp[iry + 1] - yi) * vii; }
+            else if (vv < 0) { vii = -1 / vv; diry = -1; disty = (yi - G.yp[iry]) * vii; }
+            if (ww > 0) { wii = 1 / ww; dirz = 1; distz = (G.zp[irz + 1] - zi) * wii; }
+            else if (ww < 0) { wii = -1 / ww; dirz = -1; distz = (zi - G.zp[irz]) * wii; }
+            R dtot = 0;
+            for (int guard = G.nx + G.ny + G.nz + 4; guard > 0; guard--) {
+                const R dmax = t - dtot;
+                R dl;
+                int nxt = irl;
+                if (distx < disty && distx < distz && distx < dmax) {
+                    dl = distx;
+                    irx += dirx;
+                    if (irx >= 0 && irx < G.nx) {
+                        disty -= distx; distz -= distx; nxt = irl + dirx;
+                        distx = (G.xp[irx + 1] - G.xp[irx]) * uii;
+                    } else {
+                        outside = true;
+                    }
+                } else if (disty < distz && disty < dmax) {
+                    dl = disty;
+                    iry += diry;
+                    if (iry >= 0 && iry < G.ny) {
+                        distx -= disty; distz -= disty; nxt = irl + diry * G.nx;
+                        disty = (G.yp[iry + 1] - G.yp[iry]) * vii;
+                    } else {
+                        outside = true;
+                    }
+                } else if (distz < dmax) {
+                    dl = distz;
+                    irz += dirz;
+                    if (irz >= 0 && irz < G.nz) {
+                        distx -= distz; disty -= distz; nxt = irl + dirz * G.nxy;
+                        distz = (G.zp[irz + 1] - G.zp[irz]) * wii;
+                    } else {
+                        outside = true;
+                    }
+                } else {
+                    dl = dmax;
+                }
+                score(irl, (dl > 0 ? dl : (R)0) * efrac, p.wt);
+                if (outside) break;
+                irl = nxt;
+                dtot += dl;
+                if (!(dtot < t)) break;
+            }
+        }
+        return outside ? -1 : irl;
+    }
+
     __device__ __forceinline__ void electron_step() {
         const R RM = (R)0.5109989461, RMT2 = 2 * RM, RMSQ = RM * RM;
         const R ecut = (R)P.ecut;
@@ -1097,7 +1248,9 @@ struct Shower {
         }
         R tustep = tstep < tmxs ? tstep : tmxs;
         tustep = tustep < range ? tustep : range;
-        R tperp = geom_hownear(*Gp, p.ir, p.x, p.y, p.z);
+        R tperp;
+        if constexpr (HFL) tperp = geom_box_near(*Gp, p.x, p.y, p.z);
+        else tperp = geom_hownear(*Gp, p.ir, p.x, p.y, p.z);
         p.dnear = tperp;
         R blccl = rhof * M.blcc;
         R p2 = eke * (eke + RMT2), beta2 = p2 / (p2 + RMSQ);
@@ -1135,8 +1288,13 @@ struct Shower {
         }
         int irold = p.ir, irnew = p.ir, idisc = 0;
         if (callhowfar || p.wt <= 0) {
-            int inew = geom_howfar_in(*Gp, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
-            if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
+            if constexpr (HFL) {
+                R tb = geom_box_exit(*Gp, p.x, p.y, p.z, p.u, p.v, p.w);
+                if (tb <= ustep) { ustep = tb; irnew = -1; idisc = -1; }
+            } else {
+                int inew = geom_howfar_in(*Gp, p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+                if (inew != p.ir) { irnew = inew; if (inew < 0) idisc = -1; }
+            }
         }
         if (ustep <= 0) ustep = 0;
         if (ustep == 0) {
@@ -1182,10 +1340,20 @@ struct Shower {
                 uf = p.u; vf = p.v; wf = p.w;
             }
         }
-        score(p.ir, edep, p.wt);
+        int hfl_ir = p.ir;
+        if constexpr (HFL) hfl_ir = hfl_deposit(edep, vstep, callmsdist ? tvstep : vstep,
+                                                callmsdist, xf, yf, zf, uf, vf, wf);
+        else score(p.ir, edep, p.wt);
         p.x = xf; p.y = yf; p.z = zf;
         p.u = uf; p.v = vf; p.w = wf;
         p.dnear = p.dnear - vstep;
+        if constexpr (HFL) {
+            // the voxel at the end of the step (still inside the grid)
+            if (irnew >= 0) {
+                const int v = hfl_ir >= 0 ? hfl_ir : geom_locate(*Gp, p.x, p.y, p.z);
+                p.ir = v; irnew = v; e_irl = v;
+            }
+        }
         irold = p.ir;
         peie = peie - edep;
         eie = peie;

@@ -47,6 +47,14 @@ def make_source(charge=0, energy=1.0, spectrum=None, shape="rectangle",
     return s, keep
 
 
+def make_phsp_source(x0=0.0, y0=0.0, z=0.0):
+    """A phase-space source whose particles are given by Simulation.set_phsp
+    (egs_phsp_source), its plane translated by (x0, y0, z)."""
+    s = EgsSource(0, 0, 0.0, 0, None, None, 0, x0, x0, y0, y0, z, 0.0, 0.0, 1.0)
+    s.kind = _lib.SOURCE_PHSP
+    return s, []
+
+
 class Simulation:
     def __init__(self, tables, xb, yb, zb, medium, rhor, source, transport,
                  nstreams=0, block=0, schedule=None):
@@ -70,6 +78,15 @@ class Simulation:
 
     def seed(self, ixx, jxx):
         check(_lib.lib().egsgpu_seed(self.h, ixx, jxx))
+
+    def set_phsp(self, x, y, u, v, e, wt, q):
+        """Phase-space particles (kinetic energy e, weight |wt| with the sign
+        of w, charge q); history i takes particle i mod n."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, u, v, e, wt)]
+        qq = np.ascontiguousarray(q, dtype=np.int8)
+        n = len(qq)
+        assert all(len(a) == n for a in arrs)
+        check(_lib.lib().egsgpu_set_phsp(self.h, n, *[ptr(a) for a in arrs], ptr(qq)))
 
     def set_ranlux(self, luxury, seed):
         """Drive the transport with ranlux streams: stream s is

@@ -41,6 +41,7 @@
 #include "dosxyz_input.h"
 #include "egs_input.h"
 #include "egs_phantom.h"
+#include "egs_phsp.h"
 
 using namespace egshost;
 
@@ -266,8 +267,37 @@ class EGS_GPUApplication {
         if (!s) throw std::runtime_error("simulation source not found");
         std::string lib;
         s->get("library", lib);
+        // egs_transformed_source around another source (translations only)
+        double tx = 0, ty = 0, tz = 0;
+        if (InputItem::same(lib, "egs_transformed_source")) {
+            std::string inner;
+            s->get("source name", inner);
+            std::vector<double> tr;
+            if (const InputItem *t = s->block("transformation")) {
+                if (!t->get("translation", tr) && tr.size() == 3) { tx = tr[0]; ty = tr[1]; tz = tr[2]; }
+                std::vector<double> rot;
+                if (!t->get("rotation", rot) || !t->get("rotation vector", rot))
+                    throw std::runtime_error("egs_transformed_source: only translations are supported");
+            }
+            const InputItem *in = nullptr;
+            for (auto *b : sd->blocks("source")) {
+                std::string nm;
+                b->get("name", nm);
+                if (nm == inner) in = b;
+            }
+            if (!in) throw std::runtime_error("egs_transformed_source: source " + inner + " not found");
+            s = in;
+            lib.clear();
+            s->get("library", lib);
+        }
+        if (InputItem::same(lib, "egs_phsp_source")) {
+            init_phsp_source(s);
+            src.x0 += tx; src.y0 += ty; src.z += tz;
+            return;
+        }
         if (!InputItem::same(lib, "egs_parallel_beam"))
-            throw std::runtime_error("only egs_parallel_beam is supported, got " + lib);
+            throw std::runtime_error("only egs_parallel_beam and egs_phsp_source are supported, got " +
+                                     lib);
         int q = 0;
         s->get("charge", q);
         src.charge = q;
@@ -301,6 +331,7 @@ class EGS_GPUApplication {
         } else {
             throw std::runtime_error("unsupported shape " + st);
         }
+        src.x0 += tx; src.x1 += tx; src.y0 += ty; src.y1 += ty; src.z += tz;
         const InputItem *sp = s->block("spectrum");
         if (!sp) throw std::runtime_error("source needs a spectrum");
         std::string stype;
@@ -332,6 +363,49 @@ class EGS_GPUApplication {
         } else {
             throw std::runtime_error("unsupported spectrum type " + stype);
         }
+    }
+
+    // egs_phsp_source (egs_phsp_source.cpp:455-560 input; rejectParticle
+    // :433-470): particle type and cutout filters give rejected particles
+    // weight 0 (their histories score nothing); the phase-space plane is
+    // z = 0 unless an egs_transformed_source translates it
+    EgsPhsp phsp;
+    void init_phsp_source(const InputItem *s) {
+        std::string file;
+        if (s->get("phase space file", file)) throw std::runtime_error("egs_phsp_source needs a phase space file");
+        phsp = read_egsphsp(resolve(file));
+        std::string ptype = "all";
+        s->get("particle type", ptype);
+        int want = 2;   // 2 all, 3 charged, else the charge
+        if (InputItem::same(ptype, "photons")) want = 0;
+        else if (InputItem::same(ptype, "electrons")) want = -1;
+        else if (InputItem::same(ptype, "positrons")) want = 1;
+        else if (InputItem::same(ptype, "charged")) want = 3;
+        else if (!InputItem::same(ptype, "all"))
+            throw std::runtime_error("unknown particle type " + ptype);
+        std::vector<double> cut;
+        bool cutout = !s->get("cutout", cut) && cut.size() == 4;
+        double rec = 0;
+        if (!s->get("recycle photons", rec) && rec > 0)
+            throw std::runtime_error("phase-space recycling is not supported");
+        if (!s->get("recycle electrons", rec) && rec > 0)
+            throw std::runtime_error("phase-space recycling is not supported");
+        for (int64_t k = 0; k < phsp.n; k++) {
+            bool rej = (want < 2 && phsp.q[k] != want) || (want == 3 && !phsp.q[k]) ||
+                       ((uint32_t)phsp.latch[k] & 2147483648u);
+            if (cutout && (phsp.x[k] < cut[0] || phsp.x[k] > cut[1] || phsp.y[k] < cut[2] ||
+                           phsp.y[k] > cut[3]))
+                rej = true;
+            if (rej) phsp.wt[k] = phsp.wt[k] > 0 ? 0.0f : -0.0f;
+        }
+        src = EgsSource{};
+        src.kind = EGSGPU_SOURCE_PHSP;
+        src.spectrum = EGSGPU_SPEC_MONO;
+        src.charge = 0;
+        area = 0;
+        fprintf(stderr, "phase space: %lld particles (%lld photons), %g incident particles, "
+                        "E %g-%g MeV\n", (long long)phsp.n, (long long)phsp.nphot,
+                (double)phsp.pinc, (double)phsp.emin, (double)phsp.emax);
     }
 
     void init_transport() {
@@ -417,8 +491,14 @@ class EGS_GPUApplication {
         jxx = dz.inseed2;
         if (ixx <= 0 || ixx > 31328) ixx = 1802;
         if (jxx <= 0 || jxx > 30081) jxx = 9373;
-        if (dz.ihowfarless)
-            fprintf(stderr, "note: HOWFARLESS requested; every voxel boundary is tracked\n");
+        if (dz.ihowfarless) {   // only meaningful in a homogeneous phantom
+            bool homog = true;
+            for (size_t i = 1; i < medium.size() && homog; i++)
+                homog = medium[i] == medium[0] && phantom_density[i] == phantom_density[0];
+            if (homog) tp.howfarless = 1;
+            else fprintf(stderr, "note: HOWFARLESS requested for a phantom that is not "
+                                 "homogeneous; every voxel boundary is tracked\n");
+        }
         if (dz.ireject)
             fprintf(stderr, "note: charged-particle range rejection requested; not applied\n");
         if (dz.n_split > 1)
@@ -521,6 +601,16 @@ class EGS_GPUApplication {
                src.z);
         printf("direction: %.9g %.9g %.9g\n", src.ux, src.uy, src.uz);
         printf("dsource: %.9g\n", src.dsource);
+        if (src.kind == EGSGPU_SOURCE_PHSP) {
+            long long nrej = 0, nq[3] = {0, 0, 0};
+            for (int64_t k = 0; k < phsp.n; k++) {
+                nrej += phsp.wt[k] == 0;
+                nq[phsp.q[k] + 1]++;
+            }
+            printf("phsp: %lld %lld %lld %lld %lld %.9g\n", (long long)phsp.n, nq[0], nq[1], nq[2],
+                   nrej, (double)phsp.pinc);
+            printf("phsp plane: %.9g %.9g %.9g\n", src.x0, src.y0, src.z);
+        }
         if (src.spectrum == EGSGPU_SPEC_CDFINV && !spec_e.empty()) {
             size_t n = spec_e.size(), first = 0;
             while (first < n && spec_e[first] == spec_e[0]) first++;
@@ -530,9 +620,10 @@ class EGS_GPUApplication {
         printf("ncase: %lld\nseeds: %d %d\n", (long long)ncase, ixx, jxx);
         printf("ecut: %.6g\npcut: %.6g\n", tp.ecut, tp.pcut);
         printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
-               "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g\n",
+               "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d\n",
                tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
-               tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth);
+               tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth,
+               tp.howfarless);
     }
 
     void init_run() {
@@ -596,12 +687,11 @@ class EGS_GPUApplication {
             fprintf(stderr, "note: spin effects requested but the spin data are not available; running with Spin effects = Off\n");
             tp.spin = 0;
         }
-        if (dosxyz && tp.spin) {
-            fprintf(stderr, "note: Spin effects = On requested; the spin data are not available, "
-                            "running with Spin effects = Off\n");
-            tp.spin = 0;
-        }
         check(egsgpu_create(tab, &g, &src, &tp, &h), "creating the simulation");
+        if (src.kind == EGSGPU_SOURCE_PHSP)
+            check(egsgpu_set_phsp(h, phsp.n, phsp.x.data(), phsp.y.data(), phsp.u.data(),
+                                  phsp.v.data(), phsp.e.data(), phsp.wt.data(), phsp.q.data()),
+                  "phase-space particles");
         check(egsgpu_seed(h, ixx, jxx), "seeding");
         return 0;
     }

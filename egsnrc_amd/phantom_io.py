@@ -133,6 +133,69 @@ def apply_ramp(density, ramp, epsilon=2.0 ** -50):
     return med, rel
 
 
+RM = 0.5109989461
+
+
+def read_egsphsp(path):
+    """An EGSnrc MODE0/MODE2 phase-space file (the records EGS_PhspSource
+    reads, egs++/sources/egs_phsp_source/egs_phsp_source.cpp:79-431).
+    Returns a dict of header values and per-particle arrays: latch, e
+    (kinetic, MeV), x, y, u, v, wt (sign of w), q and new_history."""
+    raw = open(path, "rb").read()
+    mode = raw[:5]
+    if mode not in (b"MODE0", b"MODE2"):
+        raise ValueError("%s is not a MODE0 or MODE2 phase-space file" % path)
+    recl = 32 if mode == b"MODE2" else 28
+    for order in ("<", ">"):
+        n, nphot = np.frombuffer(raw[5:13], dtype=order + "i4")
+        emax, emin, pinc = np.frombuffer(raw[13:25], dtype=order + "f4")
+        if n > 0 and 0 <= nphot <= n and 0 <= emin <= emax and pinc >= 0:
+            break
+    else:
+        raise ValueError("%s: meaningless phase-space header" % path)
+    n = int(n)
+    fields = [("latch", order + "i4"), ("e", order + "f4"), ("x", order + "f4"),
+              ("y", order + "f4"), ("u", order + "f4"), ("v", order + "f4"),
+              ("wt", order + "f4")]
+    if recl == 32:
+        fields.append(("zlast", order + "f4"))
+    rec = np.frombuffer(raw[recl:recl * (n + 1)], dtype=np.dtype(fields))
+    latch = rec["latch"].astype(np.int64) & 0xFFFFFFFF
+    q = np.where(latch & 1073741824, -1, np.where(latch & 536870912, 1, 0)).astype(np.int8)
+    e = rec["e"].astype(np.float64)
+    new_history = e < 0
+    e = np.abs(e) - np.where(q != 0, RM, 0.0)
+    return {"mode2": recl == 32, "n": n, "nphot": int(nphot), "emax": float(emax),
+            "emin": float(emin), "pinc": float(pinc), "latch": latch, "e": e,
+            "x": rec["x"].copy(), "y": rec["y"].copy(), "u": rec["u"].copy(),
+            "v": rec["v"].copy(), "wt": rec["wt"].copy(), "q": q, "new_history": new_history}
+
+
+def write_egsphsp(path, e, x, y, u, v, wt, q, pinc, new_history=None, latch=None):
+    """Write a MODE0 file: e kinetic (MeV), wt with the sign of w, q the
+    charge; new_history marks the first particle of each primary history
+    (negative energy), latch bits 29/30 are set from q."""
+    n = len(e)
+    q = np.asarray(q, dtype=np.int64)
+    lat = np.zeros(n, np.int64) if latch is None else np.asarray(latch, np.int64)
+    lat = lat | np.where(q < 0, 1073741824, 0) | np.where(q > 0, 536870912, 0)
+    etot = np.asarray(e, np.float64) + np.where(q != 0, RM, 0.0)
+    if new_history is not None:
+        etot = np.where(new_history, -etot, etot)
+    ekin = np.asarray(e, np.float64)
+    nphot = int(np.sum(q == 0))
+    rec = np.zeros(n, dtype=[("latch", "<i4"), ("e", "<f4"), ("x", "<f4"), ("y", "<f4"),
+                             ("u", "<f4"), ("v", "<f4"), ("wt", "<f4")])
+    rec["latch"] = lat.astype(np.uint32).view(np.int32) if lat.dtype != np.int32 else lat
+    rec["e"] = etot
+    rec["x"], rec["y"], rec["u"], rec["v"], rec["wt"] = x, y, u, v, wt
+    head = b"MODE0" + np.array([n, nphot], "<i4").tobytes() + np.array(
+        [ekin.max(), ekin.min(), pinc], "<f4").tobytes()
+    with open(path, "wb") as f:
+        f.write(head + b"\0" * (28 - len(head)))
+        f.write(rec.tobytes())
+
+
 def read_3ddose(path):
     """Returns (xb, yb, zb, dose, rel_err), x fastest."""
     with _open_text(path) as f:

@@ -73,6 +73,9 @@ typedef struct EgsXYZGeometry {
                                          xcol x ycol at dsource (< 0) from
                                          iso, rotation rot; misses are
                                          resampled and counted             */
+#define EGSGPU_SOURCE_PHSP          3 /* phase-space particles given by
+                                         egsgpu_set_phsp, translated by
+                                         (x0, y0, z)                       */
 #define EGSGPU_SHAPE_POINT     0   /* EGS_PointShape                       */
 #define EGSGPU_SHAPE_RECTANGLE 1   /* EGS_RectangleShape (z = shape_z)     */
 typedef struct EgsSource {
@@ -109,6 +112,11 @@ typedef struct EgsTransport {
     double skin_depth;           /* skindepth_for_bca (default 3)          */
     int32_t kerma;               /* 1: deposit every electron locally      */
     int32_t fp64;                /* 1: run the FP64 build of the kernels   */
+    int32_t howfarless;          /* 1: DOSXYZnrc HOWFARLESS for a homogeneous
+                                    phantom: howfar/hownear see the outer
+                                    faces only, a step's energy is shared
+                                    along its path (dosxyznrc.mortran
+                                    :3546-3765); plain history schedule */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */
@@ -230,6 +238,19 @@ int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice);
  * the streams; tallies are kept.  Same as
  * egsgpu_set_rng(h, EGSGPU_RNG_RANMAR, ixx, jxx). */
 int egsgpu_seed(egsgpu_handle *h, int32_t ixx, int32_t jxx);
+
+/* Phase-space source (EGS_PhspSource::getNextParticle/readParticle,
+ * egs++/sources/egs_phsp_source/egs_phsp_source.cpp:285-431) for a
+ * simulation created with source kind EGSGPU_SOURCE_PHSP.  History i of the
+ * simulation (counted over all its runs) takes particle i mod n: position
+ * (x + x0, y + y0, z) with x0, y0, z of the EgsSource, direction
+ * (u, v, +-sqrt(1 - u^2 - v^2)) with the sign of wt, weight |wt|, kinetic
+ * energy e and charge q (-1, 0, 1).  A particle of weight 0 (rejected by a
+ * filter) or one that misses the grid ends its history at once.  The
+ * arrays are copied; must be called before the first run. */
+int egsgpu_set_phsp(egsgpu_handle *h, int64_t n, const float *x, const float *y,
+                    const float *u, const float *v, const float *e, const float *wt,
+                    const int8_t *q);
 
 /* Choose the generator that drives the transport and seed all streams
  * (any time; tallies are kept):

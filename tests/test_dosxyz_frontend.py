@@ -204,3 +204,19 @@ def test_isource1_spectrum_and_misses_match_the_oracle(tmp_path):
     # both per unit fluence: the oracle's beam covers the 8 x 8 cm face
     compare(E, sE, m * 64.0 / (16.0 * 16.0), s * 64.0 / (16.0 * 16.0), "isource 1 + spectrum",
             100)
+
+
+def test_parse_howfarless_only_for_a_homogeneous_phantom(tmp_path):
+    """ihowfarless (record 13, field 16) switches HOWFARLESS on for a
+    homogeneous phantom and is reported and ignored otherwise."""
+    f = tmp_path / "h.egsinp"
+    write_isource1_input(f)
+    txt = f.read_text().replace("100000, 0, 500.0, 33, 97, 100.0, 0, 0, 0, 0, , 0, 0, 0, 0, 0, 0",
+                                "100000, 0, 500.0, 33, 97, 100.0, 0, 0, 0, 0, , 0, 0, 0, 0, 1, 0")
+    f.write_text(txt)
+    p, err = parse(f)
+    assert p["transport"][-2:] == ["howfarless", "0"] and "not homogeneous" in err
+    g = tmp_path / "w.egsinp"
+    g.write_text(txt.replace("3, 6, 3, 6, 4, 6, 2, 0.0", "3, 6, 3, 6, 4, 6, 1, 0.0"))
+    p, err = parse(g)
+    assert p["transport"][-2:] == ["howfarless", "1"]

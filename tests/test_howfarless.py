@@ -1,0 +1,80 @@
+"""DOSXYZnrc's HOWFARLESS transport (dosxyznrc.mortran:3546-3765,
+3830-3860): in a homogeneous phantom howfar and hownear see only the outer
+faces of the grid, photons fly to their interaction points in one step and
+a charged-particle step's energy is shared among the voxels along its
+path.  It must give the dose of ordinary voxel-by-voxel transport (the
+oracle) within statistics."""
+import os
+
+import numpy as np
+import pytest
+
+from egsnrc_amd import _lib
+from egsnrc_amd.engine import Simulation, Tables, make_source
+from tests import oracle_lib, workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BLOB700 = os.path.join(ROOT, "egsnrc_amd", "data", "700icru.egstab")
+BLOB521 = os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab")
+
+
+def run_pair(blob, n_vox, dx, src, tp, n, rho=1.0):
+    xb = np.linspace(-n_vox * dx / 2, n_vox * dx / 2, n_vox + 1)
+    zb = np.linspace(0.0, n_vox * dx, n_vox + 1)
+    med = np.zeros(n_vox ** 3, np.int32)
+    rhor = np.full(n_vox ** 3, rho)
+    g, keep = _lib.make_geometry(xb, xb.copy(), zb, med, rhor)
+    o = oracle_lib.OracleSimulation(blob, g, src[0], tp, nstreams=256, nthreads=8)
+    o.run(n)
+    a = o.tallies()
+    tp.howfarless = 1
+    sim = Simulation(Tables(blob), xb, xb.copy(), zb, med, rhor, src, tp, nstreams=16384)
+    sim.run(n)
+    b = sim.tallies()
+    c = sim.counters()
+    tp.howfarless = 0
+    assert b[2] == n and c[5] == 0 and c[6] == 0
+    return a, b, c
+
+
+def check(a, b, label, min_voxels):
+    chi2, out2, nv = workloads.compare_doses(a, b)
+    print("%s: chi2/dof %.3f  >2sigma %.3f  voxels %d" % (label, chi2, out2, nv))
+    assert nv > min_voxels
+    assert chi2 < max(1.3, 1 + 3 * np.sqrt(2.0 / nv))
+    assert out2 < max(0.09, 0.0455 + 3 * np.sqrt(0.0455 * 0.9545 / nv))
+    ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
+    assert abs(ta - tb) < 0.02 * ta
+
+
+@pytest.mark.gpu
+def test_howfarless_photon_beam_in_water():
+    """C0-like: 1.25 MeV photons, 4 x 4 cm field, 20^3 x 1 cm water."""
+    src = make_source(0, 1.25, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    tp = _lib.default_transport(ecut=0.7, pcut=0.01, rayleigh=0)
+    a, b, c = run_pair(BLOB700, 20, 1.0, src, tp, 400000)
+    # photons cross no internal boundary (only the rare electron leaving
+    # the grid counts as a crossing)
+    assert c[1] < 0.01 * c[0]
+    check(a, b, "howfarless photons", 300)
+
+
+@pytest.mark.gpu
+def test_howfarless_electron_beam_in_lung_density_water():
+    """8 MeV electrons, 2 x 2 cm field, 16^3 x 0.5 cm water at relative
+    density 0.8: the path-length sharing of the steps that cross voxels."""
+    src = make_source(-1, 8.0, x0=-1, x1=1, y0=-1, y1=1, z=-1.0)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01)
+    a, b, c = run_pair(BLOB521, 16, 0.5, src, tp, 100000, rho=0.8)
+    check(a, b, "howfarless electrons", 200)
+
+
+@pytest.mark.gpu
+def test_howfarless_rejects_a_heterogeneous_phantom():
+    xb = np.linspace(-2, 2, 5)
+    med = np.zeros(64, np.int32)
+    med[5] = 1
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01, howfarless=1)
+    with pytest.raises(_lib.EngineError, match="homogeneous"):
+        Simulation(Tables(BLOB521), xb, xb.copy(), xb + 2, med, None,
+                   make_source(0, 1.0, z=-1.0), tp)
