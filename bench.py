@@ -430,6 +430,8 @@ def main():
     ap.add_argument("--precision-report", action="store_true",
                     help="only run the workload with FP32 and FP64 transport (same seeds) "
                          "and print the per-voxel difference summary")
+    ap.add_argument("--howfarless", action="store_true",
+                    help="DOSXYZnrc HOWFARLESS transport (homogeneous workloads: c0)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (the reference "
                          "kernel for scaling HBM traffic counters)")
@@ -481,6 +483,9 @@ def main():
                                   "voxels above 10% of Dmax; z = diff / combined sigma"})
         print(json.dumps(rep))
         return
+    if a.howfarless:
+        tp.howfarless = 1
+        a.schedule = None
     sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=a.lanes, block=a.block,
                      schedule=a.schedule)
     hps = a.hist or W["hist_per_step"]
@@ -522,6 +527,8 @@ def main():
     achieved = alg_bytes / kernel_s / 1e9
     knames = ["k_transport<float>", "k_ev_source<float>", "k_ev_photon<float>",
               "k_ev_electron<float>"]
+    if tp.kerma and a.schedule in (None, "history") and not a.howfarless:
+        knames[0] = "k_kerma<float>"   # the kerma-mode history kernel
     nl = max(1, int(klaunch[kdom]))
     traffic, traffic_src = read_traffic(knames[kdom], name)
     out = {
@@ -539,7 +546,8 @@ def main():
         "tally_dtype": "f64",
         "data": "synthetic phantom and source",
         "config": dict(cfg, histories_per_step=hps, parallelism="none",
-                       schedule=a.schedule or "default", graph=a.graph,
+                       schedule=a.schedule or ("history (HOWFARLESS)" if a.howfarless
+                                               else "default"), graph=a.graph,
                        **({"autotuned": dict(zip(["waves_photon", "waves_electron",
                                                   "defer_below", "batch", "photon_steps"],
                                                  tuned))} if tuned else {})),
