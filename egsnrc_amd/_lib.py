@@ -42,7 +42,8 @@ class EgsTransport(C.Structure):
                 ("brems_angle", C.c_int32), ("spin", C.c_int32),
                 ("exact_bca", C.c_int32), ("estepe", C.c_double),
                 ("ximax", C.c_double), ("skin_depth", C.c_double),
-                ("kerma", C.c_int32), ("fp64", C.c_int32), ("howfarless", C.c_int32)]
+                ("kerma", C.c_int32), ("fp64", C.c_int32), ("howfarless", C.c_int32),
+                ("n_split", C.c_int32)]
 
 
 def default_transport(**kw):
@@ -50,7 +51,8 @@ def default_transport(**kw):
     t = EgsTransport(ecut=0.0, pcut=0.0, rayleigh=1, bound_compton=3,
                      photoelectron_angle=1, atomic_relaxation=1, pair_angle=1,
                      brems_angle=1, spin=0, exact_bca=1, estepe=0.25,
-                     ximax=0.5, skin_depth=3.0, kerma=0, fp64=0, howfarless=0)
+                     ximax=0.5, skin_depth=3.0, kerma=0, fp64=0, howfarless=0,
+                     n_split=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t

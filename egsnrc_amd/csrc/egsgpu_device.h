@@ -99,6 +99,8 @@ struct DevSource {
 
 struct DevParams {
     int rayleigh, ibcmp, iphter, iedgfl, iprdst, ibrdst, exact_bca, kerma;
+    int n_split;              // DOSXYZnrc photon splitting number (0 = off)
+    int pad;
     double ecut, pcut, skindepth;
 };
 

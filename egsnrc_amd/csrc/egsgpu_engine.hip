@@ -52,8 +52,9 @@ struct KArgs {
 };
 
 // EGS_AliasTable::sample (egs_alias_table.cpp:268-300), types 0 and 1
-template <typename R, bool EV, bool LUX, bool HFL>
-__device__ __forceinline__ R source_energy(const DevSource<R> &S, Shower<R, EV, LUX, HFL> &sh) {
+template <typename R, bool EV, bool LUX, bool HFL, bool SPLIT>
+__device__ __forceinline__ R source_energy(const DevSource<R> &S,
+                                          Shower<R, EV, LUX, HFL, SPLIT> &sh) {
     if (S.spectrum == 0) return S.energy;
     if (S.spectrum == EGSGPU_SPEC_CDFINV) {
         // DOSXYZnrc srchst (srcxyznrc.mortran:3440-3446): grid point
@@ -102,9 +103,9 @@ __device__ __forceinline__ int planes_first(double x, int n, const R *p) {
 // DOSXYZnrc isource 0, srchst (srcxyznrc.mortran:3416-3448): position on
 // the front face z = zbound(1) (x drawn, then y, each only if its range
 // is not empty), fixed direction, then the energy.
-template <typename R, bool EV, bool LUX, bool HFL>
+template <typename R, bool EV, bool LUX, bool HFL, bool SPLIT>
 __device__ __forceinline__ bool sample_dosxyz_front(const DevSource<R> &S, const GeomView<R> &G,
-                                                    Shower<R, EV, LUX, HFL> &sh, Part<R> &q) {
+                                                    Shower<R, EV, LUX, HFL, SPLIT> &sh, Part<R> &q) {
     double x = (double)S.x0, y = (double)S.y0;
     double dx = (double)S.x1 - (double)S.x0, dy = (double)S.y1 - (double)S.y0;
     if (dx != 0) x = (double)S.x0 + (double)sh.rnd() * dx;
@@ -123,9 +124,9 @@ __device__ __forceinline__ bool sample_dosxyz_front(const DevSource<R> &S, const
 // into position at distance dsource from the isocentre, then carried to
 // the first face of the grid it meets; a beam particle that misses the
 // grid is counted (sh.c_miss) and sampled again.
-template <typename R, bool EV, bool LUX, bool HFL>
+template <typename R, bool EV, bool LUX, bool HFL, bool SPLIT>
 __device__ __forceinline__ bool sample_dosxyz_beam(const DevSource<R> &S, const GeomView<R> &G,
-                                                   Shower<R, EV, LUX, HFL> &sh, Part<R> &q) {
+                                                   Shower<R, EV, LUX, HFL, SPLIT> &sh, Part<R> &q) {
     const double *r = S.rot;
     const double u = r[2], v = r[5], w = r[8];
     const double x0 = G.xp[0], x1 = G.xp[G.nx], y0 = G.yp[0], y1 = G.yp[G.ny];
@@ -217,9 +218,9 @@ __device__ __forceinline__ bool sample_phsp(const DevSource<R> &S, const GeomVie
 // The DOSXYZnrc sources (isource 0 and 1) have their own sampling; a
 // phase-space source takes particle hidx (the history's index in the
 // simulation).
-template <typename R, bool EV, bool LUX, bool HFL>
+template <typename R, bool EV, bool LUX, bool HFL, bool SPLIT>
 __device__ __forceinline__ bool sample_primary(const DevSource<R> &S, const GeomView<R> &G,
-                               Shower<R, EV, LUX, HFL> &sh, Part<R> &q, long long hidx) {
+                               Shower<R, EV, LUX, HFL, SPLIT> &sh, Part<R> &q, long long hidx) {
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_FRONT) return sample_dosxyz_front(S, G, sh, q);
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_BEAM) return sample_dosxyz_beam(S, G, sh, q);
     if (S.kind == EGSGPU_SOURCE_PHSP) return sample_phsp(S, G, hidx, q);
@@ -258,8 +259,9 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // Add this wave's event counters to the kernel's totals: one atomic per
 // counter and wave.  Called by all lanes of every wave.
-template <typename R, bool EV, bool LUX, bool HFL>
-__device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, EV, LUX, HFL> &sh,
+template <typename R, bool EV, bool LUX, bool HFL, bool SPLIT>
+__device__ __forceinline__ void add_counters(const DevTally &t,
+                                             const Shower<R, EV, LUX, HFL, SPLIT> &sh,
                                              int kernel) {
     unsigned long long *c = t.counters + 8 * kernel;
     const unsigned v[8] = {sh.c_psteps, sh.c_cross, sh.c_dep, sh.c_flush,
@@ -272,13 +274,14 @@ __device__ __forceinline__ void add_counters(const DevTally &t, const Shower<R, 
 }
 
 // History schedule: lane s runs its share of the histories one after the
-// other with ranmar stream s, like egs-parallel job s.
-template <typename R, bool LUX, bool HFL = false>
+// other with ranmar stream s, like egs-parallel job s.  SPLIT: every photon
+// goes through DOSXYZnrc's photon splitting.
+template <typename R, bool LUX, bool HFL = false, bool SPLIT = false>
 __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.L.nlanes;   // lanes past the last stream idle
-    using S = Shower<R, false, LUX, HFL>;
+    using S = Shower<R, false, LUX, HFL, SPLIT>;
     S sh(a.T, a.G, a.vox, a.P, a.L, a.tally, live ? lane : 0, rbuf_all + threadIdx.x);
     sh.grad = a.grad;
     int64_t nh = live ? a.base + (lane < a.rem ? 1 : 0) : 0;
@@ -291,7 +294,8 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
         while (sh.np > 0) {
             sh.rtopup();
             if (sh.p.iq == 0) {
-                sh.photon();
+                if constexpr (SPLIT) sh.photon_split();
+                else sh.photon();
                 sh.eph = S::kNew;
             } else if (sh.eph >= S::kBrems) {
                 sh.electron_interact();
@@ -1099,11 +1103,13 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
     L.rng = h->rng;
     L.rlx = h->rlx;
     L.mxstk = h->mxstk;
-    if ((double)h->mxstk * kStkF * h->nlanes * sizeof(R) >= 4294967296.0) {
+    // photon splitting keeps the split photon's record past the stack
+    const int nstk = h->mxstk + (h->P.n_split ? 1 : 0);
+    if ((double)nstk * kStkF * h->nlanes * sizeof(R) >= 4294967296.0) {
         set_error("particle stacks of %d streams exceed 4 GB", h->nlanes);
         return EGSGPU_E_ARG;
     }
-    CK(alloc(h, (size_t)h->mxstk * kStkF * h->nlanes, &L.stk));
+    CK(alloc(h, (size_t)nstk * kStkF * h->nlanes, &L.stk));
     // per-lane history tables: history schedule only
     L.hcap = h->schedule != EGSGPU_SCHED_EVENT ? h->hcap : 0;
     size_t nh = (size_t)L.hcap * h->nlanes;
@@ -1138,10 +1144,15 @@ int ensure_lanes(egsgpu_handle *h) {
     if (h->lanes_ready) return EGSGPU_OK;
     // default streams: what the kernels can keep resident (event kernels
     // up to 4 waves per SIMD, the history kernel 2)
-    if (!h->nlanes)
+    if (!h->nlanes) {
         h->nlanes = h->ncu * (h->schedule == EGSGPU_SCHED_EVENT ? 1024
                               : h->P.kerma && h->schedule == EGSGPU_SCHED_HISTORY
                                     ? 256 * kKermaWaves : 512);
+        // the deep stacks of photon splitting: fewer streams, stacks < 4 GB
+        const size_t per = (size_t)(h->mxstk + 1) * kStkF * (h->fp64 ? 8 : 4);
+        const int cap = (int)std::min<size_t>(4294967295ull / per / 256 * 256, 1 << 30);
+        if (h->nlanes > cap) h->nlanes = std::max(cap, 256);
+    }
     CK(alloc(h, (size_t)h->nlanes, &h->rng));
     CK(alloc(h, (size_t)h->nlanes, &h->rlx));
     if (h->fp64) CK(build_lanes(h, h->pd));
@@ -1192,7 +1203,20 @@ int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = n
     EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
     dim3 grid((h->nlanes + h->block - 1) / h->block);
     const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
-    if (h->howfarless) {   // HOWFARLESS: the plain history loop, ranmar or ranlux
+    if (h->P.n_split > 1) {   // photon splitting: the plain history loop
+        if (h->howfarless && lux)
+            hipLaunchKernelGGL((k_transport<R, true, true, true>), grid, dim3(h->block), lds,
+                               h->stream, a);
+        else if (h->howfarless)
+            hipLaunchKernelGGL((k_transport<R, false, true, true>), grid, dim3(h->block), lds,
+                               h->stream, a);
+        else if (lux)
+            hipLaunchKernelGGL((k_transport<R, true, false, true>), grid, dim3(h->block), lds,
+                               h->stream, a);
+        else
+            hipLaunchKernelGGL((k_transport<R, false, false, true>), grid, dim3(h->block), lds,
+                               h->stream, a);
+    } else if (h->howfarless) {   // HOWFARLESS: the plain history loop, ranmar or ranlux
         if (lux)
             hipLaunchKernelGGL((k_transport<R, true, true>), grid, dim3(h->block), lds, h->stream,
                                a);
@@ -1554,6 +1578,10 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         set_error("only the exact boundary-crossing algorithm is implemented");
         return EGSGPU_E_UNSUPPORTED;
     }
+    if (tp->n_split < 0 || tp->n_split > 256) {
+        set_error("n_split = %d: photon splitting takes 0..256", (int)tp->n_split);
+        return EGSGPU_E_ARG;
+    }
     if (tp->howfarless) {   // one medium and one density everywhere
         const int64_t nv = (int64_t)geom->nx * geom->ny * geom->nz;
         for (int64_t i = 1; i < nv; i++)
@@ -1615,6 +1643,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->P.iphter = tp->photoelectron_angle; h->P.iedgfl = tp->atomic_relaxation;
     h->P.iprdst = tp->pair_angle; h->P.ibrdst = tp->brems_angle;
     h->P.exact_bca = tp->exact_bca; h->P.kerma = tp->kerma;
+    h->P.n_split = tp->n_split > 1 ? tp->n_split : 0;
     h->P.skindepth = tp->skin_depth > 1e-4 ? tp->skin_depth : 3.0;
     try {
         if (h->fp64) {
@@ -1653,6 +1682,13 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->schedule = tp->kerma ? EGSGPU_SCHED_HISTORY : EGSGPU_SCHED_EVENT;
     h->howfarless = tp->howfarless != 0;
     if (h->howfarless) h->schedule = EGSGPU_SCHED_HISTORY_PLAIN;
+    if (h->P.n_split) {
+        // a split leaves about one charged particle per sub-photon on the
+        // stack; a photon from a radiative event of one of them is split
+        // on top of those
+        h->schedule = EGSGPU_SCHED_HISTORY_PLAIN;
+        h->mxstk = std::max(h->mxstk, 4 * h->P.n_split + 64);
+    }
     // electron beams touch more voxels per history than photon beams
     h->slot_hcap = tp->kerma ? 64 : (src->charge || src->kind == EGSGPU_SOURCE_PHSP ? 512 : 256);
     *out = h;
@@ -1817,8 +1853,8 @@ int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule) {
     EGS_ARG(h, "handle");
     EGS_ARG(schedule == EGSGPU_SCHED_HISTORY || schedule == EGSGPU_SCHED_EVENT ||
             schedule == EGSGPU_SCHED_HISTORY_PLAIN, "schedule");
-    if (h->howfarless && schedule != EGSGPU_SCHED_HISTORY_PLAIN) {
-        set_error("HOWFARLESS runs on the plain history schedule");
+    if ((h->howfarless || h->P.n_split) && schedule != EGSGPU_SCHED_HISTORY_PLAIN) {
+        set_error("HOWFARLESS and photon splitting run on the plain history schedule");
         return EGSGPU_E_UNSUPPORTED;
     }
     if (h->lanes_ready) {
@@ -2070,6 +2106,10 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
     }
     if (h->schedule == EGSGPU_SCHED_EVENT) {
         set_error("the density gradient replays histories: it needs the history schedule");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    if (h->P.n_split) {
+        set_error("the density gradient's score function assumes unsplit photons");
         return EGSGPU_E_UNSUPPORTED;
     }
     if (h->nvox >= (1 << kGradVoxBits)) {

@@ -36,7 +36,8 @@ __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 // may be spread over lanes and kernels; per-slot shared tally tables).
 // LUX = false: ranmar streams; true: ranlux streams.
 // HFL = true: DOSXYZnrc's HOWFARLESS transport for a homogeneous phantom.
-template <typename R, bool EV = false, bool LUX = false, bool HFL = false>
+// SPLIT = true: DOSXYZnrc's photon splitting (P.n_split > 1).
+template <typename R, bool EV = false, bool LUX = false, bool HFL = false, bool SPLIT = false>
 struct Shower {
     const DevTables<R> &T;
     const GeomView<R> *Gp;   // geometry of the current particle's simulation
@@ -311,6 +312,14 @@ struct Shower {
         q.e = sk(k, 0); q.x = sk(k, 1); q.y = sk(k, 2); q.z = sk(k, 3);
         q.u = sk(k, 4); q.v = sk(k, 5); q.w = sk(k, 6); q.wt = sk(k, 7);
         q.dnear = sk(k, 8); q.iq = bitsi(sk(k, 9)); q.ir = bitsi(sk(k, 10));
+    }
+    // push particle q; it is the new top
+    __device__ __forceinline__ bool push(const Part<R> &q) {
+        if (np >= L.mxstk) { c_ovf++; return false; }
+        if (np > 0) store(np - 1, p);
+        p = q;
+        np++;
+        return true;
     }
     // push a copy of the top ($TRANSFER PROPERTIES); the copy is the new top
     __device__ __forceinline__ bool push_copy() {
@@ -1411,11 +1420,140 @@ struct Shower {
     __device__ __forceinline__ void electron_interact() {
         const int m = e_med;
         const MedScalars<R> &M = T.ms[m];
+        const int npold = np;
         if (eph == kBrems) brems(m, M, e_elke);
         else if (eph == kMoller) moller(M);
         else if (eph == kBhabha) bhabha(M);
         else annih();
+        if constexpr (SPLIT) {   // ausgab iarg 7 and 13 (dosxyznrc.mortran:3531-3545)
+            if (eph == kBrems || eph == kAnnih) rr_photons(npold);
+        }
         eph = kNew;
+    }
+
+    // ---- DOSXYZnrc photon splitting (n_split) ---------------------------
+    // Russian roulette of the photons made by a radiative event of a
+    // charged particle (bremsstrahlung, annihilation in flight or at rest:
+    // ausgab iarg 7, 13, 14, dosxyznrc.mortran:3531-3545): each survives
+    // with probability 1/n_split and then carries n_split times its weight.
+    // Entries npold-1 .. np-1 of the stack are the event's products.
+    __device__ __forceinline__ void rr_photons(int npold) {
+        const int ns = P.n_split;
+        for (int k = npold - 1 > 0 ? npold - 1 : 0; k < np; k++) {
+            const bool top = k == np - 1;
+            const int iq = top ? p.iq : bitsi(sk(k, 9));
+            if (iq != 0) continue;
+            R xsi = rnd();
+            if (xsi * (R)ns > 1) {
+                if (top) { p.wt = 0; p.e = 0; } else { sk(k, 7) = 0; sk(k, 0) = 0; }
+            } else {
+                if (top) p.wt = p.wt * (R)ns; else sk(k, 7) = sk(k, 7) * (R)ns;
+            }
+        }
+    }
+
+    // swap stack entries a > b (entry np-1 is the top, in registers)
+    __device__ void swap_down(int a, int b) {
+        if (a == b) return;
+        Part<R> qa, qb;
+        if (a == np - 1) qa = p; else load(a, qa);
+        load(b, qb);
+        store(b, qa);
+        if (a == np - 1) p = qb; else store(a, qb);
+    }
+
+    // $SELECT-PHOTON-MFP of DOSXYZnrc with photon splitting
+    // (dosxyznrc.mortran:855-1030): the photon on top is replaced by
+    // n_split sub-photons of 1/n_split of its weight whose interaction
+    // points are stratified in the number of mean free paths along its
+    // line of flight (the i-th after -log(1 - (eta+i-1)/n_split) MFPs, one
+    // eta for all); each interacts in turn from where the previous one did.
+    // Charged products keep the sub-photon's weight; photon products (and a
+    // Rayleigh-scattered sub-photon) survive only for the one sub-photon
+    // chosen at random, with the full weight; the others are removed.  A
+    // photon then on top is split again; otherwise PHOTON returns.
+    //
+    // The survivor's photons are moved below the charged products of the
+    // same split, so that those are transported first: the stack then
+    // holds about one split's products instead of growing by the charged
+    // particles left under the survivor at every scattering generation.
+    // Only the order in which a history's particles are transported
+    // changes.  The parent's record waits in the entry past the stack.
+    __device__ __forceinline__ void photon_split() {
+        const int ns = P.n_split;
+        const int isave = L.mxstk;
+        const R d_eta = (R)1 / (R)ns;
+        for (;;) {
+            { VoxData<R> d = vox[p.ir]; p_med = d.med; p_rhof = d.rho; }
+            if (p.e <= (R)P.pcut) { score(p.ir, p.e, p.wt); pop(); return; }
+            if (p.wt == 0) { pop(); return; }
+            p_gle = dlog(p.e);
+            const R r35 = rnd() / (R)ns;
+            p.wt = p.wt / (R)ns;
+            p.dnear = 0;
+            store(isave, p);
+            pop();
+            const int zb = np;   // first stack index of this split's products
+            int nz = 0;          // survivor photons moved to zb, zb+1, ...
+            const R xxx = rnd();
+            const int isurv = 1 + (int)(xxx * (R)ns);
+            R dpold = 0, eta = 1 - r35 + d_eta;
+            for (int is = 1; is <= ns; is++) {
+                rtopup();
+                eta = eta - d_eta;
+                if (eta <= 0) break;
+                const R dp = -dlog(eta) - dpold;
+                dpold = dpold + dp;
+                {
+                    Part<R> q;
+                    load(isave, q);
+                    if (!push(q)) break;
+                }
+                { VoxData<R> d = vox[p.ir]; p_med = d.med; p_rhof = d.rho; }
+                p_cohfac = 1;
+                photon_medium();
+                p_dpmfp = dp;
+                pph = kPTrans;
+                photon_trans(1 << 30);
+                if (pph == kPEnter) break;   // it left the grid: so would the rest
+                sk(isave, 1) = p.x; sk(isave, 2) = p.y; sk(isave, 3) = p.z;
+                sk(isave, 10) = ibits(p.ir);
+                const int m = p_med;
+                const MedScalars<R> &M = T.ms[m];
+                if (pph == kPRayl) {
+                    if (is != isurv) { pop(); continue; }
+                    p.wt = p.wt * (R)ns;
+                    int li = m * kGE + p_lgle - 1;
+                    R costhe, sinthe;
+                    rayleigh_sample(m, p.e, p_gle, li, costhe, sinthe);
+                    uphi21(costhe, sinthe);
+                    swap_down(np - 1, zb + nz++);
+                    continue;
+                }
+                const int npold = np;
+                if (pph == kPPair) pair(m, M);
+                else if (pph == kPCompt) compt(m, M);
+                else photo(m, M, p_gle);
+                // photon products: removed, or the survivor's at full weight
+                int k = npold - 1;
+                while (k < np) {
+                    const bool top = k == np - 1;
+                    const int iq = top ? p.iq : bitsi(sk(k, 9));
+                    if (iq == 0 && is != isurv) {
+                        if (!top) store(k, p);   // the top takes the place
+                        pop();
+                    } else {
+                        if (iq == 0) {
+                            if (top) p.wt = p.wt * (R)ns; else sk(k, 7) = sk(k, 7) * (R)ns;
+                            swap_down(k, zb + nz++);
+                        }
+                        k++;
+                    }
+                }
+            }
+            pph = kPEnter;
+            if (np <= 0 || p.iq != 0) return;
+        }
     }
 
     // msdist_pII (egsnrc.mortran:4050-4308), spin effects off
@@ -1790,6 +1928,7 @@ struct Shower {
 
     __device__ void annih_at_rest() {
         const R RM = (R)0.5109989461;
+        const int npold = np;
         R costhe = rnd();
         costhe = 2 * costhe - 1;
         R s2 = (1 - costhe) * (1 + costhe);
@@ -1800,6 +1939,7 @@ struct Shower {
         p.u = sinthe * cphi; p.v = sinthe * sphi; p.w = costhe;
         if (!push_copy()) return;
         p.u = -p.u; p.v = -p.v; p.w = -p.w;
+        if constexpr (SPLIT) rr_photons(npold);   // ausgab iarg 14
     }
 };
 

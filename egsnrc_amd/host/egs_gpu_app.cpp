@@ -501,8 +501,9 @@ class EGS_GPUApplication {
         }
         if (dz.ireject)
             fprintf(stderr, "note: charged-particle range rejection requested; not applied\n");
-        if (dz.n_split > 1)
-            fprintf(stderr, "note: photon splitting requested; histories are not split\n");
+        // photon splitting with Russian roulette of radiative photons
+        // (dosxyznrc.mortran:855-1030, 2557-2562)
+        if (dz.n_split > 1) tp.n_split = dz.n_split;
         if (dz.i_phsp_out)
             fprintf(stderr, "note: phase-space output requested; not written\n");
         if (dz.irestart)
@@ -620,10 +621,11 @@ class EGS_GPUApplication {
         printf("ncase: %lld\nseeds: %d %d\n", (long long)ncase, ixx, jxx);
         printf("ecut: %.6g\npcut: %.6g\n", tp.ecut, tp.pcut);
         printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
-               "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d\n",
+               "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d "
+               "n_split %d\n",
                tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
                tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth,
-               tp.howfarless);
+               tp.howfarless, tp.n_split);
     }
 
     void init_run() {

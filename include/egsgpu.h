@@ -117,6 +117,12 @@ typedef struct EgsTransport {
                                     faces only, a step's energy is shared
                                     along its path (dosxyznrc.mortran
                                     :3546-3765); plain history schedule */
+    int32_t n_split;             /* DOSXYZnrc photon splitting (n_split,
+                                    dosxyznrc.mortran:855-1030, 3531-3545):
+                                    > 1 splits every photon into n_split
+                                    sub-photons and plays Russian roulette
+                                    with radiative photons; 0 or 1 = off,
+                                    at most 256; plain history schedule */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */

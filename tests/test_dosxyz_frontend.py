@@ -215,8 +215,26 @@ def test_parse_howfarless_only_for_a_homogeneous_phantom(tmp_path):
                                 "100000, 0, 500.0, 33, 97, 100.0, 0, 0, 0, 0, , 0, 0, 0, 0, 1, 0")
     f.write_text(txt)
     p, err = parse(f)
-    assert p["transport"][-2:] == ["howfarless", "0"] and "not homogeneous" in err
+    assert transport_options(p)["howfarless"] == "0" and "not homogeneous" in err
     g = tmp_path / "w.egsinp"
     g.write_text(txt.replace("3, 6, 3, 6, 4, 6, 2, 0.0", "3, 6, 3, 6, 4, 6, 1, 0.0"))
     p, err = parse(g)
-    assert p["transport"][-2:] == ["howfarless", "1"]
+    assert transport_options(p)["howfarless"] == "1"
+
+
+def transport_options(p):
+    t = p["transport"]
+    return dict(zip(t[::2], t[1::2]))
+
+
+def test_parse_photon_splitting(tmp_path):
+    """n_split (record 13, field 15) is handed to the transport; 0 and 1
+    leave photons unsplit."""
+    f = tmp_path / "s.egsinp"
+    write_isource1_input(f)
+    assert transport_options(parse(f)[0])["n_split"] == "0"
+    txt = f.read_text().replace("100000, 0, 500.0, 33, 97, 100.0, 0, 0, 0, 0, , 0, 0, 0, 0, 0, 0",
+                                "100000, 0, 500.0, 33, 97, 100.0, 0, 0, 0, 0, , 0, 0, 0, 40, 0, 0")
+    assert txt != f.read_text()
+    f.write_text(txt)
+    assert transport_options(parse(f)[0])["n_split"] == "40"

@@ -1,0 +1,151 @@
+"""DOSXYZnrc photon splitting (n_split, dosxyznrc.mortran:855-1030): every
+photon is replaced by n_split sub-photons of 1/n_split of its weight with
+stratified interaction points; photons scattered by all but one randomly
+chosen sub-photon are removed and the survivor's carry the full weight;
+photons of bremsstrahlung and annihilation in flight or at rest play
+Russian roulette (ausgab 7, 13, 14: :3531-3545, switched on at
+:2557-2562).
+
+Splitting is unbiased: the split run's expected dose is that of the
+unsplit oracle, checked voxel by voxel against its statistics, with a
+plainly wrong weight (no 1/n_split, or no n_split on a survivor) far
+outside them.  Its variance per history is lower where the dose comes from
+the photons' first interactions, which shows that the photons were split."""
+import os
+
+import numpy as np
+import pytest
+
+from egsnrc_amd import _lib
+from egsnrc_amd.engine import Simulation, Tables, make_source
+from tests import oracle_lib, workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BLOB700 = os.path.join(ROOT, "egsnrc_amd", "data", "700icru.egstab")
+BLOB521 = os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab")
+
+
+def oracle(blob, xb, yb, zb, med, rho, src, tp, n):
+    g, keep = _lib.make_geometry(xb, yb, zb, med, rho)
+    o = oracle_lib.OracleSimulation(blob, g, src[0], tp, nstreams=256, nthreads=8)
+    o.run(n)
+    return o.tallies()
+
+
+def stats(t):
+    s, s2, n = t
+    m = s / n
+    return m, np.maximum(s2 / n - m * m, 0) / (n - 1)
+
+
+def check(a, b, label, min_voxels, frac=0.1):
+    chi2, out2, nv = workloads.compare_doses(a, b, frac)
+    print("%s: chi2/dof %.3f  >2sigma %.3f  voxels %d" % (label, chi2, out2, nv))
+    assert nv >= min_voxels
+    assert chi2 < max(1.3, 1 + 3 * np.sqrt(2.0 / nv))
+    assert out2 < max(0.09, 0.0455 + 3 * np.sqrt(0.0455 * 0.9545 / nv))
+
+
+def per_history_variance_ratio(a, b, frac=0.1):
+    """Median over the dosed voxels of b's variance per history over a's."""
+    ma, va = stats(a)
+    mb, vb = stats(b)
+    keep = (ma > frac * ma.max()) & (va > 0) & (vb > 0)
+    return float(np.median((vb[keep] * b[2]) / (va[keep] * a[2])))
+
+
+@pytest.mark.gpu
+def test_split_photon_beam_matches_the_unsplit_oracle():
+    """1.25 MeV photons, 4 x 4 cm field, 16 x 16 x 20 water voxels of 1 cm
+    with a bone slab at 5-7 cm; n_split 20, FP32 and ranmar."""
+    xb = np.linspace(-8, 8, 17)
+    zb = np.linspace(0, 20, 21)
+    med = np.zeros(16 * 16 * 20, np.int32)
+    kz = np.arange(med.size) // 256
+    med[(kz == 5) | (kz == 6)] = 1
+    src = make_source(0, 1.25, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    tp = _lib.default_transport(ecut=0.7, pcut=0.01)
+    a = oracle(BLOB700, xb, xb.copy(), zb, med, None, src, tp, 400000)
+    tp.n_split = 20
+    sim = Simulation(Tables(BLOB700), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
+    n = 100000
+    sim.run(n)
+    b = sim.tallies()
+    c = sim.counters()
+    assert b[2] == n and c[5] == 0
+    check(a, b, "split photons", 300)
+    ratio = per_history_variance_ratio(a, b)
+    print("variance per history, split / unsplit: %.3f" % ratio)
+    assert ratio < 0.6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("charge,energy,fp64", [(-1, 10.0, 0), (1, 6.0, 1)])
+def test_russian_roulette_of_radiative_photons(charge, energy, fp64):
+    """Electrons (bremsstrahlung) and positrons (bremsstrahlung and
+    annihilation in flight and at rest) on water slabs, n_split 10: the
+    slabs beyond the particles' range see only photons that survived the
+    roulette, and their dose must be the oracle's."""
+    xb = np.array([-10.0, 10.0])
+    zb = np.concatenate([np.arange(0, 6.5, 0.5), [7.0, 8.0, 10.0, 13.0, 17.0, 22.0]]) \
+        * (energy / 10.0)
+    nz = len(zb) - 1
+    med = np.zeros(nz, np.int32)
+    src = make_source(charge, energy, x0=-1, x1=1, y0=-1, y1=1, z=-1.0)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01, fp64=fp64)
+    n = 400000
+    a = oracle(BLOB521, xb, xb.copy(), zb, med, None, src, tp, n)
+    tp.n_split = 10
+    sim = Simulation(Tables(BLOB521), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
+    sim.run(n)
+    b = sim.tallies()
+    assert b[2] == n and sim.counters()[5] == 0
+    check(a, b, "roulette q=%d" % charge, nz - 2, frac=1e-4)
+    # the tail slabs alone: photon dose only
+    ma, va = stats(a)
+    mb, vb = stats(b)
+    tail = slice(nz - 4, nz)
+    z = (ma[tail] - mb[tail]) / np.sqrt(va[tail] + vb[tail])
+    print("tail slabs: oracle", ma[tail], "split", mb[tail], "z", z)
+    assert np.all(ma[tail] > 0) and np.all(np.abs(z) < 4)
+
+
+@pytest.mark.gpu
+def test_split_with_howfarless_and_ranlux():
+    """HOWFARLESS in homogeneous water with ranlux streams and n_split 10:
+    photon splitting on the one-step photon flights of HOWFARLESS."""
+    xb = np.linspace(-8, 8, 17)
+    zb = np.linspace(0, 16, 17)
+    med = np.zeros(16 ** 3, np.int32)
+    src = make_source(0, 2.0, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    tp = _lib.default_transport(ecut=0.7, pcut=0.01, rayleigh=0)
+    a = oracle(BLOB700, xb, xb.copy(), zb, med, None, src, tp, 400000)
+    tp.n_split = 10
+    tp.howfarless = 1
+    sim = Simulation(Tables(BLOB700), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
+    sim.set_ranlux(1, 7)
+    n = 100000
+    sim.run(n)
+    b = sim.tallies()
+    c = sim.counters()
+    assert b[2] == n and c[5] == 0
+    check(a, b, "split + howfarless + ranlux", 200)
+    assert per_history_variance_ratio(a, b) < 0.8
+
+
+@pytest.mark.gpu
+def test_split_rejects_other_schedules_and_the_gradient():
+    xb = np.linspace(-2, 2, 5)
+    med = np.zeros(64, np.int32)
+    tp = _lib.default_transport(ecut=0.7, pcut=0.01, n_split=4)
+    with pytest.raises(_lib.EngineError, match="plain history"):
+        Simulation(Tables(BLOB700), xb, xb.copy(), xb + 2, med, None,
+                   make_source(0, 1.0, z=-1.0), tp, schedule="event")
+    tp.kerma = 1
+    sim = Simulation(Tables(BLOB700), xb, xb.copy(), xb + 2, med, None,
+                     make_source(0, 1.0, z=-1.0), tp)
+    with pytest.raises(_lib.EngineError, match="unsplit"):
+        sim.density_grad(1000, np.zeros(64))
+    with pytest.raises(_lib.EngineError, match="0..256"):
+        Simulation(Tables(BLOB700), xb, xb.copy(), xb + 2, med, None,
+                   make_source(0, 1.0, z=-1.0), _lib.default_transport(n_split=300))
