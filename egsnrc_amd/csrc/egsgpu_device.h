@@ -100,6 +100,8 @@ struct DevSource {
 struct DevParams {
     int rayleigh, ibcmp, iphter, iedgfl, iprdst, ibrdst, exact_bca, kerma;
     int n_split;              // DOSXYZnrc photon splitting number (0 = off)
+    int nbr_split;            // radiative splitting number (0 = off)
+    int play_rr;              // Russian roulette of photon-made charged particles
     int pad;
     double ecut, pcut, skindepth;
 };

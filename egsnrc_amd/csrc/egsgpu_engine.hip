@@ -294,8 +294,12 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
         while (sh.np > 0) {
             sh.rtopup();
             if (sh.p.iq == 0) {
-                if constexpr (SPLIT) sh.photon_split();
-                else sh.photon();
+                if constexpr (SPLIT) {
+                    if (a.P.n_split > 1) sh.photon_split();
+                    else sh.photon();
+                } else {
+                    sh.photon();
+                }
                 sh.eph = S::kNew;
             } else if (sh.eph >= S::kBrems) {
                 sh.electron_interact();
@@ -1104,7 +1108,7 @@ int build_lanes(egsgpu_handle *h, Precision<R> &P) {
     L.rlx = h->rlx;
     L.mxstk = h->mxstk;
     // photon splitting keeps the split photon's record past the stack
-    const int nstk = h->mxstk + (h->P.n_split ? 1 : 0);
+    const int nstk = h->mxstk + 1;
     if ((double)nstk * kStkF * h->nlanes * sizeof(R) >= 4294967296.0) {
         set_error("particle stacks of %d streams exceed 4 GB", h->nlanes);
         return EGSGPU_E_ARG;
@@ -1203,7 +1207,7 @@ int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = n
     EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
     dim3 grid((h->nlanes + h->block - 1) / h->block);
     const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
-    if (h->P.n_split > 1) {   // photon splitting: the plain history loop
+    if (h->P.n_split > 1 || h->P.nbr_split > 1) {   // splitting: the plain history loop
         if (h->howfarless && lux)
             hipLaunchKernelGGL((k_transport<R, true, true, true>), grid, dim3(h->block), lds,
                                h->stream, a);
@@ -1582,6 +1586,20 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         set_error("n_split = %d: photon splitting takes 0..256", (int)tp->n_split);
         return EGSGPU_E_ARG;
     }
+    if (tp->nbr_split < 0 || tp->nbr_split > 256) {
+        set_error("nbr_split = %d: radiative splitting takes 0..256", (int)tp->nbr_split);
+        return EGSGPU_E_ARG;
+    }
+    if (tp->n_split > 1 && tp->nbr_split > 1) {
+        set_error("photon splitting (n_split) and radiative splitting (nbr_split) are "
+                  "separate variance-reduction schemes; choose one");
+        return EGSGPU_E_ARG;
+    }
+    if (tp->play_rr && tp->nbr_split <= 1) {
+        set_error("Russian roulette (play_rr) plays with probability 1/nbr_split: it needs "
+                  "nbr_split > 1");
+        return EGSGPU_E_ARG;
+    }
     if (tp->howfarless) {   // one medium and one density everywhere
         const int64_t nv = (int64_t)geom->nx * geom->ny * geom->nz;
         for (int64_t i = 1; i < nv; i++)
@@ -1644,6 +1662,8 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->P.iprdst = tp->pair_angle; h->P.ibrdst = tp->brems_angle;
     h->P.exact_bca = tp->exact_bca; h->P.kerma = tp->kerma;
     h->P.n_split = tp->n_split > 1 ? tp->n_split : 0;
+    h->P.nbr_split = tp->nbr_split > 1 ? tp->nbr_split : 0;
+    h->P.play_rr = tp->play_rr ? 1 : 0;
     h->P.skindepth = tp->skin_depth > 1e-4 ? tp->skin_depth : 3.0;
     try {
         if (h->fp64) {
@@ -1682,6 +1702,11 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->schedule = tp->kerma ? EGSGPU_SCHED_HISTORY : EGSGPU_SCHED_EVENT;
     h->howfarless = tp->howfarless != 0;
     if (h->howfarless) h->schedule = EGSGPU_SCHED_HISTORY_PLAIN;
+    if (h->P.nbr_split) {
+        // a radiative event puts nbr_split photons (pairs) on the stack
+        h->schedule = EGSGPU_SCHED_HISTORY_PLAIN;
+        h->mxstk = std::max(h->mxstk, 4 * h->P.nbr_split + 64);
+    }
     if (h->P.n_split) {
         // a split leaves about one charged particle per sub-photon on the
         // stack; a photon from a radiative event of one of them is split
@@ -1853,8 +1878,10 @@ int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule) {
     EGS_ARG(h, "handle");
     EGS_ARG(schedule == EGSGPU_SCHED_HISTORY || schedule == EGSGPU_SCHED_EVENT ||
             schedule == EGSGPU_SCHED_HISTORY_PLAIN, "schedule");
-    if ((h->howfarless || h->P.n_split) && schedule != EGSGPU_SCHED_HISTORY_PLAIN) {
-        set_error("HOWFARLESS and photon splitting run on the plain history schedule");
+    if ((h->howfarless || h->P.n_split || h->P.nbr_split) &&
+        schedule != EGSGPU_SCHED_HISTORY_PLAIN) {
+        set_error("HOWFARLESS and photon or radiative splitting run on the plain history "
+                  "schedule");
         return EGSGPU_E_UNSUPPORTED;
     }
     if (h->lanes_ready) {
@@ -2108,7 +2135,7 @@ int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
         set_error("the density gradient replays histories: it needs the history schedule");
         return EGSGPU_E_UNSUPPORTED;
     }
-    if (h->P.n_split) {
+    if (h->P.n_split || h->P.nbr_split) {
         set_error("the density gradient's score function assumes unsplit photons");
         return EGSGPU_E_UNSUPPORTED;
     }

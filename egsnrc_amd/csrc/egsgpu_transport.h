@@ -507,12 +507,63 @@ struct Shower {
             pph = kPNewE;
             return;
         }
-        if (pph == kPPair) pair(m, M);
-        else if (pph == kPCompt) compt(m, M);
-        else photo(m, M, p_gle);
+        // with radiative splitting and Russian roulette (play_rr) the charged
+        // products are played for
+        const int npold = np;
+        if (pph == kPPair) {
+            if constexpr (SPLIT) {
+                if (P.play_rr && !rr_pair()) { pph = kPEnter; return; }
+            }
+            pair(m, M);
+        } else if (pph == kPCompt) {
+            compt(m, M);
+            if constexpr (SPLIT) {
+                if (P.play_rr) rr_electrons(npold);   // from NPold+1 (egsnrc.mortran:798)
+            }
+        } else {
+            photo(m, M, p_gle);
+            if constexpr (SPLIT) {   // from NPold (:5192): the photo-electron
+                if (P.play_rr && rr_electrons(npold - 1)) { pph = kPEnter; return; }
+            }
+        }
         if (p.iq != 0) { pph = kPEnter; return; }
         if (p.e < (R)P.pcut) { score(p.ir, p.e, p.wt); pop(); pph = kPEnter; return; }
         pph = kPNewE;
+    }
+    // $PLAY RUSSIAN ROULETTE WITH ELECTRONS FROM (first+1)
+    // (egsnrc.macros:3481-3519): each charged particle of stack entries
+    // first..np-1 survives with probability prob_RR = 1/nbr_split and then
+    // carries 1/prob_RR times its weight; a killed one is replaced by the
+    // top.  Returns true if a particle was killed.
+    __device__ bool rr_electrons(int first) {
+        const R prr = (R)1 / (R)P.nbr_split;
+        bool killed = false;
+        int k = first;
+        while (k < np) {
+            const bool top = k == np - 1;
+            const int iq = top ? p.iq : bitsi(sk(k, 9));
+            if (iq == 0) { k++; continue; }
+            R r = rnd();
+            if (r < prr) {
+                if (top) p.wt = p.wt / prr; else sk(k, 7) = sk(k, 7) / prr;
+                k++;
+            } else {
+                killed = true;
+                if (!top) store(k, p);
+                pop();
+            }
+        }
+        return killed;
+    }
+    // PAIR's own roulette, played before the sampling (egsnrc.mortran
+    // :4552-4577): the photon is removed with probability 1 - prob_RR
+    // (false), else its weight is raised by 1/prob_RR (true)
+    __device__ bool rr_pair() {
+        const R prr = (R)1 / (R)P.nbr_split;
+        R r = rnd();
+        if (r > prr) { pop(); return false; }
+        p.wt = p.wt / prr;
+        return true;
     }
     // one PHOTON call of the history schedule: runs until the photon is
     // gone or a charged particle is on top
@@ -556,7 +607,7 @@ struct Shower {
     }
 
     // ------------------------------------------------------------ COMPT
-    __device__ void compt(int m, const MedScalars<R> &M) {
+    __device__ __forceinline__ void compt(int m, const MedScalars<R> &M) {
         const R RM = (R)0.5109989461;
         R peig = p.e;
         R ko = peig / RM;
@@ -743,7 +794,7 @@ struct Shower {
     }
 
     // ------------------------------------------------------------ PHOTO
-    __device__ void photo(int m, const MedScalars<R> &M, R gle) {
+    __device__ __forceinline__ void photo(int m, const MedScalars<R> &M, R gle) {
         const R RM = (R)0.5109989461;
         R peig = p.e;
         if (peig < T.ee[1 * 16 + 1]) { p.iq = -1; p.e = peig + RM; return; }
@@ -855,7 +906,7 @@ struct Shower {
     }
 
     // ------------------------------------------------------------- PAIR
-    __device__ void pair(int m, const MedScalars<R> &M) {
+    __device__ __forceinline__ void pair(int m, const MedScalars<R> &M) {
         const R RM = (R)0.5109989461, RMT2 = 2 * RM;
         R peig = p.e, eig = peig;
         R pese1, pese2;
@@ -1426,7 +1477,7 @@ struct Shower {
         else if (eph == kBhabha) bhabha(M);
         else annih();
         if constexpr (SPLIT) {   // ausgab iarg 7 and 13 (dosxyznrc.mortran:3531-3545)
-            if (eph == kBrems || eph == kAnnih) rr_photons(npold);
+            if (P.n_split > 1 && (eph == kBrems || eph == kAnnih)) rr_photons(npold);
         }
         eph = kNew;
     }
@@ -1807,7 +1858,15 @@ struct Shower {
         R beta = dsqrt((tteie - 1) * (tteie + 1)) / tteie;
         R y2max = 2 * beta * (1 + beta) * tteie * tteie, y2maxi = 1 / y2max;
         R z2max = y2max + 1, z2maxi = dsqrt(z2max);
-        R esg, pese, ese;
+        // radiative splitting: nbr_split photons of 1/nbr_split of the
+        // weight, the electron left with the energy of the last sample
+        int nbr = 1;
+        if constexpr (SPLIT) nbr = P.nbr_split > 1 ? P.nbr_split : 1;
+        const int npold = np;
+        const R wph = p.wt / (R)nbr;
+        R pese = peie;
+        for (int ibr = 0; ibr < nbr; ibr++) {
+        R esg, ese;
         for (;;) {
             R rnno06 = rnd(), rnno07 = rnd();
             R br = brmin * dexp(rnno06 * waux);
@@ -1826,10 +1885,11 @@ struct Shower {
             if (rnno07 < rejf) break;
         }
         // the electron keeps pese; the photon goes on top
-        p.e = pese;
-        if (!push_copy()) return;
+        if (ibr == 0) p.e = pese;
+        if (!push_copy()) break;
         p.e = esg;
         p.iq = 0;
+        if constexpr (SPLIT) p.wt = wph;
         R y2tst;
         if (P.ibrdst == 1) {
             R ttese = ese / RM, esedei = ttese / tteie;
@@ -1872,9 +1932,14 @@ struct Shower {
             p.v = sinthe * sphi;
             p.w = c * costhe;
         }
+        }
+        if constexpr (SPLIT) {
+            if (nbr > 1) sk(npold - 1, 0) = pese;
+        }
     }
 
-    // ANNIH (egsnrc.mortran:70-184)
+    // ANNIH (egsnrc.mortran:70-184); with radiative splitting nbr_split
+    // photon pairs of 1/nbr_split of the positron's weight
     __device__ void annih() {
         const R RM = (R)0.5109989461;
         R pavip = p.e + RM, avip = pavip;
@@ -1889,6 +1954,11 @@ struct Shower {
             sindel = bb / sinpsi;
             cosdel = aa / sinpsi;
         }
+        int nbr = 1;
+        if constexpr (SPLIT) nbr = P.nbr_split > 1 ? P.nbr_split : 1;
+        if (nbr > 1) p.wt = p.wt / (R)nbr;
+        for (int ibr = 0; ibr < nbr; ibr++) {
+        if (ibr > 0 && !push_copy()) return;
         R ep, rejf, rnno02;
         do {
             R rnno01 = rnd();
@@ -1924,22 +1994,33 @@ struct Shower {
         costhe = (esg2 - RM) * pot / esg2;
         costhe = costhe > 1 ? (R)1 : (costhe < -1 ? (R)-1 : costhe);
         set_dir(costhe, -dsqrt(1 - costhe * costhe));
+        }
     }
 
+    // ANNIH_AT_REST (egsnrc.mortran:186-224); with radiative splitting
+    // nbr_split back-to-back pairs of 1/nbr_split of the positron's weight
     __device__ void annih_at_rest() {
         const R RM = (R)0.5109989461;
         const int npold = np;
-        R costhe = rnd();
-        costhe = 2 * costhe - 1;
-        R s2 = (1 - costhe) * (1 + costhe);
-        R sinthe = dsqrt(s2 > 0 ? s2 : (R)0);
-        R cphi, sphi;
-        azimuth(cphi, sphi);
-        p.e = RM; p.iq = 0;
-        p.u = sinthe * cphi; p.v = sinthe * sphi; p.w = costhe;
-        if (!push_copy()) return;
-        p.u = -p.u; p.v = -p.v; p.w = -p.w;
-        if constexpr (SPLIT) rr_photons(npold);   // ausgab iarg 14
+        int nbr = 1;
+        if constexpr (SPLIT) nbr = P.nbr_split > 1 ? P.nbr_split : 1;
+        if (nbr > 1) p.wt = p.wt / (R)nbr;
+        for (int ibr = 0; ibr < nbr; ibr++) {
+            if (ibr > 0 && !push_copy()) return;
+            R costhe = rnd();
+            costhe = 2 * costhe - 1;
+            R s2 = (1 - costhe) * (1 + costhe);
+            R sinthe = dsqrt(s2 > 0 ? s2 : (R)0);
+            R cphi, sphi;
+            azimuth(cphi, sphi);
+            p.e = RM; p.iq = 0;
+            p.u = sinthe * cphi; p.v = sinthe * sphi; p.w = costhe;
+            if (!push_copy()) return;
+            p.u = -p.u; p.v = -p.v; p.w = -p.w;
+        }
+        if constexpr (SPLIT) {
+            if (P.n_split > 1) rr_photons(npold);   // ausgab iarg 14
+        }
     }
 };
 

@@ -15,7 +15,9 @@
 //   run control           ncase / nbatch (egs_run_control.cpp:64-120)
 //   rng definition        ranmar initial seeds (egs_rndm.cpp:409-463)
 //   ausgab object         egs_dose_scoring .3ddose output
-//                         (egs_dose_scoring.cpp:416-479)
+//                         (egs_dose_scoring.cpp:416-479);
+//                         egs_radiative_splitting (radiative splitting and
+//                         Russian roulette, egs_radiative_splitting.cpp:80-132)
 // or, for a DOSXYZnrc input file (recognised by its records), the
 // DOSXYZnrc front end: phantom records or an .egsphant CT phantom,
 // isource 0 and 1 with monoenergetic or .spectrum energies, NCASE and the
@@ -452,6 +454,30 @@ class EGS_GPUApplication {
         tp.kerma = kerma;
     }
 
+    // egs_radiative_splitting ausgab objects (egs_radiative_splitting.h
+    // :110-121, .cpp:80-132): 'splitting = n' with n > 1 splits radiative
+    // events n-fold and plays Russian roulette with the charged particles
+    // of photon interactions (probability 1/n); n < -1 splits without
+    // roulette; 1 and -1 leave the transport alone
+    void init_variance_reduction() {
+        const InputItem *ad = input->block("ausgab object definition");
+        if (!ad) return;
+        for (const InputItem *a : ad->blocks("ausgab object")) {
+            std::string lib;
+            if (a->get("library", lib) || !InputItem::same(lib, "egs_radiative_splitting"))
+                continue;
+            double fs = 1;
+            a->get("splitting", fs);
+            int n = (int)fs;
+            if (n == 0)
+                throw std::runtime_error("egs_radiative_splitting with splitting = 0 (no radiative "
+                                         "events) is not supported");
+            if (n > 1) { tp.nbr_split = n; tp.play_rr = 1; }
+            else if (n < -1) { tp.nbr_split = -n; tp.play_rr = 0; }
+            else { tp.nbr_split = 0; tp.play_rr = 0; }
+        }
+    }
+
     // ---- DOSXYZnrc front end ---------------------------------------------
     void init_dosxyz() {
         dosxyz = true;
@@ -622,10 +648,10 @@ class EGS_GPUApplication {
         printf("ecut: %.6g\npcut: %.6g\n", tp.ecut, tp.pcut);
         printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
                "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d "
-               "n_split %d\n",
+               "n_split %d nbr_split %d play_rr %d\n",
                tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
                tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth,
-               tp.howfarless, tp.n_split);
+               tp.howfarless, tp.n_split, tp.nbr_split, tp.play_rr);
     }
 
     void init_run() {
@@ -653,6 +679,7 @@ class EGS_GPUApplication {
             init_geometry();
             init_source();
             init_transport();
+            init_variance_reduction();
             init_run();
         }
         if (parse_only) {

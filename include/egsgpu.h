@@ -123,6 +123,19 @@ typedef struct EgsTransport {
                                     sub-photons and plays Russian roulette
                                     with radiative photons; 0 or 1 = off,
                                     at most 256; plain history schedule */
+    int32_t nbr_split;           /* EGSnrc radiative splitting (nbr_split of
+                                    BREMS, ANNIH, ANNIH_AT_REST,
+                                    egsnrc.mortran:70-526; set by the
+                                    egs_radiative_splitting ausgab object,
+                                    egs_advanced_application.cpp:1224-1239):
+                                    > 1 makes nbr_split photons (pairs) of
+                                    1/nbr_split of the weight per radiative
+                                    event; 0 or 1 = off, at most 256; plain
+                                    history schedule; not with n_split */
+    int32_t play_rr;             /* 1: Russian roulette of the charged
+                                    particles made by photon interactions
+                                    with probability 1/nbr_split (i_play_RR,
+                                    egsnrc.macros:3481-3519) */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */

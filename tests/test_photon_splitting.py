@@ -1,4 +1,6 @@
-"""DOSXYZnrc photon splitting (n_split, dosxyznrc.mortran:855-1030): every
+"""Splitting and Russian roulette.
+
+DOSXYZnrc photon splitting (n_split, dosxyznrc.mortran:855-1030): every
 photon is replaced by n_split sub-photons of 1/n_split of its weight with
 stratified interaction points; photons scattered by all but one randomly
 chosen sub-photon are removed and the survivor's carry the full weight;
@@ -10,8 +12,16 @@ Splitting is unbiased: the split run's expected dose is that of the
 unsplit oracle, checked voxel by voxel against its statistics, with a
 plainly wrong weight (no 1/n_split, or no n_split on a survivor) far
 outside them.  Its variance per history is lower where the dose comes from
-the photons' first interactions, which shows that the photons were split."""
+the photons' first interactions, which shows that the photons were split.
+
+EGSnrc radiative splitting (nbr_split of BREMS, ANNIH and ANNIH_AT_REST,
+egsnrc.mortran:70-526), with or without the Russian roulette of the charged
+particles made by photon interactions (i_play_RR, egsnrc.macros:3481-3519;
+PAIR's own roulette :4552-4577), as the egs_radiative_splitting ausgab
+object sets them (egs_advanced_application.cpp:1224-1239): the same bar
+against the unsplit oracle."""
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -19,6 +29,7 @@ import pytest
 from egsnrc_amd import _lib
 from egsnrc_amd.engine import Simulation, Tables, make_source
 from tests import oracle_lib, workloads
+from tests.test_host_app import APP, EX
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BLOB700 = os.path.join(ROOT, "egsnrc_amd", "data", "700icru.egstab")
@@ -149,3 +160,127 @@ def test_split_rejects_other_schedules_and_the_gradient():
     with pytest.raises(_lib.EngineError, match="0..256"):
         Simulation(Tables(BLOB700), xb, xb.copy(), xb + 2, med, None,
                    make_source(0, 1.0, z=-1.0), _lib.default_transport(n_split=300))
+
+
+def slab_run(charge, energy, fp64, n, **split):
+    """A 2 x 2 cm beam on 20 x 20 cm water slabs: 0.5 cm slabs to 6 cm
+    (scaled by energy / 10 MeV), then thicker ones to 22 cm; oracle run
+    unsplit, engine run with the given splitting parameters."""
+    xb = np.array([-10.0, 10.0])
+    zb = np.concatenate([np.arange(0, 6.5, 0.5), [7.0, 8.0, 10.0, 13.0, 17.0, 22.0]]) \
+        * (energy / 10.0)
+    nz = len(zb) - 1
+    med = np.zeros(nz, np.int32)
+    src = make_source(charge, energy, x0=-1, x1=1, y0=-1, y1=1, z=-1.0)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01, fp64=fp64)
+    a = oracle(BLOB521, xb, xb.copy(), zb, med, None, src, tp, n)
+    for k, v in split.items():
+        setattr(tp, k, v)
+    sim = Simulation(Tables(BLOB521), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
+    sim.run(n)
+    b = sim.tallies()
+    assert b[2] == n and sim.counters()[5] == 0
+    return a, b, nz
+
+
+def tail_check(a, b, nz, label):
+    ma, va = stats(a)
+    mb, vb = stats(b)
+    tail = slice(nz - 4, nz)
+    z = (ma[tail] - mb[tail]) / np.sqrt(va[tail] + vb[tail])
+    print("%s tail slabs: oracle" % label, ma[tail], "split", mb[tail], "z", z)
+    assert np.all(ma[tail] > 0) and np.all(np.abs(z) < 4)
+    # variance per history of the bremsstrahlung tail, split over unsplit
+    return float(np.median(vb[tail] / va[tail]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("play_rr", [1, 0])
+def test_radiative_splitting_of_an_electron_beam(play_rr):
+    """10 MeV electrons, nbr_split 20, with and without the roulette of the
+    charged particles the split photons set in motion: the dose of every
+    slab is the oracle's; without roulette the bremsstrahlung tail has a
+    lower variance per history."""
+    a, b, nz = slab_run(-1, 10.0, 0, 200000, nbr_split=20, play_rr=play_rr)
+    check(a, b, "radiative splitting rr=%d" % play_rr, nz - 2, frac=1e-4)
+    ratio = tail_check(a, b, nz, "rr=%d" % play_rr)
+    print("tail variance per history, split / unsplit: %.3f" % ratio)
+    # without roulette every split photon's electrons are kept; with it
+    # about as many electrons as without splitting survive, at full weight
+    assert ratio < (0.5 if not play_rr else 2.0)
+
+
+@pytest.mark.gpu
+def test_radiative_splitting_of_annihilation():
+    """6 MeV positrons in FP64 with nbr_split 8 and roulette: annihilation
+    in flight and at rest make 8 pairs each."""
+    a, b, nz = slab_run(1, 6.0, 1, 200000, nbr_split=8, play_rr=1)
+    check(a, b, "annihilation splitting", nz - 2, frac=1e-4)
+    tail_check(a, b, nz, "annihilation")
+
+
+@pytest.mark.gpu
+def test_russian_roulette_of_photon_made_electrons():
+    """2 MeV photons on the 16 x 16 x 20 cm phantom with a bone slab,
+    nbr_split 4 with roulette: three in four Compton, photo- and pair
+    electrons are removed, the others carry four times the weight."""
+    xb = np.linspace(-8, 8, 17)
+    zb = np.linspace(0, 20, 21)
+    med = np.zeros(16 * 16 * 20, np.int32)
+    kz = np.arange(med.size) // 256
+    med[(kz == 5) | (kz == 6)] = 2
+    src = make_source(0, 2.0, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01)
+    a = oracle(BLOB521, xb, xb.copy(), zb, med, None, src, tp, 400000)
+    tp.nbr_split = 4
+    tp.play_rr = 1
+    sim = Simulation(Tables(BLOB521), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
+    n = 800000
+    sim.run(n)
+    b = sim.tallies()
+    assert b[2] == n and sim.counters()[5] == 0
+    check(a, b, "electron roulette", 300)
+    ratio = per_history_variance_ratio(a, b)
+    print("variance per history, roulette / none: %.3f" % ratio)
+    assert ratio > 1.5   # fewer, heavier electrons
+
+
+@pytest.mark.gpu
+def test_splitting_parameters_are_checked():
+    xb = np.linspace(-2, 2, 5)
+    med = np.zeros(64, np.int32)
+
+    def make(**kw):
+        Simulation(Tables(BLOB700), xb, xb.copy(), xb + 2, med, None,
+                   make_source(0, 1.0, z=-1.0), _lib.default_transport(**kw))
+    with pytest.raises(_lib.EngineError, match="choose one"):
+        make(n_split=4, nbr_split=4)
+    with pytest.raises(_lib.EngineError, match="needs nbr_split"):
+        make(play_rr=1)
+    with pytest.raises(_lib.EngineError, match="0..256"):
+        make(nbr_split=1000)
+
+
+def parse_transport(path):
+    r = subprocess.run([APP, "-i", str(path), "--parse-only"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    line = [l for l in r.stdout.splitlines() if l.startswith("transport:")][0].split()[1:]
+    return dict(zip(line[::2], line[1::2]))
+
+
+@pytest.mark.parametrize("value,nbr,rr", [("20", "20", "1"), ("-10", "10", "0"),
+                                          ("1", "0", "0")])
+def test_app_reads_egs_radiative_splitting(tmp_path, value, nbr, rr):
+    """'splitting = n' of an egs_radiative_splitting ausgab object
+    (egs_radiative_splitting.h:110-121): n > 1 splits with roulette, n < -1
+    without, 1 leaves the transport alone."""
+    txt = open(EX).read().replace(
+        ":stop ausgab object definition:",
+        "    :start ausgab object:\n        name = vr\n"
+        "        library = egs_radiative_splitting\n        splitting = %s\n"
+        "    :stop ausgab object:\n:stop ausgab object definition:" % value)
+    f = tmp_path / "vr.egsinp"
+    f.write_text(txt)
+    t = parse_transport(f)
+    assert (t["nbr_split"], t["play_rr"]) == (nbr, rr)
