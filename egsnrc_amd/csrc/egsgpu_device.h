@@ -102,8 +102,9 @@ struct DevParams {
     int n_split;              // DOSXYZnrc photon splitting number (0 = off)
     int nbr_split;            // radiative splitting number (0 = off)
     int play_rr;              // Russian roulette of photon-made charged particles
-    int pad;
+    int ireject;              // DOSXYZnrc range rejection below esave
     double ecut, pcut, skindepth;
+    double esave;             // total energy (MeV) below which range rejection applies
 };
 
 // Fields of a stacked particle.

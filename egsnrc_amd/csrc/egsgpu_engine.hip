@@ -1207,7 +1207,8 @@ int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = n
     EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
     dim3 grid((h->nlanes + h->block - 1) / h->block);
     const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
-    if (h->P.n_split > 1 || h->P.nbr_split > 1) {   // splitting: the plain history loop
+    if (h->P.n_split > 1 || h->P.nbr_split > 1 || h->P.ireject) {
+        // splitting or range rejection: the plain history loop
         if (h->howfarless && lux)
             hipLaunchKernelGGL((k_transport<R, true, true, true>), grid, dim3(h->block), lds,
                                h->stream, a);
@@ -1595,6 +1596,14 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
                   "separate variance-reduction schemes; choose one");
         return EGSGPU_E_ARG;
     }
+    if (tp->ireject && tp->howfarless) {
+        set_error("range rejection is not implemented with HOWFARLESS");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    if (tp->ireject && tp->kerma) {
+        set_error("range rejection needs charged-particle transport (kerma = 0)");
+        return EGSGPU_E_ARG;
+    }
     if (tp->play_rr && tp->nbr_split <= 1) {
         set_error("Russian roulette (play_rr) plays with probability 1/nbr_split: it needs "
                   "nbr_split > 1");
@@ -1664,6 +1673,9 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->P.n_split = tp->n_split > 1 ? tp->n_split : 0;
     h->P.nbr_split = tp->nbr_split > 1 ? tp->nbr_split : 0;
     h->P.play_rr = tp->play_rr ? 1 : 0;
+    // DOSXYZnrc switches range rejection off when ESAVE_GLOBAL <= ECUT
+    h->P.esave = tp->esave;
+    h->P.ireject = tp->ireject && tp->esave > h->P.ecut ? 1 : 0;
     h->P.skindepth = tp->skin_depth > 1e-4 ? tp->skin_depth : 3.0;
     try {
         if (h->fp64) {
@@ -1702,6 +1714,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->schedule = tp->kerma ? EGSGPU_SCHED_HISTORY : EGSGPU_SCHED_EVENT;
     h->howfarless = tp->howfarless != 0;
     if (h->howfarless) h->schedule = EGSGPU_SCHED_HISTORY_PLAIN;
+    if (h->P.ireject) h->schedule = EGSGPU_SCHED_HISTORY_PLAIN;
     if (h->P.nbr_split) {
         // a radiative event puts nbr_split photons (pairs) on the stack
         h->schedule = EGSGPU_SCHED_HISTORY_PLAIN;
@@ -1878,10 +1891,10 @@ int egsgpu_set_schedule(egsgpu_handle *h, int32_t schedule) {
     EGS_ARG(h, "handle");
     EGS_ARG(schedule == EGSGPU_SCHED_HISTORY || schedule == EGSGPU_SCHED_EVENT ||
             schedule == EGSGPU_SCHED_HISTORY_PLAIN, "schedule");
-    if ((h->howfarless || h->P.n_split || h->P.nbr_split) &&
+    if ((h->howfarless || h->P.n_split || h->P.nbr_split || h->P.ireject) &&
         schedule != EGSGPU_SCHED_HISTORY_PLAIN) {
-        set_error("HOWFARLESS and photon or radiative splitting run on the plain history "
-                  "schedule");
+        set_error("HOWFARLESS, photon or radiative splitting and range rejection run on the "
+                  "plain history schedule");
         return EGSGPU_E_UNSUPPORTED;
     }
     if (h->lanes_ready) {

@@ -1067,6 +1067,17 @@ struct Shower {
         return eloss(m, pos, rhof, tuss, eketmp, elktmp, lt) + eke - eketmp;
     }
 
+    // RANGE_ECUTRR of DOSXYZnrc (dosxyznrc.mortran:2514-2545): the
+    // electron range from ECUT down to AE in medium m at its default density
+    __device__ R range_ecut(int m, const MedScalars<R> &M) const {
+        const R eke = (R)P.ecut - (R)0.5109989461;
+        if (!(eke > earr(m, 0))) return 0;
+        R elk = dlog(eke);
+        int l = lel(M, elk);
+        R ekei = earr(m, l - 1);
+        R elkei = (R)(l - M.eke0) / M.eke1;
+        return drange(m, false, eke, ekei, l, elk, elkei) + repv(m, false, l - 1);
+    }
     __device__ __forceinline__ void ecut_discard(int lelec, R peie) {
         const R RM = (R)0.5109989461;
         R edep = peie - RM;
@@ -1312,6 +1323,19 @@ struct Shower {
         if constexpr (HFL) tperp = geom_box_near(*Gp, p.x, p.y, p.z);
         else tperp = geom_hownear(*Gp, p.ir, p.x, p.y, p.z);
         p.dnear = tperp;
+        if constexpr (SPLIT && !HFL) {
+            // DOSXYZnrc range rejection ($USER-RANGE-DISCARD,
+            // dosxyznrc.mortran:1050-1064): below ESAVE_GLOBAL a particle
+            // whose range to ECUT is within the distance to the nearest
+            // voxel boundary deposits its kinetic energy here (a positron
+            // then annihilates at rest)
+            if (P.ireject && (double)eie < P.esave &&
+                tperp >= range - range_ecut(m, M) / rhof) {
+                ecut_discard(lelec, peie);
+                eph = kNew;
+                return;
+            }
+        }
         R blccl = rhof * M.blcc;
         R p2 = eke * (eke + RMT2), beta2 = p2 / (p2 + RMSQ);
         R ssmfp = beta2 / blccl;

@@ -525,8 +525,15 @@ class EGS_GPUApplication {
             else fprintf(stderr, "note: HOWFARLESS requested for a phantom that is not "
                                  "homogeneous; every voxel boundary is tracked\n");
         }
-        if (dz.ireject)
-            fprintf(stderr, "note: charged-particle range rejection requested; not applied\n");
+        // charged-particle range rejection against voxel boundaries below
+        // ESAVE_GLOBAL (dosxyznrc.mortran:1050-1064, 2514-2545)
+        if (dz.ireject == 1) {
+            if (tp.howfarless)
+                fprintf(stderr, "note: range rejection is not applied with HOWFARLESS\n");
+            else if (dz.esave_global <= tp.ecut)
+                fprintf(stderr, "note: ESAVE_GLOBAL <= ECUT, so no range rejection\n");
+            else { tp.ireject = 1; tp.esave = dz.esave_global; }
+        }
         // photon splitting with Russian roulette of radiative photons
         // (dosxyznrc.mortran:855-1030, 2557-2562)
         if (dz.n_split > 1) tp.n_split = dz.n_split;
@@ -648,10 +655,10 @@ class EGS_GPUApplication {
         printf("ecut: %.6g\npcut: %.6g\n", tp.ecut, tp.pcut);
         printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
                "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d "
-               "n_split %d nbr_split %d play_rr %d\n",
+               "n_split %d nbr_split %d play_rr %d ireject %d esave %.6g\n",
                tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
                tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth,
-               tp.howfarless, tp.n_split, tp.nbr_split, tp.play_rr);
+               tp.howfarless, tp.n_split, tp.nbr_split, tp.play_rr, tp.ireject, tp.esave);
     }
 
     void init_run() {

@@ -136,6 +136,14 @@ typedef struct EgsTransport {
                                     particles made by photon interactions
                                     with probability 1/nbr_split (i_play_RR,
                                     egsnrc.macros:3481-3519) */
+    int32_t ireject;             /* 1: DOSXYZnrc range rejection (IREJECT,
+                                    dosxyznrc.mortran:1050-1064): a charged
+                                    particle below esave whose range to ECUT
+                                    is within the distance to the nearest
+                                    voxel boundary deposits its kinetic
+                                    energy there; plain history schedule,
+                                    not with howfarless */
+    double esave;                /* ESAVE_GLOBAL: total energy (MeV) */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */
