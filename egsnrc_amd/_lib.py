@@ -102,6 +102,7 @@ def lib():
         "egsgpu_run": [vp, i64],
         "egsgpu_get_tallies": [vp, vp, vp, vp],
         "egsgpu_reset_tallies": [vp],
+        "egsgpu_set_tallies": [vp, vp, vp, C.c_int64],
         "egsgpu_get_counters": [vp, vp],
         "egsgpu_last_kernel_ms": [vp, vp, vp],
         "egsgpu_density_grad": [vp, i64, vp, vp, vp, vp],

@@ -42,8 +42,8 @@ HOST_APP = os.path.join(HERE, "egs_gpu_app")
 def build_app():
     """The C++ host application linked against libegsgpu.so."""
     srcs = [os.path.join(HERE, "host", f)
-            for f in ("egs_input.cpp", "egs_phantom.cpp", "egs_phsp.cpp", "dosxyz_input.cpp",
-                      "egs_gpu_app.cpp")]
+            for f in ("egs_input.cpp", "egs_phantom.cpp", "egs_phsp.cpp", "egs_dat.cpp",
+                      "dosxyz_input.cpp", "egs_gpu_app.cpp")]
     cmd = ["g++", "-O2", "-std=c++17", "-o", HOST_APP + ".tmp"] + srcs + [
         "-I", os.path.join(HERE, "..", "include"), "-L", HERE, "-legsgpu", "-lz",
         "-Wl,-rpath,$ORIGIN"]

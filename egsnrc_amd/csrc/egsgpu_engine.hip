@@ -2230,6 +2230,18 @@ int egsgpu_reset_tallies(egsgpu_handle *h) {
     return EGSGPU_OK;
 }
 
+int egsgpu_set_tallies(egsgpu_handle *h, const double *sum, const double *sum2,
+                       int64_t nhist) {
+    EGS_ARG(h && sum && sum2 && nhist >= 0, "handle, sum, sum2, nhist >= 0");
+    EGS_HIP(hipMemcpyAsync(h->tally.sum, sum, sizeof(double) * h->nvox, hipMemcpyHostToDevice,
+                           h->stream));
+    EGS_HIP(hipMemcpyAsync(h->tally.sum2, sum2, sizeof(double) * h->nvox,
+                           hipMemcpyHostToDevice, h->stream));
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    h->nhist = nhist;
+    return EGSGPU_OK;
+}
+
 int egsgpu_get_counters(egsgpu_handle *h, int64_t counters[8]) {
     EGS_ARG(h && counters, "handle, counters");
     int64_t all[32];

@@ -12,7 +12,11 @@
 //                         with point/rectangle shape and monoenergetic or
 //                         tabulated spectrum (egs_spectra.cpp:482-760)
 //   MC transport parameter (egs_advanced_application.cpp:482-639)
-//   run control           ncase / nbatch (egs_run_control.cpp:64-120)
+//   run control           ncase / nbatch / calculation = first, restart,
+//                         analyze or combine (egs_run_control.cpp:64-120,
+//                         185-215), with the .egsdat state files of
+//                         EGS_Application::outputData / readData /
+//                         combineResults (egs_application.cpp:482-675)
 //   rng definition        ranmar initial seeds (egs_rndm.cpp:409-463)
 //   ausgab object         egs_dose_scoring .3ddose output
 //                         (egs_dose_scoring.cpp:416-479);
@@ -41,6 +45,7 @@
 
 #include "../../include/egsgpu.h"
 #include "dosxyz_input.h"
+#include "egs_dat.h"
 #include "egs_input.h"
 #include "egs_phantom.h"
 #include "egs_phsp.h"
@@ -94,6 +99,14 @@ class EGS_GPUApplication {
     int ixx = 1802, jxx = 9373;
     double area = 0;
     std::string dose_file;
+    // run control "calculation": 0 first, 1 restart, 2 analyze, 3 combine
+    int calculation = 0;
+    // the scoring arrays of the egs_dose_scoring object in the state file:
+    // region dose (default yes), dose file (an "output dose file" item)
+    bool region_dose = true, dose_file_array = false, medium_dose = false;
+    double cpu_before = 0, cpu_run = 0;
+    int64_t ndone_before = 0;
+    EgsDat state;                 // analyze / combine: the state read
     // DOSXYZnrc front end
     bool dosxyz = false;
     bool parse_only = false;
@@ -667,8 +680,26 @@ class EGS_GPUApplication {
             if (rc->get("number of histories", nc)) rc->get("ncase", nc);
             ncase = (int64_t)nc;
             rc->get("nbatch", nbatch);
+            std::string calc;
+            if (!rc->get("calculation", calc)) {
+                const char *kinds[] = {"first", "restart", "analyze", "combine"};
+                calculation = -1;
+                for (int k = 0; k < 4; k++)
+                    if (InputItem::same(calc, kinds[k])) calculation = k;
+                if (calculation < 0) throw std::runtime_error("calculation = " + calc);
+            }
         }
-        if (ncase <= 0) throw std::runtime_error("run control needs ncase > 0");
+        if (const InputItem *ad = input->block("ausgab object definition")) {
+            for (const InputItem *a : ad->blocks("ausgab object")) {
+                std::string lib, v;
+                if (a->get("library", lib) || !InputItem::same(lib, "egs_dose_scoring")) continue;
+                if (!a->get("region dose", v)) region_dose = !InputItem::same(v, "no");
+                if (!a->get("medium dose", v)) medium_dose = InputItem::same(v, "yes");
+                dose_file_array = a->item("output dose file") || a->block("output dose file");
+            }
+        }
+        if (ncase <= 0 && calculation < 2)
+            throw std::runtime_error("run control needs ncase > 0");
         if (nbatch < 1) nbatch = 1;
         if (const InputItem *r = input->block("rng definition")) {
             std::vector<int> s;
@@ -713,6 +744,7 @@ class EGS_GPUApplication {
         if (!phantom_density.empty())   // mass densities (.egsphant, DOSXYZnrc records)
             for (size_t i = 0; i < medium.size(); i++)
                 if (phantom_density[i] > 0) rhor[i] = phantom_density[i] / mdens[medium[i]];
+        if (!dosxyz && calculation >= 2) return load_states();   // no transport
         EgsXYZGeometry g{};
         g.nx = (int)xp.p.size() - 1; g.ny = (int)yp.p.size() - 1; g.nz = (int)zp.p.size() - 1;
         g.xb = xp.p.data(); g.yb = yp.p.data(); g.zb = zp.p.data();
@@ -729,11 +761,139 @@ class EGS_GPUApplication {
                                   phsp.v.data(), phsp.e.data(), phsp.wt.data(), phsp.q.data()),
                   "phase-space particles");
         check(egsgpu_seed(h, ixx, jxx), "seeding");
+        if (!dosxyz && calculation == 1) restart();
         return 0;
+    }
+
+    // ---- .egsdat state files ----------------------------------------------
+    int narrays() const {
+        if (medium_dose)
+            throw std::runtime_error("state files with medium dose are not supported (the "
+                                     "per-medium history sums are not kept)");
+        return (region_dose ? 1 : 0) + (dose_file_array ? 1 : 0);
+    }
+    int nvox() const {
+        return ((int)xp.p.size() - 1) * ((int)yp.p.size() - 1) * ((int)zp.p.size() - 1);
+    }
+    const EgsScoringArray &voxel_array(const EgsDat &d, const std::string &f) const {
+        if (d.arrays.empty()) throw std::runtime_error(f + ": no scoring array");
+        const EgsScoringArray &a = d.arrays.back();
+        if (a.nreg != nvox()) throw std::runtime_error(f + ": scoring array of another size");
+        return a;
+    }
+    // calculation = restart: the engine's own checkpoint (every stream
+    // continues exactly) if there is one, else the .egsdat sums, with the
+    // streams seeded apart from those of the run that made them
+    void restart() {
+        std::ifstream ck(output + ".egsgpu", std::ios::binary);
+        if (ck) {
+            std::vector<char> buf((std::istreambuf_iterator<char>(ck)),
+                                  std::istreambuf_iterator<char>());
+            check(egsgpu_restore(h, buf.data(), buf.size()), "restoring the engine checkpoint");
+            EgsDat d = read_egsdat(output + ".egsdat", narrays());
+            cpu_before = d.cpu;
+            printf("  restart from %s.egsgpu: %lld histories done\n", output.c_str(),
+                   (long long)d.ndone);
+            return;
+        }
+        EgsDat d = read_egsdat(output + ".egsdat", narrays());
+        const EgsScoringArray &a = voxel_array(d, output + ".egsdat");
+        check(egsgpu_set_tallies(h, a.sum.data(), a.sum2.data(), d.ndone), "setting tallies");
+        int is, js;
+        sequence_seeds(ixx, jxx, int64_t(1) << 20, is, js);
+        check(egsgpu_seed(h, is, js), "seeding");
+        cpu_before = d.cpu;
+        printf("  restart from %s.egsdat: %lld histories done\n", output.c_str(),
+               (long long)d.ndone);
+    }
+    // the seeds of ranmar sequence s after (ixx, jxx), as createRNG hands
+    // them to parallel jobs (egs_rndm.cpp:438-446)
+    static void sequence_seeds(int ixx, int jxx, int64_t s, int &is, int &js) {
+        if (ixx <= 0 || ixx >= 31328) ixx = 1802;
+        if (jxx <= 0 || jxx >= 30081) jxx = 9373;
+        int64_t t = (int64_t)jxx + s, carry = (t - 1) / 30080;
+        js = (int)(1 + (t - 1) % 30080);
+        is = (int)(1 + ((int64_t)ixx - 1 + carry) % 31327);
+    }
+    // calculation = analyze / combine
+    int load_states() {
+        const int na = narrays();
+        if (calculation == 2) {
+            state = read_egsdat(output + ".egsdat", na);
+        } else {   // <output>_w<j>.egsdat of parallel jobs j (combineResults)
+            int nfile = 0;
+            for (int j = 1; j <= 1024; j++) {
+                std::string f = output + "_w" + std::to_string(j) + ".egsdat";
+                std::ifstream probe(f);
+                if (!probe) continue;
+                EgsDat d = read_egsdat(f, na);
+                voxel_array(d, f);
+                printf("  %2d %-30s ncase=%lld\n", ++nfile, f.c_str(), (long long)d.ndone);
+                add_egsdat(state, d);
+            }
+            if (!nfile) throw std::runtime_error("no " + output + "_w<j>.egsdat files to combine");
+        }
+        voxel_array(state, output + ".egsdat");
+        return 0;
+    }
+    // spectrum sums of the source state: exact for monoenergetic sources,
+    // expectations for tabulated spectra
+    void spectrum_moments(double &m1, double &m2) const {
+        m1 = m2 = 0;
+        if (src.spectrum == EGSGPU_SPEC_MONO) { m1 = src.energy; m2 = m1 * m1; return; }
+        double w = 0;
+        for (size_t k = 0; k < spec_w.size(); k++) {
+            if (src.spectrum == EGSGPU_SPEC_HISTOGRAM) {
+                double a = spec_e[k], b = spec_e[k + 1], p = spec_w[k] * (b - a);
+                w += p; m1 += p * 0.5 * (a + b); m2 += p * (a * a + a * b + b * b) / 3;
+            } else if (src.spectrum == EGSGPU_SPEC_LINE) {
+                w += spec_w[k]; m1 += spec_w[k] * spec_e[k];
+                m2 += spec_w[k] * spec_e[k] * spec_e[k];
+            }
+        }
+        if (w > 0) { m1 /= w; m2 /= w; }
+    }
+    void write_state(const std::vector<double> &s, const std::vector<double> &s2, int64_t n) {
+        EgsDat d;
+        d.ndone = n;
+        d.cpu = cpu_before + cpu_run;
+        d.current_case = n;
+        // a ranmar sequence no stream of this engine has used, so that a
+        // CPU restart from this file draws fresh numbers
+        int is, js;
+        sequence_seeds(ixx, jxx, int64_t(1) << 21, is, js);
+        ranmar_initial_state(is, js, d);
+        d.src_count = n;
+        d.spec_count = n;
+        double m1, m2;
+        spectrum_moments(m1, m2);
+        d.sum_E = m1 * (double)n;
+        d.sum_E2 = m2 * (double)n;
+        d.last_case = n;
+        EgsScoringArray a;
+        a.nreg = (int)s.size();
+        a.current_ncase_short = (int)(n % 65536);
+        a.current_ncase = n;
+        a.current_ncase_65536 = n - n % 65536;
+        a.last.assign(a.nreg, n);
+        a.sum = s;
+        a.sum2 = s2;
+        for (int k = 0; k < narrays(); k++) d.arrays.push_back(a);
+        write_egsdat(output + ".egsdat", d);
+        if (h) {
+            size_t size = 0;
+            check(egsgpu_checkpoint(h, nullptr, &size), "checkpoint size");
+            std::vector<char> buf(size);
+            check(egsgpu_checkpoint(h, buf.data(), &size), "checkpoint");
+            std::ofstream ck(output + ".egsgpu", std::ios::binary);
+            ck.write(buf.data(), (std::streamsize)size);
+        }
+        printf("  state written to %s.egsdat%s\n", output.c_str(), h ? " (+ .egsgpu)" : "");
     }
 
     // runSimulation: nbatch batches of ncase/nbatch histories
     int runSimulation() {
+        if (!h) return 0;   // analyze / combine
         int64_t per = std::max<int64_t>(1, ncase / nbatch);
         int64_t done = 0;
         auto t0 = std::chrono::steady_clock::now();
@@ -744,6 +904,7 @@ class EGS_GPUApplication {
             done += n;
             double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             printf("  %5d  %10lld  %10.3f\n", b + 1, (long long)done, el);
+            cpu_run = el;
         }
         return 0;
     }
@@ -754,7 +915,16 @@ class EGS_GPUApplication {
         int nreg = nx * ny * nz;
         std::vector<double> s(nreg), s2(nreg);
         int64_t n = 0;
-        check(egsgpu_get_tallies(h, s.data(), s2.data(), &n), "tallies");
+        if (h) {
+            check(egsgpu_get_tallies(h, s.data(), s2.data(), &n), "tallies");
+        } else {   // analyze / combine: the sums of the state files
+            const EgsScoringArray &a = voxel_array(state, output + ".egsdat");
+            s = a.sum;
+            s2 = a.sum2;
+            n = state.ndone;
+            cpu_before = state.cpu;
+        }
+        if (!dosxyz && calculation != 2) write_state(s, s2, n);
         double normE = area > 0 ? area : 1.0;   // per fluence for beams with an area
         double normD = 1.602e-10 * normE;
         // DOSXYZnrc: incident fluence counts the beam particles that missed

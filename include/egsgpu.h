@@ -315,6 +315,15 @@ int egsgpu_get_tallies(egsgpu_handle *h, double *sum, double *sum2,
                        int64_t *nhist);
 int egsgpu_reset_tallies(egsgpu_handle *h);
 
+/* Set the tallies and the history count (arrays of nx*ny*nz): the
+ * restart of a run from a state file that holds the per-region sums of a
+ * scoring array (EGS_ScoringArray::setState, egs++/egs_scoring.h:340-365,
+ * read by EGS_Application::readData, egs_application.cpp:520-556).  The
+ * streams are left as they are; seed them apart from the run that made the
+ * sums. */
+int egsgpu_set_tallies(egsgpu_handle *h, const double *sum, const double *sum2,
+                       int64_t nhist);
+
 /* Checkpoint and restart (EGS_Application::outputData / readData,
  * egs_application.cpp:482-560, which store the run control, RNG, source
  * and scoring states): the tallies, the number of histories and the state
