@@ -305,14 +305,14 @@ class EGS_GPUApplication {
             lib.clear();
             s->get("library", lib);
         }
-        if (InputItem::same(lib, "egs_phsp_source")) {
-            init_phsp_source(s);
+        if (InputItem::same(lib, "egs_phsp_source") || InputItem::same(lib, "iaea_phsp_source")) {
+            init_phsp_source(s, InputItem::same(lib, "iaea_phsp_source"));
             src.x0 += tx; src.y0 += ty; src.z += tz;
             return;
         }
         if (!InputItem::same(lib, "egs_parallel_beam"))
-            throw std::runtime_error("only egs_parallel_beam and egs_phsp_source are supported, got " +
-                                     lib);
+            throw std::runtime_error("only egs_parallel_beam, egs_phsp_source and iaea_phsp_source "
+                                     "are supported, got " + lib);
         int q = 0;
         s->get("charge", q);
         src.charge = q;
@@ -385,10 +385,22 @@ class EGS_GPUApplication {
     // weight 0 (their histories score nothing); the phase-space plane is
     // z = 0 unless an egs_transformed_source translates it
     EgsPhsp phsp;
-    void init_phsp_source(const InputItem *s) {
+    // iaea_phsp_source (iaea_phsp_source.cpp:225-310 input): the same
+    // particle-type and cutout filters on an IAEA file, whose particles
+    // carry their z plane
+    void init_phsp_source(const InputItem *s, bool iaea) {
         std::string file;
-        if (s->get("phase space file", file)) throw std::runtime_error("egs_phsp_source needs a phase space file");
-        phsp = read_egsphsp(resolve(file));
+        double zplane = 0;
+        if (iaea) {
+            if (s->get("iaea phase space file", file))
+                throw std::runtime_error("iaea_phsp_source needs an iaea phase space file");
+            const std::string hb = resolve(file + ".IAEAheader");   // base name given
+            phsp = read_iaea_phsp(hb.substr(0, hb.size() - 11), zplane);
+        } else {
+            if (s->get("phase space file", file))
+                throw std::runtime_error("egs_phsp_source needs a phase space file");
+            phsp = read_egsphsp(resolve(file));
+        }
         std::string ptype = "all";
         s->get("particle type", ptype);
         int want = 2;   // 2 all, 3 charged, else the charge
@@ -417,6 +429,7 @@ class EGS_GPUApplication {
         src.kind = EGSGPU_SOURCE_PHSP;
         src.spectrum = EGSGPU_SPEC_MONO;
         src.charge = 0;
+        src.z = zplane;
         area = 0;
         fprintf(stderr, "phase space: %lld particles (%lld photons), %g incident particles, "
                         "E %g-%g MeV\n", (long long)phsp.n, (long long)phsp.nphot,
@@ -657,6 +670,12 @@ class EGS_GPUApplication {
             printf("phsp: %lld %lld %lld %lld %lld %.9g\n", (long long)phsp.n, nq[0], nq[1], nq[2],
                    nrej, (double)phsp.pinc);
             printf("phsp plane: %.9g %.9g %.9g\n", src.x0, src.y0, src.z);
+            double se = 0, sx = 0, sy = 0, su = 0, sv = 0, sw = 0;
+            for (int64_t k = 0; k < phsp.n; k++) {
+                se += phsp.e[k]; sx += phsp.x[k]; sy += phsp.y[k];
+                su += phsp.u[k]; sv += phsp.v[k]; sw += phsp.wt[k];
+            }
+            printf("phsp sums: %.9g %.9g %.9g %.9g %.9g %.9g\n", se, sx, sy, su, sv, sw);
         }
         if (src.spectrum == EGSGPU_SPEC_CDFINV && !spec_e.empty()) {
             size_t n = spec_e.size(), first = 0;

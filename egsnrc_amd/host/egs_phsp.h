@@ -23,3 +23,14 @@ struct EgsPhsp {
 // std::runtime_error.  Charged particles' total energies are converted to
 // kinetic (readParticle, :423-425).
 EgsPhsp read_egsphsp(const std::string &path);
+
+// Read an IAEA phase space (base.IAEAheader + base.IAEAphsp), the format of
+// the IAEA library the reference's iaea_phsp_source reads
+// (iaea_phsp/iaea_header.cpp:90-180 header blocks, iaea_record.cpp:157-230
+// records; charge mapping iaea_phsp_source.cpp:372-384): per record a
+// signed char type (1 photon, 2 electron, 3 positron; its sign is the sign
+// of w), then floats E (negative: a new history), and x, y, z, u, v, weight
+// where the header's RECORD_CONTENTS stores them (RECORD_CONSTANT gives the
+// others), extra floats, extra int32s.  Energies are kinetic.  The
+// particles must lie on one z plane, returned in zplane.
+EgsPhsp read_iaea_phsp(const std::string &base, double &zplane);

@@ -13,7 +13,7 @@ from egsnrc_amd import _lib
 from egsnrc_amd.engine import Simulation, Tables, make_phsp_source, make_source
 from egsnrc_amd.phantom_io import RM, read_egsphsp, write_egsphsp
 from tests import oracle_lib, workloads
-from tests.test_host_app import APP, BLOB700
+from tests.test_host_app import APP, BLOB700, read_3ddose
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BLOB521 = os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab")
@@ -178,3 +178,117 @@ def test_mixed_phsp_on_the_event_schedule_matches_the_oracle():
     assert nv > 300
     assert chi2 < max(1.3, 1 + 3 * np.sqrt(2.0 / nv))
     assert out2 < max(0.09, 0.0455 + 3 * np.sqrt(0.0455 * 0.9545 / nv))
+
+
+def write_iaea(base, e, x, y, z, u, v, wt, q, new_history=None, byteorder="<",
+               store_z=False, orig=6.0):
+    """An IAEA phase space as the IAEA library writes one
+    (iaea_record.cpp:86-135, header blocks iaea_header.cpp:720-810): the
+    sign of w in the type, a new history as a negative energy; x, y, u, v,
+    weight stored, z stored or a RECORD_CONSTANT."""
+    n = len(e)
+    nh = np.zeros(n, bool) if new_history is None else new_history
+    typ = np.choose(np.asarray(q) + 1, [2, 1, 3]).astype(np.int8)
+    w = np.sqrt(np.maximum(0, 1 - np.asarray(u) ** 2 - np.asarray(v) ** 2))
+    sign = np.where(np.asarray(wt) < 0, -1, 1)
+    typ = (typ * sign).astype(np.int8)
+    cols = [np.where(nh, -np.asarray(e), e), x, y] + ([z] if store_z else []) + \
+        [u, v, np.abs(wt)]
+    fl = np.stack([np.asarray(c, np.float32) for c in cols], 1).astype(byteorder + "f4")
+    rec = np.zeros(n, dtype=[("t", "i1"), ("f", byteorder + "f4", (fl.shape[1],))])
+    rec["t"] = typ
+    rec["f"] = fl
+    open(str(base) + ".IAEAphsp", "wb").write(rec.tobytes())
+    zc = "" if store_z else "   %g     // Constant Z\n" % z[0]
+    hdr = ("$IAEA_INDEX:\n   0\n\n$TITLE:\n   test\n\n$FILE_TYPE:\n   0\n\n"
+           "$CHECKSUM:\n   %d\n\n$RECORD_CONTENTS:\n    1     // X is stored ?\n"
+           "    1     // Y is stored ?\n    %d     // Z is stored ?\n    1     // U is stored ?\n"
+           "    1     // V is stored ?\n    1     // W is stored ?\n"
+           "    1     // Weight is stored ?\n    0     // Extra floats stored ?\n"
+           "    0     // Extra longs stored ?\n\n$RECORD_CONSTANT:\n%s\n"
+           "$RECORD_LENGTH:\n   %d\n\n$BYTE_ORDER:\n   %s\n\n$ORIG_HISTORIES:\n   %g\n\n"
+           "$PARTICLES:\n   %d\n\n" % (rec.nbytes, 1 if store_z else 0, zc,
+                                      rec.dtype.itemsize, "1234" if byteorder == "<" else "4321",
+                                      orig, n))
+    open(str(base) + ".IAEAheader", "w").write(hdr)
+
+
+IAEA_INPUT = INPUT.replace("library = egs_phsp_source", "library = iaea_phsp_source") \
+    .replace("phase space file = %s", "iaea phase space file = %s")
+
+
+def parse_phsp(path):
+    r = subprocess.run([APP, "-i", str(path), "--parse-only"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    return {l.partition(":")[0]: l.partition(":")[2].split() for l in r.stdout.splitlines()}
+
+
+@pytest.mark.parametrize("byteorder,store_z", [("<", False), (">", True)])
+def test_iaea_file_reads_as_the_same_particles(tmp_path, byteorder, store_z):
+    """The same particles as an EGSnrc MODE0 file and as an IAEA file (z
+    constant or stored, either byte order) read to the same arrays."""
+    n = 40
+    rng = np.random.default_rng(8)
+    e = rng.uniform(0.1, 6, n)
+    x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    u, v = rng.uniform(-0.3, 0.3, n), rng.uniform(-0.3, 0.3, n)
+    wt = rng.uniform(0.5, 2, n) * np.where(np.arange(n) % 7 == 3, -1, 1)
+    q = rng.choice([-1, 0, 1], n)
+    write_egsphsp(tmp_path / "p.egsphsp", e, x, y, u, v, wt, q, pinc=6.0)
+    write_iaea(tmp_path / "p", e, x, y, np.full(n, 1.0), u, v, wt, q, byteorder=byteorder,
+               store_z=store_z)
+    f1 = tmp_path / "a.egsinp"
+    f1.write_text(INPUT % ("p.egsphsp", "all"))
+    f2 = tmp_path / "b.egsinp"
+    f2.write_text(IAEA_INPUT % ("p", "all"))
+    a, b = parse_phsp(f1), parse_phsp(f2)
+    assert a["phsp"] == b["phsp"] and a["phsp sums"][1:] == b["phsp sums"][1:]
+    # MODE0 stores a charged particle's total energy in FP32: its kinetic
+    # energy comes back rounded there
+    assert float(a["phsp sums"][0]) == pytest.approx(float(b["phsp sums"][0]), rel=1e-7)
+    # the IAEA particles sit on their own plane z = 1, translated by -1
+    assert [float(t) for t in b["phsp plane"]] == [0.5, -0.25, 0.0]
+
+
+def test_iaea_header_mismatch_is_an_error(tmp_path):
+    n = 5
+    write_iaea(tmp_path / "p", np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n),
+               np.zeros(n), np.ones(n), np.zeros(n, int))
+    h = (tmp_path / "p.IAEAheader").read_text().replace("$RECORD_LENGTH:\n   25",
+                                                       "$RECORD_LENGTH:\n   29")
+    (tmp_path / "p.IAEAheader").write_text(h)
+    f = tmp_path / "b.egsinp"
+    f.write_text(IAEA_INPUT % ("p", "all"))
+    r = subprocess.run([APP, "-i", str(f), "--parse-only"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0 and "RECORD_LENGTH" in r.stderr
+
+
+@pytest.mark.gpu
+def test_iaea_and_egsphsp_sources_give_the_same_dose(tmp_path):
+    """One photon list as MODE0 and as IAEA (z = 0 plane): the same dose
+    within statistics (the event schedule hands particles to streams in an
+    order that varies from run to run, so the runs are not bit-identical)."""
+    n = 3000
+    rng = np.random.default_rng(9)
+    e = rng.uniform(0.5, 3, n)
+    x, y = rng.uniform(-1.5, 1.5, n), rng.uniform(-1.5, 1.5, n)
+    q = np.zeros(n, int)
+    write_egsphsp(tmp_path / "p.egsphsp", e, x, y, np.zeros(n), np.zeros(n), np.ones(n), q,
+                  pinc=float(n))
+    write_iaea(tmp_path / "p", e, x, y, np.zeros(n), np.zeros(n), np.zeros(n), np.ones(n), q,
+               orig=float(n))
+    outs = []
+    for name, txt in (("a", INPUT % ("p.egsphsp", "all")), ("b", IAEA_INPUT % ("p", "all"))):
+        f = tmp_path / (name + ".egsinp")
+        f.write_text(txt.replace("ncase = 1000", "ncase = 20000"))
+        r = subprocess.run([APP, "-i", str(f), "-p", BLOB700], capture_output=True, text=True,
+                           cwd=tmp_path, timeout=300)
+        assert r.returncode == 0, r.stderr
+        outs.append(read_3ddose(str(tmp_path / (name + ".3ddose"))))
+    (_, _, da, ea), (_, _, db, eb) = outs
+    m = np.maximum(da, db) > 0.05 * da.max()
+    z = (da[m] - db[m]) / np.sqrt((da[m] * ea[m]) ** 2 + (db[m] * eb[m]) ** 2)
+    print("iaea vs egsphsp: chi2/dof %.3f over %d voxels" % (np.mean(z * z), m.sum()))
+    assert m.sum() >= 20 and np.mean(z * z) < 1 + 3 * np.sqrt(2.0 / m.sum())
