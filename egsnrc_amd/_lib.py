@@ -68,6 +68,7 @@ SOURCE_PARALLEL_BEAM = 0
 SOURCE_DOSXYZ_FRONT = 1
 SOURCE_DOSXYZ_BEAM = 2
 SOURCE_PHSP = 3
+SOURCE_DOSXYZ_POINT = 4
 
 
 class EngineError(RuntimeError):

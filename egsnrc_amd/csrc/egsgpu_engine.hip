@@ -119,6 +119,37 @@ __device__ __forceinline__ bool sample_dosxyz_front(const DevSource<R> &S, const
     return true;
 }
 
+// DOSXYZnrc isource 3, srchst (srcxyznrc.mortran:3451-3495): a point
+// of the rectangle on the front face, accepted with probability
+// (SSD/r)^3 for a point source at distance SSD on the z axis (a pencil for
+// a rectangle of zero width), then the energy; direction from the source
+// point through it.
+template <typename R, bool EV, bool LUX, bool HFL, bool SPLIT>
+__device__ __forceinline__ bool sample_dosxyz_point(const DevSource<R> &S, const GeomView<R> &G,
+                                                    Shower<R, EV, LUX, HFL, SPLIT> &sh, Part<R> &q) {
+    const double ssd = S.dsource;
+    double x = (double)S.x0, y = (double)S.y0;
+    const double dx = (double)S.x1 - x, dy = (double)S.y1 - y;
+    double rxyz = sqrt(ssd * ssd + x * x + y * y);
+    if (dx != 0 && dy != 0) {
+        for (int ntry = 0; ntry < 1000000; ntry++) {
+            x = (double)sh.rnd() * dx + (double)S.x0;
+            y = (double)sh.rnd() * dy + (double)S.y0;
+            double r1 = (double)sh.rnd();
+            rxyz = sqrt(ssd * ssd + x * x + y * y);
+            double win = ssd / rxyz;
+            if (r1 < win * win * win) break;
+        }
+    }
+    R E = source_energy(S, sh);
+    int ix = planes_lower(x, G.nx, G.xp), iy = planes_lower(y, G.ny, G.yp);
+    q.e = S.charge ? E + (R)0.5109989461 : E;
+    q.x = (R)x; q.y = (R)y; q.z = G.zp[0];
+    q.u = (R)(x / rxyz); q.v = (R)(y / rxyz); q.w = (R)(ssd / rxyz);
+    q.wt = 1; q.dnear = 0; q.iq = S.charge; q.ir = ix + iy * G.nx;
+    return true;
+}
+
 // DOSXYZnrc isource 1, srchst (srcxyznrc.mortran:3551-3571, 4157-4165,
 // 4255-4392): a point of the collimator rectangle and the energy, rotated
 // into position at distance dsource from the isocentre, then carried to
@@ -223,6 +254,7 @@ __device__ __forceinline__ bool sample_primary(const DevSource<R> &S, const Geom
                                Shower<R, EV, LUX, HFL, SPLIT> &sh, Part<R> &q, long long hidx) {
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_FRONT) return sample_dosxyz_front(S, G, sh, q);
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_BEAM) return sample_dosxyz_beam(S, G, sh, q);
+    if (S.kind == EGSGPU_SOURCE_DOSXYZ_POINT) return sample_dosxyz_point(S, G, sh, q);
     if (S.kind == EGSGPU_SOURCE_PHSP) return sample_phsp(S, G, hidx, q);
     const R RM = (R)0.5109989461;
     int ireg = -1;
@@ -1569,8 +1601,11 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
             src->spectrum == EGSGPU_SPEC_LINE || src->spectrum == EGSGPU_SPEC_CDFINV) &&
             src->nbins > 0 && src->spec_e && src->spec_w),
             "source spectrum");
-    EGS_ARG(src->kind >= EGSGPU_SOURCE_PARALLEL_BEAM && src->kind <= EGSGPU_SOURCE_PHSP,
+    EGS_ARG(src->kind >= EGSGPU_SOURCE_PARALLEL_BEAM && src->kind <= EGSGPU_SOURCE_DOSXYZ_POINT,
             "source kind");
+    EGS_ARG(src->kind != EGSGPU_SOURCE_DOSXYZ_POINT ||
+            (src->dsource > 0 && src->x0 <= src->x1 && src->y0 <= src->y1),
+            "a point source needs SSD (dsource) > 0 and x0 <= x1, y0 <= y1");
     EGS_ARG(src->kind != EGSGPU_SOURCE_DOSXYZ_FRONT || src->uz > 0,
             "a front-face source must point into the grid (uz > 0)");
     int rc = require_device();

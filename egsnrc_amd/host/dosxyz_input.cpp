@@ -199,9 +199,9 @@ DosxyzInput read_dosxyz_input(const std::string &path) {
         for (size_t i = 2; i < s1.size(); i++) d.src.push_back(num(s1, i));
         while (d.src.size() < 12) d.src.push_back(0.0);
         if (d.iqin < -1 || d.iqin > 1) d.iqin = 0;
-        if (d.isource != 0 && d.isource != 1)
+        if (d.isource != 0 && d.isource != 1 && d.isource != 3)
             throw std::runtime_error("DOSXYZnrc source isource = " + std::to_string(d.isource) +
-                                     " is not supported (isource 0 and 1 are)");
+                                     " is not supported (isource 0, 1 and 3 are)");
         auto s2 = r.next_fields("enflag, mode, medsur, dsurround, dflag");
         d.enflag = inum(s2, 0);
         if (d.enflag == 0) {

@@ -570,14 +570,30 @@ class EGS_GPUApplication {
         dose_file = output + ".3ddose";
     }
 
-    // srcinit for isource 0 and 1 (srcxyznrc.mortran:2442-2539, 2771-2789)
+    // srcinit for isource 0, 1 and 3 (srcxyznrc.mortran:2442-2586, 2771-2789)
     void init_dosxyz_source() {
         const double deg = 3.141593 / 180.;   // as in srcinit
         src = EgsSource{};
         src.charge = dz.iqin;
         const int nx = (int)xp.p.size() - 1, ny = (int)yp.p.size() - 1, nz = (int)zp.p.size() - 1;
         const std::vector<double> &s = dz.src;
-        if (dz.isource == 0) {
+        if (dz.isource == 3) {   // frontal point source (:1100-1109, 2550-2586)
+            double xinl = s[0], xinu = s[1], yinl = s[2], yinu = s[3], ssd = s[4];
+            if (xinl < xp.p[0]) xinl = xp.p[0];
+            if (xinu <= xinl) xinu = xinl;
+            if (xinu > xp.p[nx]) xinu = xp.p[nx];
+            if (xinl > xp.p[nx]) xinl = xp.p[nx];
+            if (yinl < yp.p[0]) yinl = yp.p[0];
+            if (yinu <= yinl) yinu = yinl;
+            if (yinu > yp.p[ny]) yinu = yp.p[ny];
+            if (yinl > yp.p[ny]) yinl = yp.p[ny];
+            if (!(ssd > 0)) throw std::runtime_error("source 3: SSD must be > 0");
+            src.kind = EGSGPU_SOURCE_DOSXYZ_POINT;
+            src.x0 = xinl; src.x1 = xinu; src.y0 = yinl; src.y1 = yinu; src.z = zp.p[0];
+            src.dsource = ssd;
+            src.ux = 0; src.uy = 0; src.uz = 1;
+            area = (xinu - xinl) * (yinu - yinl);   // beamarea
+        } else if (dz.isource == 0) {
             double xinl = s[0], xinu = s[1], yinl = s[2], yinu = s[3];
             double thetax = s[4], thetay = s[5], thetaz = s[6];
             if (xinl < xp.p[0]) xinl = xp.p[0];

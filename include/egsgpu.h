@@ -76,6 +76,12 @@ typedef struct EgsXYZGeometry {
 #define EGSGPU_SOURCE_PHSP          3 /* phase-space particles given by
                                          egsgpu_set_phsp, translated by
                                          (x0, y0, z)                       */
+#define EGSGPU_SOURCE_DOSXYZ_POINT  4 /* DOSXYZnrc isource 3: a point at
+                                         distance dsource (SSD) in front of
+                                         the grid's first z plane on the z
+                                         axis, aimed at x0..x1, y0..y1 of
+                                         that plane with the cos^3 weight of
+                                         srcxyznrc.mortran:3451-3495       */
 #define EGSGPU_SHAPE_POINT     0   /* EGS_PointShape                       */
 #define EGSGPU_SHAPE_RECTANGLE 1   /* EGS_RectangleShape (z = shape_z)     */
 typedef struct EgsSource {

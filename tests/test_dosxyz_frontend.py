@@ -238,3 +238,50 @@ def test_parse_photon_splitting(tmp_path):
     assert txt != f.read_text()
     f.write_text(txt)
     assert transport_options(parse(f)[0])["n_split"] == "40"
+
+
+def write_isource3_input(path, field, ssd, ncase=100000):
+    write_isource1_input(path, ncase=ncase)
+    txt = path.read_text().replace("0, 1, 0.0, 0.0, 5.0, 180.0, 0.0, 3, 2, 0.0",
+                                   "0, 3, %g, %g, %g, %g, %g" % (*field, ssd))
+    assert "0, 3, " in txt
+    path.write_text(txt)
+
+
+def test_parse_isource3_clamps_the_field(tmp_path):
+    """isource 3 (srcxyznrc.mortran:1100-1109, 2550-2586): the field is
+    clamped to the front face, beamarea its area."""
+    f = tmp_path / "s3.egsinp"
+    write_isource3_input(f, (-6.0, 2.0, -1.5, 1.5), 100.0)
+    p, _ = parse(f)
+    src = dict(zip(p["source"][::2], p["source"][1::2]))
+    assert src["kind"] == "4"
+    assert [float(v) for v in p["source rectangle"][:4]] == [-4.0, 2.0, -1.5, 1.5]
+    assert float(src["area"]) == pytest.approx(18.0)
+    assert float(p["dsource"][0]) == 100.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("field,area", [((-2.0, 2.0, -2.0, 2.0), 16.0), ((0.5, 0.5, 0.5, 0.5), 0)])
+def test_isource3_far_point_source_matches_a_parallel_beam(tmp_path, field, area):
+    """A frontal point source 10^5 cm away is a parallel beam: over a 4 x 4
+    cm field it gives the oracle's 4 x 4 cm parallel beam, and with a field
+    of zero width a pencil beam at (0.5, 0.5)."""
+    f = tmp_path / "s3.egsinp"
+    n = 200000
+    write_isource3_input(f, field, 1.0e5, ncase=n)
+    r = subprocess.run([APP, "-i", str(f), "-p", BLOB700], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    med = np.zeros((10, 8, 8), np.int32)
+    med[3:6, 2:6, 2:6] = 1
+    rho = np.choose(med, [1.0, 1.85]).reshape(-1)
+    E, sE = per_fluence_from_3ddose(tmp_path / "s3.3ddose", rho, area if area else 1.0)
+    xb = np.linspace(-4, 4, 9)
+    g, keep = _lib.make_geometry(xb, xb.copy(), np.linspace(0, 10, 11), med.reshape(-1), None)
+    if area:
+        src = make_source(0, 1.25, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    else:
+        src = make_source(0, 1.25, shape="point", x0=0.5, y0=0.5, z=-1.0)
+    m, s = oracle_energy(g, src, transport(), n)
+    compare(E, sE, m, s, "isource 3 field %s" % (field,), 40 if area else 8)
