@@ -2,7 +2,7 @@
 //
 // Reads the record-oriented input of DOSXYZnrc
 // (user_codes/dosxyznrc/dosxyznrc.mortran:150-330, read at :1520-1900) and
-// of its source routines for isource 0 and 1
+// of its source routines for isource 0, 1 and 3
 // (user_codes/dosxyznrc/srcxyznrc.mortran:104-146, 606-660, read at
 // :1010-1060 and :1760-1830), followed by the EGSnrc transport parameter
 // block (":Start MC Transport Parameter:"), which is handed on as text.

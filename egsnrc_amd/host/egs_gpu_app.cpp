@@ -24,7 +24,7 @@
 //                         Russian roulette, egs_radiative_splitting.cpp:80-132)
 // or, for a DOSXYZnrc input file (recognised by its records), the
 // DOSXYZnrc front end: phantom records or an .egsphant CT phantom,
-// isource 0 and 1 with monoenergetic or .spectrum energies, NCASE and the
+// isource 0, 1 and 3 with monoenergetic or .spectrum energies, NCASE and the
 // ranmar seeds of record 13, the EGSnrc transport parameter block, and the
 // .3ddose of DOSXYZnrc normalised per incident fluence
 // (dosxyznrc.mortran:2997-3002, 3100-3130).
