@@ -274,8 +274,13 @@ def test_checkpoint_restart_continues_the_run(engine, rng):
 @pytest.mark.gpu
 def test_fp32_vs_fp64_precision_report(engine):
     """FP32 transport against FP64 transport with the same random numbers:
-    per-voxel differences stay well inside the statistical uncertainty and
-    the total deposit agrees to 1e-3."""
+    per-voxel differences stay well inside the statistical uncertainty, and
+    the total deposit agrees within 3 sigma of the total's uncertainty
+    (from the voxel uncertainties as if independent, which the positive
+    correlation of a history's deposits only makes smaller).  Once a
+    rounding sends a history another way, the two runs continue on
+    different paths, so the total's difference is partly statistical and
+    moves with any change to how the kernels are compiled."""
     from egsnrc_amd.precision_report import precision_report
     xb, yb, zb, med, rhor, src, tp = photon_coupled_case()
     rep, arrays = precision_report(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, 100000,
@@ -284,7 +289,10 @@ def test_fp32_vs_fp64_precision_report(engine):
     assert rep["voxels"] > 100
     assert rep["chi2_per_dof"] < 1.0
     assert rep["frac_beyond_2sigma"] < 0.05
-    assert abs(rep["total_rel_diff"]) < 1e-3
+    r32, s32, r64, s64 = arrays
+    sig_total = np.sqrt(np.sum(s32 ** 2) + np.sum(s64 ** 2)) / r64.sum()
+    print("total: rel diff %.5f, sigma %.5f" % (rep["total_rel_diff"], sig_total))
+    assert abs(rep["total_rel_diff"]) < 3 * sig_total
 
 
 def water_point_case(n=10, dx=1.5, energy=2.0):
