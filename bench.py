@@ -51,7 +51,7 @@ WORKLOADS = {
 def build_workload(name):
     from egsnrc_amd import _lib
     from egsnrc_amd.engine import make_source
-    from tests import workloads
+    from egsnrc_amd import workloads
     if name == "slab128":
         xb, yb, zb, med, rhor = workloads.slab_phantom(128, 0.25)
         e, w = workloads.falling_spectrum()
@@ -132,7 +132,7 @@ def grad_bench(nhist, lanes=0):
     of nhist histories (egsgpu_density_grad)."""
     from egsnrc_amd import _lib
     from egsnrc_amd.engine import Simulation, Tables, make_source
-    from tests import workloads
+    from egsnrc_amd import workloads
     n, dx = 64, 0.4
     xb, yb, zb, med, rhor = workloads.kerma_phantom(n, dx, seed=4)
     _, _, _, _, rho2 = workloads.kerma_phantom(n, dx, seed=5)
@@ -163,7 +163,7 @@ def bench_batch(a):
     per step."""
     from egsnrc_amd import _lib
     from egsnrc_amd.engine import Simulation, Tables, make_source, run_batched
-    from tests import workloads
+    from egsnrc_amd import workloads
     tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab"))
     jobs = workloads.batch_jobs(256, hist_scale=a.hist_scale)
     tp = _lib.default_transport(ecut=0.521, pcut=0.01, kerma=0)
@@ -249,7 +249,7 @@ def plain_jobs(a, jobs, tab, tp, t_batch, rel2_batch):
     timed region): time, mean relative variance and the efficiency ratio of
     the batch at equal per-voxel sigma."""
     from egsnrc_amd.engine import Simulation, make_source
-    from tests import workloads
+    from egsnrc_amd import workloads
     jobs = workloads.batch_jobs(len(jobs), hist_scale=a.hist_scale)
     T = 0.0
     rel2 = []

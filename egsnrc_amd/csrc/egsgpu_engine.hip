@@ -323,6 +323,7 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
         sh.p = q;
         sh.np = 1;
         sh.eph = S::kNew;
+        sh.score_extra(S::kIncident, q);
         while (sh.np > 0) {
             sh.rtopup();
             if (sh.p.iq == 0) {
@@ -437,6 +438,7 @@ k_kerma(KArgs<R> a, int ksteps, int batch, int max_wait) {
                 sh.np = 1;
                 sh.eph = S::kNew;
                 sh.pph = S::kPEnter;
+                sh.score_extra(S::kIncident, q);
                 inhist = true;
             }
         } else if (chosen == kEndHist) {
@@ -507,21 +509,39 @@ __device__ __forceinline__ int bank_load(const DevBank<R> &b, int i, Part<R> &q)
 // particle becomes visible.  A full bank absorbs the particle where it is
 // (counted as an overflow).
 template <typename R, bool LUX>
-__device__ void bank_push(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *count, int ph = 0,
-                          R s0 = 0, R s1 = 0, R s2 = 0, R s3 = 0) {
-    atomicAdd(sh.slots->pending + sh.slot, 1);
+__device__ bool bank_try(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *count, int ph,
+                         R s0, R s1, R s2, R s3) {
     int i = wave_claim(count);
-    if (i < b.cap) {
-        bank_store(b, i, sh.p, sh.slot);
-        b.ph[i] = ph;
-        if (ph) { b.s0[i] = s0; b.s1[i] = s1; b.s2[i] = s2; b.s3[i] = s3; }
-        return;
-    }
+    if (i >= b.cap) return false;
+    bank_store(b, i, sh.p, sh.slot);
+    b.ph[i] = ph;
+    if (ph) { b.s0[i] = s0; b.s1[i] = s1; b.s2[i] = s2; b.s3[i] = s3; }
+    return true;
+}
+template <typename R, bool LUX>
+__device__ void bank_absorb(Shower<R, true, LUX> &sh) {
     atomicSub(sh.slots->pending + sh.slot, 1);
     sh.c_ovf++;
     const R RM = (R)0.5109989461;
     R edep = sh.p.iq == 0 ? sh.p.e : (sh.p.iq < 0 ? sh.p.e - RM : sh.p.e + RM);
     sh.score(sh.p.ir, edep, sh.p.wt);
+}
+template <typename R, bool LUX>
+__device__ void bank_push(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *count, int ph = 0,
+                          R s0 = 0, R s1 = 0, R s2 = 0, R s3 = 0) {
+    atomicAdd(sh.slots->pending + sh.slot, 1);
+    if (!bank_try(sh, b, count, ph, s0, s1, s2, s3)) bank_absorb(sh);
+}
+// A particle of the other type (a photon met by the charged-particle
+// kernel, or the reverse) goes to the bank of its type; when that bank is
+// full it waits, fresh, in this launch's deferral bank, whose next drain
+// passes it on (a drained bank of the other type has room again).
+template <typename R, bool LUX>
+__device__ void bank_push_other(Shower<R, true, LUX> &sh, const EvArgs<R> &a) {
+    atomicAdd(sh.slots->pending + sh.slot, 1);
+    if (bank_try(sh, a.push, &a.ctl->n[a.ip], 0, (R)0, (R)0, (R)0, (R)0)) return;
+    if (bank_try(sh, a.defr, &a.ctl->n[a.id], 0, (R)0, (R)0, (R)0, (R)0)) return;
+    bank_absorb(sh);
 }
 
 template <typename R, bool LUX>
@@ -571,6 +591,14 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
             continue;
         }
         atomicExch(a.slots.pending + slot, 1);
+        // the primary's energy into the history's table (incident region)
+        sh.slot = slot;
+        ev_job(sh, a, slot);
+        sh.score_extra(Shower<R, true, LUX>::kIncident, q);
+        if (sh.cvox >= 0) sh.slot_add(sh.cvox, sh.cval);
+        sh.cvox = -1;
+        sh.cval = 0;
+        sh.slot = -1;
         int j = wave_claim(&a.ctl->n[a.ip]);
         if (j < a.push.cap) {
             bank_store(a.push, j, q, slot);
@@ -614,7 +642,7 @@ k_ev_photon(EvArgs<R> a) {
             bool top = true;
             while (sh.np > 0) {
                 if (sh.p.iq != 0) {
-                    bank_push(sh, a.push, &a.ctl->n[a.ip]);
+                    bank_push_other(sh, a);
                 } else if (top && sh.pph != S::kPEnter) {
                     bank_push(sh, a.defr, &a.ctl->n[a.id], sh.pph, sh.p_dpmfp, sh.p_gle);
                 } else {
@@ -647,7 +675,7 @@ k_ev_photon(EvArgs<R> a) {
             }
         }
         if (sh.p.iq != 0) {
-            bank_push(sh, a.push, &a.ctl->n[a.ip]);
+            bank_push_other(sh, a);
             sh.pop();
             sh.pph = S::kPEnter;
             continue;
@@ -698,7 +726,7 @@ k_ev_electron(EvArgs<R> a) {
             bool top = true;
             while (sh.np > 0) {
                 if (sh.p.iq == 0) {
-                    bank_push(sh, a.push, &a.ctl->n[a.ip]);
+                    bank_push_other(sh, a);
                 } else if (top && sh.eph != S::kNew) {
                     bank_push(sh, a.defr, &a.ctl->n[a.id], sh.eph | (sh.e_ctstep ? 8 : 0),
                               sh.e_demfp, sh.e_ttstep, sh.e_sig0, sh.e_elke);
@@ -735,7 +763,7 @@ k_ev_electron(EvArgs<R> a) {
             }
         }
         if (sh.p.iq == 0) {
-            bank_push(sh, a.push, &a.ctl->n[a.ip]);
+            bank_push_other(sh, a);
             sh.pop();
             sh.eph = S::kNew;
             continue;
@@ -817,6 +845,9 @@ struct Precision {
 }  // namespace egsgpu
 
 using namespace egsgpu;
+
+// tally regions past the voxels: energy escaped, energy incident
+constexpr int kExtraRegions = 2;
 
 struct egsgpu_handle {
     int fp64 = 0;
@@ -1319,8 +1350,10 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
         CK(upload(h, &jd, 1, &dj));
         h->job_self = (void *)dj;
     }
-    for (int k = 0; k < 2; k++) CK(alloc_bank(h, 2 * S.nslots, P.bank[k]));
-    for (int k = 2; k < 4; k++) CK(alloc_bank(h, 8 * S.nslots, P.bank[k]));
+    // a history in flight may hold several photons (bremsstrahlung,
+    // annihilation) as well as charged particles: 8 entries per slot for
+    // either type
+    for (int k = 0; k < 4; k++) CK(alloc_bank(h, 8 * S.nslots, P.bank[k]));
     h->pc = 0;
     h->ec = 2;
     hipLaunchKernelGGL(k_ev_init, dim3((S.nslots + 255) / 256), dim3(256), 0, h->stream,
@@ -1726,11 +1759,13 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         set_error("%s", e.what());
         return fail(EGSGPU_E_FORMAT);
     }
-    if ((rc = alloc(h, (size_t)h->nvox, &h->tally.sum))) return fail(rc);
-    if ((rc = alloc(h, (size_t)h->nvox, &h->tally.sum2))) return fail(rc);
+    // the dose regions, then the energy-bookkeeping regions (escaped,
+    // incident) of Shower::score_extra
+    if ((rc = alloc(h, (size_t)h->nvox + kExtraRegions, &h->tally.sum))) return fail(rc);
+    if ((rc = alloc(h, (size_t)h->nvox + kExtraRegions, &h->tally.sum2))) return fail(rc);
     if ((rc = alloc(h, 32, &h->tally.counters))) return fail(rc);
-    if (hipMemset(h->tally.sum, 0, sizeof(double) * h->nvox) != hipSuccess ||
-        hipMemset(h->tally.sum2, 0, sizeof(double) * h->nvox) != hipSuccess ||
+    if (hipMemset(h->tally.sum, 0, sizeof(double) * (h->nvox + kExtraRegions)) != hipSuccess ||
+        hipMemset(h->tally.sum2, 0, sizeof(double) * (h->nvox + kExtraRegions)) != hipSuccess ||
         hipMemset(h->tally.counters, 0, 256) != hipSuccess) {
         set_error("tally clear failed");
         return fail(EGSGPU_E_HIP);
@@ -1984,12 +2019,13 @@ int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice) {
     int rc = ensure_lanes(h);
     if (rc) return rc;
     // the probes must not change the tallies: keep them aside
-    size_t nb = sizeof(double) * (size_t)h->nvox;
+    const size_t nt = (size_t)h->nvox + kExtraRegions;
+    size_t nb = sizeof(double) * nt;
     double *keep = nullptr;
     EGS_HIP(hipMalloc(&keep, 2 * nb));
     auto done = [&](int r) { (void)hipFree(keep); return r; };
     if (hipMemcpyAsync(keep, h->tally.sum, nb, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(keep + h->nvox, h->tally.sum2, nb, hipMemcpyDeviceToDevice, h->stream) !=
+        hipMemcpyAsync(keep + nt, h->tally.sum2, nb, hipMemcpyDeviceToDevice, h->stream) !=
             hipSuccess) {
         set_error("tally save failed");
         return done(EGSGPU_E_HIP);
@@ -2027,7 +2063,7 @@ int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice) {
         h->ev_wp = h->ev_we = wpe;
     }
     if (hipMemcpyAsync(h->tally.sum, keep, nb, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(h->tally.sum2, keep + h->nvox, nb, hipMemcpyDeviceToDevice, h->stream) !=
+        hipMemcpyAsync(h->tally.sum2, keep + nt, nb, hipMemcpyDeviceToDevice, h->stream) !=
             hipSuccess ||
         hipMemcpy(h->tally.counters, cnt, sizeof(cnt), hipMemcpyHostToDevice) != hipSuccess) {
         set_error("tally restore failed");
@@ -2067,14 +2103,17 @@ int egsgpu_run(egsgpu_handle *h, int64_t nhist) {
 
 // ---- checkpoint / restart ----------------------------------------------
 // Layout: "EGSGPUCK", u32 version, u32 nvox, u32 nlanes, u32 rng kind,
-// i64 nhist, i32 seeds[2] (ixx, jxx or luxury, seed), f64 sum[nvox],
-// f64 sum2[nvox], then the stream records: RanmarRec[nlanes] or
-// RanluxRec[nlanes] (streams that have not run yet are stored seeded).
-// Version 1 (ranmar only, kind word 0) is still read.
+// i64 nhist, i32 seeds[2] (ixx, jxx or luxury, seed), f64 sum[nvox + 2],
+// f64 sum2[nvox + 2] (the dose regions and the energy bookkeeping), then
+// the stream records: RanmarRec[nlanes] or RanluxRec[nlanes] (streams
+// that have not run yet are stored seeded).  Versions 1 (ranmar only, kind
+// word 0) and 2 (sums of the dose regions only; the bookkeeping restarts)
+// are still read.
 namespace {
-constexpr uint32_t kCkVersion = 2;
-size_t ck_size(const egsgpu_handle *h, int nlanes, int kind) {
-    return 8 + 4 * 4 + 8 + 2 * 4 + 2 * sizeof(double) * (size_t)h->nvox +
+constexpr uint32_t kCkVersion = 3;
+size_t ck_size(const egsgpu_handle *h, int nlanes, int kind, uint32_t version = kCkVersion) {
+    const size_t nt = (size_t)h->nvox + (version >= 3 ? kExtraRegions : 0);
+    return 8 + 4 * 4 + 8 + 2 * 4 + 2 * sizeof(double) * nt +
            (kind == EGSGPU_RNG_RANLUX ? sizeof(RanluxRec) : sizeof(RanmarRec)) * (size_t)nlanes;
 }
 }  // namespace
@@ -2099,10 +2138,11 @@ int egsgpu_checkpoint(egsgpu_handle *h, void *buf, size_t *size) {
     memcpy(p, &h->nhist, 8); p += 8;
     int32_t seeds[2] = {lux ? h->lux : h->ixx, lux ? h->lux_seed : h->jxx};
     memcpy(p, seeds, 8); p += 8;
-    EGS_HIP(hipMemcpy(p, h->tally.sum, sizeof(double) * h->nvox, hipMemcpyDeviceToHost));
-    p += sizeof(double) * h->nvox;
-    EGS_HIP(hipMemcpy(p, h->tally.sum2, sizeof(double) * h->nvox, hipMemcpyDeviceToHost));
-    p += sizeof(double) * h->nvox;
+    const size_t nt = (size_t)h->nvox + kExtraRegions;
+    EGS_HIP(hipMemcpy(p, h->tally.sum, sizeof(double) * nt, hipMemcpyDeviceToHost));
+    p += sizeof(double) * nt;
+    EGS_HIP(hipMemcpy(p, h->tally.sum2, sizeof(double) * nt, hipMemcpyDeviceToHost));
+    p += sizeof(double) * nt;
     size_t rb = 0;
     void *recs = rng_records(h, &rb);
     EGS_HIP(hipMemcpy(p, recs, rb, hipMemcpyDeviceToHost));
@@ -2119,7 +2159,7 @@ int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size) {
     }
     uint32_t hd[4];
     memcpy(hd, p + 8, sizeof(hd));
-    if (hd[0] != kCkVersion && hd[0] != 1) {
+    if (hd[0] < 1 || hd[0] > kCkVersion) {
         set_error("checkpoint version %u", hd[0]);
         return EGSGPU_E_FORMAT;
     }
@@ -2137,7 +2177,7 @@ int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size) {
         return EGSGPU_E_ARG;
     }
     if (!h->lanes_ready) h->nlanes = (int)hd[2];
-    if (size < ck_size(h, (int)hd[2], kind)) {
+    if (size < ck_size(h, (int)hd[2], kind, hd[0])) {
         set_error("checkpoint truncated");
         return EGSGPU_E_FORMAT;
     }
@@ -2155,10 +2195,15 @@ int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size) {
         h->ixx = seeds[0];
         h->jxx = seeds[1];
     }
-    EGS_HIP(hipMemcpy(h->tally.sum, p, sizeof(double) * h->nvox, hipMemcpyHostToDevice));
-    p += sizeof(double) * h->nvox;
-    EGS_HIP(hipMemcpy(h->tally.sum2, p, sizeof(double) * h->nvox, hipMemcpyHostToDevice));
-    p += sizeof(double) * h->nvox;
+    const size_t nt = (size_t)h->nvox + (hd[0] >= 3 ? kExtraRegions : 0);
+    if (hd[0] < 3) {
+        EGS_HIP(hipMemset(h->tally.sum + h->nvox, 0, sizeof(double) * kExtraRegions));
+        EGS_HIP(hipMemset(h->tally.sum2 + h->nvox, 0, sizeof(double) * kExtraRegions));
+    }
+    EGS_HIP(hipMemcpy(h->tally.sum, p, sizeof(double) * nt, hipMemcpyHostToDevice));
+    p += sizeof(double) * nt;
+    EGS_HIP(hipMemcpy(h->tally.sum2, p, sizeof(double) * nt, hipMemcpyHostToDevice));
+    p += sizeof(double) * nt;
     size_t rb = 0;
     void *recs = rng_records(h, &rb);
     EGS_HIP(hipMemcpy(recs, p, rb, hipMemcpyHostToDevice));
@@ -2257,8 +2302,9 @@ int egsgpu_get_tallies(egsgpu_handle *h, double *sum, double *sum2, int64_t *nhi
 
 int egsgpu_reset_tallies(egsgpu_handle *h) {
     EGS_ARG(h, "handle");
-    EGS_HIP(hipMemsetAsync(h->tally.sum, 0, sizeof(double) * h->nvox, h->stream));
-    EGS_HIP(hipMemsetAsync(h->tally.sum2, 0, sizeof(double) * h->nvox, h->stream));
+    const size_t nt = (size_t)h->nvox + kExtraRegions;
+    EGS_HIP(hipMemsetAsync(h->tally.sum, 0, sizeof(double) * nt, h->stream));
+    EGS_HIP(hipMemsetAsync(h->tally.sum2, 0, sizeof(double) * nt, h->stream));
     EGS_HIP(hipMemsetAsync(h->tally.counters, 0, 256, h->stream));
     EGS_HIP(hipStreamSynchronize(h->stream));
     h->nhist = 0;
@@ -2272,8 +2318,24 @@ int egsgpu_set_tallies(egsgpu_handle *h, const double *sum, const double *sum2,
                            h->stream));
     EGS_HIP(hipMemcpyAsync(h->tally.sum2, sum2, sizeof(double) * h->nvox,
                            hipMemcpyHostToDevice, h->stream));
+    // the energy bookkeeping restarts with the run that follows
+    EGS_HIP(hipMemsetAsync(h->tally.sum + h->nvox, 0, sizeof(double) * kExtraRegions, h->stream));
+    EGS_HIP(hipMemsetAsync(h->tally.sum2 + h->nvox, 0, sizeof(double) * kExtraRegions, h->stream));
     EGS_HIP(hipStreamSynchronize(h->stream));
     h->nhist = nhist;
+    return EGSGPU_OK;
+}
+
+int egsgpu_get_energy_balance(egsgpu_handle *h, double out[4]) {
+    EGS_ARG(h && out, "handle, out");
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    double s[kExtraRegions], s2[kExtraRegions];
+    EGS_HIP(hipMemcpy(s, h->tally.sum + h->nvox, sizeof(s), hipMemcpyDeviceToHost));
+    EGS_HIP(hipMemcpy(s2, h->tally.sum2 + h->nvox, sizeof(s2), hipMemcpyDeviceToHost));
+    out[0] = s[1];
+    out[1] = s2[1];
+    out[2] = s[0];
+    out[3] = s2[0];
     return EGSGPU_OK;
 }
 

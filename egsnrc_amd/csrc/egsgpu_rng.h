@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ranmar_seed.h"
+
 namespace egsgpu {
 
 constexpr int kRNB = 16;          // ranmar outputs per refill
@@ -27,40 +29,17 @@ struct RanmarRec {
 };
 static_assert(sizeof(RanmarRec) == 400, "ranmar record is 400 bytes");
 
-// Seed pair of stream s (createRNG(input, s) for jxx+s <= 30080,
-// egs_rndm.cpp:438-446; the excess carries into ixx).
-__host__ __device__ inline void stream_seeds(int ixx, int jxx, int64_t s,
-                                             int &is, int &js) {
-    if (ixx <= 0 || ixx >= 31328) ixx = 1802;
-    if (jxx <= 0 || jxx >= 30081) jxx = 9373;
-    int64_t t = (int64_t)jxx + s;
-    int64_t carry = (t - 1) / 30080;
-    js = (int)(1 + (t - 1) % 30080);
-    is = (int)(1 + ((int64_t)ixx - 1 + carry) % 31327);
+// Seed pair of stream s and the lag table of a seed pair: ranmar_seed.h
+// (createRNG(input, s), egs_rndm.cpp:438-446; EGS_Ranmar::setState,
+// egs_rndm.cpp:303-336).
+__host__ __device__ inline void stream_seeds(int ixx, int jxx, int64_t s, int &is, int &js) {
+    ranmar_stream_seeds(ixx, jxx, s, is, js);
 }
-
-// EGS_Ranmar::setState (egs_rndm.cpp:303-336).
 __device__ inline void ranmar_seed(RanmarRec *r, int ixx, int jxx) {
-    if (ixx <= 0 || ixx >= 31328) ixx = 1802;
-    if (jxx <= 0 || jxx >= 30081) jxx = 9373;
-    int i = (ixx / 177) % 177 + 2;
-    int j = ixx % 177 + 2;
-    int k = (jxx / 169) % 178 + 1;
-    int l = jxx % 169;
-    for (int ii = 0; ii < 97; ii++) {
-        int s = 0, t = 8388608;
-        for (int jj = 0; jj < 24; jj++) {
-            int m = (((i * j) % 179) * k) % 179;
-            i = j; j = k; k = m;
-            l = (53 * l + 1) % 169;
-            if ((l * m) % 64 >= 32) s += t;
-            t >>= 1;
-        }
-        r->u[ii] = s;
-    }
-    r->c = 362436;
-    r->ix = 96;
-    r->jx = 32;
+    ranmar_seed_words(ixx, jxx, r->u);
+    r->c = kRanmarC0;
+    r->ix = kRanmarIx0;
+    r->jx = kRanmarJx0;
 }
 
 // Produce NB consecutive outputs of the stream (EGS_Ranmar::fillArray,

@@ -215,7 +215,11 @@ struct Shower {
         }
         if (ir < 0 || !(edep > 0)) return;
         c_dep++;
-        double e = (double)edep * (double)wt;
+        score_raw(ir, (double)edep * (double)wt);
+    }
+    // add e to region ir of the current history (run-length cache in front
+    // of the history's table)
+    __device__ __forceinline__ void score_raw(int ir, double e) {
         if (ir == cvox) { cval += e; return; }
         if (cvox >= 0) {
             if constexpr (EV) slot_add(cvox, cval);
@@ -223,6 +227,21 @@ struct Shower {
         }
         cvox = ir;
         cval = e;
+    }
+    // Energy bookkeeping of the simulation, scored like deposits into two
+    // regions past the voxels so that each has a history-by-history sum:
+    // region nvox + kIncident the energy brought in by the primaries that
+    // enter the grid, region nvox + kEscaped the energy carried out of it.
+    // Without variance reduction the two and the dose sums balance:
+    // incident = deposited + escaped.  A particle's energy here is its
+    // kinetic energy, plus 2 m c^2 for a positron (its annihilation).
+    enum { kEscaped = 0, kIncident = 1 };
+    __device__ __forceinline__ void score_extra(int k, const Part<R> &q) {
+        if (grad) return;
+        const R RM = (R)0.5109989461;
+        const R e = q.iq == 0 ? q.e : (q.iq < 0 ? q.e - RM : q.e + RM);
+        if (!(e > 0) || !(q.wt > 0)) return;
+        score_raw(Gp->nxy * Gp->nz + k, (double)e * (double)q.wt);
     }
     // event schedule: add e to voxel vox of the current slot's table
     __device__ void slot_add(int vox, double e) {
@@ -442,7 +461,7 @@ struct Shower {
                 p.z = p.z + p.w * ustep;
                 R d = p_dpmfp - ustep / gmfp;
                 p_dpmfp = d > 0 ? d : (R)0;
-                if (out) { pop(); pph = kPEnter; return; }
+                if (out) { score_extra(kEscaped, p); pop(); pph = kPEnter; return; }
                 p.ir = geom_locate(G, p.x, p.y, p.z);
                 if (p_dpmfp <= (R)1.e-8) { photon_choose(); return; }
                 continue;
@@ -470,8 +489,8 @@ struct Shower {
                 if (irnew >= 0) { VoxData<R> d = vox[irnew]; p_med = d.med; p_rhof = d.rho; }
                 else p_med = -1;
             }
-            // left the grid: ausgab(3), not scored
-            if (idisc < 0) { pop(); pph = kPEnter; return; }
+            // left the grid: ausgab(3), not scored as dose
+            if (idisc < 0) { score_extra(kEscaped, p); pop(); pph = kPEnter; return; }
             if (p_med != medold) { photon_medium(); continue; }
             if (p_dpmfp <= (R)1.e-8) {   // interaction: choose its type
                 photon_choose();
@@ -1078,9 +1097,26 @@ struct Shower {
         R elkei = (R)(l - M.eke0) / M.eke1;
         return drange(m, false, eke, ekei, l, elk, elkei) + repv(m, false, l - 1);
     }
+    // :ECUT-DISCARD: (egsnrc.mortran:1773-1800).  A particle whose last
+    // step left the grid is outside: nothing is scored as dose (its energy
+    // escapes), and a positron still annihilates with the same random
+    // numbers, but its photons start outside and are discarded at once, as
+    // egsHowfar discards a particle outside the geometry
+    // (egs_advanced_application.cpp:1355-1358).
     __device__ __forceinline__ void ecut_discard(int lelec, R peie) {
         const R RM = (R)0.5109989461;
         R edep = peie - RM;
+        if (p.ir < 0) {
+            score_extra(kEscaped, p);
+            if (lelec > 0 && edep < peie) {
+                const int n0 = np;
+                annih_at_rest();
+                while (np >= n0) pop();
+                return;
+            }
+            pop();
+            return;
+        }
         score(p.ir, edep, p.wt);
         if (lelec > 0 && edep < peie) { annih_at_rest(); return; }
         pop();
@@ -1452,7 +1488,7 @@ struct Shower {
             if (irnew >= 0) { VoxData<R> d = vox[irnew]; e_med = d.med; e_rhof = d.rho; }
         }
         if (eie <= ecut) { ecut_discard(lelec, peie); eph = kNew; return; }
-        if (idisc < 0) { pop(); eph = kNew; return; }
+        if (idisc < 0) { score_extra(kEscaped, p); pop(); eph = kNew; return; }
         if (e_med != m) { eph = kTstep; return; }
         e_demfp = e_demfp - save_de * sig;
         e_ttstep = e_ttstep - tvstep * rhof;

@@ -119,6 +119,20 @@ class Simulation:
         check(_lib.lib().egsgpu_get_counters(self.h, ptr(c)))
         return c
 
+    def energy_balance(self):
+        """(incident, escaped, deposited) energy per history (MeV) and the
+        standard errors of incident and escaped (egsgpu_get_energy_balance;
+        deposited = the sum of the dose regions)."""
+        b = np.zeros(4)
+        check(_lib.lib().egsgpu_get_energy_balance(self.h, ptr(b)))
+        s, s2, n = self.tallies()
+        n = max(n, 1)
+
+        def err(t, t2):
+            return np.sqrt(max(t2 / n - (t / n) ** 2, 0.0) / max(n - 1, 1))
+        return dict(incident=b[0] / n, escaped=b[2] / n, deposited=float(s.sum()) / n,
+                    incident_err=err(b[0], b[1]), escaped_err=err(b[2], b[3]))
+
     def kernel_ms(self):
         ms = C.c_double(0)
         n = C.c_int32(0)

@@ -1,6 +1,8 @@
 // egs_dat.cpp -- .egsdat state files (see egs_dat.h).
 #include "egs_dat.h"
 
+#include "../csrc/ranmar_seed.h"
+
 #include <fstream>
 #include <iomanip>
 #include <stdexcept>
@@ -31,23 +33,12 @@ T get(std::istream &in, const char *what) {
 void ranmar_initial_state(int ixx, int jxx, EgsDat &d) {
     if (ixx <= 0 || ixx >= 31328) ixx = 1802;
     if (jxx <= 0 || jxx >= 30081) jxx = 9373;
-    int i = (ixx / 177) % 177 + 2, j = ixx % 177 + 2;
-    int k = (jxx / 169) % 178 + 1, l = jxx % 169;
-    d.u.assign(97, 0.0);
-    for (int ii = 0; ii < 97; ii++) {
-        int s = 0, t = 8388608;
-        for (int jj = 0; jj < 24; jj++) {
-            int m = (((i * j) % 179) * k) % 179;
-            i = j; j = k; k = m;
-            l = (53 * l + 1) % 169;
-            if ((l * m) % 64 >= 32) s += t;
-            t /= 2;
-        }
-        d.u[ii] = s;
-    }
-    d.c = 362436;
-    d.ix = 96;
-    d.jx = 32;
+    int u[97];
+    ranmar_seed_words(ixx, jxx, u);
+    d.u.assign(u, u + 97);
+    d.c = kRanmarC0;
+    d.ix = kRanmarIx0;
+    d.jx = kRanmarJx0;
     d.iseed1 = ixx;
     d.iseed2 = jxx;
     d.high_res = 0;

@@ -16,21 +16,31 @@ from .engine import Simulation, dose_stats
 
 
 def run_pair(tables, xb, yb, zb, medium, rhor, source, transport, nhist,
-             nstreams=16384, seeds=(1802, 9373)):
-    """Run the workload in FP32 and FP64; returns ((r32, s32), (r64, s64),
-    n) with per-history mean deposits r and their standard errors s."""
-    out = []
+             nstreams=16384, seeds=(1802, 9373), nbatch=10):
+    """Run the workload in FP32 and FP64, each in nbatch launches of the
+    same sizes (so that batch k of both runs holds the same histories);
+    returns ((r32, s32), (r64, s64), n, t32, t64) with per-history mean
+    deposits r, their standard errors s, and the per-batch total deposits
+    t (MeV per history)."""
+    out, tot = [], []
+    sizes = [nhist // nbatch + (1 if k < nhist % nbatch else 0) for k in range(nbatch)]
     for fp64 in (0, 1):
         tp = _lib.EgsTransport.from_buffer_copy(transport)
         tp.fp64 = fp64
         sim = Simulation(tables, xb, yb, zb, medium, rhor, source, tp, nstreams=nstreams,
                          schedule="history")
         sim.seed(*seeds)
-        sim.run(nhist)
+        t, prev = [], 0.0
+        for m in sizes:
+            sim.run(m)
+            cur = float(sim.tallies()[0].sum())
+            t.append((cur - prev) / m)
+            prev = cur
         s, s2, n = sim.tallies()
         out.append(dose_stats(s, s2, n))
+        tot.append(np.array(t))
         sim.close()
-    return out[0], out[1], nhist
+    return out[0], out[1], nhist, tot[0], tot[1]
 
 
 def report(r32, s32, r64, s64, frac=0.1):
@@ -54,9 +64,15 @@ def report(r32, s32, r64, s64, frac=0.1):
 
 
 def precision_report(tables, xb, yb, zb, medium, rhor, source, transport, nhist,
-                     nstreams=16384, frac=0.1):
-    (r32, s32), (r64, s64), n = run_pair(tables, xb, yb, zb, medium, rhor, source,
-                                          transport, nhist, nstreams)
+                     nstreams=16384, frac=0.1, nbatch=10):
+    """The per-voxel report, plus the relative difference of the total
+    deposit and its standard error from the paired per-batch differences
+    (which carries both the correlation of a history's deposits over
+    voxels and the correlation of the two runs)."""
+    (r32, s32), (r64, s64), n, t32, t64 = run_pair(tables, xb, yb, zb, medium, rhor, source,
+                                                    transport, nhist, nstreams, nbatch=nbatch)
     rep = report(r32, s32, r64, s64, frac)
     rep["histories"] = n
+    d = (t32 - t64) / t64.mean()
+    rep["total_rel_diff_sigma"] = float(d.std(ddof=1) / np.sqrt(len(d)))
     return rep, (r32, s32, r64, s64)

@@ -356,6 +356,19 @@ int egsgpu_restore(egsgpu_handle *h, const void *buf, size_t size);
 int egsgpu_density_grad(egsgpu_handle *h, int64_t nhist, const double *target,
                         double *grad, double *dose, double *loss);
 
+/* Energy bookkeeping since the last reset (history-by-history FP64 sums,
+ * weight included): out[0], out[1] = sum and sum of per-history squares of
+ * the energy brought into the grid by the primaries; out[2], out[3] = the
+ * same for the energy carried out of the grid by escaping particles.  A
+ * particle's energy is its kinetic energy, plus 2 m c^2 for a positron (the
+ * energy of its annihilation).  Without variance reduction, incident =
+ * deposited (the sum of egsgpu_get_tallies) + escaped, per history, up to
+ * the rounding of the transport arithmetic: the energy conservation of
+ * the shower (EGSnrc passes every particle's energy to ausgab when it is
+ * deposited or discarded, egsnrc.mortran:1800-1816, 6600-6643).  Restored
+ * with a checkpoint; restarted by egsgpu_set_tallies. */
+int egsgpu_get_energy_balance(egsgpu_handle *h, double out[4]);
+
 /* Engine counters: [0] photon steps, [1] voxel crossings, [2] deposits,
  * [3] history flushes (voxel,history pairs), [4] electron steps,
  * [5] stack overflows, [6] tally-table overflows, [7] primaries that

@@ -12,20 +12,15 @@
 #pragma once
 #include <cstdint>
 
+#include "../egsnrc_amd/csrc/ranmar_seed.h"
+
 namespace oracle {
 
-// Seed pair of stream s.  For jxx+s <= 30080 this is exactly what
-// EGS_RandomGenerator::createRNG(input, s) builds (egs_rndm.cpp:438-446):
-// (ixx, jxx+s).  Beyond that the excess is carried into ixx so that every
-// stream keeps a distinct valid seed pair (setState would otherwise reset
-// them all to 9373, egs_rndm.cpp:307-309).
+// Seed pair of stream s: (ixx, jxx+s) as createRNG(input, s) builds it
+// (egs_rndm.cpp:438-446), the excess above 30080 carried into ixx
+// (ranmar_seed.h, shared with the engine).
 inline void stream_seeds(int ixx, int jxx, int64_t s, int &is, int &js) {
-    if (ixx <= 0 || ixx >= 31328) ixx = 1802;
-    if (jxx <= 0 || jxx >= 30081) jxx = 9373;
-    int64_t t = (int64_t)jxx + s;              // >= 1
-    int64_t carry = (t - 1) / 30080;
-    js = (int)(1 + (t - 1) % 30080);
-    is = (int)(1 + ((int64_t)ixx - 1 + carry) % 31327);
+    ranmar_stream_seeds(ixx, jxx, s, is, js);
 }
 
 struct Ranmar {
@@ -33,24 +28,8 @@ struct Ranmar {
     int c, ix, jx;
     // egs_rndm.cpp:303-344
     void set_state(int ixx, int jxx) {
-        if (ixx <= 0 || ixx >= 31328) ixx = 1802;
-        if (jxx <= 0 || jxx >= 30081) jxx = 9373;
-        int i = (ixx / 177) % 177 + 2;
-        int j = ixx % 177 + 2;
-        int k = (jxx / 169) % 178 + 1;
-        int l = jxx % 169;
-        for (int ii = 0; ii < 97; ii++) {
-            int s = 0, t = 8388608;
-            for (int jj = 0; jj < 24; jj++) {
-                int m = (((i * j) % 179) * k) % 179;
-                i = j; j = k; k = m;
-                l = (53 * l + 1) % 169;
-                if ((l * m) % 64 >= 32) s += t;
-                t /= 2;
-            }
-            u[ii] = s;
-        }
-        c = 362436; ix = 96; jx = 32;
+        ranmar_seed_words(ixx, jxx, u);
+        c = kRanmarC0; ix = kRanmarIx0; jx = kRanmarJx0;
     }
     // one step of egs_rndm.cpp:338-370; returns the 24-bit integer
     int next_int() {

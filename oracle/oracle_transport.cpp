@@ -1405,9 +1405,19 @@ struct Shower {
     }
 
     // :ECUT-DISCARD: / :POSITRON-ANNIHILATION: (egsnrc.mortran:1773-1800)
+    // A particle whose last step left the grid is outside: no dose; a
+    // positron still annihilates (same random numbers) but its photons
+    // start outside and egsHowfar discards them at once
+    // (egs_advanced_application.cpp:1355-1358).
     void ecut_discard(int lelec, double peie) {
         Particle &p = top();
         double edep = peie - PRM;   // e-: e - prm; e+: $POSITRON-ECUT-DISCARD
+        if (p.ir < 0) {
+            const size_t n0 = stk.size();
+            if (lelec > 0 && edep < peie) annih_at_rest();
+            while (stk.size() >= n0) stk.pop_back();
+            return;
+        }
         score(p.ir, edep, p.wt);
         if (lelec > 0 && edep < peie) { annih_at_rest(); return; }
         stk.pop_back();

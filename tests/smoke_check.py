@@ -8,7 +8,8 @@ def run():
     import numpy as np
     from egsnrc_amd import _lib
     from egsnrc_amd.engine import Simulation, Tables, make_source
-    from tests import oracle_lib, workloads
+    from egsnrc_amd import workloads
+    from tests import oracle_lib
     xb = np.linspace(-2, 2, 9)
     g, keep = _lib.make_geometry(xb, xb, np.linspace(0, 4, 9))
     x = np.array([[0.1, 0.2, -1.0], [-3.0, 0.3, 1.1]])
@@ -25,7 +26,8 @@ def run():
     xb, yb, zb, med, rhor = workloads.kerma_phantom(8, 2.0)
     src = make_source(0, 2.0, x0=-4, x1=4, y0=-4, y1=4, z=-1.0)
     tp = _lib.default_transport(ecut=20.0, kerma=1)
-    blob = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "521icru.egstab")
+    blob = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "egsnrc_amd",
+                        "data", "521icru.egstab")
     sim = Simulation(Tables(blob), xb, yb, zb, med, rhor, src, tp, nstreams=4096)
     sim.run(100000)
     s, s2, n = sim.tallies()

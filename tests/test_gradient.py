@@ -10,7 +10,8 @@ import pytest
 
 from egsnrc_amd import _lib
 from egsnrc_amd.engine import make_source
-from tests import oracle_lib, workloads
+from egsnrc_amd import workloads
+from tests import oracle_lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BLOB = os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab")

@@ -11,7 +11,9 @@ import pytest
 
 from egsnrc_amd import _lib
 from egsnrc_amd.engine import Simulation, Tables, make_source
-from tests import oracle_lib, workloads
+from egsnrc_amd import workloads
+from tests import oracle_lib
+from tests.parity import P2, check_parity
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BLOB700 = os.path.join(ROOT, "egsnrc_amd", "data", "700icru.egstab")
@@ -37,36 +39,47 @@ def run_pair(blob, n_vox, dx, src, tp, n, rho=1.0):
     return a, b, c
 
 
-def check(a, b, label, min_voxels):
-    chi2, out2, nv = workloads.compare_doses(a, b)
-    print("%s: chi2/dof %.3f  >2sigma %.3f  voxels %d" % (label, chi2, out2, nv))
-    assert nv > min_voxels
-    assert chi2 < max(1.3, 1 + 3 * np.sqrt(2.0 / nv))
-    assert out2 < max(0.09, 0.0455 + 3 * np.sqrt(0.0455 * 0.9545 / nv))
-    ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
-    assert abs(ta - tb) < 0.02 * ta
+def check(a, b, label, min_voxels=1800):
+    check_parity(a, b, label=label, min_voxels=min_voxels, total_rtol=0.02)
+
+
+# HOWFARLESS is an approximation for charged particles: a step that crosses
+# voxels shares its energy along a two-piece hinge path instead of the
+# voxel-limited steps of the exact transport (the deposit per step is the
+# same; where it lands differs at the scale of a step).  In 0.5 cm voxels at
+# density 0.8 that leaves a systematic difference of a few tenths of a
+# percent in the dose gradients, which adds about 10 % to chi^2/dof at 1e5
+# histories; the electron comparison allows chi^2/dof up to 1.25 for it.
+HFL_ELECTRON_CHI2 = 1.25
 
 
 @pytest.mark.gpu
 def test_howfarless_photon_beam_in_water():
-    """C0-like: 1.25 MeV photons, 4 x 4 cm field, 20^3 x 1 cm water."""
-    src = make_source(0, 1.25, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    """C0-like: 1.25 MeV photons, 12 x 12 cm field, 20^3 x 1 cm water."""
+    src = make_source(0, 1.25, x0=-6, x1=6, y0=-6, y1=6, z=-1.0)
     tp = _lib.default_transport(ecut=0.7, pcut=0.01, rayleigh=0)
     a, b, c = run_pair(BLOB700, 20, 1.0, src, tp, 400000)
     # photons cross no internal boundary (only the rare electron leaving
     # the grid counts as a crossing)
     assert c[1] < 0.01 * c[0]
-    check(a, b, "howfarless photons", 300)
+    check(a, b, "howfarless photons")
 
 
 @pytest.mark.gpu
 def test_howfarless_electron_beam_in_lung_density_water():
-    """8 MeV electrons, 2 x 2 cm field, 16^3 x 0.5 cm water at relative
+    """8 MeV electrons, 8 x 8 cm field, 24^3 x 0.5 cm water at relative
     density 0.8: the path-length sharing of the steps that cross voxels."""
-    src = make_source(-1, 8.0, x0=-1, x1=1, y0=-1, y1=1, z=-1.0)
+    src = make_source(-1, 8.0, x0=-4, x1=4, y0=-4, y1=4, z=-1.0)
     tp = _lib.default_transport(ecut=0.521, pcut=0.01)
-    a, b, c = run_pair(BLOB521, 16, 0.5, src, tp, 100000, rho=0.8)
-    check(a, b, "howfarless electrons", 200)
+    a, b, c = run_pair(BLOB521, 24, 0.5, src, tp, 100000, rho=0.8)
+    chi2, out2, nv = workloads.compare_doses(a, b)
+    band = 3 * np.sqrt(P2 * (1 - P2) / nv)
+    print("howfarless electrons: chi2/dof %.4f  >2sigma %.4f  voxels %d" % (chi2, out2, nv))
+    assert nv >= 1800
+    assert chi2 <= HFL_ELECTRON_CHI2
+    assert out2 <= P2 + 2 * band
+    ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
+    assert abs(ta - tb) < 0.005 * ta
 
 
 @pytest.mark.gpu

@@ -12,7 +12,9 @@ import pytest
 from egsnrc_amd import _lib
 from egsnrc_amd.engine import Simulation, Tables, make_phsp_source, make_source
 from egsnrc_amd.phantom_io import RM, read_egsphsp, write_egsphsp
-from tests import oracle_lib, workloads
+from egsnrc_amd import workloads
+from tests import oracle_lib
+from tests.parity import check_parity, check_parity_means
 from tests.test_host_app import APP, BLOB700, read_3ddose
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -113,29 +115,25 @@ def oracle_run(xb, yb, zb, med, src, tp, n, blob):
 
 @pytest.mark.gpu
 def test_phsp_weights_and_rejections_match_the_oracle():
-    """1.25 MeV photons of a 4 x 4 cm parallel beam given as phase-space
+    """1.25 MeV photons of a 6 x 6 cm parallel beam given as phase-space
     particles, alternately of weight 2 and of weight 0 (a rejected
     particle): the same expected dose as the oracle's parallel beam."""
-    n = 400000
+    n = 800000
     rng = np.random.default_rng(5)
-    x, y, u, v = beam_particles(n, rng)
+    x, y, u, v = beam_particles(n, rng, half=3.0)
     wt = np.where(np.arange(n) % 2 == 0, 2.0, 0.0)
-    xb = np.linspace(-4, 4, 9)
-    zb = np.linspace(0, 10, 11)
-    med = np.zeros(8 * 8 * 10, np.int32)
+    xb = np.linspace(-4, 4, 17)
+    zb = np.linspace(0, 10, 21)
+    med = np.zeros(16 * 16 * 20, np.int32)
     tp = _lib.default_transport(ecut=0.7, pcut=0.01)
     sim = Simulation(Tables(BLOB700), xb, xb.copy(), zb, med, None, make_phsp_source(z=-1.0), tp,
                      nstreams=8192, schedule="history")
     sim.set_phsp(x, y, u, v, np.full(n, 1.25), wt, np.zeros(n))
     sim.run(n)
     b = sim.tallies()
-    a = oracle_run(xb, xb.copy(), zb, med, make_source(0, 1.25, x0=-2, x1=2, y0=-2, y1=2,
+    a = oracle_run(xb, xb.copy(), zb, med, make_source(0, 1.25, x0=-3, x1=3, y0=-3, y1=3,
                                                          z=-1.0), tp, n, BLOB700)
-    chi2, out2, nv = workloads.compare_doses(a, b)
-    print("phsp weights: chi2/dof %.3f  >2sigma %.3f  voxels %d" % (chi2, out2, nv))
-    assert nv > 60
-    assert chi2 < max(1.3, 1 + 3 * np.sqrt(2.0 / nv))
-    assert out2 < max(0.09, 0.0455 + 3 * np.sqrt(0.0455 * 0.9545 / nv))
+    check_parity(a, b, sim.counters(), label="phsp weights")
 
 
 @pytest.mark.gpu
@@ -146,7 +144,7 @@ def test_mixed_phsp_on_the_event_schedule_matches_the_oracle():
     electron-beam dose."""
     n = 200000
     rng = np.random.default_rng(6)
-    x, y, u, v = beam_particles(n, rng)
+    x, y, u, v = beam_particles(n, rng, half=3.9)
     q = np.where(np.arange(n) % 2 == 0, 0, -1)
     e = np.where(q == 0, 2.0, 6.0)
     xb = np.linspace(-4, 4, 17)
@@ -165,19 +163,13 @@ def test_mixed_phsp_on_the_event_schedule_matches_the_oracle():
     half = n // 2
     tabs = []
     for charge, energy in ((0, 2.0), (-1, 6.0)):
-        src = make_source(charge, energy, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+        src = make_source(charge, energy, x0=-3.9, x1=3.9, y0=-3.9, y1=3.9, z=-1.0)
         tabs.append(oracle_run(xb, xb.copy(), zb, med, src, tp, half, BLOB521))
     mb = b[0] / b[2]
     vb = np.maximum(b[1] / b[2] - mb * mb, 0) / (b[2] - 1)
     ma = 0.5 * sum(t[0] / t[2] for t in tabs)
     va = 0.25 * sum(np.maximum(t[1] / t[2] - (t[0] / t[2]) ** 2, 0) / (t[2] - 1) for t in tabs)
-    keep = (np.maximum(ma, mb) > 0.1 * max(ma.max(), mb.max())) & (va + vb > 0)
-    z = (ma[keep] - mb[keep]) / np.sqrt(va[keep] + vb[keep])
-    chi2, out2, nv = float(np.mean(z * z)), float(np.mean(np.abs(z) > 2)), int(keep.sum())
-    print("mixed phsp: chi2/dof %.3f  >2sigma %.3f  voxels %d" % (chi2, out2, nv))
-    assert nv > 300
-    assert chi2 < max(1.3, 1 + 3 * np.sqrt(2.0 / nv))
-    assert out2 < max(0.09, 0.0455 + 3 * np.sqrt(0.0455 * 0.9545 / nv))
+    check_parity_means(ma, np.sqrt(va), mb, np.sqrt(vb), "mixed phsp", frac=0.05)
 
 
 def write_iaea(base, e, x, y, z, u, v, wt, q, new_history=None, byteorder="<",

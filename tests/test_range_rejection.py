@@ -16,7 +16,9 @@ import pytest
 
 from egsnrc_amd import _lib
 from egsnrc_amd.engine import Simulation, Tables, make_source
-from tests import oracle_lib, workloads
+from egsnrc_amd import workloads
+from tests import oracle_lib
+from tests.parity import check_parity
 from tests.test_dosxyz_frontend import parse, write_isource1_input
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,7 +58,7 @@ def run(tp, n, ireject):
     med = np.zeros(16 * 16 * 20, np.int32)
     kz = np.arange(med.size) // 256
     med[(kz >= 6) & (kz < 10)] = 2   # bone at 3-5 cm
-    src = make_source(0, 1.25, x0=-2, x1=2, y0=-2, y1=2, z=-1.0)
+    src = make_source(0, 1.25, x0=-3.5, x1=3.5, y0=-3.5, y1=3.5, z=-1.0)
     tp.ireject = ireject
     tp.esave = 2.0
     sim = Simulation(Tables(BLOB521), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
@@ -73,7 +75,7 @@ def test_range_rejection_keeps_the_dose_and_cuts_electron_steps():
     0.521, ESAVE_GLOBAL 2 MeV: dose against the oracle, electron steps
     against the same engine without rejection."""
     tp = _lib.default_transport(ecut=0.521, pcut=0.01)
-    n = 400000
+    n = 800000
     b, cb, (g, keep, src) = run(tp, n, 1)
     _, c0, _ = run(_lib.default_transport(ecut=0.521, pcut=0.01), n, 0)
     o = oracle_lib.OracleSimulation(BLOB521, g, src[0], _lib.default_transport(ecut=0.521,
@@ -81,14 +83,8 @@ def test_range_rejection_keeps_the_dose_and_cuts_electron_steps():
                                     nstreams=256, nthreads=8)
     o.run(n)
     a = o.tallies()
-    chi2, out2, nv = workloads.compare_doses(a, b)
-    print("range rejection: chi2/dof %.3f  >2sigma %.3f  voxels %d; electron steps %d -> %d"
-          % (chi2, out2, nv, c0[4], cb[4]))
-    assert nv > 300 and cb[5] == 0
-    assert chi2 < max(1.3, 1 + 3 * np.sqrt(2.0 / nv))
-    assert out2 < max(0.09, 0.0455 + 3 * np.sqrt(0.0455 * 0.9545 / nv))
-    ta, tb = a[0].sum() / a[2], b[0].sum() / b[2]
-    assert abs(ta - tb) < 0.01 * ta
+    print("range rejection: electron steps %d -> %d" % (c0[4], cb[4]))
+    check_parity(a, b, cb, label="range rejection", total_rtol=0.01)
     assert cb[4] < 0.7 * c0[4]
 
 
