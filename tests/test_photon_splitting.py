@@ -38,9 +38,14 @@ BLOB700 = os.path.join(ROOT, "egsnrc_amd", "data", "700icru.egstab")
 BLOB521 = os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab")
 
 
-def oracle(blob, xb, yb, zb, med, rho, src, tp, n):
+NULL_SEEDS = (4711, 20000)   # a second oracle run, independent of the first
+
+
+def oracle(blob, xb, yb, zb, med, rho, src, tp, n, seeds=None):
     g, keep = _lib.make_geometry(xb, yb, zb, med, rho)
     o = oracle_lib.OracleSimulation(blob, g, src[0], tp, nstreams=256, nthreads=8)
+    if seeds:
+        o.seed(*seeds)
     o.run(n)
     return o.tallies()
 
@@ -51,8 +56,8 @@ def stats(t):
     return m, np.maximum(s2 / n - m * m, 0) / (n - 1)
 
 
-def check(a, b, label, frac=0.1, counters=None):
-    check_parity(a, b, counters, label=label, frac=frac)
+def check(a, b, label, frac=0.1, counters=None, null=None):
+    check_parity(a, b, counters, label=label, frac=frac, null=null)
 
 
 def per_history_variance_ratio(a, b, frac=0.1):
@@ -75,6 +80,7 @@ def test_split_photon_beam_matches_the_unsplit_oracle():
     src = make_source(0, 1.25, x0=-6, x1=6, y0=-6, y1=6, z=-1.0)
     tp = _lib.default_transport(ecut=0.7, pcut=0.01)
     a = oracle(BLOB700, xb, xb.copy(), zb, med, None, src, tp, 400000)
+    a0 = oracle(BLOB700, xb, xb.copy(), zb, med, None, src, tp, 400000, NULL_SEEDS)
     tp.n_split = 20
     sim = Simulation(Tables(BLOB700), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
     n = 100000
@@ -82,7 +88,7 @@ def test_split_photon_beam_matches_the_unsplit_oracle():
     b = sim.tallies()
     c = sim.counters()
     assert b[2] == n and c[5] == 0
-    check(a, b, "split photons", counters=c)
+    check(a, b, "split photons", counters=c, null=a0)
     ratio = per_history_variance_ratio(a, b)
     print("variance per history, split / unsplit: %.3f" % ratio)
     assert ratio < 0.6
@@ -110,6 +116,7 @@ def test_split_with_howfarless_and_ranlux():
     src = make_source(0, 2.0, x0=-6, x1=6, y0=-6, y1=6, z=-1.0)
     tp = _lib.default_transport(ecut=0.7, pcut=0.01, rayleigh=0)
     a = oracle(BLOB700, xb, xb.copy(), zb, med, None, src, tp, 400000)
+    a0 = oracle(BLOB700, xb, xb.copy(), zb, med, None, src, tp, 400000, NULL_SEEDS)
     tp.n_split = 10
     tp.howfarless = 1
     sim = Simulation(Tables(BLOB700), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
@@ -119,7 +126,7 @@ def test_split_with_howfarless_and_ranlux():
     b = sim.tallies()
     c = sim.counters()
     assert b[2] == n and c[5] == 0
-    check(a, b, "split + howfarless + ranlux", counters=c)
+    check(a, b, "split + howfarless + ranlux", counters=c, null=a0)
     assert per_history_variance_ratio(a, b) < 0.8
 
 
@@ -255,6 +262,7 @@ def test_russian_roulette_of_photon_made_electrons():
     src = make_source(0, 2.0, x0=-6, x1=6, y0=-6, y1=6, z=-1.0)
     tp = _lib.default_transport(ecut=0.521, pcut=0.01)
     a = oracle(BLOB521, xb, xb.copy(), zb, med, None, src, tp, 400000)
+    a0 = oracle(BLOB521, xb, xb.copy(), zb, med, None, src, tp, 400000, NULL_SEEDS)
     tp.nbr_split = 4
     tp.play_rr = 1
     sim = Simulation(Tables(BLOB521), xb, xb.copy(), zb, med, None, src, tp, nstreams=16384)
@@ -262,7 +270,7 @@ def test_russian_roulette_of_photon_made_electrons():
     sim.run(n)
     b = sim.tallies()
     assert b[2] == n and sim.counters()[5] == 0
-    check(a, b, "electron roulette", counters=sim.counters())
+    check(a, b, "electron roulette", counters=sim.counters(), null=a0)
     ratio = per_history_variance_ratio(a, b)
     print("variance per history, roulette / none: %.3f" % ratio)
     assert ratio > 1.5   # fewer, heavier electrons

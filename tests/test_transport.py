@@ -18,6 +18,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BLOB521 = os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab")
 
 
+def oracle_threads():
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
 def kerma_case(n=32, dx=0.8):
     xb, yb, zb, med, rhor = workloads.kerma_phantom(n, dx)
     src = make_source(0, 2.0, x0=-4, x1=4, y0=-4, y1=4, z=-1.0)
@@ -60,7 +64,7 @@ def test_oracle_kerma_uniform_slab_attenuation():
     assert np.all(d > 0)
 
 
-KERMA_HIST = 1000000
+KERMA_HIST = 4000000
 
 
 @pytest.mark.gpu
@@ -142,7 +146,7 @@ def test_oracle_electron_depth_dose_20mev():
     assert 7.9 < r50 < 8.8, r50
 
 
-ELECTRON_HIST = 100000
+ELECTRON_HIST = 300000
 
 
 @pytest.mark.gpu
@@ -150,16 +154,15 @@ ELECTRON_HIST = 100000
 def test_electron_dose_parity(engine, schedule):
     xb, yb, zb, med, rhor, src, tp = electron_case()
     g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
-    o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
-    o.run(ELECTRON_HIST)
-    a = o.tallies()
+    a = _oracle_cached("electron", electron_case, ELECTRON_HIST)
     sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192,
                      schedule=schedule)
     sim.run(ELECTRON_HIST // 2)
     sim.run(ELECTRON_HIST - ELECTRON_HIST // 2)
     b = sim.tallies()
     assert b[2] == ELECTRON_HIST
-    check_parity(a, b, sim.counters(), label="electrons/" + schedule)
+    check_parity(a, b, sim.counters(), label="electrons/" + schedule,
+                 null=_oracle_cached("electron", electron_case, ELECTRON_HIST, NULL_SEEDS))
 
 
 def photon_coupled_case(n=32, dx=0.5):
@@ -177,19 +180,31 @@ def photon_coupled_case(n=32, dx=0.5):
     return xb, xb.copy(), zb, med, None, src, tp
 
 
-PHOTON_HIST = 800000
+PHOTON_HIST = 2000000
 _ORACLE_RUNS = {}
 
 
-def _oracle_photon_coupled(n):
-    """The oracle's tallies for photon_coupled_case (one run per n)."""
-    if n not in _ORACLE_RUNS:
-        xb, yb, zb, med, rhor, src, tp = photon_coupled_case()
+NULL_SEEDS = (4711, 20000)   # a second oracle run, independent of the first
+
+
+def _oracle_cached(key, case, n, seeds=None):
+    """The oracle's tallies for case() at n histories (one run per key, n
+    and seeds, shared by the tests that compare against it)."""
+    if (key, n, seeds) not in _ORACLE_RUNS:
+        xb, yb, zb, med, rhor, src, tp = case()
+        tp.ecut, tp.pcut = max(tp.ecut, 0.521), max(tp.pcut, 0.01)
         g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
-        o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
+        o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256,
+                                        nthreads=oracle_threads())
+        if seeds:
+            o.seed(*seeds)
         o.run(n)
-        _ORACLE_RUNS[n] = o.tallies()
-    return _ORACLE_RUNS[n]
+        _ORACLE_RUNS[(key, n, seeds)] = o.tallies()
+    return _ORACLE_RUNS[(key, n, seeds)]
+
+
+def _oracle_photon_coupled(n):
+    return _oracle_cached("photon slabs", photon_coupled_case, n)
 
 
 @pytest.mark.gpu
@@ -206,7 +221,8 @@ def test_photon_coupled_dose_parity(engine, schedule):
     sim.run(PHOTON_HIST)
     b = sim.tallies()
     assert b[2] == PHOTON_HIST
-    check_parity(a, b, sim.counters(), label="photons coupled/" + schedule)
+    check_parity(a, b, sim.counters(), label="photons coupled/" + schedule,
+                 null=_oracle_cached("photon slabs", photon_coupled_case, PHOTON_HIST, NULL_SEEDS))
 
 
 @pytest.mark.gpu
@@ -222,7 +238,8 @@ def test_ranlux_driven_transport_parity(engine, schedule, lux):
     sim.run(PHOTON_HIST)
     b = sim.tallies()
     assert b[2] == PHOTON_HIST
-    check_parity(a, b, sim.counters(), label="ranlux %d %s" % (lux, schedule))
+    check_parity(a, b, sim.counters(), label="ranlux %d %s" % (lux, schedule),
+                 null=_oracle_cached("photon slabs", photon_coupled_case, PHOTON_HIST, NULL_SEEDS))
 
 
 @pytest.mark.gpu
@@ -294,7 +311,7 @@ def test_batched_jobs_match_oracle(engine):
     each job's dose agrees with the oracle run of that job alone."""
     from egsnrc_amd.engine import run_batched
     cases = [electron_case(), photon_coupled_case(), water_beam_case()]
-    nh = [ELECTRON_HIST, PHOTON_HIST, 200000]
+    nh = [ELECTRON_HIST, PHOTON_HIST, 1000000]
     tab = Tables(BLOB521)
     sims = []
     for c in cases:
@@ -303,14 +320,12 @@ def test_batched_jobs_match_oracle(engine):
         sims.append(Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=8192,
                                schedule="event"))
     run_batched(sims, nh)
-    for c, sim, n, label in zip(cases, sims, nh, ["electron", "photon slabs", "water beam"]):
-        xb, yb, zb, med, rhor, src, tp = c
-        g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
-        o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
-        o.run(n)
+    for case, sim, n, label in zip((electron_case, photon_coupled_case, water_beam_case), sims,
+                                   nh, ["electron", "photon slabs", "water beam"]):
         b = sim.tallies()
         assert b[2] == n
-        check_parity(o.tallies(), b, sims[0].counters(), label="batch/" + label)
+        check_parity(_oracle_cached(label, case, n), b, sims[0].counters(),
+                     label="batch/" + label, null=_oracle_cached(label, case, n, NULL_SEEDS))
     # jobs must share transport parameters
     xb, yb, zb, med, rhor, src, tp = water_beam_case()
     tp.ecut = 0.7
@@ -353,13 +368,19 @@ def test_c0_pencil_beam_parity(engine):
     tp = _lib.default_transport(ecut=0.7, pcut=0.01, rayleigh=0)
     g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
     n = 6000000
-    o = oracle_lib.OracleSimulation(BLOB700, g, src[0], tp, nstreams=256,
-                                    nthreads=oracle_threads())
-    o.run(n)
+    ref = []
+    for seeds in (None, NULL_SEEDS):
+        o = oracle_lib.OracleSimulation(BLOB700, g, src[0], tp, nstreams=256,
+                                        nthreads=oracle_threads())
+        if seeds:
+            o.seed(*seeds)
+        o.run(n)
+        ref.append(o.tallies())
+        del o
     sim = Simulation(Tables(BLOB700), xb, yb, zb, med, rhor, src, tp)
     sim.run(n)
-    check_parity(o.tallies(), sim.tallies(), sim.counters(), label="c0", frac=0.002,
-                 total_rtol=0.003)
+    check_parity(ref[0], sim.tallies(), sim.counters(), label="c0", frac=0.002,
+                 total_rtol=0.003, null=ref[1])
 
 
 @pytest.mark.gpu
@@ -375,10 +396,6 @@ def test_c2_electron_beam_reduced_parity(engine):
     sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=16384)
     sim.run(100000)
     check_parity(o.tallies(), sim.tallies(), sim.counters(), label="c2 reduced")
-
-
-def oracle_threads():
-    return max(1, min(os.cpu_count() or 1, 16))
 
 
 @pytest.mark.gpu
@@ -397,7 +414,7 @@ def test_c1_full_size_parity(engine):
     n = 4000000
     g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
     ref = []
-    for seeds in ((1802, 9373), (4711, 20000)):
+    for seeds in ((1802, 9373), NULL_SEEDS):
         o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256,
                                         nthreads=oracle_threads())
         o.seed(*seeds)
@@ -430,7 +447,7 @@ def test_c3_batch256(engine):
     order = sorted(range(len(jobs)), key=lambda j: jobs[j]["n"])
     photons = [j for j in order if jobs[j]["src"]["charge"] == 0][:2]
     electrons = [j for j in order if jobs[j]["src"]["charge"] != 0][:1]
-    picked = {photons[0]: 300000, photons[1]: 300000, electrons[0]: 60000}
+    picked = {photons[0]: 1000000, photons[1]: 1000000, electrons[0]: 200000}
     sims, keep = [], {}
     for k, j in enumerate(jobs):
         s = j["src"]
@@ -462,9 +479,15 @@ def test_c3_batch256(engine):
     for k, n in picked.items():
         xb, yb, zb, med, rhor, src = keep[k]
         g, gk = _lib.make_geometry(xb, yb, zb, med, rhor)
-        o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256,
-                                        nthreads=oracle_threads())
-        o.run(n)
-        check_parity(o.tallies(), sims[k].tallies(), label="c3 job %d (n=%d, %s)"
+        ref = []
+        for seeds in (None, NULL_SEEDS):
+            o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256,
+                                            nthreads=oracle_threads())
+            if seeds:
+                o.seed(*seeds)
+            o.run(n)
+            ref.append(o.tallies())
+            del o
+        check_parity(ref[0], sims[k].tallies(), label="c3 job %d (n=%d, %s)"
                      % (k, jobs[k]["n"], "photons" if jobs[k]["src"]["charge"] == 0
-                        else "electrons %.1f MeV" % jobs[k]["src"]["energy"]))
+                        else "electrons %.1f MeV" % jobs[k]["src"]["energy"]), null=ref[1])
