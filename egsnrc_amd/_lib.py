@@ -106,6 +106,7 @@ def lib():
         "egsgpu_set_tallies": [vp, vp, vp, C.c_int64],
         "egsgpu_get_counters": [vp, vp],
         "egsgpu_get_energy_balance": [vp, vp],
+        "egsgpu_run_static": [vp, i64],
         "egsgpu_last_kernel_ms": [vp, vp, vp],
         "egsgpu_density_grad": [vp, i64, vp, vp, vp, vp],
         "egsgpu_tables_medium_info": [vp, i32, vp, C.c_size_t, vp],

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -826,6 +827,23 @@ __global__ void k_ev_init(DevSlots S, EvCtl *c) {
     }
 }
 
+// ---- the library's static scratch (egsgpu_run_static) -------------------
+// Device memory of a fixed size that the library never allocates: each GPU
+// the library runs on holds one copy until the process ends.  Host code
+// never uses the array's name as an address (on the host it is only a
+// stand-in); it takes the device address once per device with
+// hipGetSymbolAddress and passes that.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Zero n 16-byte words (the history-table keys must start at zero); a
+// kernel rather than a memset so that callers may capture it in a graph.
+__global__ void k_zero16(uint4 *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // ------------------------------------------------------------ host side
 struct DevBuf {
     void *p = nullptr;
@@ -1260,14 +1278,14 @@ int take_time(egsgpu_handle *h, int k) {
 
 template <typename R>
 int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = nullptr,
-           long long hist0 = -1) {
+           long long hist0 = -1, bool timed = true) {
     KArgs<R> a = kargs(h, P);
     if (hist0 >= 0) a.hist0 = hist0;   // a replay of histories already counted
     a.base = n / h->nlanes;
     a.rem = (int)(n % h->nlanes);
     a.grad = grad;
     size_t lds = (size_t)kRing * h->block * sizeof(float);
-    EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
+    if (timed) EGS_HIP(hipEventRecord(h->kev[0][0], h->stream));
     dim3 grid((h->nlanes + h->block - 1) / h->block);
     const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
     if (h->P.n_split > 1 || h->P.nbr_split > 1 || h->P.ireject) {
@@ -1304,7 +1322,7 @@ int launch(egsgpu_handle *h, Precision<R> &P, int64_t n, const DevGrad *grad = n
         hipLaunchKernelGGL((k_transport<R, false>), grid, dim3(h->block), lds, h->stream, a);
     }
     EGS_HIP(hipGetLastError());
-    EGS_HIP(hipEventRecord(h->kev[0][1], h->stream));
+    if (timed) EGS_HIP(hipEventRecord(h->kev[0][1], h->stream));
     return EGSGPU_OK;
 }
 
@@ -2097,6 +2115,99 @@ int egsgpu_run(egsgpu_handle *h, int64_t nhist) {
     EGS_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
     h->last_launches = h->klaunch[0] + h->klaunch[1] + h->klaunch[2] + h->klaunch[3];
+    h->nhist += nhist;
+    return EGSGPU_OK;
+}
+
+}  // extern "C"
+
+// ---- egsgpu_run without allocation ---------------------------------------
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (taken once per
+// device, with that device current).
+int scratch_address(void **out) {
+    int dev = 0;
+    EGS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d: the static scratch is kept for devices 0..%d", dev, kMaxDevices - 1);
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        EGS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("hipGetSymbolAddress returned no address for the static scratch");
+            return EGSGPU_E_HIP;
+        }
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return EGSGPU_OK;
+}
+
+// Bytes of scratch a history-schedule run of h needs: per stream its
+// history table (entries and list) and its particle stack (with the entry
+// past the stack that photon splitting keeps).
+size_t static_bytes(const egsgpu_handle *h, size_t *tab_bytes, size_t *list_bytes) {
+    const size_t rsz = h->fp64 ? sizeof(double) : sizeof(float);
+    const size_t n = (size_t)h->nlanes;
+    *tab_bytes = n * (size_t)h->hcap * sizeof(SlotEntry);
+    *list_bytes = (n * (size_t)h->hcap * sizeof(int) + 255) / 256 * 256;
+    return *tab_bytes + *list_bytes + n * (size_t)(h->mxstk + 1) * kStkF * rsz;
+}
+
+template <typename R>
+int run_static(egsgpu_handle *h, Precision<R> &P, int64_t nhist, unsigned char *base) {
+    size_t tb = 0, lb = 0;
+    static_bytes(h, &tb, &lb);
+    Precision<R> Q = P;
+    Q.L.hcap = h->hcap;
+    Q.L.hent = (SlotEntry *)base;
+    Q.L.hlist = (int *)(base + tb);
+    Q.L.stk = (R *)(base + tb + lb);
+    const size_t words = tb / sizeof(uint4);
+    const int blocks = (int)std::min<size_t>((words + 255) / 256, (size_t)h->ncu * 8);
+    hipLaunchKernelGGL(k_zero16, dim3(std::max(blocks, 1)), dim3(256), 0, h->stream,
+                       (uint4 *)base, words);
+    EGS_HIP(hipGetLastError());
+    return launch(h, Q, nhist, nullptr, -1, false);
+}
+}  // namespace
+
+extern "C" {
+
+int egsgpu_run_static(egsgpu_handle *h, int64_t nhist) {
+    EGS_ARG(h && nhist >= 0, "handle, nhist >= 0");
+    if (!h->lanes_ready) {
+        set_error("egsgpu_run_static needs the handle's streams: run egsgpu_run(h, 0) once "
+                  "first");
+        return EGSGPU_E_ARG;
+    }
+    if (h->schedule == EGSGPU_SCHED_EVENT) {
+        set_error("egsgpu_run_static runs the history schedules; choose one with "
+                  "egsgpu_set_schedule before the first run");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    size_t tb = 0, lb = 0;
+    const size_t need = static_bytes(h, &tb, &lb);
+    if (need > kScratchBytes) {
+        set_error("egsgpu_run_static: %d streams need %zu bytes of scratch, the static array "
+                  "holds %zu (use fewer streams: egsgpu_set_streams)",
+                  h->nlanes, need, kScratchBytes);
+        return EGSGPU_E_ARG;
+    }
+    void *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    rc = h->fp64 ? run_static(h, h->pd, nhist, (unsigned char *)base)
+                 : run_static(h, h->pf, nhist, (unsigned char *)base);
+    if (rc) return rc;
+    EGS_HIP(hipStreamSynchronize(h->stream));
     h->nhist += nhist;
     return EGSGPU_OK;
 }

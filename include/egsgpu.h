@@ -304,6 +304,26 @@ int egsgpu_set_rng(egsgpu_handle *h, int32_t kind, int32_t a, int32_t b);
  * streams; tallies accumulate history-by-history (egs_scoring.h:63-125). */
 int egsgpu_run(egsgpu_handle *h, int64_t nhist);
 
+/* egsgpu_run without allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Runs nhist further
+ * histories exactly as egsgpu_run does on the history schedules
+ * (EGSGPU_SCHED_HISTORY or EGSGPU_SCHED_HISTORY_PLAIN; same histories, same
+ * random numbers, same tallies up to the order of FP64 additions), with the
+ * per-stream history tables and particle stacks in a static device array
+ * of the library (64 MiB, held by each GPU the library runs on until the
+ * process ends) instead of the handle's own.  It calls no hipMalloc,
+ * hipFree, event or stream creation.  Requirements: the handle's streams
+ * exist (a first egsgpu_run, e.g. of 0 histories, makes them) and fit:
+ * streams x (history-table entries x 20 B + stack entries x 44 B in FP32,
+ * 88 B in FP64) <= 64 MiB, otherwise EGSGPU_E_ARG; e.g. up to about 10000
+ * streams in kerma mode (256 entries, 33 stack entries) and 1500 with
+ * coupled transport (2048 entries, 65 stack entries).  Every call on a GPU
+ * shares that GPU's array, which keeps the last call's contents; the
+ * history tables are cleared at the start of each call.  Calls that
+ * overlap on one GPU (two host threads, or two handles' streams) race. */
+int egsgpu_run_static(egsgpu_handle *h, int64_t nhist);
+
 /* Run a batch of simulations together (many small jobs): nhist[j] further
  * histories of simulation jobs[j], j < njobs, in one event schedule on the
  * streams and history slots of jobs[0].  The jobs share tables, precision
