@@ -1805,7 +1805,8 @@ int oracle_destroy(void *h) {
 // Sampler hooks for the oracle's own pinning tests: run one interaction
 // routine n times on a particle moving along +z in an infinite medium and
 // return per-sample outputs.  kind:
-//   0 COMPT   -> [k'/k, cos(theta_gamma), T_electron, Uj_deposit]
+//   0 COMPT (param = IBCMP: 0 Klein-Nishina, 1 bound Compton with rejected
+//            interactions returned as -9, 3 bound Compton resampled) -> [k'/k, cos(theta_gamma), T_electron, Uj_deposit]
 //   1 MOLLER  -> [T_secondary, T_primary, cos theta_1, cos theta_2]
 //   2 BHABHA  -> [T_of_particle_2, T_of_particle_1, q_2, q_1]
 //   3 ANNIH   -> [E_gamma1, E_gamma2, cos1, cos2]
@@ -1815,6 +1816,11 @@ int oracle_destroy(void *h) {
 //                step of param cm (one call per sample, find_index reset)
 //   7 BREMS   -> [E_gamma, E_electron_total, cos theta_gamma, 0]
 //   8 RAYLEIGH-> [cos theta, 0, 0, 0]
+//   9 SSCAT   -> [cos theta, sin theta, 0, 0] for screening parameter
+//                chia2 = param
+//  10 MSDIST  -> [x, y, z, w] after one msdist_pII step of param cm (no
+//                energy loss, unit density) from the origin along +z, at
+//                kinetic energy E
 extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
                              double energy, double param, int64_t n,
                              int32_t seed, double *out) {
@@ -1836,7 +1842,7 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
         double ecut = 0, pcut = 0;
         for (auto &m : pb.T.med) { ecut = std::max(ecut, m.ae); pcut = std::max(pcut, m.ap); }
         P = Params{ecut, pcut, 1, 3, 1, 1, 1, 1, 0, 1, 0.25, 0.5, 3.0, 0};
-        if (kind == 0 && param == 0) P.ibcmp = 0;      // Klein-Nishina
+        if (kind == 0) P.ibcmp = (int)param;           // 0 KN, 1 bound with rejection, 3 norej
         std::vector<double> tmp(1, 0.0);
         std::vector<int> touched;
         Shower sh(pb, tmp, touched);
@@ -1899,6 +1905,15 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
                 double c, s;
                 sh.rayleigh_sample(M, energy, gle, lgle, c, s);
                 o[0] = c; o[1] = s; o[2] = 0; o[3] = 0;
+            } else if (kind == 9) {
+                double c, s;
+                sh.sscat(param, c, s);
+                o[0] = c; o[1] = s; o[2] = 0; o[3] = 0;
+            } else if (kind == 10) {
+                double us, vs, ws, xf, yf, zf, ustep;
+                sh.msdist_pII(M, energy, 0.0, param, 1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, us, vs, ws,
+                              xf, yf, zf, ustep);
+                o[0] = xf; o[1] = yf; o[2] = zf; o[3] = ws;
             } else {
                 throw std::runtime_error("unknown sampler kind");
             }

@@ -1,0 +1,324 @@
+"""Pins of the table builder and of the oracle's samplers that the
+conservation and forward-peaking checks left open.  Each compares with a
+quantity fixed independently of the code under test:
+
+  * photon tables: an independent sum over the EGSnrc data files
+    (tests/golden/photon_xs_521icru.json, written by
+    tests/golden/make_photon_xs.py) and NIST XCOM attenuation of water;
+  * electron ranges: numerical integration of 1/(dE/dx);
+  * Rayleigh: the cumulative of the tabulated form factors
+    (tests/golden/rayleigh_ff.json) and the angular distribution
+    (1 + cos^2) F^2(x) / 2;
+  * PAIR and BREMS: the Bethe-Heitler cross sections with the screening
+    functions of the EGSnrc manual (PIRS-701, section 2.7) and Tsai's
+    radiation logarithms, written out in their textbook form;
+  * PHOTO: Sauter's K-shell angular distribution;
+  * sscat: the screened Rutherford distribution;
+  * msdist_pII: the spatial moments of Lewis's theory;
+  * bound Compton: the fraction of interactions the impulse approximation
+    accepts equals the bound-to-free cross-section ratio of
+    data/compton_sigma.data.
+(CPU only.)"""
+import json
+import os
+
+import numpy as np
+from scipy import integrate, stats
+
+from tests import oracle_lib
+from tests.blob import medium_names, pwlf, read_blob
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BLOB = os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+RM = 0.5109989461
+WATER = 0
+WATER_ATOMS = [(1, 2.0), (8, 1.0)]   # (Z, atoms per molecule) of H2O521ICRU
+
+
+def blob():
+    return read_blob(BLOB)
+
+
+def chi2_pvalue(samples, edges, probs):
+    """chi^2 p-value of a histogram against bin probabilities."""
+    h, _ = np.histogram(samples, edges)
+    exp = probs / probs.sum() * len(samples)
+    m = exp > 5
+    chi2 = np.sum((h[m] - exp[m]) ** 2 / exp[m])
+    return stats.chi2.sf(chi2, m.sum() - 1)
+
+
+def quantile_edges(pdf, lo, hi, nbins=40, npts=20001):
+    """Bin edges of equal expected content for an unnormalised pdf, and the
+    bin probabilities integrated with quad."""
+    x = np.linspace(lo, hi, npts)
+    c = integrate.cumulative_trapezoid(pdf(x), x, initial=0)
+    c /= c[-1]
+    edges = np.interp(np.linspace(0, 1, nbins + 1), c, x)
+    edges[0], edges[-1] = lo, hi
+    p = np.array([integrate.quad(pdf, a, b, limit=200)[0] for a, b in zip(edges[:-1], edges[1:])])
+    return edges, p
+
+
+# ---------------------------------------------------------------- tables
+def test_photon_tables_match_an_independent_xcom_sum():
+    """gmfp, the branching ratios and the Rayleigh fraction of every medium
+    of the blob equal the independent sum over the XCOM element data
+    within the piecewise-linear interpolation of the 2000-point table."""
+    b = blob()
+    fx = json.load(open(os.path.join(GOLDEN, "photon_xs_521icru.json")))
+    for m, name in enumerate(medium_names(b)):
+        rows = fx["media"][name]["rows"]
+        x = np.log([r["e"] for r in rows])
+        for f, tol in (("gmfp", 1e-4), ("gbr2", 1e-4), ("cohe", 2e-5)):
+            exp = np.array([r[f] for r in rows])
+            got = pwlf(b, m, f, x)
+            assert np.all(np.abs(got - exp) <= tol * exp), (name, f)
+        exp = np.array([r["gbr1"] for r in rows])
+        got = pwlf(b, m, "gbr1", x)
+        assert np.all(np.abs(got - exp) <= 1e-6 + 2e-4 * exp), (name, "gbr1")
+
+
+def test_water_attenuation_matches_nist_xcom():
+    """Total attenuation (with coherent scattering) of water against NIST
+    XCOM (Berger et al., XCOM: Photon Cross Sections Database): 0.8096,
+    0.1707, 0.07072 and 0.02219 cm^2/g at 0.02, 0.1, 1 and 10 MeV.  The
+    blob's bound-Compton correction is that of compton_sigma.data rather
+    than XCOM's incoherent scattering function, hence 1 %."""
+    b = blob()
+    for e, mu in ((0.02, 0.8096), (0.1, 0.1707), (1.0, 0.07072), (10.0, 0.02219)):
+        x = np.log([e])
+        got = 1.0 / (pwlf(b, WATER, "gmfp", x) * pwlf(b, WATER, "cohe", x))[0]
+        assert abs(got / mu - 1) < 0.01, (e, got, mu)
+
+
+def test_electron_range_is_the_integral_of_the_stopping_power():
+    """range_ep(j) - range_ep(0) = integral of dE / (dE/dx) from the first
+    energy of the grid to E_j, with dE/dx the blob's own piecewise-linear
+    restricted stopping power (quad per interval, not the series expansion
+    of $COMPUTE-DRANGE), for electrons and positrons."""
+    b = blob()
+    for m in range(len(medium_names(b))):
+        e0, e1 = b["m%d.eke0" % m][0], b["m%d.eke1" % m][0]
+        n = int(b["m%d.meke" % m][0])
+        earr = b["m%d.e_array" % m][:n]
+        for q, field in ((0, "ededx"), (1, "pdedx")):
+            rep = b["m%d.range_ep%d" % (m, q)][:n]
+            a0, a1 = b["m%d.%s0" % (m, field)], b["m%d.%s1" % (m, field)]
+            acc = 0.0
+            for j in range(1, n, 7):
+                acc = 0.0
+                for i in range(j):   # interval i: ln E in [(i+1-e0)/e1, (i+2-e0)/e1)
+                    lo, hi = np.log(earr[i]), np.log(earr[i + 1])
+                    acc += integrate.quad(lambda x: np.exp(x) / (a1[i] * x + a0[i]), lo, hi)[0]
+                got = rep[j] - rep[0]
+                assert abs(got - acc) <= 2e-5 * acc + 1e-12, (m, q, j, got, acc)
+
+
+def form_factor_sq(x, atoms):
+    """sum over atoms of F^2(x) (independent atoms), F interpolated
+    log-log on the pgs4form.dat grid."""
+    d = json.load(open(os.path.join(GOLDEN, "rayleigh_ff.json")))
+    xg = np.array(d["x"], float)
+    xg[0] = 1e-4
+    out = np.zeros_like(np.asarray(x, float))
+    for z, n in atoms:
+        f = np.maximum(np.array(d["F"][str(z)], float), 1e-300)
+        lf = np.interp(np.log(np.maximum(x, 1e-4)), np.log(xg), np.log(f))
+        out += n * np.exp(2 * lf)
+    return out
+
+
+def test_rayleigh_cumulative_form_factor():
+    """fcum(x_i) = int_0^{x_i} F^2 d(x^2) / int_0^{x_max} F^2 d(x^2), with
+    x_max the momentum transfer of backscattering at the top of the
+    photon table, from the tabulated form factors of water."""
+    b = blob()
+    xg = b["m0.ray_x"]
+    fcum = b["m0.ray_fcum"]
+    up = np.exp((2000 - b["m0.ge0"][0]) / b["m0.ge1"][0])
+    xmax = 20.607544 * 2 * up / RM
+    t = np.exp(np.linspace(np.log(1e-4), np.log(xmax), 400001))
+
+    def cum(xx):
+        tt = t[t <= xx]
+        return integrate.trapezoid(form_factor_sq(tt, WATER_ATOMS) * 2 * tt, tt)
+    norm = cum(xmax)
+    for i in range(1, 100, 9):
+        if xg[i] > xmax:
+            break
+        exp = cum(xg[i]) / norm
+        assert abs(fcum[i] - exp) < 2e-3 * exp + 1e-7, (i, xg[i], fcum[i], exp)
+
+
+# -------------------------------------------------------------- samplers
+def fcoulc(z):
+    """Coulomb correction of Davies, Bethe and Maximon."""
+    a2 = (z / 137.03604) ** 2
+    return a2 * (1 / (1 + a2) + 0.20206 + a2 * (-0.0369 + a2 * (0.0083 + a2 * (-0.002))))
+
+
+def xsif(z):
+    """Tsai's ratio of the radiation logarithms L'_rad / (L_rad - f_c)
+    (Rev. Mod. Phys. 46 (1974) 815, table B.2 for Z <= 4)."""
+    if z <= 4:
+        lrad, lradp = (5.31, 4.79, 4.74, 4.71)[z - 1], (6.144, 5.621, 5.805, 5.924)[z - 1]
+        return lradp / (lrad - fcoulc(z))
+    return np.log(1194.0 * z ** (-2 / 3)) / (np.log(184.15 * z ** (-1 / 3)) - fcoulc(z))
+
+
+def screening_constants(atoms):
+    """Z_g = <ln Z^(-1/3)> and Z_v = Z_g - <f_c>, weighted by Z (Z + xi)."""
+    zt = sum(n * z * (z + xsif(z)) for z, n in atoms)
+    zb = -sum(n * z * (z + xsif(z)) * np.log(z) / 3 for z, n in atoms)
+    zf = sum(n * z * (z + xsif(z)) * fcoulc(z) for z, n in atoms)
+    return zb / zt, (zb - zf) / zt
+
+
+def phi12(delta, zc):
+    """The screening functions of the EGSnrc manual plus 4 Z_c."""
+    delta = np.asarray(delta, float)
+    small = delta <= 1
+    d = np.where(small, delta, 0.0)
+    big = 21.12 - 4.184 * np.log(np.maximum(delta, 1.0) + 0.952)
+    p1 = np.where(small, 20.863 - 3.242 * d + 0.625 * d * d, big) + 4 * zc
+    p2 = np.where(small, 20.029 - 1.930 * d - 0.086 * d * d, big) + 4 * zc
+    return p1, p2
+
+
+def test_screening_radius_of_the_blob():
+    """delcm = 136 m exp(Z_g) from the composition (to the single
+    precision of the PEGS4 file it comes from)."""
+    zg, zv = screening_constants(WATER_ATOMS)
+    assert abs(blob()["m0.delcm"][0] / (136 * RM * np.exp(zg)) - 1) < 1e-4
+
+
+def test_pair_energy_sharing_bethe_heitler():
+    """Electron energy fraction eps of pair production against
+    d sigma / d eps ~ (eps^2 + (1-eps)^2) Phi1 + 2/3 eps (1-eps) Phi2,
+    delta = 136 m exp(Z_g) / (k eps (1-eps)) (Z_v with the Coulomb
+    correction from 50 MeV)."""
+    zg, zv = screening_constants(WATER_ATOMS)
+    for k in (6.0, 20.0, 80.0):
+        zc = zg if k < 50 else zv
+        out = oracle_lib.sample(BLOB, 4, WATER, k, 200000, seed=int(k * 100))
+        em = np.where(out[:, 2] < 0, out[:, 0], out[:, 1])
+        eps = em / k
+
+        def pdf(e):
+            delta = 136 * RM * np.exp(zg) / (k * e * (1 - e))
+            p1, p2 = phi12(delta, zc)
+            return np.maximum((e * e + (1 - e) ** 2) * p1 + 2 / 3 * e * (1 - e) * p2, 0)
+        edges, p = quantile_edges(pdf, RM / k, 1 - RM / k)
+        assert chi2_pvalue(eps, edges, p) > 1e-3, k
+
+
+def test_brems_photon_spectrum_bethe_heitler():
+    """Bremsstrahlung photon energy k against
+    d sigma / d k ~ [(1 + a^2) Phi1 - 2/3 a Phi2] / k, a = E'/E (total
+    energies), delta = 136 m exp(Z_g) k / (E E')."""
+    zg, zv = screening_constants(WATER_ATOMS)
+    ap = blob()["m0.ap"][0]
+    for t in (10.0, 70.0):
+        e = t + RM
+        zc = zg if e < 50 else zv
+        out = oracle_lib.sample(BLOB, 7, WATER, t, 200000, seed=int(t))
+        lk = np.log(out[:, 0])
+
+        def pdf(lkk):
+            kk = np.exp(lkk)
+            ep = e - kk
+            a = ep / e
+            delta = 136 * RM * np.exp(zg) * kk / (e * ep)
+            p1, p2 = phi12(delta, zc)
+            return np.maximum((1 + a * a) * p1 - 2 / 3 * a * p2, 0)   # d sigma / d ln k
+        edges, p = quantile_edges(pdf, np.log(ap), np.log(t))
+        assert chi2_pvalue(lk, edges, p) > 1e-3, t
+
+
+def test_photoelectron_direction_sauter():
+    """Photo-electron polar angle against Sauter's K-shell distribution
+    sin^2 / (1 - b c)^4 [1 + g (g-1) (g-2) (1 - b c) / 2]."""
+    for k in (0.1, 1.0):
+        out = oracle_lib.sample(BLOB, 5, WATER, k, 200000)
+        g = np.median(out[:, 0]) / RM
+        beta = np.sqrt(1 - 1 / g ** 2)
+        c = out[:, 2]
+
+        def pdf(x):
+            return (1 - x * x) / (1 - beta * x) ** 4 * (1 + 0.5 * g * (g - 1) * (g - 2) *
+                                                        (1 - beta * x))
+        edges, p = quantile_edges(pdf, -1.0, 1.0)
+        assert chi2_pvalue(c, edges, p) > 1e-3, k
+
+
+def test_rayleigh_angle_form_factor():
+    """Rayleigh scattering angle against (1 + cos^2) F^2(x), x =
+    sin(theta/2) E / (h c), with the form factors of the fixture."""
+    for e in (0.03, 0.1, 0.3):
+        out = oracle_lib.sample(BLOB, 8, WATER, e, 200000)
+        c = out[:, 0]
+
+        def pdf(x):
+            q = 80.65506856998 * e * np.sqrt(np.maximum((1 - x) / 2, 0))
+            return (1 + x * x) * form_factor_sq(q, WATER_ATOMS)
+        edges, p = quantile_edges(pdf, -1.0, 1.0, npts=200001)
+        assert chi2_pvalue(c, edges, p) > 1e-3, e
+
+
+def test_single_scattering_screened_rutherford():
+    """sscat's t = 1 - cos(theta) against 1 / (t + 2 chi_a^2)^2 on [0, 2]."""
+    for c2 in (1e-4, 1e-2, 0.3):
+        out = oracle_lib.sample(BLOB, 9, WATER, 1.0, 200000, param=c2)
+        t = 1 - out[:, 0]
+        edges, p = quantile_edges(lambda x: 1 / (x + 2 * c2) ** 2, 0.0, 2.0, npts=400001)
+        assert chi2_pvalue(t, edges, p) > 1e-3, c2
+        assert np.allclose(out[:, 0] ** 2 + out[:, 1] ** 2, 1, atol=1e-12)
+
+
+def test_msdist_lewis_moments():
+    """After one PRESTA-II step of length s without energy loss: <w> =
+    exp(-G1), <z> = s (1 - exp(-G1)) / G1 and <x^2 + y^2> = <r^2> - <z^2>
+    of Lewis's theory, with G_l = 2 lambda q_l for the 2 lambda elastic
+    mean free paths of the step and q_1, q_2 the screened-Rutherford
+    transport coefficients."""
+    b = blob()
+    blcc, xcc = b["m0.blcc"][0], b["m0.xcc"][0]
+    for e, s in ((1.0, 0.01), (1.0, 0.1), (5.0, 0.8), (0.2, 0.005), (1.0, 0.25)):
+        out = oracle_lib.sample(BLOB, 10, WATER, e, 300000, param=s)
+        p2 = e * (e + 2 * RM)
+        beta2 = p2 / (p2 + RM * RM)
+        c = xcc / (4 * p2 * blcc)
+        lam = 0.5 * s * blcc / beta2 / (1 + c)
+        lg = np.log(1 + 1 / c)
+        q1 = 2 * c * ((1 + c) * lg - 1)
+        q2 = 6 * c * (1 + c) * ((1 + 2 * c) * lg - 2)
+        g1, g2 = 2 * lam * q1, 2 * lam * q2
+        k1, k2 = g1 / s, g2 / s
+        zl = s * (1 - np.exp(-g1)) / g1
+        r2 = 2 * s * s / g1 ** 2 * (g1 - 1 + np.exp(-g1))
+        z2 = 2 / k1 * integrate.quad(lambda t: (1 - np.exp(-k1 * (s - t))) *
+                                     (1 / 3 + 2 / 3 * np.exp(-k2 * t)), 0, s)[0]
+        n = len(out)
+        z, rr, w = out[:, 2], out[:, 0] ** 2 + out[:, 1] ** 2, out[:, 3]
+        assert abs(w.mean() - np.exp(-g1)) < 4 * w.std() / np.sqrt(n) + 5e-4 * np.exp(-g1)
+        assert abs(z.mean() - zl) < 4 * z.std() / np.sqrt(n) + 5e-4 * zl
+        assert abs(rr.mean() - (r2 - z2)) < 4 * rr.std() / np.sqrt(n) + 3e-3 * (r2 - z2)
+
+
+def test_bound_compton_acceptance_equals_compton_sigma():
+    """With IBCMP = 1 a Klein-Nishina interaction is rejected when the
+    sampled shell cannot be ionised or the momentum profile rejects it, so
+    the fraction accepted is sigma_bound / sigma_KN of the impulse
+    approximation: the ratio of data/compton_sigma.data for the mixture."""
+    fx = json.load(open(os.path.join(GOLDEN, "photon_xs_521icru.json")))
+    rows = {round(r["e"], 6): r for r in fx["media"]["H2O521ICRU"]["rows"]}
+    n = 200000
+    for e in (0.015, 0.03, 0.06, 0.15, 0.5):
+        r = rows[min(rows, key=lambda x: abs(x - e))]
+        out = oracle_lib.sample(BLOB, 0, WATER, r["e"], n, param=1.0)
+        acc = np.mean(out[:, 0] > -1)
+        sig = np.sqrt(acc * (1 - acc) / n)
+        print("E %.4f accepted %.4f ratio %.4f" % (r["e"], acc, r["compton_ratio"]))
+        assert abs(acc - r["compton_ratio"]) < 4 * sig + 3e-3, (r["e"], acc, r["compton_ratio"])
