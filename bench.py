@@ -102,14 +102,19 @@ def efficiency(sum_, sum2, n, seconds):
 
 
 def cpu_baseline(name, seconds_target=15.0):
-    """The FP64 oracle on the host cores, run on a bounded sample."""
+    """The FP64 oracle on the host cores, run on a bounded sample.  Returns
+    the baseline record and the sample's tallies (sum, sum2, histories)."""
     from egsnrc_amd import _lib
     from tests import oracle_lib
     xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
     g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
-    cores = min(os.cpu_count() or 1, 16)
+    # every core of the host, up to the 16 that one GPU's share of the box
+    # provides
+    cores = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                else (os.cpu_count() or 1), 16)
     blob = os.path.join(ROOT, "egsnrc_amd", "data", WORKLOADS[name]["blob"])
     o = oracle_lib.OracleSimulation(blob, g, src[0], tp, nstreams=4 * cores, nthreads=cores)
+    o.seed(4711, 20000)   # independent of the engine's default streams
     n = 20000
     t = time.perf_counter()
     o.run(n)
@@ -119,12 +124,70 @@ def cpu_baseline(name, seconds_target=15.0):
     t = time.perf_counter()
     o.run(n2)
     dt = time.perf_counter() - t
-    return {"value": n2 / dt, "unit": "histories/s", "cores": cores, "kind": "port",
-            "sample": "%d histories of the same workload (FP64 oracle, %d threads, %.1f s)"
-                      % (n2, cores, dt)}
+    rec = {"value": n2 / dt, "unit": "histories/s", "cores": cores, "kind": "port",
+           "sample": "%d histories of the same workload (FP64 oracle, %d threads, %.1f s)"
+                     % (n2, cores, dt)}
+    return rec, o.tallies()
 
 
-def grad_bench(nhist, lanes=0):
+def dose_agreement(gpu, ref, frac=0.1):
+    """Per-voxel z of the engine's tallies against the oracle's over the
+    voxels above frac of Dmax: chi^2/dof and the fraction beyond 2 sigma
+    (4.55 % for agreeing normal z)."""
+    def st(t):
+        s_, s2, n = t
+        r = s_ / n
+        v = s2 / n - r * r
+        return r, np.where(v > 0, v / (n - 1), 0.0)
+    ra, va = st(gpu)
+    rb, vb = st(ref)
+    m = (np.maximum(ra, rb) > frac * max(ra.max(), rb.max())) & (va + vb > 0)
+    z = (ra[m] - rb[m]) / np.sqrt(va[m] + vb[m])
+    return {"chi2_per_dof": float(np.mean(z * z)), "beyond_2sigma": float(np.mean(np.abs(z) > 2)),
+            "voxels": int(z.size), "histories_engine": int(gpu[2]),
+            "histories_oracle": int(ref[2]),
+            "total_rel_diff": float(ra.sum() / rb.sum() - 1.0),
+            "definition": "z = (D_engine - D_oracle) / sqrt(s_e^2 + s_o^2) over voxels above "
+                          "10% of Dmax"}
+
+
+def fp64_throughput(tab, name, hps, a):
+    """Histories/s of the same workload with FP64 transport arithmetic
+    (tallies are FP64 in both builds): one warm-up and two timed steps."""
+    from egsnrc_amd.engine import Simulation
+    xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
+    tp.fp64 = 1
+    sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=a.lanes, block=a.block,
+                     schedule=a.schedule)
+    if a.woodcock is not None:
+        sim.set_woodcock(a.woodcock)
+    sim.run(hps // 4)   # egsgpu_run returns when the launch has completed
+    t0 = time.perf_counter()
+    for _ in range(2):
+        sim.run(hps)
+    T = time.perf_counter() - t0
+    sim.close()
+    return {"histories_per_s": 2 * hps / T, "steps": 2, "histories_per_step": hps,
+            "dtype": "f64"}
+
+
+def atomic_bench():
+    """Rate of FP64 atomic adds on this device (egsgpu_atomic_rate): the
+    roof of the tally flushes and of the density-gradient sweep."""
+    from egsnrc_amd import _lib
+    out = {"unit": "G adds/s"}
+    for key, pattern, naddr, nops in (("scattered_64cube", 0, 64 ** 3, 1 << 28),
+                                      ("scattered_128cube", 0, 128 ** 3, 1 << 28),
+                                      ("coalesced_wave", 2, 1 << 24, 1 << 28),
+                                      ("one_address_per_wave", 1, 1 << 20, 1 << 24)):
+        rate, total, issued = _lib.atomic_rate(pattern, naddr, nops)
+        if total != 2 * issued:
+            raise RuntimeError("atomic_rate: %s lost adds (%g of %d)" % (key, total, 2 * issued))
+        out[key] = rate / 1e9
+    return out
+
+
+def grad_bench(nhist, lanes=0, woodcock=None):
     """Density-gradient evaluations/s on the C4 problem: 64^3 x 0.4 cm
     water with densities from a smoothed random field in [0.2, 2.0], 2 MeV
     8x8 cm^2 parallel beam, kerma mode; target dose from a second field
@@ -141,10 +204,14 @@ def grad_bench(nhist, lanes=0):
     tp = _lib.default_transport(ecut=20.0, kerma=1)
     tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab"))
     st = Simulation(tab, xb, yb, zb, med, trho, src, tp, nstreams=lanes)
+    if woodcock is not None:
+        st.set_woodcock(woodcock)
     st.run(min(nhist, 10_000_000))
     s_, _, nn = st.tallies()
     st.close()
     sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=lanes)
+    if woodcock is not None:
+        sim.set_woodcock(woodcock)
     vol = np.prod([np.diff(b)[0] for b in (xb, yb, zb)])
     dstar = s_ / nn / (vol * trho)            # water, nominal density 1
     sim.density_grad(1_000_000, dstar)        # warm-up
@@ -155,6 +222,8 @@ def grad_bench(nhist, lanes=0):
                         "beam, kerma mode, target from a 10% perturbed field",
             "histories": int(nhist), "seconds": T, "evals_per_s": 1.0 / T,
             "histories_per_s": 2.0 * nhist / T, "loss": L,
+            "photon_tracking": ("woodcock x%.2f" % (woodcock or 1.25)
+                                if woodcock is not None else "voxel"),
             "note": "one evaluation = forward transport + replay from the saved RNG states"}
 
 
@@ -430,6 +499,9 @@ def main():
     ap.add_argument("--precision-report", action="store_true",
                     help="only run the workload with FP32 and FP64 transport (same seeds) "
                          "and print the per-voxel difference summary")
+    ap.add_argument("--woodcock", type=float, default=None,
+                    help="photon flights by Woodcock tracking with this majorant factor "
+                         "(0 = library default 1.25); also used by the gradient evaluation")
     ap.add_argument("--howfarless", action="store_true",
                     help="DOSXYZnrc HOWFARLESS transport (homogeneous workloads: c0)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -497,6 +569,8 @@ def main():
         _lib.check(_lib.lib().egsgpu_set_event_batching(sim.h, ks, bt, mw, db))
     if not a.graph:
         _lib.check(_lib.lib().egsgpu_set_event_graph(sim.h, 0))
+    if a.woodcock is not None:
+        sim.set_woodcock(a.woodcock)
     tuned = sim.autotune(a.autotune) if a.autotune else None
     for _ in range(a.warmup):
         sim.run(hps)
@@ -575,10 +649,17 @@ def main():
             sim.run(hps)
         out["graph_mode"] = {"histories_per_s": 2 * hps / (time.perf_counter() - t0),
                              "steps": 2, "note": "same steps replayed from a HIP graph"}
+    if a.woodcock is not None:
+        out["config"]["photon_tracking"] = "woodcock (majorant x %s)" % (a.woodcock or 1.25)
     if a.grad_hist and name in ("slab128", "kerma64"):
-        out["density_grad"] = grad_bench(a.grad_hist, a.lanes)
+        sim.close()   # the gradient run gets the device to itself
+        out["density_grad"] = grad_bench(a.grad_hist, a.lanes, a.woodcock)
+    if name in ("slab128", "kerma64") and not a.howfarless:
+        out["fp64_transport"] = fp64_throughput(tab, name, hps, a)
+        out["fp64_atomics"] = atomic_bench()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(name)
+        out["cpu_baseline"], ref = cpu_baseline(name)
+        out["dose_agreement_vs_oracle"] = dose_agreement((s, s2, n), ref)
     print(json.dumps(out))
 
 

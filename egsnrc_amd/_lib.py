@@ -91,6 +91,7 @@ def lib():
         "egsgpu_ranlux_fill": [i32, i32, i32, i64, vp],
         "egsgpu_trace_rays": [vp, i32, vp, vp, i32, i32, vp, vp, vp],
         "egsgpu_hownear": [vp, i32, vp, vp, i32, vp],
+        "egsgpu_atomic_rate": [i32, i32, i64, vp, vp],
         "egsgpu_tables_load": [C.c_char_p, vp],
         "egsgpu_tables_free": [vp],
         "egsgpu_tables_nmedia": [vp, vp],
@@ -107,6 +108,7 @@ def lib():
         "egsgpu_get_counters": [vp, vp],
         "egsgpu_get_energy_balance": [vp, vp],
         "egsgpu_run_static": [vp, i64],
+        "egsgpu_set_photon_tracking": [vp, i32, C.c_double],
         "egsgpu_last_kernel_ms": [vp, vp, vp],
         "egsgpu_density_grad": [vp, i64, vp, vp, vp, vp],
         "egsgpu_tables_medium_info": [vp, i32, vp, C.c_size_t, vp],
@@ -203,3 +205,14 @@ def hownear(geom, x, ireg, precision):
     check(lib().egsgpu_hownear(C.byref(geom), x.shape[0], ptr(x), ptr(ireg),
                                precision, ptr(out)))
     return out
+
+
+def atomic_rate(pattern, naddr, nops):
+    """FP64 atomic adds per second of egsgpu_atomic_rate, and the number of
+    adds the table holds afterwards (both launches)."""
+    ms = C.c_double(0)
+    total = C.c_double(0)
+    check(lib().egsgpu_atomic_rate(pattern, naddr, int(nops), C.byref(ms), C.byref(total)))
+    per = 64
+    issued = ((int(nops) + per - 1) // per + 255) // 256 * 256 * per
+    return issued / (ms.value * 1e-3), total.value, issued

@@ -80,6 +80,11 @@ struct DevTables {
     const float *ms_u, *ms_f, *ms_w;
     const int *ms_i;
     float ms_dllambi, ms_dqmsi;
+    // Woodcock tracking: the majorant attenuation coefficient (1/cm), one
+    // value per interval of ln E, interval L = int(wm1 ln E + wm0) in 1..wmn
+    const R *wmaj;
+    R wm0, wm1;
+    int wmn;
 };
 
 template <typename R>
@@ -99,6 +104,7 @@ struct DevSource {
 
 struct DevParams {
     int rayleigh, ibcmp, iphter, iedgfl, iprdst, ibrdst, exact_bca, kerma;
+    int woodcock;             // photon flights by Woodcock (delta) tracking
     int n_split;              // DOSXYZnrc photon splitting number (0 = off)
     int nbr_split;            // radiative splitting number (0 = off)
     int play_rr;              // Russian roulette of photon-made charged particles
@@ -109,6 +115,10 @@ struct DevParams {
 
 // Fields of a stacked particle.
 constexpr int kStkF = 11;
+
+// Copies of each energy-bookkeeping tally region (a power of two): the
+// histories add into copy lane % kExtraCopies.
+constexpr int kExtraCopies = 64;
 
 // One entry of a history slot's table: the voxel's key and its energy in
 // one 16-byte record, so that a deposit touches one cache line.

@@ -510,7 +510,7 @@ __device__ __forceinline__ int bank_load(const DevBank<R> &b, int i, Part<R> &q)
 // particle becomes visible.  A full bank absorbs the particle where it is
 // (counted as an overflow).
 template <typename R, bool LUX>
-__device__ bool bank_try(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *count, int ph,
+__device__ __forceinline__ bool bank_try(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *count, int ph,
                          R s0, R s1, R s2, R s3) {
     int i = wave_claim(count);
     if (i >= b.cap) return false;
@@ -520,7 +520,7 @@ __device__ bool bank_try(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *cou
     return true;
 }
 template <typename R, bool LUX>
-__device__ void bank_absorb(Shower<R, true, LUX> &sh) {
+__device__ __forceinline__ void bank_absorb(Shower<R, true, LUX> &sh) {
     atomicSub(sh.slots->pending + sh.slot, 1);
     sh.c_ovf++;
     const R RM = (R)0.5109989461;
@@ -528,7 +528,7 @@ __device__ void bank_absorb(Shower<R, true, LUX> &sh) {
     sh.score(sh.p.ir, edep, sh.p.wt);
 }
 template <typename R, bool LUX>
-__device__ void bank_push(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *count, int ph = 0,
+__device__ __forceinline__ void bank_push(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *count, int ph = 0,
                           R s0 = 0, R s1 = 0, R s2 = 0, R s3 = 0) {
     atomicAdd(sh.slots->pending + sh.slot, 1);
     if (!bank_try(sh, b, count, ph, s0, s1, s2, s3)) bank_absorb(sh);
@@ -538,7 +538,7 @@ __device__ void bank_push(Shower<R, true, LUX> &sh, const DevBank<R> &b, int *co
 // full it waits, fresh, in this launch's deferral bank, whose next drain
 // passes it on (a drained bank of the other type has room again).
 template <typename R, bool LUX>
-__device__ void bank_push_other(Shower<R, true, LUX> &sh, const EvArgs<R> &a) {
+__device__ __forceinline__ void bank_push_other(Shower<R, true, LUX> &sh, const EvArgs<R> &a) {
     atomicAdd(sh.slots->pending + sh.slot, 1);
     if (bank_try(sh, a.push, &a.ctl->n[a.ip], 0, (R)0, (R)0, (R)0, (R)0)) return;
     if (bank_try(sh, a.defr, &a.ctl->n[a.id], 0, (R)0, (R)0, (R)0, (R)0)) return;
@@ -864,8 +864,9 @@ struct Precision {
 
 using namespace egsgpu;
 
-// tally regions past the voxels: energy escaped, energy incident
-constexpr int kExtraRegions = 2;
+// tally entries past the voxels: the energy-bookkeeping regions (escaped,
+// incident), kExtraCopies entries each
+constexpr int kExtraRegions = 2 * kExtraCopies;
 
 struct egsgpu_handle {
     int fp64 = 0;
@@ -897,6 +898,8 @@ struct egsgpu_handle {
     int last_launches = 0;
     // host copies for dose normalisation and the gradient
     std::vector<double> hx, hy, hz, hrhor, hmass;
+    std::vector<double> rho_max;        // largest relative density of each medium (0: absent)
+    double woodcock_factor = 0;         // majorant over the largest attenuation (0: off)
     RanmarRec *rng_saved = nullptr;
     RanluxRec *rlx_saved = nullptr;
     DevGrad dgrad{};
@@ -1116,6 +1119,17 @@ int build_geom(egsgpu_handle *h, const EgsXYZGeometry *g, const std::vector<uint
     if (tol <= 0) tol = sizeof(R) == 4 ? 1.0 / (1 << 21) : 1.0 / 4294967296.0;
     v.btol = (R)tol;
     v.hbtol = v.btol / (R)2;
+    // equally spaced planes: the inverse spacing, for the voxel lookup of
+    // Woodcock tracking (0 when the spacing varies by more than 1e-9)
+    auto inv_spacing = [](const double *b, int n) -> R {
+        const double d = (b[n] - b[0]) / n;
+        for (int i = 0; i < n; i++)
+            if (std::fabs((b[i + 1] - b[i]) - d) > 1e-9 * std::fabs(d)) return (R)0;
+        return (R)(1.0 / d);
+    };
+    v.ixs = inv_spacing(g->xb, g->nx);
+    v.iys = inv_spacing(g->yb, g->ny);
+    v.izs = inv_spacing(g->zb, g->nz);
     std::vector<VoxData<R>> vox((size_t)h->nvox);
     for (int i = 0; i < h->nvox; i++) {
         vox[i].rho = g->rhor ? (R)g->rhor[i] : (R)1;
@@ -1737,8 +1751,10 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->hz.assign(geom->zb, geom->zb + geom->nz + 1);
     h->hrhor.assign(h->nvox, 1.0);
     h->hmass.assign(h->nvox, 0.0);
+    h->rho_max.assign(nmed, 0.0);
     for (int i = 0; i < h->nvox; i++) {
         if (geom->rhor) h->hrhor[i] = geom->rhor[i];
+        h->rho_max[med[i]] = std::max(h->rho_max[med[i]], h->hrhor[i]);
         int iz = i / (geom->nx * geom->ny), iy = (i / geom->nx) % geom->ny, ix = i % geom->nx;
         double vol = (geom->xb[ix + 1] - geom->xb[ix]) * (geom->yb[iy + 1] - geom->yb[iy]) *
                      (geom->zb[iz + 1] - geom->zb[iz]);
@@ -1913,6 +1929,11 @@ int egsgpu_run_batched(egsgpu_handle *const *jobs, int32_t njobs, const int64_t 
     }
     if (h->schedule != EGSGPU_SCHED_EVENT) {
         set_error("a batch runs on the event schedule of job 0");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    if (h->P.woodcock) {
+        set_error("Woodcock tracking needs a majorant for each phantom; a batch shares job "
+                  "0's: run batches with voxel-by-voxel tracking");
         return EGSGPU_E_UNSUPPORTED;
     }
     int rc = ensure_lanes(h);
@@ -2093,6 +2114,93 @@ int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice) {
         choice[3] = h->ev_batch; choice[4] = h->ev_ksteps;
     }
     return done(EGSGPU_OK);
+}
+
+namespace {
+// The majorant of Woodcock tracking on a grid of kWoodcockN intervals of
+// ln E from the lowest AP to the highest UP of the media present: in each
+// interval, factor times the largest attenuation coefficient any voxel can
+// have, rho_max(m) / (gmfp_m(E) cohe_m(E)) evaluated exactly as the
+// kernels do (the piecewise-linear tables), at the interval's ends and at
+// every table node inside it, between which it is monotone.
+constexpr int kWoodcockN = 4096;
+int build_majorant(egsgpu_handle *h, double factor) {
+    const Blob &b = h->tab->blob;
+    const int nmed = (int)h->rho_max.size();
+    double lo = 1e300, hi = -1e300;
+    for (int m = 0; m < nmed; m++) {
+        if (h->rho_max[m] <= 0) continue;
+        lo = std::min(lo, std::log(b.f1(Blob::mk(m, "ap"))));
+        const double ge0 = b.f1(Blob::mk(m, "ge0")), ge1 = b.f1(Blob::mk(m, "ge1"));
+        hi = std::max(hi, (kGE - ge0) / ge1);
+    }
+    if (!(hi > lo)) {
+        set_error("no medium with a positive density: no majorant");
+        return EGSGPU_E_ARG;
+    }
+    const double w1 = kWoodcockN / (hi - lo), w0 = 1 - w1 * lo;
+    std::vector<double> maj(kWoodcockN, 0.0);
+    auto add = [&](int L, double v) {
+        L = std::min(std::max(L, 1), kWoodcockN);
+        maj[L - 1] = std::max(maj[L - 1], v);
+    };
+    for (int m = 0; m < nmed; m++) {
+        const double rmax = h->rho_max[m];
+        if (rmax <= 0) continue;
+        const double ge0 = b.f1(Blob::mk(m, "ge0")), ge1 = b.f1(Blob::mk(m, "ge1"));
+        const auto &g0 = b.F(Blob::mk(m, "gmfp0")), &g1 = b.F(Blob::mk(m, "gmfp1"));
+        const auto &c0 = b.F(Blob::mk(m, "cohe0")), &c1 = b.F(Blob::mk(m, "cohe1"));
+        auto mu_at = [&](double x, int node) {   // node: the table interval (1-based)
+            node = std::min(std::max(node, 1), kGE);
+            double gm = g1[node - 1] * x + g0[node - 1];
+            double ch = h->P.rayleigh ? c1[node - 1] * x + c0[node - 1] : 1.0;
+            return rmax / (gm * ch);
+        };
+        auto mu = [&](double x) { return mu_at(x, (int)(ge1 * x + ge0)); };
+        for (int L = 1; L <= kWoodcockN; L++) {
+            const double xa = (L - w0) / w1, xb = (L + 1 - w0) / w1;
+            add(L, mu(xa));
+            add(L, mu(xb));
+        }
+        for (int j = 1; j <= kGE; j++) {   // table nodes, both one-sided limits
+            const double x = (j - ge0) / ge1;
+            const int L = (int)(w1 * x + w0);
+            add(L, mu_at(x, j));
+            add(L, mu_at(x, j - 1));
+        }
+    }
+    for (double &v : maj) v *= factor;
+    if (h->fp64) {
+        std::vector<double> v(maj);
+        CK(upload(h, v.data(), v.size(), &h->pd.T.wmaj));
+        h->pd.T.wm0 = w0; h->pd.T.wm1 = w1; h->pd.T.wmn = kWoodcockN;
+    } else {
+        std::vector<float> v(maj.begin(), maj.end());
+        CK(upload(h, v.data(), v.size(), &h->pf.T.wmaj));
+        h->pf.T.wm0 = (float)w0; h->pf.T.wm1 = (float)w1; h->pf.T.wmn = kWoodcockN;
+    }
+    return EGSGPU_OK;
+}
+}  // namespace
+
+int egsgpu_set_photon_tracking(egsgpu_handle *h, int32_t mode, double factor) {
+    EGS_ARG(h, "handle");
+    EGS_ARG(mode == EGSGPU_TRACK_VOXEL || mode == EGSGPU_TRACK_WOODCOCK, "tracking mode");
+    if (mode == EGSGPU_TRACK_VOXEL) {
+        h->P.woodcock = 0;
+        return EGSGPU_OK;
+    }
+    if (factor == 0) factor = 1.25;
+    EGS_ARG(factor >= 1.0 && factor <= 100.0, "majorant factor 1..100 (0 = 1.25)");
+    if (h->howfarless || h->P.n_split) {
+        set_error("Woodcock tracking is not combined with HOWFARLESS or photon splitting, "
+                  "which step photons along their own paths");
+        return EGSGPU_E_UNSUPPORTED;
+    }
+    CK(build_majorant(h, factor));
+    h->woodcock_factor = factor;
+    h->P.woodcock = 1;
+    return EGSGPU_OK;
 }
 
 int egsgpu_run(egsgpu_handle *h, int64_t nhist) {
@@ -2443,10 +2551,13 @@ int egsgpu_get_energy_balance(egsgpu_handle *h, double out[4]) {
     double s[kExtraRegions], s2[kExtraRegions];
     EGS_HIP(hipMemcpy(s, h->tally.sum + h->nvox, sizeof(s), hipMemcpyDeviceToHost));
     EGS_HIP(hipMemcpy(s2, h->tally.sum2 + h->nvox, sizeof(s2), hipMemcpyDeviceToHost));
-    out[0] = s[1];
-    out[1] = s2[1];
-    out[2] = s[0];
-    out[3] = s2[0];
+    for (int k = 0; k < 4; k++) out[k] = 0;
+    for (int c = 0; c < kExtraCopies; c++) {   // region 0 escaped, 1 incident
+        out[2] += s[c];
+        out[3] += s2[c];
+        out[0] += s[kExtraCopies + c];
+        out[1] += s2[kExtraCopies + c];
+    }
     return EGSGPU_OK;
 }
 

@@ -19,6 +19,9 @@ struct GeomView {
     int nx, ny, nz, nxy;
     const R *xp, *yp, *zp;       // plane positions (nx+1, ny+1, nz+1)
     R btol, hbtol;
+    // equally spaced planes on every axis: 1 / spacing per axis (0 if the
+    // planes of that axis are not equally spaced)
+    R ixs, iys, izs;
 };
 
 template <typename R>
@@ -177,6 +180,27 @@ __host__ __device__ inline int geom_locate(const GeomView<R> &g, R x, R y, R z) 
     iy = iy < 0 ? 0 : (iy >= g.ny ? g.ny - 1 : iy);
     iz = iz < 0 ? 0 : (iz >= g.nz ? g.nz - 1 : iz);
     return ix + iy * g.nx + iz * g.nxy;
+}
+
+// geom_locate with a first guess from the spacing when the planes of an
+// axis are equally spaced: the guess is moved by at most one plane so that
+// the result is the binary search's (x <= p[i+1] belongs to voxel i).
+template <typename R>
+__host__ __device__ inline int axis_find(R x, int n, const R *__restrict__ p, R inv) {
+    if (inv > 0) {
+        int i = (int)((x - p[0]) * inv);
+        i = i < 0 ? 0 : (i >= n ? n - 1 : i);
+        if (i > 0 && x <= p[i]) i--;
+        else if (i < n - 1 && x > p[i + 1]) i++;
+        return i;
+    }
+    int i = planes_find(x, n, p);
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+template <typename R>
+__host__ __device__ inline int geom_locate_fast(const GeomView<R> &g, R x, R y, R z) {
+    return axis_find(x, g.nx, g.xp, g.ixs) + axis_find(y, g.ny, g.yp, g.iys) * g.nx +
+           axis_find(z, g.nz, g.zp, g.izs) * g.nxy;
 }
 
 // howfarFromOut: entry into the grid from outside (t overwritten only on a

@@ -308,6 +308,67 @@ int egsgpu_ranlux_fill(int32_t lux, int32_t seed, int32_t nstreams, int64_t n,
     return EGSGPU_OK;
 }
 
+// ---- FP64 atomic rate ---------------------------------------------------
+// Throughput of the FP64 atomic adds the tallies and the density-gradient
+// sweep are made of.  Every thread issues `per` adds of 1.0 into a table of
+// naddr doubles; pattern 0 spreads them over the table by a multiplicative
+// hash (the tallies' access pattern: unrelated voxels per lane), pattern 1
+// sends all lanes of a wave to one address per add (the worst case of a
+// shared counter), pattern 2 gives the lanes of a wave 64 consecutive
+// addresses (one coalesced atomic per wave).
+__global__ void __launch_bounds__(256) k_atomic_rate(double *t, uint32_t naddr, int per,
+                                                     int pattern) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wave = g >> 6, lane = g & 63;
+    for (int i = 0; i < per; i++) {
+        uint32_t a;
+        if (pattern == 0) a = (g * 2654435761u + (uint32_t)i * 40503u) % naddr;
+        else if (pattern == 1) a = (wave * 97u + (uint32_t)i) % naddr;
+        else a = ((wave * 64u + (uint32_t)i * 4096u) + lane) % naddr;
+        atomicAdd(t + a, 1.0);
+    }
+}
+
+int egsgpu_atomic_rate(int32_t pattern, int32_t naddr, int64_t nops, double *ms,
+                       double *check_sum) {
+    EGS_ARG(pattern >= 0 && pattern <= 2, "pattern 0..2");
+    EGS_ARG(naddr >= 64 && nops > 0 && ms && check_sum, "naddr >= 64, nops > 0, outputs");
+    int rc = require_device();
+    if (rc) return rc;
+    const int per = 64;
+    const int64_t threads = (nops + per - 1) / per;
+    EGS_ARG(threads <= ((int64_t)1 << 31) - 256, "nops too large");
+    const int blocks = (int)((threads + 255) / 256);
+    double *t = nullptr;
+    EGS_HIP(hipMalloc(&t, sizeof(double) * (size_t)naddr));
+    EGS_HIP(hipMemset(t, 0, sizeof(double) * (size_t)naddr));
+    hipEvent_t e0, e1;
+    EGS_HIP(hipEventCreate(&e0));
+    EGS_HIP(hipEventCreate(&e1));
+    // one untimed launch brings the table into the caches
+    hipLaunchKernelGGL(k_atomic_rate, dim3(blocks), dim3(256), 0, 0, t, (uint32_t)naddr, per,
+                       pattern);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipEventRecord(e0, 0));
+    hipLaunchKernelGGL(k_atomic_rate, dim3(blocks), dim3(256), 0, 0, t, (uint32_t)naddr, per,
+                       pattern);
+    EGS_HIP(hipGetLastError());
+    EGS_HIP(hipEventRecord(e1, 0));
+    EGS_HIP(hipEventSynchronize(e1));
+    float f = 0;
+    EGS_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    std::vector<double> h((size_t)naddr);
+    EGS_HIP(hipMemcpy(h.data(), t, sizeof(double) * (size_t)naddr, hipMemcpyDeviceToHost));
+    double sum = 0;
+    for (double v : h) sum += v;
+    *check_sum = sum;   // = 2 * blocks * 256 * per: every add landed
+    EGS_HIP(hipEventDestroy(e0));
+    EGS_HIP(hipEventDestroy(e1));
+    EGS_HIP(hipFree(t));
+    return EGSGPU_OK;
+}
+
 int egsgpu_trace_rays(const EgsXYZGeometry *g, int32_t nrays, const double *x,
                       const double *u, int32_t max_steps, int32_t precision,
                       int32_t *regions, double *steps, int32_t *nsteps) {

@@ -37,6 +37,13 @@ __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
 // LUX = false: ranmar streams; true: ranlux streams.
 // HFL = true: DOSXYZnrc's HOWFARLESS transport for a homogeneous phantom.
 // SPLIT = true: DOSXYZnrc's photon splitting (P.n_split > 1).
+// The kernels must inline every member function they call: a call that is
+// not inlined passes the shower by address, which moves the whole state
+// from registers to scratch memory (about 1.5 KB per lane).  The helpers
+// with several call sites, and the large phases the history kernels call
+// (photon_trans, photon_interact, msdist_pII, ...), whose size the
+// inliner's budget would otherwise refuse, are forced inline; the compiler
+// inlines the rest.
 template <typename R, bool EV = false, bool LUX = false, bool HFL = false, bool SPLIT = false>
 struct Shower {
     const DevTables<R> &T;
@@ -169,7 +176,7 @@ struct Shower {
     // this lane's history table (history schedule)
     __device__ __forceinline__ SlotEntry *hent() const { return L.hent + (size_t)lane * L.hcap; }
     __device__ __forceinline__ int *hlist() const { return L.hlist + (size_t)lane * L.hcap; }
-    __device__ void hash_add(int vox, double e) {
+    __device__ __forceinline__ void hash_add(int vox, double e) {
         SlotEntry *hent = this->hent();
         int *hlist = this->hlist();
         unsigned key = (unsigned)vox + 1u;
@@ -193,19 +200,57 @@ struct Shower {
             flush_table();
         }
     }
-    __device__ void flush_table() {
+    __device__ __forceinline__ void flush_table() {
         SlotEntry *hent = this->hent();
         int *hlist = this->hlist();
         for (int k = 0; k < hused; k++) {
             int s = hlist[k];
-            int vox = hent[s].key - 1;
-            double e = hent[s].val;
-            atomicAdd(tsum + vox, e);
-            atomicAdd(tsum2 + vox, e * e);
+            tally_add(hent[s].key - 1, hent[s].val);
             hent[s].key = 0;
         }
         c_flush += hused;
         hused = 0;
+    }
+    // The history's sum e of region r into the tallies (sum and sum of
+    // squares).  The dose regions are the voxels; the bookkeeping regions
+    // past them are spread over kExtraCopies copies (by lane), so that the
+    // end of every history does not add into the same two addresses.
+    __device__ __forceinline__ void tally_add(int r, double e) {
+        const int nv = Gp->nxy * Gp->nz;
+        const size_t i = r < nv ? (size_t)r
+                                : (size_t)nv + (size_t)(r - nv) * kExtraCopies +
+                                      (size_t)(lane & (kExtraCopies - 1));
+        atomicAdd(tsum + i, e);
+        atomicAdd(tsum2 + i, e * e);
+    }
+    // History schedule: the first kSmall regions a history scores into are
+    // kept in registers (a kerma-mode history touches a handful); later ones
+    // go to the lane's table in memory.  A region is in one or the other.
+    static constexpr int kSmall = EV ? 1 : 8;
+    int sv[kSmall];
+    double se[kSmall];
+    int sn = 0;
+    __device__ __forceinline__ void small_add(int r, double e) {
+        bool done = false;
+#pragma unroll
+        for (int i = 0; i < kSmall; i++)
+            if (!done && i < sn && sv[i] == r) { se[i] += e; done = true; }
+        if (done) return;
+        if (sn < kSmall) {
+#pragma unroll
+            for (int i = 0; i < kSmall; i++)
+                if (i == sn) { sv[i] = r; se[i] = e; }
+            sn++;
+            return;
+        }
+        hash_add(r, e);
+    }
+    __device__ __forceinline__ void flush_small() {
+#pragma unroll
+        for (int i = 0; i < kSmall; i++)
+            if (i < sn) tally_add(sv[i], se[i]);
+        c_flush += sn;
+        sn = 0;
     }
     // ausgab with iarg <= 4: score edep*wt in region ir (>= 0)
     __device__ __forceinline__ void score(int ir, R edep, R wt) {
@@ -223,7 +268,7 @@ struct Shower {
         if (ir == cvox) { cval += e; return; }
         if (cvox >= 0) {
             if constexpr (EV) slot_add(cvox, cval);
-            else hash_add(cvox, cval);
+            else small_add(cvox, cval);
         }
         cvox = ir;
         cval = e;
@@ -244,7 +289,7 @@ struct Shower {
         score_raw(Gp->nxy * Gp->nz + k, (double)e * (double)q.wt);
     }
     // event schedule: add e to voxel vox of the current slot's table
-    __device__ void slot_add(int vox, double e) {
+    __device__ __forceinline__ void slot_add(int vox, double e) {
         const DevSlots &S = *slots;
         int key = vox + 1;
         size_t base = (size_t)slot * S.hcap;
@@ -268,11 +313,10 @@ struct Shower {
         }
         // table full: score this part outside the history grouping (counted)
         c_hovf++;
-        atomicAdd(tsum + vox, e);
-        atomicAdd(tsum2 + vox, e * e);
+        tally_add(vox, e);
     }
     // event schedule: this lane is done with its particles of the slot
-    __device__ void slot_release() {
+    __device__ __forceinline__ void slot_release() {
         if (cvox >= 0) slot_add(cvox, cval);
         cvox = -1;
         cval = 0;
@@ -288,8 +332,7 @@ struct Shower {
                                           __HIP_MEMORY_SCOPE_AGENT);
                 int key = atomicExch(&S.ent[base + h].key, 0);
                 double e = atomicExch(&S.ent[base + h].val, 0.0);
-                atomicAdd(tsum + key - 1, e);
-                atomicAdd(tsum2 + key - 1, e * e);
+                tally_add(key - 1, e);
             }
             c_flush += n;
             unsigned long long at = atomicAdd(&ev_ctl->free_tail, 1ull);
@@ -300,10 +343,11 @@ struct Shower {
     }
     __device__ __forceinline__ void end_history() {
         hsteps = 0;
-        if (cvox >= 0) hash_add(cvox, cval);
+        if (cvox >= 0) small_add(cvox, cval);
         cvox = -1;
         cval = 0;
-        flush_table();
+        flush_small();
+        if (hused) flush_table();
     }
 
     // ------------------------------------------------------------ stack
@@ -419,13 +463,13 @@ struct Shower {
         if (P.rayleigh) p_cohfac = g[kCohe + 1] * p_gle + g[kCohe];
     }
     // kPEnter -> kPNewE, or the photon is absorbed below PCUT
-    __device__ void photon_enter() {
+    __device__ __forceinline__ void photon_enter() {
         { VoxData<R> d = vox[p.ir]; p_med = d.med; p_rhof = d.rho; }
         if (p.e <= (R)P.pcut) { score(p.ir, p.e, p.wt); pop(); return; }
         pph = kPNewE;
     }
     // kPNewE -> kPTrans
-    __device__ void photon_newe() {
+    __device__ __forceinline__ void photon_newe() {
         if (p.wt == 0) { pop(); pph = kPEnter; return; }
         p_gle = dlog(p.e);
         R rnno35 = rnd();
@@ -437,7 +481,7 @@ struct Shower {
     }
     // kPTrans: up to kmax voxel steps; at the interaction point the type is
     // sampled (kPRayl..kPPair); a photon leaving the grid is discarded
-    __device__ void photon_trans(int kmax) {
+    __device__ __forceinline__ void photon_trans(int kmax) {
         const GeomView<R> &G = *Gp;
         int iz = p.ir / G.nxy, iy = p.ir - iz * G.nxy;
         int ix = iy;
@@ -464,6 +508,43 @@ struct Shower {
                 if (out) { score_extra(kEscaped, p); pop(); pph = kPEnter; return; }
                 p.ir = geom_locate(G, p.x, p.y, p.z);
                 if (p_dpmfp <= (R)1.e-8) { photon_choose(); return; }
+                continue;
+            }
+            if (P.woodcock) {
+                // Woodcock (delta) tracking: a flight of p_dpmfp mean free
+                // paths of the majorant attenuation coefficient ends at a
+                // tentative collision, which is a real one with probability
+                // mu / mu_maj (mu of the voxel there); flights do not stop
+                // at voxel faces.  Same expected dose as the voxel-by-voxel
+                // flight below.
+                int wl = (int)(T.wm1 * p_gle + T.wm0);
+                wl = wl < 1 ? 1 : (wl > T.wmn ? T.wmn : wl);
+                const R mumaj = T.wmaj[wl - 1];
+                const R dist = p_dpmfp / mumaj;
+                const R tb = geom_box_exit(G, p.x, p.y, p.z, p.u, p.v, p.w);
+                if (tb <= dist) {   // the flight leaves the grid
+                    score_extra(kEscaped, p);
+                    pop();
+                    pph = kPEnter;
+                    return;
+                }
+                p.x = p.x + p.u * dist;
+                p.y = p.y + p.v * dist;
+                p.z = p.z + p.w * dist;
+                p.dnear = 0;
+                p.ir = geom_locate_fast(G, p.x, p.y, p.z);
+                c_cross++;
+                { VoxData<R> d = vox[p.ir]; p_med = d.med; p_rhof = d.rho; }
+                photon_medium();
+                const R mu = p_rhof / (p_gmfpr0 * p_cohfac);
+                if (mu > mumaj) c_ovf++;   // a majorant below mu (never expected)
+                const R r = rnd();
+                if (r * mumaj < mu) { photon_choose(); return; }   // real collision
+                // virtual collision: d log(1 - mu/mu_maj) / d rho
+                if (grad) log_event(1, p.ir, -(mu / p_rhof) / (mumaj - mu));
+                R rn = rnd();
+                if (rn == 0) rn = (R)1.e-30;
+                p_dpmfp = -dlog(rn);
                 continue;
             }
             R gmfp = p_gmfpr0 / p_rhof;
@@ -515,7 +596,7 @@ struct Shower {
     }
     // kPRayl..kPPair: the interaction; then the photon continues
     // (:PNEWENERGY:) or the stack top changed (kPEnter)
-    __device__ void photon_interact() {
+    __device__ __forceinline__ void photon_interact() {
         const int m = p_med;
         const MedScalars<R> &M = T.ms[m];
         if (pph == kPRayl) {
@@ -1088,7 +1169,7 @@ struct Shower {
 
     // RANGE_ECUTRR of DOSXYZnrc (dosxyznrc.mortran:2514-2545): the
     // electron range from ECUT down to AE in medium m at its default density
-    __device__ R range_ecut(int m, const MedScalars<R> &M) const {
+    __device__ __forceinline__ R range_ecut(int m, const MedScalars<R> &M) const {
         const R eke = (R)P.ecut - (R)0.5109989461;
         if (!(eke > earr(m, 0))) return 0;
         R elk = dlog(eke);
@@ -1106,19 +1187,15 @@ struct Shower {
     __device__ __forceinline__ void ecut_discard(int lelec, R peie) {
         const R RM = (R)0.5109989461;
         R edep = peie - RM;
-        if (p.ir < 0) {
-            score_extra(kEscaped, p);
-            if (lelec > 0 && edep < peie) {
-                const int n0 = np;
-                annih_at_rest();
-                while (np >= n0) pop();
-                return;
-            }
-            pop();
+        const bool outside = p.ir < 0;
+        if (outside) score_extra(kEscaped, p);
+        else score(p.ir, edep, p.wt);
+        if (lelec > 0 && edep < peie) {
+            const int n0 = np;
+            annih_at_rest();
+            if (outside) while (np >= n0) pop();
             return;
         }
-        score(p.ir, edep, p.wt);
-        if (lelec > 0 && edep < peie) { annih_at_rest(); return; }
         pop();
     }
 
@@ -1668,7 +1745,7 @@ struct Shower {
     }
 
     // msdist_pII (egsnrc.mortran:4050-4308), spin effects off
-    __device__ void msdist_pII(const MedScalars<R> &M, R e0, R el, R tustep, R rhof,
+    __device__ __forceinline__ void msdist_pII(const MedScalars<R> &M, R e0, R el, R tustep, R rhof,
                                R &us, R &vs, R &ws, R &xf, R &yf, R &zf, R &ustep) {
         const R PRM = (R)0.5109989461, RMT2 = 2 * PRM, RMSQ = PRM * PRM;
         R blccc = M.blcc, xcccc = M.xcc;

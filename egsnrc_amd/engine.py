@@ -88,6 +88,10 @@ class Simulation:
         assert all(len(a) == n for a in arrs)
         check(_lib.lib().egsgpu_set_phsp(self.h, n, *[ptr(a) for a in arrs], ptr(qq)))
 
+    def set_woodcock(self, factor=0.0):
+        """Photon flights by Woodcock tracking (egsgpu_set_photon_tracking)."""
+        check(_lib.lib().egsgpu_set_photon_tracking(self.h, 1, float(factor)))
+
     def set_ranlux(self, luxury, seed):
         """Drive the transport with ranlux streams: stream s is
         init_ranlux(luxury, seed + s)."""

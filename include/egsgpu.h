@@ -267,6 +267,26 @@ int egsgpu_set_event_graph(egsgpu_handle *h, int32_t on);
  * schedule.  No counterpart in the reference. */
 int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice);
 
+/* How photons fly between interactions.
+ *  EGSGPU_TRACK_VOXEL (default): voxel by voxel, each step ending at the
+ *    next voxel face (howfar) or at the interaction point, as PHOTON does
+ *    (egsnrc.mortran:6439-6643) with EGS_XYZGeometry::howfar.
+ *  EGSGPU_TRACK_WOODCOCK: Woodcock (delta) tracking: a flight of one mean
+ *    free path of a majorant attenuation coefficient mu_maj(E) -- factor
+ *    times the largest mu any voxel of the simulation has at E (0 = 1.25)
+ *    -- ends at a tentative collision that is a real interaction with
+ *    probability mu/mu_maj of the voxel there.  The same expected dose
+ *    (statistically identical), with a few voxel lookups per photon
+ *    instead of one step per voxel crossed; for the density gradient the
+ *    score function then sits at the tentative collisions.  The random
+ *    numbers of a history differ from the voxel-by-voxel flight.  Not with
+ *    HOWFARLESS, photon splitting or batched runs.  No counterpart in the
+ *    reference.  Counter 5 also counts tentative collisions where mu
+ *    exceeded the majorant (none expected). */
+#define EGSGPU_TRACK_VOXEL    0
+#define EGSGPU_TRACK_WOODCOCK 1
+int egsgpu_set_photon_tracking(egsgpu_handle *h, int32_t mode, double factor);
+
 /* Seed all streams: ranmar initial seeds (ixx, jxx).  Re-seeding resets
  * the streams; tallies are kept.  Same as
  * egsgpu_set_rng(h, EGSGPU_RNG_RANMAR, ixx, jxx). */
@@ -436,6 +456,16 @@ int egsgpu_trace_rays(const EgsXYZGeometry *g, int32_t nrays,
                       const double *x, const double *u, int32_t max_steps,
                       int32_t precision, int32_t *regions, double *steps,
                       int32_t *nsteps);
+
+/* Measurement hook, no counterpart in the reference: the rate of FP64
+ * atomic adds on this device.  Two launches of about nops adds of 1.0 each
+ * into a table of naddr doubles (the second is timed: *ms); pattern 0
+ * scatters the adds over the table by a hash (the tallies' pattern),
+ * 1 sends the 64 lanes of a wave to one address, 2 gives them 64
+ * consecutive addresses.  *check_sum = the table's sum after both
+ * launches (2 x the adds issued). */
+int egsgpu_atomic_rate(int32_t pattern, int32_t naddr, int64_t nops, double *ms,
+                       double *check_sum);
 
 /* hownear for nrays points with known regions (egs_nd_geometry.h:1106). */
 int egsgpu_hownear(const EgsXYZGeometry *g, int32_t n, const double *x,
