@@ -1511,6 +1511,9 @@ struct Shower {
             de = eloss_g(m, M, pos, rhof, range, tvstep, eke, elke, lelke);
         } else {
             tvstep = tustep;
+            // a single-scattering step that stays clear of the boundary
+            // loses energy too (egsnrc.mortran:1444-1452)
+            if (!callmsdist) de = eloss_g(m, M, pos, rhof, range, tvstep, eke, elke, lelke);
         }
         R save_de = de;
         R edep = de;
