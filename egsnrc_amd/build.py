@@ -56,8 +56,11 @@ def build_app():
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
-                  if f.endswith(".hip") or f.endswith(".cpp"))
+    srcs = [os.path.join(CSRC, f) for f in os.listdir(CSRC)
+            if f.endswith(".hip") or f.endswith(".cpp")]
+    # the charged-particle and history kernels take longest: start them first
+    order = ("_el_", "_split_", "_hist_", "_elw_", "_kerma_", "_ev_", "engine")
+    srcs.sort(key=lambda f: next((i for i, k in enumerate(order) if k in f), len(order)))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

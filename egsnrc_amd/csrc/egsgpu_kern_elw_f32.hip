@@ -1,0 +1,7 @@
+// Explicit instantiations of the event-schedule charged-particle kernels built for 3 and 4 waves per SIMD in float;
+// the kernel lists and templates are in egsgpu_kernels.h.
+#include "egsgpu_kernels.h"
+
+namespace egsgpu {
+EGSGPU_ELECTRON_WAVE_KERNELS(EGSGPU_INSTANTIATE, float)
+}  // namespace egsgpu

@@ -267,11 +267,42 @@ struct Shower {
     __device__ __forceinline__ void score_raw(int ir, double e) {
         if (ir == cvox) { cval += e; return; }
         if (cvox >= 0) {
-            if constexpr (EV) slot_add(cvox, cval);
+            if constexpr (EV) pend_add(cvox, cval);
             else small_add(cvox, cval);
         }
         cvox = ir;
         cval = e;
+    }
+    // Event schedule: a deposit that leaves the run-length cache waits in a
+    // short register queue and is added to the slot table by flush_pend()
+    // at the top of the kernel's next loop turn, where the transport
+    // temporaries are dead and the wave has re-converged (adding it in the
+    // middle of a step raised the charged-particle kernel above its
+    // register budget).  One turn scores into at most three regions; a
+    // fourth waits too, and a deposit beyond that (none is expected) is
+    // added outside the history grouping and counted as a table overflow.
+    static constexpr int kPend = EV ? 4 : 1;
+    int pv[kPend];
+    double pe[kPend];
+    int pn = 0;
+    __device__ __forceinline__ void pend_add(int v, double e) {
+        if (pn < kPend) {
+#pragma unroll
+            for (int i = 0; i < kPend; i++)
+                if (i == pn) { pv[i] = v; pe[i] = e; }
+            pn++;
+            return;
+        }
+        c_hovf++;
+        tally_add(v, e);
+    }
+    __device__ __forceinline__ void flush_pend() {
+        if constexpr (EV) {
+#pragma unroll
+            for (int i = 0; i < kPend; i++)
+                if (i < pn) slot_add(pv[i], pe[i]);
+            pn = 0;
+        }
     }
     // Energy bookkeeping of the simulation, scored like deposits into two
     // regions past the voxels so that each has a history-by-history sum:
@@ -317,6 +348,7 @@ struct Shower {
     }
     // event schedule: this lane is done with its particles of the slot
     __device__ __forceinline__ void slot_release() {
+        flush_pend();
         if (cvox >= 0) slot_add(cvox, cval);
         cvox = -1;
         cval = 0;
@@ -1508,13 +1540,13 @@ struct Shower {
         if (callhowfar) {
             tvstep = vstep;
             if (tvstep != tustep) dosingle = false;
-            de = eloss_g(m, M, pos, rhof, range, tvstep, eke, elke, lelke);
         } else {
             tvstep = tustep;
-            // a single-scattering step that stays clear of the boundary
-            // loses energy too (egsnrc.mortran:1444-1452)
-            if (!callmsdist) de = eloss_g(m, M, pos, rhof, range, tvstep, eke, elke, lelke);
         }
+        // the energy loss of every step that did not call msdist: a step
+        // cut by howfar and a single-scattering step that stays clear of the
+        // boundary alike (egsnrc.mortran:1437-1452)
+        if (!callmsdist) de = eloss_g(m, M, pos, rhof, range, tvstep, eke, elke, lelke);
         R save_de = de;
         R edep = de;
         R ekef = eke - de;
