@@ -121,6 +121,7 @@ def lib():
         "egsgpu_run_batched": [vp, i32, vp],
         "egsgpu_autotune": [vp, i64, vp],
         "egsgpu_set_event_graph": [vp, i32],
+        "egsgpu_graph_stats": [vp, vp, vp, vp],
         "egsgpu_restore": [vp, vp, C.c_size_t],
     }
     for name, args in sig.items():

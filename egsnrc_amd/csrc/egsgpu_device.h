@@ -193,6 +193,8 @@ struct EvCtl {
     long long plan_base;            // first history index
     int plan_n;                     // and number of histories
     int stall;                      // no free slot while all banks are empty
+    int done;                       // graph-driven runs: the run has ended
+    int iters;                      // graph-driven runs: iterations that had work
 };
 
 // One simulation of a batch (egsgpu_run_batched): its geometry, voxels,

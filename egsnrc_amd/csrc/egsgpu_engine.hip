@@ -53,15 +53,25 @@ __global__ void k_ev_ctl(EvCtl *c, int b) {
 
 // Plan the next source launch of a graph-driven run: as many new histories
 // as there are free slots, histories left and room in the bank they go to.
+// Once no history is left to start and every bank is empty the run has
+// ended: the plan sets `done`, and the kernels of the iterations still
+// queued in the graph return at once.
 __global__ void k_ev_plan(EvCtl *c, int bank, int cap) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (c->done) { c->plan_n = 0; return; }
+    if (c->remaining == 0 && c->n[0] == 0 && c->n[1] == 0 && c->n[2] == 0 && c->n[3] == 0) {
+        c->done = 1;
+        c->plan_n = 0;
+        return;
+    }
+    c->iters++;
     unsigned long long fr = c->free_tail - c->free_head;
     long long inj = (long long)fr < c->remaining ? (long long)fr : c->remaining;
     long long room = (long long)cap - c->n[bank];
     inj = inj < room ? inj : room;
     inj = inj > 0 ? inj : 0;
     bool empty = c->n[0] == 0 && c->n[1] == 0 && c->n[2] == 0 && c->n[3] == 0;
-    if (inj == 0 && empty && c->remaining > 0) c->stall = 1;
+    if (inj == 0 && empty && c->remaining > 0) { c->stall = 1; c->done = 1; }
     c->plan_n = (int)inj;
     c->plan_head = c->free_head;
     c->plan_base = c->hist_base;
@@ -76,6 +86,8 @@ __global__ void k_ev_setrun(EvCtl *c, long long nhist, unsigned long long free_h
     c->hist_base = 0;
     c->free_head = free_head;
     c->stall = 0;
+    c->done = 0;
+    c->iters = 0;
 }
 
 __global__ void k_ev_init(DevSlots S, EvCtl *c) {
@@ -132,6 +144,8 @@ using namespace egsgpu;
 // incident), kExtraCopies entries each
 constexpr int kExtraRegions = 2 * kExtraCopies;
 
+constexpr int kMaxGraphIters = 512;   // iterations one run graph may hold
+
 struct egsgpu_handle {
     int fp64 = 0;
     int nvox = 0;
@@ -151,6 +165,9 @@ struct egsgpu_handle {
     int pc = 0, ec = 2;                 // banks to drain next (photon, charged)
     int *slot_job = nullptr;            // [nslots] job of each slot (batched runs)
     bool use_graph = true;              // event schedule: replay captured iterations
+    int ev_graph_iters = 16;            // iterations captured in the run's graph
+    int graph_launches = 0;             // graph launches of the last run
+    int graph_iters_used = 0;           // iterations with work in the last run
     void *job_self = nullptr;           // device JobDesc of this simulation alone
     const egsgpu_tables *tab = nullptr; // the tables the simulation was made with
     int ixx = 1802, jxx = 9373;
@@ -726,6 +743,7 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
     a.batch = h->ev_batch;
     a.max_wait = h->ev_max_wait;
     a.defer_below = h->ev_defer;
+    a.gated = 0;
     const dim3 grid((h->nlanes + h->block - 1) / h->block), block(h->block);
     // persistent kernels: launch what stays resident at the chosen variant
     auto resident = [&](int w) {
@@ -832,6 +850,7 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
     a.batch = h->ev_batch;
     a.max_wait = h->ev_max_wait;
     a.defer_below = h->ev_defer;
+    a.gated = 1;      // the kernels return at once after the run has ended
     const dim3 grid((h->nlanes + h->block - 1) / h->block), block(h->block);
     auto resident = [&](int w) {
         int n = std::min(h->nlanes, h->ncu * 4 * 64 * w);
@@ -851,13 +870,18 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
     hipLaunchKernelGGL(k_ev_setrun, dim3(1), dim3(64), 0, h->stream, h->ctl, (long long)nhist,
                        h->free_head);
     EGS_HIP(hipGetLastError());
-    // capture two iterations
+    // One graph holds as many iterations as the run is expected to need
+    // (the count of the previous run, with a margin; an even number, so that
+    // the bank roles are back where they started); the device plans every
+    // iteration and, once the run has ended, the kernels still queued return
+    // at once.  A run that needs more iterations launches the graph again.
+    const int iters = h->ev_graph_iters;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     EGS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int pc = h->pc, ec = h->ec;
     hipError_t cerr = hipSuccess;
-    for (int it = 0; it < 2 && cerr == hipSuccess; it++) {
+    for (int it = 0; it < iters && cerr == hipSuccess; it++) {
         int sb = photon_src ? pc : ec;
         hipLaunchKernelGGL(k_ev_plan, dim3(1), dim3(64), 0, h->stream, h->ctl, sb,
                            P.bank[sb].cap);
@@ -889,12 +913,14 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
     EvCtl c;
     int rc = EGSGPU_OK;
     const unsigned long long ns = (unsigned long long)h->slots.nslots;
-    for (int64_t rep = 0;; rep++) {
+    int launches = 0;
+    for (;;) {
         if (hipGraphLaunch(exec, h->stream) != hipSuccess) {
             set_error("graph launch failed");
             rc = EGSGPU_E_HIP;
             break;
         }
+        launches++;
         if ((rc = read_ctl(h, c))) break;
         if (c.stall) {
             set_error("event schedule: no free history slot with empty banks");
@@ -906,6 +932,13 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
     }
     cleanup();
     if (rc) return rc;
+    h->graph_launches = launches;
+    h->graph_iters_used = c.iters;
+    // size the next run's graph from this run: 25 % more iterations plus
+    // two, rounded up to an even number
+    int want = c.iters + c.iters / 4 + 2;
+    want = std::max(4, std::min(kMaxGraphIters, want + (want & 1)));
+    h->ev_graph_iters = want;
     h->free_head = c.free_head;
     if (c.free_tail - h->free_head != ns) {
         set_error("event schedule ended with %llu of %llu history slots in use",
@@ -1297,6 +1330,15 @@ int egsgpu_set_event_shape(egsgpu_handle *h, int32_t waves_photon, int32_t waves
 int egsgpu_set_event_graph(egsgpu_handle *h, int32_t on) {
     EGS_ARG(h, "handle");
     h->use_graph = on != 0;
+    return EGSGPU_OK;
+}
+
+int egsgpu_graph_stats(egsgpu_handle *h, int32_t *launches, int32_t *iterations,
+                       int32_t *captured) {
+    EGS_ARG(h && launches && iterations && captured, "null argument");
+    *launches = h->graph_launches;
+    *iterations = h->graph_iters_used;
+    *captured = h->ev_graph_iters;
     return EGSGPU_OK;
 }
 

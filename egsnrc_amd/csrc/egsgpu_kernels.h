@@ -455,6 +455,7 @@ struct EvArgs {
     const long long *job_first;     // [njobs+1] first history index of each job
     int njobs;
     long long hist_base;            // index of this source launch's first history
+    int gated;                      // graph-driven: return at once when ctl->done
 };
 
 // Point the shower at the simulation of the history in `slot`.
@@ -541,6 +542,7 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
     const bool live = lane < a.k.L.nlanes;
     Shower<R, true, LUX> sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally,
                             live ? lane : 0, rbuf_all + threadIdx.x);
+    if (a.gated && a.ctl->done) return;
     ev_setup(sh, a);
     const unsigned long long ns = (unsigned long long)a.slots.nslots;
     // a negative count: the plan made on the device by k_ev_plan
@@ -608,6 +610,7 @@ k_ev_photon(EvArgs<R> a) {
     using S = Shower<R, true, LUX>;
     S sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
+    if (a.gated && a.ctl->done) return;
     ev_setup(sh, a);
     const int n = min(a.ctl->n[a.ic], a.cons.cap);
     // deferral starts only after a lane of this wave found the bank empty,
@@ -694,6 +697,7 @@ k_ev_electron(EvArgs<R> a) {
     using S = Shower<R, true, LUX>;
     S sh(a.k.T, a.jobs[0].G, a.jobs[0].vox, a.k.P, a.k.L, a.k.tally, live ? lane : 0,
          rbuf_all + threadIdx.x);
+    if (a.gated && a.ctl->done) return;
     ev_setup(sh, a);
     const int n = min(a.ctl->n[a.ic], a.cons.cap);
     // deferral starts only after a lane of this wave found the bank empty,

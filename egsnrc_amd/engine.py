@@ -175,6 +175,13 @@ class Simulation:
         check(_lib.lib().egsgpu_kernel_time(self.h, kernel, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def graph_stats(self):
+        """The last graph-driven run: (graph launches, iterations with work,
+        iterations the next run's graph holds) (egsgpu_graph_stats)."""
+        v = [C.c_int32(0) for _ in range(3)]
+        check(_lib.lib().egsgpu_graph_stats(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     def close(self):
         if self.h.value:
             _lib.lib().egsgpu_destroy(self.h)

@@ -247,15 +247,24 @@ int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps,
                               int32_t batch, int32_t max_wait,
                               int32_t defer_below);
 
-/* Event schedule driven by a HIP graph (on = 1, the default): one
- * iteration (source planned on the device, photon kernel, charged-particle
- * kernel) is captured per bank parity and the graph is replayed until the
- * run is complete, with one read of the control words per replay; on = 0
+/* Event schedule driven by a HIP graph (on = 1, the default): a run (or a
+ * batch of runs, egsgpu_run_batched) is one launch of a graph of as many
+ * iterations (source planned on the device, photon kernel, charged-particle
+ * kernel) as the previous run needed plus a margin; the device decides when
+ * the run has ended, after which the kernels still queued return at once,
+ * and the host reads the control words once per launch (a run that needs
+ * more iterations than the graph holds launches it again).  on = 0
  * launches the kernels one by one from the host, with HIP events around
  * each launch for egsgpu_kernel_time (in graph mode only the whole run is
  * timed, egsgpu_last_kernel_ms).  Same results either way.
  * No counterpart in the reference. */
 int egsgpu_set_event_graph(egsgpu_handle *h, int32_t on);
+
+/* The last graph-driven run: graph launches it took, iterations that had
+ * work, and iterations the next run's graph will hold.
+ * No counterpart in the reference. */
+int egsgpu_graph_stats(egsgpu_handle *h, int32_t *launches, int32_t *iterations,
+                       int32_t *captured);
 
 /* Autotuner of the event schedule: times short runs of nhist_probe
  * histories while varying, one after the other, the register budget (2 or

@@ -474,9 +474,11 @@ def test_c3_batch256(engine):
     """C3: the 256-job random-field workload (edges 32-160 voxels of 1-5 mm,
     air / lung / water / bone with +-5 % density noise, photon spectra and
     4-20 MeV electron beams) in one batched event schedule, at 1 % of each
-    job's histories, except three small jobs run at full parity size.
-    Every job: its history count, energy balance (incident = deposited +
-    escaped) and no overflow; the three: dose parity with the oracle."""
+    job's histories, except three small jobs run at parity size; the batch
+    is run twice, the second time as one graph launch.  Every job: its
+    history count, energy balance (incident = deposited + escaped) and no
+    overflow; the three: dose parity with the oracle (the engine's tallies
+    hold twice the oracle's histories)."""
     from egsnrc_amd.engine import run_batched
     jobs = workloads.batch_jobs(256, hist_scale=0.01)
     tab = Tables(BLOB521)
@@ -500,13 +502,21 @@ def test_c3_batch256(engine):
     print("batch: %d jobs, %d histories, %d voxels"
           % (len(jobs), nh.sum(), sum(s.nvox for s in sims)))
     run_batched(sims, nh)
+    first = sims[0].graph_stats()
+    # the second run of the batch is one graph launch: its graph was sized
+    # from the first run's iteration count, and the device ended it
+    run_batched(sims, nh)
+    launches, iters, captured = sims[0].graph_stats()
+    print("graph launches: first run %d (%d iterations); second run %d (%d of %d iterations)"
+          % (first[0], first[1], launches, iters, captured))
+    assert launches == 1 and 0 < iters <= captured
     c = sims[0].counters()
     assert c[5] == 0, "stack or bank overflow"
     assert c[6] == 0, "history tally table overflow"
     worst = 0.0
     for j, s in enumerate(sims):
         t = s.tallies()
-        assert t[2] == nh[j]
+        assert t[2] == 2 * nh[j]
         eb = s.energy_balance()
         assert eb["incident"] > 0 and eb["deposited"] > 0
         r = abs(eb["incident"] - eb["deposited"] - eb["escaped"]) / eb["incident"]
