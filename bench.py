@@ -187,12 +187,17 @@ def atomic_bench():
     return out
 
 
-def grad_bench(nhist, lanes=0, woodcock=None):
+def grad_bench(nhist, lanes=0, fd_seeds=4):
     """Density-gradient evaluations/s on the C4 problem: 64^3 x 0.4 cm
     water with densities from a smoothed random field in [0.2, 2.0], 2 MeV
     8x8 cm^2 parallel beam, kerma mode; target dose from a second field
     with a 10 % perturbation.  One evaluation = forward transport + replay
-    of nhist histories (egsgpu_density_grad)."""
+    of nhist histories (egsgpu_density_grad), with Woodcock photon tracking
+    (the default here) and with voxel-by-voxel tracking.  The finite-
+    difference check: the score-function directional derivatives along
+    five smooth random directions against central differences of the loss
+    with common random numbers, h = 3e-2 rho, averaged over fd_seeds seeds
+    (relative L2 of the difference)."""
     from egsnrc_amd import _lib
     from egsnrc_amd.engine import Simulation, Tables, make_source
     from egsnrc_amd import workloads
@@ -203,28 +208,134 @@ def grad_bench(nhist, lanes=0, woodcock=None):
     src = make_source(0, 2.0, x0=-4, x1=4, y0=-4, y1=4, z=-1.0)
     tp = _lib.default_transport(ecut=20.0, kerma=1)
     tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab"))
-    st = Simulation(tab, xb, yb, zb, med, trho, src, tp, nstreams=lanes)
-    if woodcock is not None:
-        st.set_woodcock(woodcock)
-    st.run(min(nhist, 10_000_000))
-    s_, _, nn = st.tallies()
-    st.close()
-    sim = Simulation(tab, xb, yb, zb, med, rhor, src, tp, nstreams=lanes)
-    if woodcock is not None:
-        sim.set_woodcock(woodcock)
-    vol = np.prod([np.diff(b)[0] for b in (xb, yb, zb)])
-    dstar = s_ / nn / (vol * trho)            # water, nominal density 1
-    sim.density_grad(1_000_000, dstar)        # warm-up
+    vol = dx ** 3
+
+    def sim_for(r, seed, woodcock):
+        s = Simulation(tab, xb, yb, zb, med, r, src, tp, nstreams=lanes, schedule="history")
+        if woodcock:
+            s.set_woodcock(0.0)
+        s.seed(seed, 9373)
+        return s
+
+    def dose(r, seed, nh):
+        s = sim_for(r, seed, True)
+        s.run(nh)
+        su, _, nn = s.tallies()
+        s.close()
+        return su / nn / (vol * r)
+
+    dstar = dose(trho, 77, min(nhist, 100_000_000))
+    out = {"workload": "C4: 64^3 x 0.4 cm water, density field [0.2, 2.0], 2 MeV 8x8 cm2 "
+                       "beam, kerma mode, L = sum (D - D*)^2 with D* from a 10% perturbed field",
+           "histories": int(nhist),
+           "note": "one evaluation = forward transport + replay from the saved RNG states"}
+    for key, wc in (("woodcock", True), ("voxel", False)):
+        sim = sim_for(rhor, 1802, wc)
+        sim.density_grad(1_000_000, dstar)        # warm-up
+        t0 = time.perf_counter()
+        g, d, L = sim.density_grad(nhist, dstar)
+        T = time.perf_counter() - t0
+        sim.close()
+        out[key] = {"seconds": T, "evals_per_s": 1.0 / T, "histories_per_s": 2.0 * nhist / T,
+                    "loss": L}
+    out["evals_per_s"] = out["woodcock"]["evals_per_s"]
+    out["photon_tracking"] = "woodcock (majorant x 1.25); voxel-by-voxel beside it"
+    if fd_seeds > 0:
+        dirs = []
+        for k in range(5):
+            f = workloads.grf_density(n, 3.0, 100 + k)
+            dirs.append(rhor * np.transpose((f - f.mean()) / f.std(), (2, 1, 0)).reshape(-1))
+        h = 3e-2
+        sf, fd = [], []
+        for seed in range(1802, 1802 + 1000 * fd_seeds, 1000):
+            s = sim_for(rhor, seed, True)
+            g, _, _ = s.density_grad(nhist, dstar)
+            s.close()
+            sf.append([float(g @ dl) for dl in dirs])
+            fd.append([float((np.sum((dose(rhor + h * dl, seed, nhist) - dstar) ** 2) -
+                              np.sum((dose(rhor - h * dl, seed, nhist) - dstar) ** 2)) / (2 * h))
+                       for dl in dirs])
+        sf, fd = np.mean(sf, 0), np.mean(fd, 0)
+        out["fd_check"] = {"h": h, "seeds": fd_seeds, "directions": 5,
+                           "rel_l2": float(np.linalg.norm(sf - fd) / np.linalg.norm(fd)),
+                           "score_function": sf.tolist(), "finite_differences": fd.tolist()}
+    return out
+
+
+def _rel2(sim):
+    """Mean relative variance of the mean dose over D > 0.5 Dmax."""
+    su, s2, n = sim.tallies()
+    r = su / n
+    v = np.where(s2 / n - r * r > 0, (s2 / n - r * r) / (n - 1), 0.0)
+    m = r > 0.5 * r.max()
+    return float(np.mean(v[m] / (r[m] * r[m])))
+
+
+def c3_compare(hist_scale, lanes=0):
+    """C3 in the batched persistent form against the plain form.
+
+    Batched: the 256 jobs in one egsgpu_run_batched call (one graph launch;
+    a first run of the same batch sizes the graph and is not timed).
+    Plain: every job its own simulation, run alone, with the event schedule
+    and with the history schedule (simulations made outside the timed
+    region).  Efficiency 1/(sigma^2 T) with sigma^2 the mean over jobs of
+    the mean relative variance over D > 0.5 Dmax, so the ratio compares the
+    forms at equal per-voxel sigma."""
+    from egsnrc_amd import _lib
+    from egsnrc_amd.engine import Simulation, Tables, make_source, run_batched
+    from egsnrc_amd import workloads
+    tab = Tables(os.path.join(ROOT, "egsnrc_amd", "data", "521icru.egstab"))
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01, kerma=0)
+
+    def make(j, lanes, schedule):
+        s = j["src"]
+        src = make_source(s["charge"], s["energy"], spectrum=s["spectrum"], x0=s["x0"],
+                          x1=s["x1"], y0=s["y0"], y1=s["y1"], z=s["z"])
+        return Simulation(tab, j["xb"], j["yb"], j["zb"], j["med"], j["rhor"], src, tp,
+                          nstreams=lanes, schedule=schedule)
+
+    jobs = workloads.batch_jobs(256, hist_scale=hist_scale)
+    nh = np.array([j["nhist"] for j in jobs], dtype=np.int64)
+    sims = [make(j, lanes if k == 0 else 0, "event") for k, j in enumerate(jobs)]
+    run_batched(sims, nh)                       # sizes the run's graph
+    for s in sims:
+        _lib.check(_lib.lib().egsgpu_reset_tallies(s.h))
     t0 = time.perf_counter()
-    g, d, L = sim.density_grad(nhist, dstar)
-    T = time.perf_counter() - t0
-    return {"workload": "C4: 64^3 x 0.4 cm water, density field [0.2, 2.0], 2 MeV 8x8 cm2 "
-                        "beam, kerma mode, target from a 10% perturbed field",
-            "histories": int(nhist), "seconds": T, "evals_per_s": 1.0 / T,
-            "histories_per_s": 2.0 * nhist / T, "loss": L,
-            "photon_tracking": ("woodcock x%.2f" % (woodcock or 1.25)
-                                if woodcock is not None else "voxel"),
-            "note": "one evaluation = forward transport + replay from the saved RNG states"}
+    run_batched(sims, nh)
+    t_batch = time.perf_counter() - t0
+    launches, iters, captured = sims[0].graph_stats()
+    rel2_batch = float(np.mean([_rel2(s) for s in sims]))
+    for s in sims:
+        s.close()
+    del sims
+    out = {"workload": "batch256: 256 jobs, 32-160^3 voxels of 1-5 mm, random "
+                       "air/lung/water/bone fields, photon 0.1-6 MeV and electron 4-20 MeV "
+                       "beams, histories log-uniform 1e5-1e7 x %g" % hist_scale,
+           "histories": int(nh.sum()),
+           "batched": {"seconds": t_batch, "sigma2": rel2_batch,
+                       "efficiency": 1.0 / (rel2_batch * t_batch),
+                       "histories_per_s": float(nh.sum()) / t_batch,
+                       "graph_launches": launches, "graph_iterations": iters,
+                       "graph_iterations_captured": captured}}
+    for schedule in ("event", "history"):
+        T, rel2 = 0.0, []
+        for j, n in zip(jobs, nh):
+            sim = make(j, 0, schedule)
+            sim.run(min(int(n), 1000))          # allocation and first launch
+            _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
+            t0 = time.perf_counter()
+            sim.run(int(n))
+            T += time.perf_counter() - t0
+            rel2.append(_rel2(sim))
+            sim.close()
+        rel2 = float(np.mean(rel2))
+        out["plain_" + schedule] = {"seconds": T, "sigma2": rel2, "efficiency": 1.0 / (rel2 * T),
+                                    "histories_per_s": float(nh.sum()) / T}
+    best = max(out["plain_event"]["efficiency"], out["plain_history"]["efficiency"])
+    out["batched_over_best_plain_efficiency"] = out["batched"]["efficiency"] / best
+    out["note"] = ("plain = each job alone in its own simulation, one after the other; "
+                   "the better of the event and history schedules is the reference")
+    return out
 
 
 def bench_batch(a):
@@ -308,46 +419,8 @@ def bench_batch(a):
                      "tally_overflows": int(call[6])},
     }
     if a.compare_plain:
-        out["plain"] = plain_jobs(a, jobs, tab, tp, T, rel2)
+        out["c3_compare"] = c3_compare(a.hist_scale, a.lanes)
     print(json.dumps(out))
-
-
-def plain_jobs(a, jobs, tab, tp, t_batch, rel2_batch):
-    """The same jobs one after the other, each a separate simulation run by
-    the plain history-schedule kernel (created and destroyed outside the
-    timed region): time, mean relative variance and the efficiency ratio of
-    the batch at equal per-voxel sigma."""
-    from egsnrc_amd.engine import Simulation, make_source
-    from egsnrc_amd import workloads
-    jobs = workloads.batch_jobs(len(jobs), hist_scale=a.hist_scale)
-    T = 0.0
-    rel2 = []
-    for j in jobs:
-        s = j["src"]
-        src = make_source(s["charge"], s["energy"], spectrum=s["spectrum"], x0=s["x0"],
-                          x1=s["x1"], y0=s["y0"], y1=s["y1"], z=s["z"])
-        sim = Simulation(tab, j["xb"], j["yb"], j["zb"], j["med"], j["rhor"], src, tp,
-                         schedule="history")
-        j["med"] = j["rhor"] = None
-        sim.run(1000)                       # allocation and first launch
-        from egsnrc_amd import _lib
-        _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            sim.run(j["nhist"])
-        T += time.perf_counter() - t0
-        su, s2, n = sim.tallies()
-        sim.close()
-        r = su / n
-        v = np.where(s2 / n - r * r > 0, (s2 / n - r * r) / (n - 1), 0.0)
-        m = r > 0.5 * r.max()
-        rel2.append(float(np.mean(v[m] / (r[m] * r[m]))))
-    rel2 = float(np.mean(rel2))
-    eff_plain = 1.0 / (rel2 * T)
-    eff_batch = 1.0 / (rel2_batch * t_batch)
-    return {"schedule": "history, one job at a time", "seconds": T, "sigma2": rel2,
-            "efficiency": eff_plain, "batch_over_plain_efficiency": eff_batch / eff_plain,
-            "histories_per_s": sum(j["nhist"] for j in jobs) * a.steps / T}
 
 
 def _kernel_key(name):
@@ -494,6 +567,11 @@ def main():
     ap.add_argument("--grad-hist", type=int, default=100_000_000,
                     help="histories of the density-gradient evaluation reported beside "
                          "the main line (C4 problem); 0 = skip")
+    ap.add_argument("--grad-fd-seeds", type=int, default=4,
+                    help="seeds of the finite-difference check of the gradient; 0 = skip")
+    ap.add_argument("--c3-scale", type=float, default=0.01,
+                    help="slab128: also run the C3 batch (256 jobs) at this fraction of its "
+                         "histories in the batched form and job by job; 0 = skip")
     ap.add_argument("--hist-scale", type=float, default=0.01,
                     help="batch256: fraction of each job's histories run per step")
     ap.add_argument("--precision-report", action="store_true",
@@ -653,10 +731,12 @@ def main():
         out["config"]["photon_tracking"] = "woodcock (majorant x %s)" % (a.woodcock or 1.25)
     if a.grad_hist and name in ("slab128", "kerma64"):
         sim.close()   # the gradient run gets the device to itself
-        out["density_grad"] = grad_bench(a.grad_hist, a.lanes, a.woodcock)
+        out["density_grad"] = grad_bench(a.grad_hist, a.lanes, a.grad_fd_seeds)
     if name in ("slab128", "kerma64") and not a.howfarless:
         out["fp64_transport"] = fp64_throughput(tab, name, hps, a)
         out["fp64_atomics"] = atomic_bench()
+    if name == "slab128" and a.c3_scale > 0:
+        out["c3_batched_vs_plain"] = c3_compare(a.c3_scale, a.lanes)
     if not a.no_cpu:
         out["cpu_baseline"], ref = cpu_baseline(name)
         out["dose_agreement_vs_oracle"] = dose_agreement((s, s2, n), ref)

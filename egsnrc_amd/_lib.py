@@ -31,7 +31,8 @@ class EgsSource(C.Structure):
                 ("y1", C.c_double), ("z", C.c_double),
                 ("ux", C.c_double), ("uy", C.c_double), ("uz", C.c_double),
                 ("kind", C.c_int32), ("rot", C.c_double * 9), ("iso", C.c_double * 3),
-                ("xcol", C.c_double), ("ycol", C.c_double), ("dsource", C.c_double)]
+                ("xcol", C.c_double), ("ycol", C.c_double), ("dsource", C.c_double),
+                ("z1", C.c_double), ("rho_max", C.c_double)]
 
 
 class EgsTransport(C.Structure):
@@ -69,6 +70,7 @@ SOURCE_DOSXYZ_FRONT = 1
 SOURCE_DOSXYZ_BEAM = 2
 SOURCE_PHSP = 3
 SOURCE_DOSXYZ_POINT = 4
+SOURCE_FANO = 5
 
 
 class EngineError(RuntimeError):

@@ -96,6 +96,7 @@ struct DevSource {
     int kind;                // EGSGPU_SOURCE_*
     double rot[9];           // DOSXYZnrc isource 1: r_11..r_33
     double iso[3], xcol, ycol, dsource;
+    R z1, rho_max;           // egs_fano_source: box z range end, max mass density
     // phase-space particles (kind EGSGPU_SOURCE_PHSP)
     long long nphsp;
     const float *px, *py, *pu, *pv, *pe, *pw;

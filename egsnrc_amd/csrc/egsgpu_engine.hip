@@ -438,6 +438,7 @@ int build_source(egsgpu_handle *h, const EgsSource *s, Precision<R> &P) {
     for (int k = 0; k < 9; k++) S.rot[k] = s->rot[k];
     for (int k = 0; k < 3; k++) S.iso[k] = s->iso[k];
     S.xcol = s->xcol; S.ycol = s->ycol; S.dsource = s->dsource;
+    S.z1 = (R)s->z1; S.rho_max = (R)s->rho_max;
     if (s->spectrum == EGSGPU_SPEC_MONO) return EGSGPU_OK;
     if (s->spectrum == EGSGPU_SPEC_CDFINV) {
         S.np = s->nbins;
@@ -963,8 +964,11 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
             src->spectrum == EGSGPU_SPEC_LINE || src->spectrum == EGSGPU_SPEC_CDFINV) &&
             src->nbins > 0 && src->spec_e && src->spec_w),
             "source spectrum");
-    EGS_ARG(src->kind >= EGSGPU_SOURCE_PARALLEL_BEAM && src->kind <= EGSGPU_SOURCE_DOSXYZ_POINT,
+    EGS_ARG(src->kind >= EGSGPU_SOURCE_PARALLEL_BEAM && src->kind <= EGSGPU_SOURCE_FANO,
             "source kind");
+    EGS_ARG(src->kind != EGSGPU_SOURCE_FANO ||
+            (src->x0 < src->x1 && src->y0 < src->y1 && src->z < src->z1 && src->rho_max > 0),
+            "a Fano source needs a box x0 < x1, y0 < y1, z < z1 and rho_max > 0");
     EGS_ARG(src->kind != EGSGPU_SOURCE_DOSXYZ_POINT ||
             (src->dsource > 0 && src->x0 <= src->x1 && src->y0 <= src->y1),
             "a point source needs SSD (dsource) > 0 and x0 <= x1, y0 <= y1");
@@ -1056,6 +1060,13 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         double vol = (geom->xb[ix + 1] - geom->xb[ix]) * (geom->yb[iy + 1] - geom->yb[iy]) *
                      (geom->zb[iz + 1] - geom->zb[iz]);
         h->hmass[i] = vol * h->hrhor[i] * b.f1(Blob::mk(med[i], "rho"));
+        // egs_fano_source: the acceptance ratio must stay <= 1 in every voxel
+        if (src->kind == EGSGPU_SOURCE_FANO &&
+            h->hrhor[i] * b.f1(Blob::mk(med[i], "rho")) > src->rho_max * (1 + 1e-9)) {
+            set_error("Fano source: voxel %d has mass density %g g/cm^3 above rho_max %g", i,
+                      h->hrhor[i] * b.f1(Blob::mk(med[i], "rho")), src->rho_max);
+            return fail(EGSGPU_E_ARG);
+        }
     }
     // cut-offs: $adjust_rhor_ecut_pcut
     double ecut = 0, pcut = 0;

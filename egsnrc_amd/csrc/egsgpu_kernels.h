@@ -229,9 +229,53 @@ __device__ __forceinline__ bool sample_phsp(const DevSource<R> &S, const GeomVie
 // The DOSXYZnrc sources (isource 0 and 1) have their own sampling; a
 // phase-space source takes particle hidx (the history's index in the
 // simulation).
+// egs_fano_source (egs_fano_source.h:128-182, with the energy drawn first
+// as EGS_BaseSimpleSource::getNextParticle does, egs_base_source.h:684-692):
+// a point uniform in the box, drawn again while it lies outside the grid,
+// kept with probability rho/rho_max (rho the mass density of its voxel:
+// the medium's density times the voxel's relative density), so that
+// starting points are distributed in proportion to mass; then an
+// isotropic direction (cos theta uniform in [-1, 1], phi uniform).
+template <typename R, bool EV, bool LUX, bool HFL, bool SPLIT>
+__device__ __forceinline__ bool sample_fano(const DevSource<R> &S, const GeomView<R> &G,
+                                            const VoxData<R> *vox,
+                                            Shower<R, EV, LUX, HFL, SPLIT> &sh, Part<R> &q) {
+    const R RM = (R)0.5109989461;
+    const R E = source_energy(S, sh);
+    R x = 0, y = 0, z = 0;
+    int ireg = -1;
+    for (int ntry = 0; ntry < 1000000; ntry++) {
+        x = S.x0 + (S.x1 - S.x0) * sh.rnd();
+        y = S.y0 + (S.y1 - S.y0) * sh.rnd();
+        z = S.z + (S.z1 - S.z) * sh.rnd();
+        const int ir = geom_is_where(G, x, y, z);
+        if (ir < 0) continue;
+        const VoxData<R> d = vox[ir];
+        const R rho = sh.T.ms[d.med].rho * d.rho;
+        if (!(sh.rnd() * S.rho_max > rho)) { ireg = ir; break; }
+    }
+    if (ireg < 0) return false;
+    const R w = 1 - 2 * sh.rnd();
+    R u = 0, v = 0, sinz = 1 - w * w;
+    if (sinz > (R)1e-10) {
+        sinz = dsqrt(sinz);
+        const R phi = (R)6.283185307179586 * sh.rnd();
+        u = sinz * cos(phi);
+        v = sinz * sin(phi);
+    }
+    q.e = S.charge ? E + RM : E;
+    q.x = x; q.y = y; q.z = z;
+    q.u = u; q.v = v; q.w = w;
+    q.wt = 1; q.dnear = 0; q.iq = S.charge; q.ir = ireg;
+    return true;
+}
+
 template <typename R, bool EV, bool LUX, bool HFL, bool SPLIT>
 __device__ __forceinline__ bool sample_primary(const DevSource<R> &S, const GeomView<R> &G,
-                               Shower<R, EV, LUX, HFL, SPLIT> &sh, Part<R> &q, long long hidx) {
+                                               const VoxData<R> *vox,
+                                               Shower<R, EV, LUX, HFL, SPLIT> &sh, Part<R> &q,
+                                               long long hidx) {
+    if (S.kind == EGSGPU_SOURCE_FANO) return sample_fano(S, G, vox, sh, q);
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_FRONT) return sample_dosxyz_front(S, G, sh, q);
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_BEAM) return sample_dosxyz_beam(S, G, sh, q);
     if (S.kind == EGSGPU_SOURCE_DOSXYZ_POINT) return sample_dosxyz_point(S, G, sh, q);
@@ -299,7 +343,7 @@ __global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
     int64_t nh = live ? a.base + (lane < a.rem ? 1 : 0) : 0;
     for (int64_t k = 0; k < nh; k++) {
         Part<R> q;
-        if (!sample_primary(a.S, a.G, sh, q, a.hist0 + k * a.L.nlanes + lane)) continue;
+        if (!sample_primary(a.S, a.G, a.vox, sh, q, a.hist0 + k * a.L.nlanes + lane)) continue;
         sh.p = q;
         sh.np = 1;
         sh.eph = S::kNew;
@@ -411,7 +455,7 @@ k_kerma(KArgs<R> a, int ksteps, int batch, int max_wait) {
             sh.photon_trans(ksteps);
         } else if (chosen == kNewHist) {
             Part<R> q;
-            const bool hit = sample_primary(a.S, a.G, sh, q, a.hist0 + k * a.L.nlanes + lane);
+            const bool hit = sample_primary(a.S, a.G, a.vox, sh, q, a.hist0 + k * a.L.nlanes + lane);
             k++;
             if (hit) {
                 sh.p = q;
@@ -567,7 +611,7 @@ __global__ void __launch_bounds__(256) k_ev_source(EvArgs<R> a) {
         // the history's index in its own simulation
         const long long hsim = a.njobs > 1 ? a.jobs[lo].hist0 + hist_base + i - a.job_first[lo]
                                            : a.k.hist0 + hist_base + i;
-        if (!sample_primary(*src, *geo, sh, q, hsim)) {
+        if (!sample_primary(*src, *geo, a.jobs[lo].vox, sh, q, hsim)) {
             unsigned long long at = atomicAdd(&a.ctl->free_tail, 1ull);
             a.slots.ring[at % ns] = slot;
             continue;

@@ -47,6 +47,18 @@ def make_source(charge=0, energy=1.0, spectrum=None, shape="rectangle",
     return s, keep
 
 
+def make_fano_source(charge, energy, box, rho_max, spectrum=None):
+    """egs_fano_source: particles start uniformly per unit mass in
+    box = (x0, x1, y0, y1, z0, z1), with isotropic directions; rho_max (g/cm^3)
+    must be at least the largest voxel mass density in the box."""
+    s, keep = make_source(charge, energy, spectrum=spectrum, x0=box[0], x1=box[1],
+                          y0=box[2], y1=box[3], z=box[4])
+    s.kind = _lib.SOURCE_FANO
+    s.z1 = box[5]
+    s.rho_max = rho_max
+    return s, keep
+
+
 def make_phsp_source(x0=0.0, y0=0.0, z=0.0):
     """A phase-space source whose particles are given by Simulation.set_phsp
     (egs_phsp_source), its plane translated by (x0, y0, z)."""

@@ -310,12 +310,38 @@ class EGS_GPUApplication {
             src.x0 += tx; src.y0 += ty; src.z += tz;
             return;
         }
-        if (!InputItem::same(lib, "egs_parallel_beam"))
-            throw std::runtime_error("only egs_parallel_beam, egs_phsp_source and iaea_phsp_source "
-                                     "are supported, got " + lib);
+        const bool fano = InputItem::same(lib, "egs_fano_source");
+        if (!InputItem::same(lib, "egs_parallel_beam") && !fano)
+            throw std::runtime_error("only egs_parallel_beam, egs_fano_source, egs_phsp_source and "
+                                     "iaea_phsp_source are supported, got " + lib);
         int q = 0;
         s->get("charge", q);
         src.charge = q;
+        if (fano) {
+            // egs_fano_source (egs_fano_source.cpp input): a box shape, the
+            // maximum mass density and an isotropic direction
+            src.kind = EGSGPU_SOURCE_FANO;
+            if (s->get("max mass density", src.rho_max) || !(src.rho_max > 0))
+                throw std::runtime_error("egs_fano_source needs max mass density > 0");
+            const InputItem *sh = s->block("shape");
+            if (!sh) throw std::runtime_error("egs_fano_source needs a shape");
+            std::string st;
+            sh->get("type", st);
+            if (st.empty()) sh->get("library", st);
+            if (!InputItem::same(st, "box") && !InputItem::same(st, "egs_box"))
+                throw std::runtime_error("egs_fano_source: only a box shape is supported");
+            std::vector<double> bs, tr{0, 0, 0};
+            if (sh->get("box size", bs) || (bs.size() != 1 && bs.size() != 3))
+                throw std::runtime_error("box shape needs box size (1 or 3 values)");
+            if (bs.size() == 1) bs = {bs[0], bs[0], bs[0]};
+            if (const InputItem *t = sh->block("transformation")) t->get("translation", tr);
+            if (tr.size() != 3) throw std::runtime_error("box translation needs 3 values");
+            src.x0 = tr[0] - bs[0] / 2 + tx; src.x1 = tr[0] + bs[0] / 2 + tx;
+            src.y0 = tr[1] - bs[1] / 2 + ty; src.y1 = tr[1] + bs[1] / 2 + ty;
+            src.z = tr[2] - bs[2] / 2 + tz; src.z1 = tr[2] + bs[2] / 2 + tz;
+            src.ux = 0; src.uy = 0; src.uz = 1;
+            area = 0;
+        }
         std::vector<double> dir{0, 0, 1};
         s->get("direction", dir);
         double nrm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
@@ -326,7 +352,9 @@ class EGS_GPUApplication {
         sh->get("type", st);
         if (st.empty()) sh->get("library", st);
         std::vector<double> pos;
-        if (InputItem::same(st, "point")) {
+        if (fano) {
+            // the box was read above
+        } else if (InputItem::same(st, "point")) {
             if (sh->get("position", pos) || pos.size() != 3) throw std::runtime_error("point shape needs position");
             src.shape = EGSGPU_SHAPE_POINT;
             src.x0 = src.x1 = pos[0]; src.y0 = src.y1 = pos[1]; src.z = pos[2];
@@ -346,7 +374,7 @@ class EGS_GPUApplication {
         } else {
             throw std::runtime_error("unsupported shape " + st);
         }
-        src.x0 += tx; src.x1 += tx; src.y0 += ty; src.y1 += ty; src.z += tz;
+        if (!fano) { src.x0 += tx; src.x1 += tx; src.y0 += ty; src.y1 += ty; src.z += tz; }
         const InputItem *sp = s->block("spectrum");
         if (!sp) throw std::runtime_error("source needs a spectrum");
         std::string stype;

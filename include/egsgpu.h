@@ -82,6 +82,13 @@ typedef struct EgsXYZGeometry {
                                          axis, aimed at x0..x1, y0..y1 of
                                          that plane with the cos^3 weight of
                                          srcxyznrc.mortran:3451-3495       */
+#define EGSGPU_SOURCE_FANO          5 /* egs_fano_source (egs_fano_source.h:
+                                         128-182): a point uniform in the box
+                                         [x0,x1]x[y0,y1]x[z,z1], kept with
+                                         probability (mass density of its
+                                         voxel)/rho_max, so that particles
+                                         start in proportion to mass; the
+                                         direction isotropic               */
 #define EGSGPU_SHAPE_POINT     0   /* EGS_PointShape                       */
 #define EGSGPU_SHAPE_RECTANGLE 1   /* EGS_RectangleShape (z = shape_z)     */
 typedef struct EgsSource {
@@ -98,6 +105,9 @@ typedef struct EgsSource {
     double rot[9];               /* DOSXYZ_BEAM: r_11, r_12, ..., r_33     */
     double iso[3];               /* DOSXYZ_BEAM: isocentre                 */
     double xcol, ycol, dsource;  /* DOSXYZ_BEAM: widths, source distance   */
+    double z1;                   /* FANO: far z face of the box            */
+    double rho_max;              /* FANO: "max mass density" (g/cm^3), at
+                                    least the largest voxel mass density  */
 } EgsSource;
 
 /* ---- transport parameters (":start MC transport parameter:",
