@@ -90,6 +90,14 @@ def build_workload(name):
     raise SystemExit("unknown workload " + name)
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print("[bench %.0fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
+
+
+T_START = time.perf_counter()
+
+
 def efficiency(sum_, sum2, n, seconds):
     """1/(sigma^2 T): sigma^2 = mean relative variance over voxels with
     D > 0.5 Dmax."""
@@ -255,6 +263,7 @@ def grad_bench(nhist, lanes=0, fd_seeds=4):
             fd.append([float((np.sum((dose(rhor + h * dl, seed, nhist) - dstar) ** 2) -
                               np.sum((dose(rhor - h * dl, seed, nhist) - dstar) ** 2)) / (2 * h))
                        for dl in dirs])
+            progress("gradient finite-difference check: seed %d done" % seed)
         sf, fd = np.mean(sf, 0), np.mean(fd, 0)
         out["fd_check"] = {"h": h, "seeds": fd_seeds, "directions": 5,
                            "rel_l2": float(np.linalg.norm(sf - fd) / np.linalg.norm(fd)),
@@ -271,14 +280,19 @@ def _rel2(sim):
     return float(np.mean(v[m] / (r[m] * r[m])))
 
 
-def c3_compare(hist_scale, lanes=0):
+def c3_compare(hist_scale, lanes=0, schedules=("event",)):
     """C3 in the batched persistent form against the plain form.
 
     Batched: the 256 jobs in one egsgpu_run_batched call (one graph launch;
     a first run of the same batch sizes the graph and is not timed).
-    Plain: every job its own simulation, run alone, with the event schedule
-    and with the history schedule (simulations made outside the timed
-    region).  Efficiency 1/(sigma^2 T) with sigma^2 the mean over jobs of
+    Plain: every job its own simulation, run alone, with the given
+    schedules (the event schedule, the faster of the two for these jobs, by
+    default; --workload batch256 --compare-plain adds the history
+    schedule), each with as many transport lanes as
+    suit its size (a power of two near a quarter of its histories, between
+    4096 and the default), so that a small job neither waits for nor
+    allocates the state of a full-device launch (simulations made outside
+    the timed region).  Efficiency 1/(sigma^2 T) with sigma^2 the mean over jobs of
     the mean relative variance over D > 0.5 Dmax, so the ratio compares the
     forms at equal per-voxel sigma."""
     from egsnrc_amd import _lib
@@ -317,10 +331,12 @@ def c3_compare(hist_scale, lanes=0):
                        "histories_per_s": float(nh.sum()) / t_batch,
                        "graph_launches": launches, "graph_iterations": iters,
                        "graph_iterations_captured": captured}}
-    for schedule in ("event", "history"):
+    progress("c3: batched run done (%.2f s); plain runs" % t_batch)
+    for schedule in schedules:
         T, rel2 = 0.0, []
         for j, n in zip(jobs, nh):
-            sim = make(j, 0, schedule)
+            lanes_j = int(min(262144, max(4096, 2 ** round(np.log2(max(1, n) / 4)))))
+            sim = make(j, lanes_j, schedule)
             sim.run(min(int(n), 1000))          # allocation and first launch
             _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
             t0 = time.perf_counter()
@@ -331,10 +347,12 @@ def c3_compare(hist_scale, lanes=0):
         rel2 = float(np.mean(rel2))
         out["plain_" + schedule] = {"seconds": T, "sigma2": rel2, "efficiency": 1.0 / (rel2 * T),
                                     "histories_per_s": float(nh.sum()) / T}
-    best = max(out["plain_event"]["efficiency"], out["plain_history"]["efficiency"])
+        progress("c3: plain %s schedule done (%.2f s)" % (schedule, T))
+    best = max(out["plain_" + sc]["efficiency"] for sc in schedules)
     out["batched_over_best_plain_efficiency"] = out["batched"]["efficiency"] / best
-    out["note"] = ("plain = each job alone in its own simulation, one after the other; "
-                   "the better of the event and history schedules is the reference")
+    out["note"] = ("plain = each job alone in its own simulation, one after the other, "
+                   "with lanes sized to the job; the best of the plain schedules run (%s) is "
+                   "the reference" % ", ".join(schedules))
     return out
 
 
@@ -419,7 +437,7 @@ def bench_batch(a):
                      "tally_overflows": int(call[6])},
     }
     if a.compare_plain:
-        out["c3_compare"] = c3_compare(a.hist_scale, a.lanes)
+        out["c3_compare"] = c3_compare(a.hist_scale, a.lanes, ("event", "history"))
     print(json.dumps(out))
 
 
@@ -430,17 +448,25 @@ def _kernel_key(name):
 
 def _pmc_per_dispatch(d, counter):
     """Per-kernel (dispatches, mean counter value) from one rocprofv3 --pmc
-    output directory written with `-o run --output-format csv`."""
+    output directory written with `-o run --output-format csv`.  Dispatches
+    whose value is below 1e-4 of the kernel's largest are left out: they are
+    the iterations of a run's graph that the device found already finished,
+    whose kernels return at once."""
     import csv
-    tot, disp = {}, {}
+    per = {}
     with open(os.path.join(d, "run_counter_collection.csv")) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] != counter:
                 continue
             k = _kernel_key(r["Kernel_Name"])
-            tot[k] = tot.get(k, 0.0) + float(r["Counter_Value"])
-            disp.setdefault(k, set()).add(r["Dispatch_Id"])
-    return {k: (len(disp[k]), tot[k] / len(disp[k])) for k in tot}
+            per.setdefault(k, {})
+            per[k][r["Dispatch_Id"]] = per[k].get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    out = {}
+    for k, dv in per.items():
+        mx = max(dv.values())
+        v = [x for x in dv.values() if x >= 1e-4 * mx] or [0.0]
+        out[k] = (len(v), sum(v) / len(v))
+    return out
 
 
 def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, command):
@@ -485,6 +511,60 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, command):
     return "\n".join(out + [""] + lines) + "\n"
 
 
+SQ_COUNTERS = ("SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_ACTIVE_INST_VALU "
+               "SQ_THREAD_CYCLES_VALU SQ_INSTS_VALU GRBM_GUI_ACTIVE")
+
+
+def sq_summary(sq_dir, command):
+    """Markdown summary of one rocprofv3 --pmc pass over SQ_COUNTERS: per
+    kernel, the VALU lane utilisation SQ_THREAD_CYCLES_VALU / (64
+    SQ_ACTIVE_INST_VALU) (the share of the 64 lanes doing work when a VALU
+    instruction issues) and the share of wave cycles spent waiting
+    (SQ_WAIT_ANY / SQ_WAVE_CYCLES).  The line `valu_lane_utilisation
+    <kernel>: <fraction>` is what the bench reads back."""
+    import csv
+    agg, disp = {}, {}
+    with open(os.path.join(sq_dir, "run_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            k = _kernel_key(r["Kernel_Name"])
+            agg.setdefault(k, {})
+            agg[k][r["Counter_Name"]] = agg[k].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            disp.setdefault(k, set()).add(r["Dispatch_Id"])
+    out = ["Command (from the repository root, `TMPDIR=/tmp`):", "",
+           "    rocprofv3 --pmc %s -d %s -o run --output-format csv -- %s"
+           % (SQ_COUNTERS, sq_dir, command), "",
+           "| kernel | dispatches | waves/dispatch | VALU instructions/dispatch | VALU lane "
+           "utilisation | waiting share of wave cycles |", "|---|---|---|---|---|---|"]
+    lines = []
+    for k in sorted(agg, key=lambda k: -agg[k].get("SQ_WAVE_CYCLES", 0)):
+        d, n = agg[k], len(disp[k])
+        if d.get("SQ_ACTIVE_INST_VALU", 0) <= 0 or not k.startswith("k_"):
+            continue
+        util = d["SQ_THREAD_CYCLES_VALU"] / (64.0 * d["SQ_ACTIVE_INST_VALU"])
+        wait = d["SQ_WAIT_ANY"] / d["SQ_WAVE_CYCLES"]
+        out.append("| `%s` | %d | %.0f | %.4g | %.3f | %.3f |"
+                   % (k, n, d["SQ_WAVES"] / n, d["SQ_INSTS_VALU"] / n, util, wait))
+        lines.append("valu_lane_utilisation %s: %.4f" % (k, util))
+        lines.append("wait_share %s: %.4f" % (k, wait))
+    return "\n".join(out + [""] + lines) + "\n"
+
+
+def read_profile_value(kind, kernel, workload, stem):
+    """A `<kind> <kernel>: <value>` line written by sq_summary or
+    pmc_summary under profiles/<stem>_<workload>.md (kernel matched by name,
+    template arguments aside), with the file it came from."""
+    import re
+    p = os.path.join(ROOT, "profiles", "%s_%s.md" % (stem, workload))
+    if not os.path.exists(p):
+        return None, None
+    base = _kernel_key(kernel).split("<")[0]
+    for l in open(p):
+        m = re.match(r"%s (.+): (\S+)\s*$" % kind, l)
+        if m and m.group(1).split("<")[0] == base and "double" not in m.group(1):
+            return float(m.group(2)), os.path.relpath(p, ROOT)
+    return None, None
+
+
 def trace_summary(trace_dir, bench_log, command):
     """Markdown summary of a `rocprofv3 --kernel-trace --stats` run
     (`-o run --output-format csv`) next to the bench line printed by the
@@ -494,6 +574,15 @@ def trace_summary(trace_dir, bench_log, command):
     rows = list(csv.DictReader(open(os.path.join(trace_dir, "run_kernel_stats.csv"))))
     line = json.loads([l for l in open(bench_log) if l.startswith("{")][-1])
     rf = line["roofline"]
+    # the dominant kernel's launches that did work (the iterations of a run's
+    # graph that the device found finished return within microseconds)
+    work = []
+    for r in csv.DictReader(open(os.path.join(trace_dir, "run_kernel_trace.csv"))):
+        if _kernel_key(r["Kernel_Name"]) == _kernel_key(rf["kernel"]).replace(
+                "<float>", "<float, 2, false>"):
+            dt = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+            if dt > 0.05:
+                work.append(dt)
     out = ["Command (from the repository root, `TMPDIR=/tmp`):", "",
            "    rocprofv3 --kernel-trace --stats -d %s -o run --output-format csv -- %s"
            % (trace_dir, command), "",
@@ -509,12 +598,14 @@ def trace_summary(trace_dir, bench_log, command):
         if k == _kernel_key(rf["kernel"]):
             dom = float(r["AverageNs"]) / 1e6
     out += ["", "Bench line of the same run: %.0f histories/s, %.1f ms per step; "
-            "dominant kernel `%s`, %d launches, HIP-event average %.3f ms%s; "
+            "dominant kernel `%s`, %d launches, HIP-event average %.3f ms%s%s; "
             "roofline achieved %.1f GB/s (%.2f %% of %.0f GB/s)."
             % (line["value"], line["ms_per_step"], rf["kernel"], rf["kernel_launches"],
                rf["kernel_ms_avg"],
-               "" if dom is None else ", trace average %.3f ms (ratio %.3f)"
-               % (dom, rf["kernel_ms_avg"] / dom),
+               "" if dom is None else ", trace average over all launches %.3f ms" % dom,
+               "" if not work else "; trace average over the %d launches that did work "
+               "(longer than 50 us) %.3f ms (HIP-event / trace = %.3f)"
+               % (len(work), sum(work) / len(work), rf["kernel_ms_avg"] / (sum(work) / len(work))),
                rf["achieved"], 100 * rf["frac"], rf["peak"])]
     return "\n".join(out) + "\n"
 
@@ -588,6 +679,9 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, default=None,
                     metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="only print the HBM traffic summary of four rocprofv3 --pmc runs")
+    ap.add_argument("--sq-summary", default=None, metavar="SQ_DIR",
+                    help="only print the VALU summary of a rocprofv3 --pmc run over the SQ "
+                         "counters")
     ap.add_argument("--trace-summary", nargs=2, default=None, metavar=("TRACE_DIR", "BENCH_LOG"),
                     help="only print the summary of a rocprofv3 --kernel-trace --stats run")
     ap.add_argument("--profiled-command", default="python3 bench.py",
@@ -598,6 +692,9 @@ def main():
         return
     if a.trace_summary:
         sys.stdout.write(trace_summary(*a.trace_summary, a.profiled_command))
+        return
+    if a.sq_summary:
+        sys.stdout.write(sq_summary(a.sq_summary, a.profiled_command))
         return
     if a.copy_calibration:
         import torch
@@ -652,6 +749,7 @@ def main():
     tuned = sim.autotune(a.autotune) if a.autotune else None
     for _ in range(a.warmup):
         sim.run(hps)
+    progress("warm-up done; timing %d steps" % a.steps)
     _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
     kms = np.zeros(4)
     klaunch = np.zeros(4, dtype=np.int64)
@@ -729,15 +827,39 @@ def main():
                              "steps": 2, "note": "same steps replayed from a HIP graph"}
     if a.woodcock is not None:
         out["config"]["photon_tracking"] = "woodcock (majorant x %s)" % (a.woodcock or 1.25)
+    progress("timed steps done: %.0f histories/s" % hist_s)
     if a.grad_hist and name in ("slab128", "kerma64"):
         sim.close()   # the gradient run gets the device to itself
+        progress("density gradient")
         out["density_grad"] = grad_bench(a.grad_hist, a.lanes, a.grad_fd_seeds)
     if name in ("slab128", "kerma64") and not a.howfarless:
+        progress("FP64 transport")
         out["fp64_transport"] = fp64_throughput(tab, name, hps, a)
         out["fp64_atomics"] = atomic_bench()
     if name == "slab128" and a.c3_scale > 0:
+        progress("C3 batched vs plain")
         out["c3_batched_vs_plain"] = c3_compare(a.c3_scale, a.lanes)
+    if "fp64_atomics" in out:
+        # FP64 atomic adds of the dominant kernel into the dose tallies (sum
+        # and sum of squares at every history-voxel flush) against the rate
+        # of scattered FP64 atomic adds over a grid of the same size
+        peak = out["fp64_atomics"]["scattered_128cube"]
+        ach = 2.0 * dc[3] / kernel_s / 1e9
+        out["atomic_roofline"] = {"achieved": ach, "peak": peak, "unit": "G adds/s",
+                                  "frac": ach / peak,
+                                  "kernel": knames[kdom],
+                                  "what": "FP64 atomicAdd into the dose tallies, 2 per "
+                                          "(history, voxel) flush; peak = measured rate of "
+                                          "scattered FP64 atomic adds over 128^3 doubles"}
+    valu, vsrc = read_profile_value("valu_lane_utilisation", knames[kdom], name, "valu")
+    wait, _ = read_profile_value("wait_share", knames[kdom], name, "valu")
+    if valu is not None:
+        out["valu"] = {"lane_utilisation": valu, "wait_share": wait, "kernel": knames[kdom],
+                       "source": vsrc,
+                       "definition": "SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU); "
+                                     "SQ_WAIT_ANY / SQ_WAVE_CYCLES"}
     if not a.no_cpu:
+        progress("CPU baseline")
         out["cpu_baseline"], ref = cpu_baseline(name)
         out["dose_agreement_vs_oracle"] = dose_agreement((s, s2, n), ref)
     print(json.dumps(out))
