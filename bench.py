@@ -441,6 +441,17 @@ def bench_batch(a):
     print(json.dumps(out))
 
 
+def _same_kernel(a, b):
+    """Kernel names that agree in base name and first template argument
+    (the bench names k_ev_electron<float>, the profiler
+    k_ev_electron<float, 2, false>)."""
+    def parts(n):
+        n = _kernel_key(n)
+        base, _, rest = n.partition("<")
+        return base, rest.split(",")[0].rstrip(">").strip()
+    return parts(a) == parts(b)
+
+
 def _kernel_key(name):
     """Kernel name without return type, namespace and argument list."""
     return name.split("(")[0].replace("void ", "").replace("egsgpu::", "").strip()
@@ -578,8 +589,7 @@ def trace_summary(trace_dir, bench_log, command):
     # graph that the device found finished return within microseconds)
     work = []
     for r in csv.DictReader(open(os.path.join(trace_dir, "run_kernel_trace.csv"))):
-        if _kernel_key(r["Kernel_Name"]) == _kernel_key(rf["kernel"]).replace(
-                "<float>", "<float, 2, false>"):
+        if _same_kernel(r["Kernel_Name"], rf["kernel"]):
             dt = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
             if dt > 0.05:
                 work.append(dt)
@@ -595,7 +605,7 @@ def trace_summary(trace_dir, bench_log, command):
                    % (k, r["Calls"], float(r["TotalDurationNs"]) / 1e3,
                       float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
                       float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
-        if k == _kernel_key(rf["kernel"]):
+        if _same_kernel(k, rf["kernel"]):
             dom = float(r["AverageNs"]) / 1e6
     out += ["", "Bench line of the same run: %.0f histories/s, %.1f ms per step; "
             "dominant kernel `%s`, %d launches, HIP-event average %.3f ms%s%s; "
