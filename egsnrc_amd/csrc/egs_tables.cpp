@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <limits>
 #include <sstream>
 
@@ -863,6 +864,116 @@ int tables_build(const std::string &hh, const std::string &pegs_file,
             out.i["edge_number"] = en;
             out.f["edge_a"] = ea; out.f["edge_b"] = eb; out.f["edge_c"] = ec;
             out.f["edge_d"] = ed; out.f["edge_energies"] = ee;
+        }
+
+        // ---- egs_init_relax (:5794-5998): EADL relaxation data of every
+        //      element of the media, from relax.data (4-byte records: the
+        //      element count, one record per element pointing at its block;
+        //      per shell the EADL type, the transition count, the binding
+        //      energy, then (final state, probability) per transition, final
+        //      states counted from 0 as C arrays do)
+        {
+            std::ifstream f(data + "relax.data", std::ios::binary);
+            if (!f) throw std::runtime_error("cannot open relax.data");
+            std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+            const size_t nw = raw.size() / 4;
+            auto word = [&](size_t rec) -> int32_t {   // rec: 1-based record
+                if (rec < 1 || rec > nw) throw std::runtime_error("relax.data: record out of range");
+                int32_t v;
+                std::memcpy(&v, raw.data() + 4 * (rec - 1), 4);
+                return v;
+            };
+            auto real = [&](size_t rec) -> double {
+                int32_t v = word(rec);
+                float x;
+                std::memcpy(&x, &v, 4);
+                return (double)x;
+            };
+            const double min_be = 1e-3;   // $RELAX-CUTOFF
+            const std::vector<double> &bind = out.F("binding_energies");
+            std::vector<int32_t> zfirst(101, -1), znsh(101, 0), shell_num, rfirst, rntran, rstate,
+                ratbin;
+            std::vector<double> shell_be, rprob;
+            for (int m = 0; m < nmed; m++) {
+                std::vector<double> zs = out.F(Blob::mk(m, "z"));
+                std::sort(zs.begin(), zs.end());
+                for (double zz : zs) {
+                    const int iZ = (int)(zz + 0.5);
+                    if (iZ < 1 || iZ > 100) throw std::runtime_error("relax.data: Z out of range");
+                    if (zfirst[iZ] >= 0) continue;
+                    size_t rec = (size_t)word((size_t)iZ + 1);
+                    const int nshell = word(rec);
+                    if (nshell < 1 || nshell > kMXSHXSEC)
+                        throw std::runtime_error("relax.data: unexpected shell count");
+                    zfirst[iZ] = (int32_t)shell_be.size();
+                    znsh[iZ] = nshell;
+                    for (int s = 1; s <= nshell; s++) {
+                        rec++;                          // EADL shell type (not used)
+                        const int ntran0 = word(++rec);
+                        double be = real(++rec);
+                        // XCOM edge energies where EDGSET has one
+                        if (bind[(size_t)iZ * kMXSHXSEC + s - 1] > 0)
+                            be = bind[(size_t)iZ * kMXSHXSEC + s - 1];
+                        std::vector<int32_t> st(ntran0);
+                        std::vector<double> w(ntran0);
+                        for (int k = 0; k < ntran0; k++) {
+                            int32_t t = word(++rec);
+                            w[k] = real(++rec);
+                            st[k] = t < 64 ? t + 1 : t + 65;   // 1-based shells
+                        }
+                        shell_be.push_back(be);
+                        shell_num.push_back(s);
+                        if (be < min_be || ntran0 < 1) {
+                            rfirst.push_back(-1);
+                            rntran.push_back(-1);
+                            continue;
+                        }
+                        double sumw = 0;
+                        for (double x : w) sumw += x;
+                        if (sumw > 1) {
+                            for (double &x : w) x /= sumw;
+                        } else if (sumw < 1) {   // the rest: no new vacancy
+                            st.push_back(-1);
+                            w.push_back(1 - sumw);
+                        }
+                        const int n = (int)w.size();
+                        // prepare_alias_histogram (egs_utilities.mortran:1940-1991)
+                        std::vector<int32_t> bin(n, -1);
+                        double avg = 0;
+                        for (double x : w) avg += x;
+                        avg /= n;
+                        for (int i = 0; i < n - 1; i++) {
+                            int jh = -1, jl = -1;
+                            for (int j = 0; j < n; j++)
+                                if (bin[j] < 0 && w[j] > avg) { jh = j; break; }
+                            for (int j = 0; j < n; j++)
+                                if (bin[j] < 0 && w[j] < avg) { jl = j; break; }
+                            if (jh < 0 || jl < 0) break;   // the rest are at the average
+                            w[jh] -= avg - w[jl];
+                            w[jl] /= avg;
+                            bin[jl] = jh + 1;
+                        }
+                        for (int j = 0; j < n; j++)
+                            if (bin[j] < 0) { bin[j] = j + 1; w[j] = 1; }
+                        rfirst.push_back((int32_t)rprob.size());
+                        rntran.push_back(n);
+                        for (int j = 0; j < n; j++) {
+                            rprob.push_back(w[j]);
+                            ratbin.push_back(bin[j]);
+                            rstate.push_back(st[j]);
+                        }
+                    }
+                }
+            }
+            out.i["relax_zfirst"] = zfirst;   // first shell of element Z (0-based), -1: none
+            out.i["relax_nshell"] = znsh;
+            out.f["shell_be"] = shell_be;     // MeV
+            out.i["shell_num"] = shell_num;   // shell number within its element (1-based)
+            out.i["relax_first"] = rfirst;    // first transition (0-based), -1: none
+            out.i["relax_ntran"] = rntran;
+            out.f["relax_prob"] = rprob;      // alias-histogram weights
+            out.i["relax_atbin"] = ratbin;    // alias bins (1-based)
+            out.i["relax_state"] = rstate;    // final state: shell (<= 64) or 64*s1 + s2
         }
 
         // ---- init_ms_SR (:3267-3306): q^(2+) surface, single precision

@@ -67,11 +67,18 @@ struct DevTables {
     const R *eno;
     const int *eno_atbin, *shell;
     const R *cs_be, *cs_Jo, *cs_erfJo;
+    const int *cs_iz, *cs_shn;   // element and shell number of Compton shell j
     // elements [m*kMaxNE + k]
     const R *z, *pz;
     // EDGE data per element [iz*30 + k], [iz*6 + k], [iz*16 + k]
     const R *bind, *iprob, *ea, *eb, *ec, *ed, *ee;
     const int *enumb;
+    // EADL relaxation (egs_init_relax): element Z's shells start at
+    // relax_zfirst[Z] (-1: none) in the shell arrays; a shell's transitions
+    // are an alias histogram at relax_first (-1: none)
+    const int *relax_zfirst, *relax_nshell, *shell_num, *relax_first, *relax_ntran,
+        *relax_atbin, *relax_state;
+    const R *shell_be, *relax_prob;
     // brems/pair fits [m*48 + k*8 + l]
     const R *dl;
     // electron PWLF rows [(m*kMaxEKE + L-1)*kERow + field]

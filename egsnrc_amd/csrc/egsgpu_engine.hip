@@ -340,6 +340,12 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
     CK(pack_glob<R>(h, b, "cs_be", &T.cs_be));
     CK(pack_glob<R>(h, b, "cs_Jo", &T.cs_Jo));
     CK(pack_glob<R>(h, b, "cs_erfJo", &T.cs_erfJo));
+    {
+        const auto &iz = b.I("cs_iz"), &shn = b.I("cs_shn");
+        std::vector<int> a(iz.begin(), iz.end()), c(shn.begin(), shn.end());
+        CK(upload(h, a.data(), a.size(), &T.cs_iz));
+        CK(upload(h, c.data(), c.size(), &T.cs_shn));
+    }
     CK(pack_med<R>(h, b, nmed, "z", kMaxNE, &T.z));
     CK(pack_med<R>(h, b, nmed, "pz", kMaxNE, &T.pz));
     CK(pack_glob<R>(h, b, "binding_energies", &T.bind));
@@ -353,6 +359,26 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
         const auto &en = b.I("edge_number");
         std::vector<int> v(en.begin(), en.end());
         CK(upload(h, v.data(), v.size(), &T.enumb));
+    }
+    if (b.has("relax_zfirst")) {
+        auto up_i = [&](const char *k, const int **dst) {
+            const auto &a = b.I(k);
+            std::vector<int> v(a.begin(), a.end());
+            return upload(h, v.data(), v.size(), dst);
+        };
+        CK(up_i("relax_zfirst", &T.relax_zfirst));
+        CK(up_i("relax_nshell", &T.relax_nshell));
+        CK(up_i("shell_num", &T.shell_num));
+        CK(up_i("relax_first", &T.relax_first));
+        CK(up_i("relax_ntran", &T.relax_ntran));
+        CK(up_i("relax_atbin", &T.relax_atbin));
+        CK(up_i("relax_state", &T.relax_state));
+        CK(pack_glob<R>(h, b, "shell_be", &T.shell_be));
+        CK(pack_glob<R>(h, b, "relax_prob", &T.relax_prob));
+    } else {   // tables made without relaxation data: vacancies absorbed on the spot
+        T.relax_zfirst = T.relax_nshell = T.shell_num = T.relax_first = T.relax_ntran =
+            T.relax_atbin = T.relax_state = nullptr;
+        T.shell_be = T.relax_prob = nullptr;
     }
     CK(pack_med<R>(h, b, nmed, "dl", 48, &T.dl));
     {
@@ -1243,6 +1269,20 @@ int egsgpu_run_batched(egsgpu_handle *const *jobs, int32_t njobs, const int64_t 
         set_error("Woodcock tracking needs a majorant for each phantom; a batch shares job "
                   "0's: run batches with voxel-by-voxel tracking");
         return EGSGPU_E_UNSUPPORTED;
+    }
+    // the history slots (job 0's) must hold the voxels a history of any job
+    // touches: the largest table any job asks for (charged-particle sources
+    // ask for more than photon beams)
+    int hcap = h->slot_hcap;
+    for (int j = 1; j < njobs; j++) hcap = std::max(hcap, jobs[j]->slot_hcap);
+    if (hcap > h->slot_hcap) {
+        if (h->ev_ready) {
+            set_error("job 0 already ran with history tables of %d entries; this batch needs "
+                      "%d (a job with a charged-particle source): run the batch before job 0 "
+                      "runs alone", h->slot_hcap, hcap);
+            return EGSGPU_E_ARG;
+        }
+        h->slot_hcap = hcap;
     }
     int rc = ensure_lanes(h);
     if (rc) return rc;

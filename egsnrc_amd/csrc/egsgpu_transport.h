@@ -920,8 +920,11 @@ struct Shower {
         p.e = pese;
         p.iq = -1;
         if (ibcmpl == 1 || ibcmpl == 3) {
-            // vacancy energy absorbed locally (relaxation products below cut-offs)
-            if (Uj > 0) score(p.ir, Uj, p.wt);
+            // shell vacancy (:778-794): above 1 keV filled by the EADL
+            // cascade, whose sub-threshold energy is absorbed here; below,
+            // Uj is absorbed
+            R edep = Uj > (R)1e-3 ? eadl_relax(T.cs_iz[j], T.cs_shn[j]) : Uj;
+            if (edep > 0) score(p.ir, edep, p.wt);
         }
     }
 
@@ -932,8 +935,8 @@ struct Shower {
         if (peig < T.ee[1 * 16 + 1]) { p.iq = -1; p.e = peig + RM; return; }
         bool do_relax = false;
         R e_vac = 0;
+        int iZ = 0, vac_shell = 0;
         if (P.iedgfl) {
-            int iZ;
             if (M.nne == 1) {
                 iZ = (int)(T.z[m * kMaxNE] + (R)0.5);
             } else {
@@ -990,6 +993,7 @@ struct Shower {
                     e_vac = T.bind[iZ * 30 + k - 1];
                     p.e = peig - e_vac + RM;
                     do_relax = true;
+                    vac_shell = k;
                     p.iq = -1;
                 }
             }
@@ -998,7 +1002,103 @@ struct Shower {
             p.iq = -1;
         }
         if (P.iphter) photoelectron_direction();
-        if (do_relax) score(p.ir, e_vac, p.wt);
+        if (do_relax) {   // RELAX (:5191-5193)
+            R edep = eadl_relax(iZ, vac_shell);
+            if (edep > 0) score(p.ir, edep, p.wt);
+        }
+    }
+
+    // egs_eadl_relax (egsnrc.mortran:6020-6216): fill a vacancy in shell
+    // shell_egs of element iZ with fluorescent photons and Auger / Coster-
+    // Kronig electrons (EADL transitions sampled from alias histograms),
+    // last vacancy first.  Products above PCUT / ECUT go on the stack with
+    // the position, region and weight of the particle on top when the
+    // cascade starts and isotropic directions; the energy of the others and
+    // of vacancies below 1 keV is returned to be absorbed on the spot.  The
+    // open vacancies (at most kRelaxVac; 4 are the most a K vacancy of lead
+    // needs) are packed 16 bits each into two registers.  A transition from
+    // the bin that completes probabilities summing to less than 1 gives its
+    // whole energy to an electron.
+    static constexpr int kRelaxVac = 8;
+    __device__ __forceinline__ R eadl_relax(int iZ, int shell_egs) {
+        const R RM = (R)0.5109989461;
+        if (!T.relax_zfirst) {   // no relaxation data: absorbed on the spot
+            return shell_egs >= 1 && shell_egs <= 30 ? T.bind[iZ * 30 + shell_egs - 1] : (R)0;
+        }
+        if (iZ < 1 || iZ > 100) return 0;
+        const int zf = T.relax_zfirst[iZ];
+        if (zf < 0 || shell_egs < 1 || shell_egs > T.relax_nshell[iZ]) return 0;
+        int vac = zf + shell_egs - 1;
+        R Evac = T.shell_be[vac];
+        if (shell_egs > 4) return Evac;
+        const R Ec = (R)P.ecut - RM, Pc = (R)P.pcut, min_E = (R)1e-3;
+        const Part<R> save = p;
+        R edep = 0;
+        unsigned long long vs0 = 0, vs1 = 0;   // vacancy stack, 16 bits per entry
+        int nvac = 0;
+        auto vpush = [&](int v) {
+            if (nvac >= kRelaxVac) { c_ovf++; edep += T.shell_be[v]; return; }
+            vs1 = (vs1 << 16) | (vs0 >> 48);
+            vs0 = (vs0 << 16) | (unsigned long long)(unsigned)v;
+            nvac++;
+        };
+        for (int guard = 0; guard < 256; guard++) {
+            if (Evac < min_E || T.relax_ntran[vac] < 1) {
+                edep += Evac;
+            } else {
+                const int first = T.relax_first[vac], n = T.relax_ntran[vac];
+                // sample_alias_histogram (egs_utilities.mortran:1994-2019)
+                const R r1 = rnd(), r2 = rnd();
+                int ibin = 1 + (int)((R)n * r1);
+                ibin = ibin > n ? n : ibin;
+                if (r2 > T.relax_prob[first + ibin - 1]) ibin = T.relax_atbin[first + ibin - 1];
+                const int st = T.relax_state[first + ibin - 1];
+                const int base = vac - T.shell_num[vac];
+                R Ef, Ecc;
+                int iqf;
+                if (st < 0) {
+                    Ef = 0; iqf = -1; Ecc = Ec;
+                } else if (st <= 64) {   // fluorescence
+                    iqf = 0;
+                    Ef = T.shell_be[base + st];
+                    vpush(base + st);
+                    Ecc = Pc;
+                } else {                  // Auger or Coster-Kronig
+                    iqf = -1;
+                    const int n1 = base + st / 64, n2 = base + st % 64;
+                    Ef = T.shell_be[n1] + T.shell_be[n2];
+                    vpush(n1);
+                    vpush(n2);
+                    Ecc = Ec;
+                }
+                const R Ex = Evac - Ef;
+                if (Ex > Ecc) {
+                    Part<R> q = save;
+                    q.iq = iqf;
+                    const R cost = 2 * rnd() - 1;
+                    R sint = 1 - cost * cost;
+                    q.u = 0; q.v = 0; q.w = cost;
+                    if (sint > 0) {
+                        sint = dsqrt(sint);
+                        R cphi, sphi;
+                        azimuth(cphi, sphi);
+                        q.u = sint * cphi;
+                        q.v = sint * sphi;
+                    }
+                    q.e = iqf == 0 ? Ex : Ex + RM;
+                    if (!push(q)) edep += Ex;   // stack full: absorbed (counted)
+                } else {
+                    edep += Ex;
+                }
+            }
+            if (nvac == 0) break;
+            vac = (int)(vs0 & 0xffffull);
+            vs0 = (vs0 >> 16) | (vs1 << 48);
+            vs1 >>= 16;
+            nvac--;
+            Evac = T.shell_be[vac];
+        }
+        return edep;
     }
 
     __device__ void photoelectron_direction() {

@@ -20,6 +20,9 @@ BLOBS = {
                         "AIR521ICRU", "AL521ICRU"]),
     "700icru.egstab": ("700icru.pegs4dat",
                        ["H2O700ICRU", "ICRPBONE700ICRU", "AIR700ICRU", "LUNG700ICRU"]),
+    # high-Z media of the tutorial data set (AE 0.7, AP 0.01 MeV): tantalum
+    # of tutor2, lead for the atomic relaxation cascade, and water
+    "tutor.egstab": ("tutor_data.pegs4dat", ["TA", "PB", "H2O"]),
 }
 
 

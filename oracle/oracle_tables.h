@@ -96,6 +96,11 @@ struct Tables {
     // EDGE common
     std::vector<double> bind, iprob, ea, eb, ec, ed, ee;
     std::vector<int32_t> enumb;
+    // EADL relaxation (egs_init_relax): shells of the media's elements and
+    // their transitions as alias histograms
+    std::vector<int32_t> relax_zfirst, relax_nshell, shell_num, relax_first, relax_ntran,
+        relax_atbin, relax_state;
+    std::vector<double> shell_be, relax_prob;
     // MS data
     std::vector<double> ms_u, ms_f, ms_w, ms_const;
     std::vector<int32_t> ms_i;
@@ -157,6 +162,11 @@ struct Tables {
         bind = b.F("binding_energies"); iprob = b.F("interaction_prob");
         ea = b.F("edge_a"); eb = b.F("edge_b"); ec = b.F("edge_c");
         ed = b.F("edge_d"); ee = b.F("edge_energies"); enumb = b.I("edge_number");
+        relax_zfirst = b.I("relax_zfirst"); relax_nshell = b.I("relax_nshell");
+        shell_num = b.I("shell_num"); relax_first = b.I("relax_first");
+        relax_ntran = b.I("relax_ntran"); relax_atbin = b.I("relax_atbin");
+        relax_state = b.I("relax_state"); shell_be = b.F("shell_be");
+        relax_prob = b.F("relax_prob");
         ms_u = b.F("ms_u"); ms_f = b.F("ms_f"); ms_w = b.F("ms_w");
         ms_i = b.I("ms_i"); ms_const = b.F("ms_const");
         esige_max = b.F("esige_max")[0]; psige_max = b.F("psige_max")[0];

@@ -74,6 +74,8 @@ struct Shower {
     std::vector<Ev> *evlog = nullptr;
     int64_t n_overflow = 0;
     int64_t c_esteps = 0;
+    int64_t c_relax_out = 0;   // relaxation products put on the stack
+    int relax_depth = 0;       // largest vacancy-stack depth of the cascades
 
     Shower(const Problem &p, std::vector<double> &t, std::vector<int> &tl)
         : pb(p), T(p.T), tmp(t), touched(tl) {}
@@ -458,10 +460,10 @@ struct Shower {
             top().e = pese;
             top().iq = -1;
             if (ibcmpl == 1 || ibcmpl == 3) {
-                // RELAX / egs_eadl_relax: every relaxation product of the
-                // media handled here is below the cut-offs, so the vacancy
-                // energy is absorbed on the spot (ausgab iarg = 4).
-                double edep = Uj;
+                // shell vacancy (:778-794): above 1 keV filled by RELAX
+                // (egs_eadl_relax), its sub-threshold products absorbed on
+                // the spot; below, Uj is absorbed (ausgab iarg = 4)
+                double edep = Uj > 1e-3 ? eadl_relax(T.cs_iz[j], T.cs_shn[j]) : Uj;
                 if (edep > 0) score(top().ir, edep, top().wt);
             }
         }
@@ -477,7 +479,7 @@ struct Shower {
             p.e = peig + PRM;
             return;
         }
-        int iZ = 0;
+        int iZ = 0, vac_shell = 0;
         bool do_relax = false;
         double e_vac = 0;
         if (pb.P.iedgfl) {
@@ -536,6 +538,7 @@ struct Shower {
                     e_vac = T.bind[iZ * 30 + k - 1];
                     p.e = peig - e_vac + PRM;
                     do_relax = true;
+                    vac_shell = k;
                     p.iq = -1;
                 }
             }
@@ -544,7 +547,90 @@ struct Shower {
             p.iq = -1;
         }
         if (pb.P.iphter) photoelectron_direction();
-        if (do_relax) score(top().ir, e_vac, top().wt);   // relaxation below cut-offs
+        if (do_relax) {   // RELAX (:5191-5193): the cascade; what stays is absorbed here
+            double edep = eadl_relax(iZ, vac_shell);
+            if (edep > 0) score(top().ir, edep, top().wt);
+        }
+    }
+
+    // egs_eadl_relax (egsnrc.mortran:6020-6216): fill a vacancy in shell
+    // shell_egs of element iZ by fluorescent photons and Auger and Coster-
+    // Kronig electrons (EADL transition data as alias histograms).  Products
+    // above PCUT / ECUT go on the stack, with the position, region and
+    // weight of the particle on top when the cascade starts and isotropic
+    // directions; the energy of the others and of vacancies below 1 keV is
+    // returned for local absorption.  Vacancies are filled last in, first
+    // out.  A transition drawn from the bin that completes probabilities
+    // summing to less than 1 gives its whole energy to an electron.
+    double eadl_relax(int iZ, int shell_egs) {
+        if (iZ < 1 || iZ > 100 || T.relax_zfirst[iZ] < 0 || shell_egs < 1 ||
+            shell_egs > T.relax_nshell[iZ])
+            return 0;   // unknown vacancy: nothing is done (:6108)
+        const int shell = T.relax_zfirst[iZ] + shell_egs - 1;
+        const double Ec = pb.P.ecut - RM, Pc = pb.P.pcut, min_E = 1e-3;
+        double Evac = T.shell_be[shell];
+        if (shell_egs > 4) return Evac;   // below L3: absorbed (:6126-6139)
+        const Particle save = top();
+        double edep = 0;
+        int vacs[128];
+        int nvac = 0, vac = shell;
+        for (;;) {
+            if (Evac < min_E || T.relax_ntran[vac] < 1) {
+                edep += Evac;
+            } else {
+                const int first = T.relax_first[vac], n = T.relax_ntran[vac];
+                // sample_alias_histogram (egs_utilities.mortran:1994-2019)
+                double r1 = rnd(), r2 = rnd();
+                int ibin = 1 + (int)(n * r1);
+                if (r2 > T.relax_prob[first + ibin - 1]) ibin = T.relax_atbin[first + ibin - 1];
+                const int st = T.relax_state[first + ibin - 1];
+                const int base = vac - T.shell_num[vac];   // shell 0 of the element
+                double Ef, Ecc;
+                int iqf;
+                if (st < 0) {
+                    Ef = 0; iqf = -1; Ecc = Ec;
+                } else if (st <= 64) {                        // fluorescence
+                    iqf = 0;
+                    const int nw = base + st;
+                    Ef = T.shell_be[nw];
+                    vacs[nvac++] = nw;
+                    Ecc = Pc;
+                } else {                                      // Auger, Coster-Kronig
+                    iqf = -1;
+                    const int n1 = base + st / 64, n2 = base + st % 64;
+                    Ef = T.shell_be[n1] + T.shell_be[n2];
+                    vacs[nvac++] = n1;
+                    vacs[nvac++] = n2;
+                    Ecc = Ec;
+                }
+                if (nvac > 126) throw std::runtime_error("relaxation: too many vacancies");
+                relax_depth = std::max(relax_depth, nvac);
+                const double Ex = Evac - Ef;
+                if (Ex > Ecc) {
+                    Particle q = save;
+                    q.iq = iqf;
+                    const double cost = 2 * rnd() - 1;
+                    double sint = 1 - cost * cost;
+                    if (sint > 0) {
+                        sint = std::sqrt(sint);
+                        double cphi, sphi;
+                        azimuth(cphi, sphi);
+                        q.u = sint * cphi; q.v = sint * sphi; q.w = cost;
+                    } else {
+                        q.u = 0; q.v = 0; q.w = cost;
+                    }
+                    q.e = iqf == 0 ? Ex : Ex + RM;
+                    stk.push_back(q);
+                    c_relax_out++;
+                } else {
+                    edep += Ex;
+                }
+            }
+            if (nvac == 0) break;
+            vac = vacs[--nvac];
+            Evac = T.shell_be[vac];
+        }
+        return edep;
     }
 
     // $SELECT-PHOTOELECTRON-DIRECTION (egsnrc.macros:1916-1950)
@@ -1914,6 +2000,25 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
                 sh.msdist_pII(M, energy, 0.0, param, 1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, us, vs, ws,
                               xf, yf, zf, ustep);
                 o[0] = xf; o[1] = yf; o[2] = zf; o[3] = ws;
+            } else if (kind == 11) {
+                // egs_eadl_relax of a vacancy in shell `param` of element
+                // Z = energy (the medium sets the cut-offs): the first
+                // product (its energy, negative for an electron, 0 if none),
+                // photons + 1000 electrons put on the stack, their kinetic
+                // energy and the energy absorbed on the spot
+                p.e = 0; p.iq = 0; sh.stk.push_back(p);
+                sh.relax_depth = 0;
+                double edep = sh.eadl_relax((int)(energy + 0.5), (int)param);
+                double ekin = 0, first = 0;
+                int nph = 0, nel = 0;
+                for (size_t i = 1; i < sh.stk.size(); i++) {
+                    const Particle &q = sh.stk[i];
+                    double ek = q.iq == 0 ? q.e : q.e - PRM;
+                    ekin += ek;
+                    if (q.iq == 0) nph++; else nel++;
+                    if (i == 1) first = q.iq == 0 ? ek : -ek;
+                }
+                o[0] = first; o[1] = nph + 1000.0 * nel + 1e6 * sh.relax_depth; o[2] = ekin; o[3] = edep;
             } else {
                 throw std::runtime_error("unknown sampler kind");
             }

@@ -128,9 +128,16 @@ def test_pair_energy_conservation_and_symmetry():
 
 
 def test_photo_energy_conservation():
+    """Photo-electron plus the energy absorbed on the spot equals the photon
+    energy, except for a vacancy in a shell without an XCOM edge (carbon's
+    L3): PHOTO gives the electron the whole photon energy (its binding
+    energy there is 0) and the relaxation then absorbs the EADL binding
+    energy of the shell (8.98 eV), as egs_eadl_relax does."""
     for k in (0.02, 0.1, 1.0):
         out = oracle_lib.sample(BLOB, 5, 2, k, 20000)   # ICRP bone
-        assert np.allclose(out[:, 0] - RM + out[:, 1], k, atol=1e-12)
+        r = out[:, 0] - RM + out[:, 1] - k
+        assert np.all((np.abs(r) < 1e-12) | (np.abs(r - 8.98e-6) < 1e-9)), r[np.abs(r) > 1e-12]
+        assert np.mean(np.abs(r) > 1e-12) < 1e-3
 
 
 def test_brems_kinematics():

