@@ -332,8 +332,15 @@ __device__ __forceinline__ void add_counters(const DevTally &t,
 // History schedule: lane s runs its share of the histories one after the
 // other with ranmar stream s, like egs-parallel job s.  SPLIT: every photon
 // goes through DOSXYZnrc's photon splitting.
+// Resident waves per SIMD the history kernels are compiled for: 2 (a
+// 256-register budget with a few hundred bytes of scratch) ran the slab
+// workload 1.45x and HOWFARLESS 1.6x faster than 1 (344 registers, no
+// scratch); 3 was slower than either.
+constexpr int kHistWaves = 2;
+
 template <typename R, bool LUX, bool HFL = false, bool SPLIT = false>
-__global__ void __launch_bounds__(256) k_transport(KArgs<R> a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHistWaves)))
+k_transport(KArgs<R> a) {
     extern __shared__ float rbuf_all[];
     int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = lane < a.L.nlanes;   // lanes past the last stream idle

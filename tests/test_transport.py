@@ -436,6 +436,33 @@ def test_c2_electron_beam_reduced_parity(engine):
 
 
 @pytest.mark.gpu
+def test_c2_full_size_parity(engine):
+    """C2 at full size: 20 MeV electrons, 10 x 10 cm^2, on 256^3 x 0.2 cm
+    water with a 3 cm aluminium cube at 3 cm depth (spin effects off: the
+    reference tree has no spin data), 2e5 histories on the default event
+    schedule against the oracle, with a second oracle run for the 2-sigma
+    fraction (a voxel of this phantom gets its dose from about 80
+    histories)."""
+    xb, yb, zb, med, rhor = workloads.al_cube_phantom(256, 0.2)
+    src = make_source(-1, 20.0, x0=-5, x1=5, y0=-5, y1=5, z=-1.0)
+    tp = _lib.default_transport(ecut=0.521, pcut=0.01)
+    n = 200000
+    g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
+    ref = []
+    for seeds in ((1802, 9373), NULL_SEEDS):
+        o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256,
+                                        nthreads=oracle_threads())
+        o.seed(*seeds)
+        o.run(n)
+        ref.append(o.tallies())
+        del o
+    sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp)
+    sim.run(n)
+    check_parity(ref[0], sim.tallies(), sim.counters(), label="c2 full size",
+                 total_rtol=0.003, null=ref[1])
+
+
+@pytest.mark.gpu
 def test_c1_full_size_parity(engine):
     """C1 at full size: 128^3 x 0.25 cm water / lung (0.26) / bone (1.85) /
     water slabs, 10 x 10 cm^2 parallel beam of the 60-bin falling 0.1-6 MeV
