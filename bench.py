@@ -134,7 +134,12 @@ def cpu_baseline(name, seconds_target=15.0):
     dt = time.perf_counter() - t
     rec = {"value": n2 / dt, "unit": "histories/s", "cores": cores, "kind": "port",
            "sample": "%d histories of the same workload (FP64 oracle, %d threads, %.1f s)"
-                     % (n2, cores, dt)}
+                     % (n2, cores, dt),
+           "note": "the oracle (a scalar FP64 restatement of the reference's routines) "
+                   "stands in for the reference's CPU path, which cannot be built here: "
+                   "egs++ needs the configure-generated egs_config1.h and the Mortran "
+                   "core mortran3-generated Fortran; 16 threads = one GPU's share of the "
+                   "host"}
     return rec, o.tallies()
 
 
