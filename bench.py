@@ -665,9 +665,11 @@ def main():
     ap.add_argument("--batching", default="",
                     help="event kernels: photon steps per turn, interaction batch, max wait, "
                          "defer below")
-    ap.add_argument("--autotune", type=int, default=0,
-                    help="run the event-schedule autotuner with this many probe histories "
-                         "before the warm-up")
+    ap.add_argument("--autotune", type=int, default=500000,
+                    help="run the event-schedule autotuner (egsgpu_autotune: register budget, "
+                         "queue capacity, workgroup size, deferral threshold, interaction batch, "
+                         "photon steps) with this many probe histories before the warm-up, "
+                         "outside the timed region; 0 = library defaults")
     ap.add_argument("--compare-plain", action="store_true",
                     help="batch256: also run the jobs one at a time with the plain kernel")
     ap.add_argument("--grad-hist", type=int, default=100_000_000,
@@ -814,7 +816,8 @@ def main():
                        schedule=a.schedule or ("history (HOWFARLESS)" if a.howfarless
                                                else "default"), graph=a.graph,
                        **({"autotuned": dict(zip(["waves_photon", "waves_electron",
-                                                  "defer_below", "batch", "photon_steps"],
+                                                  "defer_below", "batch", "photon_steps",
+                                                  "slots_per_stream", "workgroup"],
                                                  tuned))} if tuned else {})),
         "efficiency": {"value": eff, "unit": "1/(sigma^2 s)",
                        "sigma2": rel2, "definition": "mean relative variance over D > 0.5 Dmax"},

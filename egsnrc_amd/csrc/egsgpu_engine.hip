@@ -189,6 +189,8 @@ struct egsgpu_handle {
     int schedule = EGSGPU_SCHED_EVENT;
     bool howfarless = false;            // DOSXYZnrc HOWFARLESS (homogeneous phantom)
     bool ev_ready = false;
+    size_t ev_alloc_first = 0, ev_alloc_end = 0;   // the event state's entries of allocs
+    int ev_block = 0;                   // event kernels' workgroup size (0: block)
     int nslots = 0, slot_hcap = 256;
     DevSlots slots{};
     EvCtl *ctl = nullptr;
@@ -665,6 +667,7 @@ int alloc_bank(egsgpu_handle *h, int cap, DevBank<R> &b) {
 template <typename R>
 int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     if (h->ev_ready) return EGSGPU_OK;
+    h->ev_alloc_first = h->allocs.size();
     DevSlots &S = h->slots;
     S.nslots = h->nslots > 0 ? h->nslots : 4 * h->nlanes;
     S.hcap = h->slot_hcap;
@@ -702,6 +705,23 @@ int ensure_event(egsgpu_handle *h, Precision<R> &P) {
     EGS_HIP(hipStreamSynchronize(h->stream));
     h->free_head = 0;
     h->ev_ready = true;
+    h->ev_alloc_end = h->allocs.size();
+    return EGSGPU_OK;
+}
+
+// Free the event schedule's slots, banks and control words (between runs,
+// when every bank is empty), so that the next run makes them again with
+// another number of slots.
+int release_event(egsgpu_handle *h) {
+    if (!h->ev_ready) return EGSGPU_OK;
+    EGS_HIP(hipStreamSynchronize(h->stream));
+    for (size_t i = h->ev_alloc_first; i < h->ev_alloc_end; i++) (void)hipFree(h->allocs[i]);
+    h->allocs.erase(h->allocs.begin() + (long)h->ev_alloc_first,
+                    h->allocs.begin() + (long)h->ev_alloc_end);
+    if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+    h->ctl_host = nullptr;
+    h->ctl = nullptr;
+    h->ev_ready = false;
     return EGSGPU_OK;
 }
 
@@ -771,18 +791,19 @@ int run_event(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
     a.max_wait = h->ev_max_wait;
     a.defer_below = h->ev_defer;
     a.gated = 0;
-    const dim3 grid((h->nlanes + h->block - 1) / h->block), block(h->block);
+    const int eb = h->ev_block ? h->ev_block : h->block;   // workgroup size
+    const dim3 grid((h->nlanes + eb - 1) / eb), block(eb);
     // persistent kernels: launch what stays resident at the chosen variant
     auto resident = [&](int w) {
         int n = std::min(h->nlanes, h->ncu * 4 * 64 * w);
-        return dim3((n + h->block - 1) / h->block);
+        return dim3((n + eb - 1) / eb);
     };
     const dim3 grid_p = resident(h->ev_wp), grid_e = resident(h->ev_we);
     const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
     auto kp = photon_kernel<R>(h->ev_wp, lux);
     auto ke = electron_kernel<R>(h->ev_we, lux);
     auto ks = lux ? k_ev_source<R, true> : k_ev_source<R, false>;
-    const size_t lds = (size_t)kRing * h->block * sizeof(float);
+    const size_t lds = (size_t)kRing * eb * sizeof(float);
     const bool photon_src = P.S.charge == 0;
     auto setb = [&](int ic, int ip, int id) {
         a.ic = ic; a.ip = ip; a.id = id;
@@ -878,17 +899,18 @@ int run_event_graph(egsgpu_handle *h, Precision<R> &P, int64_t nhist,
     a.max_wait = h->ev_max_wait;
     a.defer_below = h->ev_defer;
     a.gated = 1;      // the kernels return at once after the run has ended
-    const dim3 grid((h->nlanes + h->block - 1) / h->block), block(h->block);
+    const int eb = h->ev_block ? h->ev_block : h->block;   // workgroup size
+    const dim3 grid((h->nlanes + eb - 1) / eb), block(eb);
     auto resident = [&](int w) {
         int n = std::min(h->nlanes, h->ncu * 4 * 64 * w);
-        return dim3((n + h->block - 1) / h->block);
+        return dim3((n + eb - 1) / eb);
     };
     const dim3 grid_p = resident(h->ev_wp), grid_e = resident(h->ev_we);
     const bool lux = h->rng_kind == EGSGPU_RNG_RANLUX;
     auto kp = photon_kernel<R>(h->ev_wp, lux);
     auto ke = electron_kernel<R>(h->ev_we, lux);
     auto ks = lux ? k_ev_source<R, true> : k_ev_source<R, false>;
-    const size_t lds = (size_t)kRing * h->block * sizeof(float);
+    const size_t lds = (size_t)kRing * eb * sizeof(float);
     const bool photon_src = P.S.charge == 0;
     auto setb = [&](int ic, int ip, int id) {
         a.ic = ic; a.ip = ip; a.id = id;
@@ -1409,7 +1431,8 @@ int egsgpu_set_event_batching(egsgpu_handle *h, int32_t photon_steps, int32_t ba
 int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice) {
     EGS_ARG(h && nhist_probe > 0, "handle, nhist_probe > 0");
     if (h->schedule != EGSGPU_SCHED_EVENT) {   // nothing to choose
-        if (choice) { choice[0] = choice[1] = choice[2] = choice[3] = choice[4] = 0; }
+        if (choice)
+            for (int k = 0; k < 7; k++) choice[k] = 0;
         return EGSGPU_OK;
     }
     int rc = ensure_lanes(h);
@@ -1440,23 +1463,41 @@ int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice) {
     double best, ms;
     if ((rc = probe(best))) return done(rc);   // warm-up
     if ((rc = probe(best))) return done(rc);
-    // coordinate search: register budget, deferral, interaction batch,
-    // photon steps per turn
-    struct Knob { int *v; std::vector<int> opts; };
+    // coordinate search: register budget, queue capacity (history slots
+    // per stream; the banks hold 8 particles per slot), workgroup size,
+    // deferral threshold (a wave hands its particles on when fewer lanes are
+    // busy), interaction batch, photon steps per turn
+    struct Knob { int *v; std::vector<int> opts; bool slots; };
     int wpe = h->ev_wp;
-    std::vector<Knob> knobs = {{&wpe, {2, 3}}, {&h->ev_defer, {16, 32, 48}},
-                               {&h->ev_batch, {8, 16, 24}}, {&h->ev_ksteps, {4, 8, 16}}};
+    int spl = h->nslots > 0 ? std::max(1, h->nslots / h->nlanes) : 4;
+    int eb = h->ev_block ? h->ev_block : h->block;
+    std::vector<Knob> knobs = {{&wpe, {2, 3}, false},
+                               {&spl, {2, 4, 8}, true},
+                               {&eb, {128, 256}, false},
+                               {&h->ev_defer, {16, 32, 48}, false},
+                               {&h->ev_batch, {8, 16, 24}, false},
+                               {&h->ev_ksteps, {4, 8, 16}, false}};
+    auto apply = [&]() -> int {
+        h->ev_wp = h->ev_we = wpe;
+        h->ev_block = eb;
+        if (h->nslots != spl * h->nlanes) {
+            CK(release_event(h));
+            h->nslots = spl * h->nlanes;
+        }
+        return EGSGPU_OK;
+    };
     for (auto &k : knobs) {
         int cur = *k.v, pick = cur;
         for (int o : k.opts) {
             if (o == cur) continue;
             *k.v = o;
-            h->ev_wp = h->ev_we = wpe;
+            if ((rc = apply())) return done(rc);
+            if (k.slots && (rc = probe(ms))) return done(rc);   // the new slots' first run
             if ((rc = probe(ms))) return done(rc);
             if (ms < best) { best = ms; pick = o; }
         }
         *k.v = pick;
-        h->ev_wp = h->ev_we = wpe;
+        if ((rc = apply())) return done(rc);
     }
     if (hipMemcpyAsync(h->tally.sum, keep, nb, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
         hipMemcpyAsync(h->tally.sum2, keep + nt, nb, hipMemcpyDeviceToDevice, h->stream) !=
@@ -1469,6 +1510,7 @@ int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice) {
     if (choice) {
         choice[0] = h->ev_wp; choice[1] = h->ev_we; choice[2] = h->ev_defer;
         choice[3] = h->ev_batch; choice[4] = h->ev_ksteps;
+        choice[5] = spl; choice[6] = eb;
     }
     return done(EGSGPU_OK);
 }

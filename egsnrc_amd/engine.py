@@ -157,8 +157,9 @@ class Simulation:
 
     def autotune(self, nhist_probe=200000):
         """egsgpu_autotune; returns the chosen (waves photon, waves electron,
-        defer_below, batch, photon steps)."""
-        c = np.zeros(5, dtype=np.int32)
+        defer_below, batch, photon steps, slots per stream, workgroup
+        size)."""
+        c = np.zeros(7, dtype=np.int32)
         check(_lib.lib().egsgpu_autotune(self.h, int(nhist_probe), ptr(c)))
         return tuple(int(x) for x in c)
 

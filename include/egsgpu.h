@@ -278,12 +278,16 @@ int egsgpu_graph_stats(egsgpu_handle *h, int32_t *launches, int32_t *iterations,
 
 /* Autotuner of the event schedule: times short runs of nhist_probe
  * histories while varying, one after the other, the register budget (2 or
- * 3 waves per SIMD), the deferral threshold, the interaction batch and the
- * photon steps per turn, and keeps the fastest setting.  The probes
+ * 3 waves per SIMD), the queue capacity (2, 4 or 8 history slots per
+ * stream, the particle banks holding 8 per slot), the workgroup size (128
+ * or 256), the deferral threshold (the number of busy lanes below which a
+ * wave hands its particles to the next launch), the interaction batch and
+ * the photon steps per turn, and keeps the fastest setting.  The probes
  * advance the RNG streams but leave tallies, history count and counters as
- * they were.  choice (5 ints, may be NULL) receives waves photon, waves
- * electron, defer_below, batch, photon steps.  A no-op for the history
- * schedule.  No counterpart in the reference. */
+ * they were.  choice (7 ints, may be NULL) receives waves photon, waves
+ * electron, defer_below, batch, photon steps, slots per stream, workgroup
+ * size.  A no-op for the history schedule.  No counterpart in the
+ * reference. */
 int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice);
 
 /* How photons fly between interactions.

@@ -373,6 +373,9 @@ def test_batched_jobs_match_oracle(engine):
 
 @pytest.mark.gpu
 def test_autotune_leaves_tallies_alone(engine):
+    """The probes leave tallies, history count and counters as they were;
+    the choices are among the offered ones; the tuned schedule passes the
+    coupled-photon parity."""
     xb, yb, zb, med, rhor, src, tp = photon_coupled_case()
     sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=8192,
                      schedule="event")
@@ -385,9 +388,17 @@ def test_autotune_leaves_tallies_alone(engine):
     np.testing.assert_array_equal(after[0], before[0])
     np.testing.assert_array_equal(after[1], before[1])
     np.testing.assert_array_equal(sim.counters(), c0)
-    wp, we, defer, batch, ks = choice
+    wp, we, defer, batch, ks, spl, wg = choice
     assert wp in (2, 3) and we == wp and defer in (16, 32, 48) and batch in (8, 16, 24)
-    assert ks in (4, 8, 16)
+    assert ks in (4, 8, 16) and spl in (2, 4, 8) and wg in (128, 256)
+    print("autotuned:", choice)
+    # the tuned schedule (queue capacity and workgroup size included) still
+    # transports correctly
+    _lib.check(_lib.lib().egsgpu_reset_tallies(sim.h))
+    sim.run(PHOTON_HIST)
+    check_parity(_oracle_photon_coupled(PHOTON_HIST), sim.tallies(), sim.counters(),
+                 label="photons coupled/autotuned",
+                 null=_oracle_cached("photon slabs", photon_coupled_case, PHOTON_HIST, NULL_SEEDS))
 
 
 BLOB700 = os.path.join(os.path.dirname(BLOB521), "700icru.egstab")
