@@ -57,3 +57,31 @@ def test_app_runs_c0_input(tmp_path):
     axis = d3[:, 14, 14]
     assert axis.max() == d.max()
     assert np.all(axis[:25] > 0)
+    # dose = r * 1.602e-10 / (V rho) in Gy per history with the relative
+    # uncertainty beside it (egs_dose_scoring.cpp:416-479): the same problem
+    # run through the C ABI from Python agrees voxel by voxel, and in the
+    # energy deposited per history
+    from egsnrc_amd import _lib, workloads
+    from egsnrc_amd.engine import Simulation, Tables, make_source
+    from tests.parity import check_parity_means
+    xb, yb, zb, med, rhor = workloads.water_cube(30, 1.0)
+    src = make_source(0, 1.25, shape="point", x0=0.0, y0=0.0, z=-1.0)
+    tp = _lib.default_transport(ecut=0.7, pcut=0.01, rayleigh=0)
+    mass = 1.0 * 1.0                      # 1 cm^3 of water at 1 g/cm^3
+
+    def python_run(seeds):
+        sim = Simulation(Tables(BLOB700), xb, yb, zb, med, rhor, src, tp)
+        sim.seed(*seeds)
+        sim.run(200000)
+        su, s2, n = sim.tallies()
+        sim.close()
+        r = su / n
+        var = np.maximum(s2 / n - r * r, 0) / (n - 1)
+        return r * 1.602e-10 / mass, np.sqrt(var) * 1.602e-10 / mass
+    dg, sg = python_run((4711, 20000))
+    null = python_run((9001, 30000))      # the spread of two runs of the same problem
+    sa = d * e                            # the file's relative uncertainty
+    check_parity_means(dg, sg, d, sa, label="c0 .3ddose vs C ABI", frac=0.002,
+                       min_voxels=400, null=null)
+    ea, eg = d.sum() / 1.602e-10, dg.sum() / 1.602e-10     # MeV per history
+    assert abs(ea - eg) < 0.01 * eg, (ea, eg)   # ~0.1 % statistical; a wrong norm is far off
