@@ -45,7 +45,8 @@ class EgsTransport(C.Structure):
                 ("ximax", C.c_double), ("skin_depth", C.c_double),
                 ("kerma", C.c_int32), ("fp64", C.c_int32), ("howfarless", C.c_int32),
                 ("n_split", C.c_int32), ("nbr_split", C.c_int32), ("play_rr", C.c_int32),
-                ("ireject", C.c_int32), ("esave", C.c_double)]
+                ("ireject", C.c_int32), ("esave", C.c_double),
+                ("step_algorithm", C.c_int32)]
 
 
 def default_transport(**kw):
@@ -54,7 +55,8 @@ def default_transport(**kw):
                      photoelectron_angle=1, atomic_relaxation=1, pair_angle=1,
                      brems_angle=1, spin=0, exact_bca=1, estepe=0.25,
                      ximax=0.5, skin_depth=3.0, kerma=0, fp64=0, howfarless=0,
-                     n_split=0, nbr_split=0, play_rr=0, ireject=0, esave=0.0)
+                     n_split=0, nbr_split=0, play_rr=0, ireject=0, esave=0.0,
+                     step_algorithm=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t

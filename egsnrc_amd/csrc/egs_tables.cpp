@@ -61,6 +61,20 @@ int blob_write(const Blob &b, const std::string &path) {
     return EGSGPU_OK;
 }
 
+double bca_blcmin(const Blob &b, const std::vector<int> &media, double ecutmn) {
+    // the blob keeps blcc and xcc as mscati leaves them after this point:
+    // blcc times exp(2 gamma_E - 1) and xcc squared
+    const double beta2 = (ecutmn - kRM) * (ecutmn + kRM) / (ecutmn * ecutmn);
+    double best = 1e30;
+    for (int m : media) {
+        const double blcc = b.f1(Blob::mk(m, "blcc")) / 1.16699413758864573;
+        const double xcc2 = b.f1(Blob::mk(m, "xcc"));
+        const double t = blcc * beta2 * ecutmn * ecutmn / xcc2;
+        best = std::min(best, std::log(t / std::log(t)));
+    }
+    return best;
+}
+
 int blob_read(const std::string &path, Blob &b) {
     std::ifstream in(path, std::ios::binary);
     if (!in) { set_error("cannot open table blob %s", path.c_str()); return EGSGPU_E_IO; }

@@ -52,6 +52,12 @@ int tables_build(const std::string &hen_house, const std::string &pegs_file,
                  const std::vector<std::string> &media, const BuildOptions &o,
                  Blob &out);
 
+// mscati's default minimum step for PRESTA-I boundary crossing
+// (egsnrc.mortran:2790-2817): BLCMIN = min over `media` of log(t / log t),
+// t = blcc (E^2 - m^2) / xcc^2 at the lowest ECUT E, with the PEGS4 blcc and
+// xcc; the skin depth is exp(BLCMIN) elastic mean free paths.
+double bca_blcmin(const Blob &b, const std::vector<int> &media, double ecutmn);
+
 // Dimensions fixed by the reference data layout.
 constexpr int kMXGE = 2000;      // photon log-energy grid ($MXGE)
 constexpr int kMXRAYFF = 100;    // Rayleigh form factor points

@@ -117,6 +117,7 @@ struct DevParams {
     int nbr_split;            // radiative splitting number (0 = off)
     int play_rr;              // Russian roulette of photon-made charged particles
     int ireject;              // DOSXYZnrc range rejection below esave
+    int presta1;              // transport_algorithm PRESTA-I (msdist_pI)
     double ecut, pcut, skindepth;
     double esave;             // total energy (MeV) below which range rejection applies
 };

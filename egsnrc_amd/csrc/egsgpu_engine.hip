@@ -1028,9 +1028,9 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         set_error("spin effects need the spinms data, which the engine does not have");
         return EGSGPU_E_UNSUPPORTED;
     }
-    if (!tp->kerma && !tp->exact_bca) {
-        set_error("only the exact boundary-crossing algorithm is implemented");
-        return EGSGPU_E_UNSUPPORTED;
+    if (tp->step_algorithm != 0 && tp->step_algorithm != 1) {
+        set_error("step_algorithm = %d: 0 is PRESTA-II, 1 is PRESTA-I", (int)tp->step_algorithm);
+        return EGSGPU_E_ARG;
     }
     if (tp->n_split < 0 || tp->n_split > 256) {
         set_error("n_split = %d: photon splitting takes 0..256", (int)tp->n_split);
@@ -1134,7 +1134,23 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     // DOSXYZnrc switches range rejection off when ESAVE_GLOBAL <= ECUT
     h->P.esave = tp->esave;
     h->P.ireject = tp->ireject && tp->esave > h->P.ecut ? 1 : 0;
-    h->P.skindepth = tp->skin_depth > 1e-4 ? tp->skin_depth : 3.0;
+    h->P.presta1 = tp->step_algorithm == 1;
+    // mscati (egsnrc.mortran:2790-2817): skindepth_for_bca <= 1e-4 selects
+    // 3 elastic mean free paths with exact boundary crossing and
+    // exp(BLCMIN) at the lowest ECUT with PRESTA-I, the minimum taken over
+    // the media of the problem (those the voxels use)
+    if (tp->skin_depth > 1e-4) {
+        h->P.skindepth = tp->skin_depth;
+    } else if (tp->exact_bca) {
+        h->P.skindepth = 3.0;
+    } else {
+        std::vector<char> seen(nmed, 0);
+        for (int i = 0; i < h->nvox; i++) seen[med[i]] = 1;
+        std::vector<int> used;
+        for (int m = 0; m < nmed; m++)
+            if (seen[m]) used.push_back(m);
+        h->P.skindepth = std::exp(bca_blcmin(b, used, h->P.ecut));
+    }
     try {
         if (h->fp64) {
             if ((rc = build_tables(h, b, h->pd.T))) return fail(rc);

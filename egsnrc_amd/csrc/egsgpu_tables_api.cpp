@@ -1,4 +1,5 @@
 // egsgpu_tables_api.cpp -- C ABI for the cross-section table blob.
+#include <cmath>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -71,6 +72,26 @@ int egsgpu_tables_medium_info(const egsgpu_tables *t, int32_t i, char *name,
         snprintf(name, n, "%s", s.c_str());
     }
     if (rho) *rho = t->blob.f1(Blob::mk(i, "rho"));
+    return EGSGPU_OK;
+}
+
+int egsgpu_tables_bca_skindepth(const egsgpu_tables *t, double ecut, int32_t nmed,
+                                const int32_t *media, double *blcmin, double *skindepth) {
+    EGS_ARG(t && blcmin && skindepth, "tables, blcmin and skindepth");
+    EGS_ARG(ecut > kRM, "ecut is a total energy (MeV) above the electron rest mass");
+    const int all = t->blob.i1("nmed");
+    std::vector<int> use;
+    if (media) {
+        EGS_ARG(nmed > 0, "nmed > 0");
+        for (int i = 0; i < nmed; i++) {
+            EGS_ARG(media[i] >= 0 && media[i] < all, "medium index out of range");
+            use.push_back(media[i]);
+        }
+    } else {
+        for (int m = 0; m < all; m++) use.push_back(m);
+    }
+    *blcmin = bca_blcmin(t->blob, use, ecut);
+    *skindepth = std::exp(*blcmin);
     return EGSGPU_OK;
 }
 

@@ -1587,15 +1587,26 @@ struct Shower {
         R skindepth = (R)P.skindepth * ssmfp;
         R mx = tperp > skindepth ? tperp : skindepth;
         tustep = tustep < mx ? tustep : mx;
-        bool callhowfar, dosingle = false, callmsdist;
+        bool callhowfar, dosingle = false, domultiple = false, callmsdist;
         R ustep, de = 0;
         R us = 0, vs = 0, ws = 0, xt = 0, yt = 0, zt = 0;
+        // screening angle, beta^2 and xi of a PRESTA-I approximate step
+        R ms_chia2 = 0, ms_beta2 = 1, ms_xi = 0;
+        const bool exact = P.exact_bca != 0;
         // the CH-versus-boundary decision is taken in FP64
-        if ((double)tustep <= (double)tperp && (double)tustep > (double)skindepth) {
+        if ((double)tustep <= (double)tperp && (!exact || (double)tustep > (double)skindepth)) {
             callhowfar = false;
             callmsdist = true;
             de = eloss_g(m, M, pos, rhof, range, tustep, eke, elke, lelke);
-            msdist_pII(M, eke, de, tustep, rhof, us, vs, ws, xt, yt, zt, ustep);
+            if (P.presta1) msdist_pI(M, eke, de, tustep, rhof, us, vs, ws, xt, yt, zt, ustep);
+            else msdist_pII(M, eke, de, tustep, rhof, us, vs, ws, xt, yt, zt, ustep);
+        } else if (!exact) {
+            // PRESTA-I boundary crossing: an approximate step whose multiple
+            // scattering is done at its end (egsnrc.mortran:1338-1346)
+            callmsdist = false;
+            domultiple = true;
+            ustep = set_ustep(blccl, rhof * M.xcc, eke, tustep, dedx, ms_chia2, ms_beta2, ms_xi);
+            callhowfar = !(ustep < tperp);
         } else {
             callmsdist = false;
             R rnnoss = rnd();
@@ -1616,6 +1627,7 @@ struct Shower {
             callhowfar = !(ustep < tperp);
         }
         int irold = p.ir, irnew = p.ir, idisc = 0;
+        const R ustep0 = ustep;
         if (callhowfar || p.wt <= 0) {
             if constexpr (HFL) {
                 R tb = geom_box_exit(*Gp, p.x, p.y, p.z, p.u, p.v, p.w);
@@ -1638,8 +1650,14 @@ struct Shower {
         }
         R vstep = ustep, tvstep;
         if (callhowfar) {
-            tvstep = vstep;
-            if (tvstep != tustep) dosingle = false;
+            if (exact) {
+                tvstep = vstep;
+                if (tvstep != tustep) dosingle = false;
+            } else {
+                tvstep = set_tvstep(blccl, rhof * M.xcc, eke, tustep, dedx, vstep, ustep0,
+                                    ms_chia2, ms_beta2, ms_xi);
+                if (tvstep < 0) { c_ovf++; np = 0; eph = kNew; return; }
+            }
         } else {
             tvstep = tustep;
         }
@@ -1657,7 +1675,19 @@ struct Shower {
             xf = p.x + p.u * vstep;
             yf = p.y + p.v * vstep;
             zf = p.z + p.w * vstep;
-            if (dosingle) {
+            if (domultiple) {
+                // multiple scattering over the approximate step, then the
+                // deflection (egsnrc.mortran:1484-1502, 1564-1570)
+                R lambda = blccl * tvstep / ms_beta2 / (1 + ms_chia2);
+                bool find_index = true;
+                int msi = 0, msj = 0;
+                R omega2 = 0, costhe, sinthe;
+                mscat(lambda, ms_chia2, ms_xi, find_index, msi, msj, omega2, costhe, sinthe);
+                R ut = p.u, vt = p.v, wt = p.w;
+                uphi21(costhe, sinthe);
+                uf = p.u; vf = p.v; wf = p.w;
+                p.u = ut; p.v = vt; p.w = wt;
+            } else if (dosingle) {
                 R ekems = ekef > ecut - RM ? ekef : ecut - RM;
                 R p2s = ekems * (ekems + RMT2);
                 R chia2 = M.xcc / (4 * M.blcc * p2s);
@@ -1953,8 +1983,97 @@ struct Shower {
         zf = p.z + tustep * wt;
     }
 
+    // msdist_pI (egsnrc.mortran:4310-4515), spin effects off: one mscat
+    // angle over the step (elastic mean free paths corrected for energy
+    // loss), the exact Lewis <z> along the initial direction and a lateral
+    // offset of half the sine of the angle
+    __device__ __forceinline__ void msdist_pI(const MedScalars<R> &M, R e0, R el, R tustep, R rhof,
+                                           R &us, R &vs, R &ws, R &xf, R &yf, R &zf, R &ustep) {
+        const R PRM = (R)0.5109989461, RMT2 = 2 * PRM, RMSQ = PRM * PRM;
+        R e = e0 - (R)0.5 * el;
+        R p2 = e * (e + RMT2);
+        R beta2 = p2 / (p2 + RMSQ);
+        R lambda = tustep * rhof * M.blcc / beta2;
+        R factor = 1 / (1 + (R)0.9784671 * e);   // 1 / (1 + e / 2m)
+        R eps = el / e0;
+        eps = eps / (1 - (R)0.5 * eps);
+        lambda = lambda * (1 + (R)0.25 * (1 - factor * (1 - (R)0.333333 * factor)) * eps * eps);
+        R chia2 = M.xcc / (4 * M.blcc * p2);
+        lambda = lambda / (1 + chia2);
+        R chilog = dlog(1 + 1 / chia2);
+        R xi = 2 * chia2 * (chilog * (1 + chia2) - 1) * lambda;
+        bool find_index = true;
+        int msi = 0, msj = 0;
+        R omega2 = 0, sint, cphi, sphi;
+        mscat(lambda, chia2, xi, find_index, msi, msj, omega2, ws, sint);
+        azimuth(cphi, sphi);
+        us = sint * cphi;
+        vs = sint * sphi;
+        R z = xi < (R)0.1 ? 1 - xi * ((R)0.5 - xi * ((R)0.166666667 - (R)0.041666667 * xi))
+                          : (1 - dexp(-xi)) / xi;
+        R r = (R)0.5 * sint, r2 = r * r, z2 = z * z;
+        if (1 - z2 < r2) { r2 = 1 - z2; r = dsqrt(r2); }
+        R ut = r * cphi, vt = r * sphi, wt = z;
+        ustep = dsqrt(z2 + r2) * tustep;
+        R u0 = p.u, v0 = p.v, w0 = p.w;
+        R sint02 = u0 * u0 + v0 * v0;
+        if (sint02 > (R)1e-20) {
+            R sint0 = dsqrt(sint02), sint0i = 1 / sint0;
+            R cphi0 = sint0i * u0, sphi0 = sint0i * v0;
+            R u2p = w0 * us + sint0 * ws;
+            ws = w0 * ws - sint0 * us;
+            us = u2p * cphi0 - vs * sphi0;
+            vs = u2p * sphi0 + vs * cphi0;
+            u2p = w0 * ut + sint0 * wt;
+            wt = w0 * wt - sint0 * ut;
+            ut = u2p * cphi0 - vt * sphi0;
+            vt = u2p * sphi0 + vt * cphi0;
+        } else {
+            wt = w0 * wt;
+            ws = w0 * ws;
+        }
+        xf = p.x + tustep * ut;
+        yf = p.y + tustep * vt;
+        zf = p.z + tustep * wt;
+    }
+
+    // $CALCULATE-XI (egsnrc.macros:1324-1348), spin effects off: the first
+    // GS moment of a path len at kinetic energy ekems; blccl, xccl are
+    // rhof * blcc, xcc
+    __device__ static R calc_xi(R blccl, R xccl, R ekems, R len, R &chia2, R &beta2) {
+        const R PRM = (R)0.5109989461, RMT2 = 2 * PRM, RMSQ = PRM * PRM;
+        R p2 = ekems * (ekems + RMT2);
+        beta2 = p2 / (p2 + RMSQ);
+        chia2 = xccl / (4 * blccl * p2);
+        R xi = (R)0.5 * xccl / p2 / beta2 * len;
+        return xi * (dlog(1 + 1 / chia2) - 1 / (1 + chia2));
+    }
+
+    // $SET-USTEP (egsnrc.macros:1308-1322): mean straight-line distance of
+    // the curved path tustep
+    __device__ static R set_ustep(R blccl, R xccl, R eke, R tustep, R dedx, R &chia2, R &beta2,
+                                  R &xi) {
+        R ekems = eke - (R)0.5 * tustep * dedx;
+        xi = calc_xi(blccl, xccl, ekems, tustep, chia2, beta2);
+        if (xi < (R)0.1) return tustep * (1 - xi * ((R)0.5 - xi * (R)0.166667));
+        return tustep * (1 - dexp(-xi)) / xi;
+    }
+
+    // $SET-TVSTEP (egsnrc.macros:1350-1384): curved path of a step that
+    // howfar cut from ustep0 to vstep; -1 for xi >= 1, where the reference
+    // stops with a fatal error
+    __device__ static R set_tvstep(R blccl, R xccl, R eke, R tustep, R dedx, R vstep, R ustep0,
+                                   R &chia2, R &beta2, R &xi) {
+        if (!(vstep < ustep0)) return tustep;
+        R ekems = eke - (R)0.5 * tustep * vstep / ustep0 * dedx;
+        xi = calc_xi(blccl, xccl, ekems, vstep, chia2, beta2);
+        if (xi < (R)0.1) return vstep * (1 + xi * ((R)0.5 + xi * (R)0.333333));
+        if (xi < (R)0.999999) return -vstep * dlog(1 - xi) / xi;
+        return -1;
+    }
+
     // mscat (egsnrc.mortran:2984-3160), spin effects off
-    __device__ void mscat(R lambda, R chia2, R q1, bool &find_index, int &msi, int &msj,
+    __device__ __forceinline__ void mscat(R lambda, R chia2, R q1, bool &find_index, int &msi, int &msj,
                           R &omega2, R &cost, R &sint) {
         if (lambda <= (R)13.8) {
             R sprob = rnd();

@@ -488,14 +488,23 @@ class EGS_GPUApplication {
             else if (InputItem::same(v, "norej")) tp.bound_compton = 3;
             else throw std::runtime_error("Bound Compton scattering = " + v + " not supported");
         }
-        if (!t->get("Boundary crossing algorithm", v) && !InputItem::same(v, "Exact")) {
-            if (!dosxyz)
-                throw std::runtime_error("only the Exact boundary crossing algorithm is supported");
-            fprintf(stderr, "note: Boundary crossing algorithm = %s requested; the engine "
-                            "implements EXACT only and runs with it\n", v.c_str());
+        // egs_advanced_application.cpp:552-612: the step algorithm takes
+        // EGSnrc / PRESTA-I / PRESTA-II / default (all but PRESTA-I mean
+        // PRESTA-II), boundary crossing Exact / PRESTA-I
+        if (!t->get("Boundary crossing algorithm", v)) {
+            if (InputItem::same(v, "Exact")) tp.exact_bca = 1;
+            else if (InputItem::same(v, "PRESTA-I")) tp.exact_bca = 0;
+            else throw std::runtime_error("Boundary crossing algorithm = " + v +
+                                          ": Exact or PRESTA-I");
         }
-        if (!t->get("Electron-step algorithm", v) && !InputItem::same(v, "PRESTA-II"))
-            throw std::runtime_error("only PRESTA-II is supported");
+        if (!t->get("Electron-step algorithm", v)) {
+            if (InputItem::same(v, "PRESTA-I")) tp.step_algorithm = 1;
+            else if (InputItem::same(v, "PRESTA-II") || InputItem::same(v, "EGSnrc") ||
+                     InputItem::same(v, "default"))
+                tp.step_algorithm = 0;
+            else throw std::runtime_error("Electron-step algorithm = " + v +
+                                          ": PRESTA-II, PRESTA-I, EGSnrc or default");
+        }
         if (!t->get("Brems angular sampling", v)) tp.brems_angle = InputItem::same(v, "Simple") ? 0 : 1;
         if (!t->get("Pair angular sampling", v)) tp.pair_angle = InputItem::same(v, "Off") ? 0 : 1;
         if (!t->get("Brems cross sections", v) && !InputItem::same(v, "BH"))
@@ -731,10 +740,12 @@ class EGS_GPUApplication {
         printf("ecut: %.6g\npcut: %.6g\n", tp.ecut, tp.pcut);
         printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
                "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d "
-               "n_split %d nbr_split %d play_rr %d ireject %d esave %.6g\n",
+               "n_split %d nbr_split %d play_rr %d ireject %d esave %.6g exact_bca %d "
+               "step_algorithm %d\n",
                tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
                tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth,
-               tp.howfarless, tp.n_split, tp.nbr_split, tp.play_rr, tp.ireject, tp.esave);
+               tp.howfarless, tp.n_split, tp.nbr_split, tp.play_rr, tp.ireject, tp.esave,
+               tp.exact_bca, tp.step_algorithm);
     }
 
     void init_run() {

@@ -122,10 +122,18 @@ typedef struct EgsTransport {
     int32_t pair_angle;          /* IPRDST 0..2                            */
     int32_t brems_angle;         /* IBRDST 0/1                             */
     int32_t spin;                /* spin effects (needs spinms data)       */
-    int32_t exact_bca;           /* 1 exact boundary crossing (default)    */
+    int32_t exact_bca;           /* 1 exact boundary crossing (default);
+                                    0 PRESTA-I boundary crossing (bca_algorithm
+                                    1: ELECTR's inexact branches and
+                                    $SET-USTEP / $SET-TVSTEP,
+                                    egsnrc.mortran:1247-1440)              */
     double estepe;               /* ESTEPE (default 0.25)                  */
     double ximax;                /* XIMAX (default 0.5)                    */
-    double skin_depth;           /* skindepth_for_bca (default 3)          */
+    double skin_depth;           /* skindepth_for_bca (default 3); <= 1e-4
+                                    selects mscati's default: 3 with exact
+                                    boundary crossing, exp(BLCMIN) at the
+                                    lowest ECUT with PRESTA-I
+                                    (egsnrc.mortran:2790-2817)             */
     int32_t kerma;               /* 1: deposit every electron locally      */
     int32_t fp64;                /* 1: run the FP64 build of the kernels   */
     int32_t howfarless;          /* 1: DOSXYZnrc HOWFARLESS for a homogeneous
@@ -160,6 +168,9 @@ typedef struct EgsTransport {
                                     energy there; plain history schedule,
                                     not with howfarless */
     double esave;                /* ESAVE_GLOBAL: total energy (MeV) */
+    int32_t step_algorithm;      /* transport_algorithm: 0 PRESTA-II
+                                    (msdist_pII, default), 1 PRESTA-I
+                                    (msdist_pI, egsnrc.mortran:4310-4515) */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */
@@ -193,6 +204,16 @@ int egsgpu_tables_nmedia(const egsgpu_tables *t, int32_t *n);
  * (egsAddMedium, egs_advanced_application.cpp:460-464). */
 int egsgpu_tables_medium_info(const egsgpu_tables *t, int32_t i, char *name,
                               size_t n, double *rho);
+
+/* mscati's default skin depth for PRESTA-I boundary crossing
+ * (egsnrc.mortran:2790-2817): BLCMIN = min over media of
+ * log(t / log t), t = blcc (E^2 - m^2) / xcc^2 at E = ecut (total energy,
+ * MeV) with the PEGS4 blcc and xcc, and skindepth = exp(BLCMIN) elastic
+ * mean free paths.  The media are the nmed indices in media, or every
+ * medium of the blob when media is NULL (egsgpu_create takes the media its
+ * voxels use, the media of the problem). */
+int egsgpu_tables_bca_skindepth(const egsgpu_tables *t, double ecut, int32_t nmed,
+                                const int32_t *media, double *blcmin, double *skindepth);
 
 /* ---- run control ------------------------------------------------------ */
 /* Create a simulation: EGS_AdvancedApplication::initSimulation analogue

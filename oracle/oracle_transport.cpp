@@ -44,6 +44,7 @@ struct Params {
     int rayleigh, ibcmp, iphter, iedgfl, iprdst, ibrdst, spin, exact_bca;
     double estepe, ximax, skindepth;
     int kerma;
+    int presta1;   // transport_algorithm PRESTA-I (msdist_pI)
 };
 
 struct Problem {
@@ -932,16 +933,32 @@ struct Shower {
                 double ssmfp = beta2 / blccl;
                 double skindepth = pb.P.skindepth * ssmfp;
                 tustep = std::min(tustep, std::max(tperp, skindepth));
-                bool callhowfar, dosingle = false, callmsdist;
+                bool callhowfar, dosingle = false, domultiple = false, callmsdist;
                 double ustep, de;
                 double uscat = 0, vscat = 0, wscat = 0, xtrans = 0, ytrans = 0, ztrans = 0;
-                if (tustep <= tperp && tustep > skindepth) {
+                // the approximate step's screening angle, beta^2 and xi
+                // ($SET-USTEP / $SET-TVSTEP)
+                double ms_chia2 = 0, ms_beta2 = 0, ms_xi = 0;
+                const bool exact = pb.P.exact_bca != 0;
+                if (tustep <= tperp && (!exact || tustep > skindepth)) {
                     callhowfar = false;
                     dosingle = false;
                     callmsdist = true;
                     de = compute_eloss_g(*M, lelec, qel, rhof, range, tustep, eke, elke, lelke);
-                    msdist_pII(*M, eke, de, tustep, rhof, p.u, p.v, p.w, p.x, p.y, p.z,
-                               uscat, vscat, wscat, xtrans, ytrans, ztrans, ustep);
+                    if (pb.P.presta1)
+                        msdist_pI(*M, eke, de, tustep, rhof, p.u, p.v, p.w, p.x, p.y, p.z,
+                                  uscat, vscat, wscat, xtrans, ytrans, ztrans, ustep);
+                    else
+                        msdist_pII(*M, eke, de, tustep, rhof, p.u, p.v, p.w, p.x, p.y, p.z,
+                                   uscat, vscat, wscat, xtrans, ytrans, ztrans, ustep);
+                } else if (!exact) {
+                    // boundary crossing a la EGS4/PRESTA-I with the exact
+                    // path-length correction (egsnrc.mortran:1338-1346)
+                    callmsdist = false;
+                    domultiple = true;
+                    ustep = set_ustep(blccl, rhof * M->xcc, eke, tustep, dedx, ms_chia2,
+                                      ms_beta2, ms_xi);
+                    callhowfar = !(ustep < tperp);
                 } else {
                     callmsdist = false;
                     double rnnoss = rnd();
@@ -961,6 +978,7 @@ struct Shower {
                     callhowfar = !(ustep < tperp);
                 }
                 int irold = p.ir, irnew = p.ir, idisc = 0;
+                const double ustep0 = ustep;
                 if (callhowfar || p.wt <= 0) {
                     int inew = pb.geo.howfar(p.ir, p.x, p.y, p.z, p.u, p.v, p.w, ustep);
                     if (inew != p.ir) {
@@ -980,8 +998,13 @@ struct Shower {
                 }
                 double vstep = ustep, tvstep;
                 if (callhowfar) {
-                    tvstep = vstep;
-                    if (tvstep != tustep) dosingle = false;
+                    if (exact) {
+                        tvstep = vstep;
+                        if (tvstep != tustep) dosingle = false;
+                    } else {
+                        tvstep = set_tvstep(blccl, rhof * M->xcc, eke, tustep, dedx, vstep,
+                                            ustep0, ms_chia2, ms_beta2, ms_xi);
+                    }
                     de = compute_eloss_g(*M, lelec, qel, rhof, range, tvstep, eke, elke, lelke);
                 } else {
                     tvstep = tustep;
@@ -992,7 +1015,13 @@ struct Shower {
                 double edep = de;
                 double ekef = eke - de;
                 double costhe = 1, sinthe = 0;
-                if (!callmsdist && dosingle) {
+                if (!callmsdist && domultiple) {
+                    // multiple scattering over the approximate step
+                    // (egsnrc.mortran:1484-1502)
+                    double lambda = blccl * tvstep / ms_beta2 / (1 + ms_chia2);
+                    bool fi = true;
+                    mscat(lambda, ms_chia2, ms_xi, fi, costhe, sinthe);
+                } else if (!callmsdist && dosingle) {
                     double ekems = std::max(ekef, ecut - RM);
                     double p2s = ekems * (ekems + RMT2);
                     double beta2s = p2s / (p2s + RMSQ);
@@ -1008,7 +1037,7 @@ struct Shower {
                     xf = p.x + p.u * vstep;
                     yf = p.y + p.v * vstep;
                     zf = p.z + p.w * vstep;
-                    if (dosingle) {
+                    if (domultiple || dosingle) {
                         double ut = p.u, vt = p.v, wt = p.w;
                         uphi21(costhe, sinthe);
                         uf = p.u; vf = p.v; wf = p.w;
@@ -1169,6 +1198,100 @@ struct Shower {
         xf = x0 + tustep * ut;
         yf = y0 + tustep * vt;
         zf = z0 + tustep * wt;
+    }
+
+    // msdist_pI (egsnrc.mortran:4310-4515), spin effects off: one mscat
+    // angle over the whole step, the number of elastic mean free paths
+    // corrected for energy loss, the exact Lewis <z> along the initial
+    // direction and a lateral offset of half the sine of the angle.
+    void msdist_pI(const Medium &M, double e0, double eloss, double tustep, double rhof,
+                   double u0, double v0, double w0, double x0, double y0, double z0,
+                   double &us, double &vs, double &ws, double &xf, double &yf,
+                   double &zf, double &ustep) {
+        double blccc = M.blcc, xcccc = M.xcc;
+        double e = e0 - 0.5 * eloss;
+        double p2 = e * (e + RMT2);
+        double p2i = 1 / p2;
+        double beta2 = p2 / (p2 + RMSQ);
+        double lambda = tustep * rhof * blccc / beta2;
+        double factor = 1 / (1 + 0.9784671 * e);   // 1 / (1 + e / 2m)
+        double epsilon = eloss / e0;
+        epsilon = epsilon / (1 - 0.5 * epsilon);
+        double temp = 0.25 * (1 - factor * (1 - 0.333333 * factor)) * epsilon * epsilon;
+        lambda = lambda * (1 + temp);
+        double chia2 = xcccc * p2i / (4 * blccc);
+        lambda = lambda / (1 + chia2);
+        double chilog = std::log(1 + 1 / chia2);
+        double q1 = 2 * chia2 * (chilog * (1 + chia2) - 1);
+        double xi = q1 * lambda;
+        bool find_index = true;
+        double sint, cphi, sphi;
+        mscat(lambda, chia2, xi, find_index, ws, sint);
+        azimuth(cphi, sphi);
+        us = sint * cphi;
+        vs = sint * sphi;
+        double z = xi < 0.1 ? 1 - xi * (0.5 - xi * (0.166666667 - 0.041666667 * xi))
+                            : (1 - std::exp(-xi)) / xi;
+        double r = 0.5 * sint, r2 = r * r, z2 = z * z;
+        double r2max = 1 - z2;
+        if (r2max < r2) { r2 = r2max; r = std::sqrt(r2); }
+        double ut = r * cphi, vt = r * sphi, wt = z;
+        ustep = std::sqrt(z2 + r2) * tustep;
+        double sint02 = u0 * u0 + v0 * v0;
+        if (sint02 > 1e-20) {
+            double sint0 = std::sqrt(sint02), sint0i = 1 / sint0;
+            double cphi0 = sint0i * u0, sphi0 = sint0i * v0;
+            double u2p = w0 * us + sint0 * ws;
+            ws = w0 * ws - sint0 * us;
+            us = u2p * cphi0 - vs * sphi0;
+            vs = u2p * sphi0 + vs * cphi0;
+            u2p = w0 * ut + sint0 * wt;
+            wt = w0 * wt - sint0 * ut;
+            ut = u2p * cphi0 - vt * sphi0;
+            vt = u2p * sphi0 + vt * cphi0;
+        } else {
+            wt = w0 * wt;
+            ws = w0 * ws;
+        }
+        xf = x0 + tustep * ut;
+        yf = y0 + tustep * vt;
+        zf = z0 + tustep * wt;
+    }
+
+    // $CALCULATE-XI (egsnrc.macros:1324-1348), spin effects off: the first
+    // GS moment xi of a path `len` at kinetic energy ekems, with the
+    // screening angle chia2 and beta^2 left for the multiple-scattering
+    // call of an approximate step.  blccl, xccl are rhof * blcc, xcc.
+    static double calc_xi(double blccl, double xccl, double ekems, double len,
+                          double &chia2, double &beta2) {
+        double p2 = ekems * (ekems + RMT2);
+        beta2 = p2 / (p2 + RMSQ);
+        chia2 = xccl / (4 * blccl * p2);
+        double xi = 0.5 * xccl / p2 / beta2 * len;
+        return xi * (std::log(1 + 1. / chia2) - 1 / (1 + chia2));
+    }
+
+    // $SET-USTEP (egsnrc.macros:1308-1322): the average straight-line
+    // distance along the initial direction of a curved path tustep.
+    static double set_ustep(double blccl, double xccl, double eke, double tustep, double dedx,
+                            double &chia2, double &beta2, double &xi) {
+        double ekems = eke - 0.5 * tustep * dedx;
+        xi = calc_xi(blccl, xccl, ekems, tustep, chia2, beta2);
+        if (xi < 0.1) return tustep * (1 - xi * (0.5 - xi * 0.166667));
+        return tustep * (1 - std::exp(-xi)) / xi;
+    }
+
+    // $SET-TVSTEP (egsnrc.macros:1350-1384): the curved path of a step that
+    // howfar cut from ustep0 to vstep.
+    static double set_tvstep(double blccl, double xccl, double eke, double tustep, double dedx,
+                             double vstep, double ustep0, double &chia2, double &beta2,
+                             double &xi) {
+        if (!(vstep < ustep0)) return tustep;
+        double ekems = eke - 0.5 * tustep * vstep / ustep0 * dedx;
+        xi = calc_xi(blccl, xccl, ekems, vstep, chia2, beta2);
+        if (xi < 0.1) return vstep * (1 + xi * (0.5 + xi * 0.333333));
+        if (xi < 0.999999) return -vstep * std::log(1 - xi) / xi;
+        throw std::runtime_error("SET-TVSTEP: xi > 1");
     }
 
     // mscat (egsnrc.mortran:2984-3160), spin effects off.  ms_i/ms_j/
@@ -1781,8 +1904,28 @@ struct OTransport {
     int32_t rayleigh, bound_compton, photoelectron_angle, atomic_relaxation,
         pair_angle, brems_angle, spin, exact_bca;
     double estepe, ximax, skin_depth;
-    int32_t kerma, fp64;
+    int32_t kerma, fp64, howfarless, n_split, nbr_split, play_rr, ireject;
+    double esave;
+    int32_t step_algorithm;
 };
+
+// mscati's default PRESTA-I skin depth (egsnrc.mortran:2790-2817) over the
+// media `use` at the lowest ECUT: the blob holds blcc and xcc as mscati
+// leaves them (blcc times exp(2 gamma_E - 1), xcc squared), so the PEGS4
+// values are recovered first.
+static double default_blcmin(const Tables &T, const std::vector<int> &use, double ecutmn,
+                             double prm = PRM) {
+    double tstbmn = 1e30;
+    for (int m : use) {
+        const Medium &M = T.med[m];
+        double blcc = M.blcc / 1.16699413758864573, xcc = std::sqrt(M.xcc);
+        double tstbm = (ecutmn - prm) * (ecutmn + prm) / (ecutmn * ecutmn);
+        tstbm = blcc * tstbm * (ecutmn / xcc) * (ecutmn / xcc);
+        tstbm = std::log(tstbm / std::log(tstbm));
+        tstbmn = std::min(tstbmn, tstbm);
+    }
+    return tstbmn;
+}
 
 static thread_local std::string g_oerr;
 
@@ -1818,6 +1961,13 @@ int oracle_create(const char *blob, const OGeom *g, const OSource *s,
         P.iprdst = tp->pair_angle; P.ibrdst = tp->brems_angle; P.spin = tp->spin;
         P.exact_bca = tp->exact_bca; P.estepe = tp->estepe; P.ximax = tp->ximax;
         P.skindepth = tp->skin_depth; P.kerma = tp->kerma;
+        P.presta1 = tp->step_algorithm == 1;
+        if (P.skindepth <= 1e-4) {   // over the media the voxels use
+            std::vector<int> used;
+            for (int m = 0; m < pb.T.nmed; m++)
+                if (std::find(pb.med.begin(), pb.med.end(), m) != pb.med.end()) used.push_back(m);
+            P.skindepth = P.exact_bca ? 3.0 : std::exp(default_blcmin(pb.T, used, P.ecut));
+        }
         Source &S = pb.src;
         S.charge = s->charge; S.spectrum = s->spectrum; S.energy = s->energy;
         S.nbins = s->nbins; S.shape = s->shape;
@@ -1907,6 +2057,29 @@ int oracle_destroy(void *h) {
 //  10 MSDIST  -> [x, y, z, w] after one msdist_pII step of param cm (no
 //                energy loss, unit density) from the origin along +z, at
 //                kinetic energy E
+// mscati's default PRESTA-I BLCMIN and skin depth for the media `media`
+// (every medium when NULL) of a blob at the lowest ECUT ecutmn, with the
+// electron rest mass prm (<= 0: egsnrc.macros:1140's value)
+extern "C" int oracle_bca_skindepth(const char *blob, double ecutmn, double prm, int32_t nmed,
+                                    const int32_t *media, double *blcmin, double *skindepth) {
+    try {
+        Tables T;
+        T.load(blob);
+        std::vector<int> use;
+        for (int i = 0; i < (media ? nmed : T.nmed); i++) {
+            int m = media ? media[i] : i;
+            if (m < 0 || m >= T.nmed) throw std::runtime_error("medium index out of range");
+            use.push_back(m);
+        }
+        *blcmin = default_blcmin(T, use, ecutmn, prm > 0 ? prm : PRM);
+        *skindepth = std::exp(*blcmin);
+        return 0;
+    } catch (const std::exception &ex) {
+        g_oerr = ex.what();
+        return -1;
+    }
+}
+
 extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
                              double energy, double param, int64_t n,
                              int32_t seed, double *out) {
@@ -2019,6 +2192,25 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
                     if (i == 1) first = q.iq == 0 ? ek : -ek;
                 }
                 o[0] = first; o[1] = nph + 1000.0 * nel + 1e6 * sh.relax_depth; o[2] = ekin; o[3] = edep;
+            } else if (kind == 12) {
+                // one PRESTA-I step (msdist_pI) of length param without
+                // energy loss along +z from the origin
+                double us, vs, ws, xf, yf, zf, ustep;
+                sh.msdist_pI(M, energy, 0.0, param, 1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, us, vs, ws,
+                             xf, yf, zf, ustep);
+                o[0] = xf; o[1] = yf; o[2] = zf; o[3] = ws;
+            } else if (kind == 13) {
+                // $SET-USTEP for a curved path param at this energy with the
+                // electron stopping power of the medium, then $SET-TVSTEP
+                // for a boundary just short of the straight-line step
+                int lk = sh.lel(M, std::log(energy));
+                double dedx = Shower::ev(M.ededx0, M.ededx1, lk, std::log(energy));
+                double chia2, beta2, xi;
+                double ustep = Shower::set_ustep(M.blcc, M.xcc, energy, param, dedx, chia2, beta2, xi);
+                double xi_t = xi;
+                double tv = Shower::set_tvstep(M.blcc, M.xcc, energy, param, dedx,
+                                               ustep * (1 - 1e-12), ustep, chia2, beta2, xi);
+                o[0] = ustep; o[1] = tv; o[2] = xi_t; o[3] = chia2;
             } else {
                 throw std::runtime_error("unknown sampler kind");
             }

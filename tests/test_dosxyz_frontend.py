@@ -53,8 +53,10 @@ def test_parse_reference_example():
     # INSEED1 = -97 is out of range and takes the default 1802
     assert p["seeds"] == ["1802", "33"]
     assert float(p["ecut"][0]) == 0.7 and float(p["pcut"][0]) == 0.01
-    # PRESTA-I is replaced by EXACT (reported)
-    assert "EXACT" in err
+    # PRESTA-I boundary crossing with mscati's default skin depth, PRESTA-II
+    # steps
+    t = transport_options(p)
+    assert (t["exact_bca"], t["skin"], t["step_algorithm"]) == ("0", "0", "0")
 
 
 def write_isource1_input(path, spectrum=None, xcol=3.0, ycol=2.0, ncase=100000, seeds=(33, 97)):
@@ -166,9 +168,11 @@ def test_reference_example_runs_and_matches_the_oracle(tmp_path):
     E, sE = per_fluence_from_3ddose(tmp_path / "air_bone.3ddose", rho, 2.5 * 4.5)
     g, keep = _lib.make_geometry(xb, xb.copy(), zb, med, None)
     src = make_source(-1, 20.0, x0=-1.25, x1=1.25, y0=-2.25, y1=2.25, z=-1.0)
-    # "Brems angular sampling= Simple" in the example
-    m, s = oracle_energy(g, src, transport(brems_angle=0), n)
-    null = oracle_energy(g, src, transport(brems_angle=0), n, seeds=(4711, 20000))
+    # "Brems angular sampling= Simple", "Boundary crossing algorithm=
+    # PRESTA-I" and "Skin depth for BCA= 0" in the example
+    tp = transport(brems_angle=0, exact_bca=0, skin_depth=0.0)
+    m, s = oracle_energy(g, src, tp, n)
+    null = oracle_energy(g, src, tp, n, seeds=(4711, 20000))
     compare(E, sE, m, s, "dosxyznrc example, isource 0", 200, null=null)
 
 
