@@ -118,36 +118,6 @@ int blob_read(const std::string &path, Blob &b) {
     return EGSGPU_OK;
 }
 
-// ------------------------------------------------------- record reading
-// Fortran list-directed / E-format records: every READ starts on a new
-// line and drops what is left of its last line.
-struct Records {
-    std::ifstream in;
-    std::string path;
-    explicit Records(const std::string &p) : in(p), path(p) {}
-    bool ok() const { return (bool)in; }
-    bool line(std::string &s) { return (bool)std::getline(in, s); }
-    void skip(int n = 1) { std::string s; for (int k = 0; k < n; k++) std::getline(in, s); }
-    std::vector<double> reals(size_t n) {
-        std::vector<double> v;
-        std::string s;
-        while (v.size() < n) {
-            if (!std::getline(in, s)) throw std::runtime_error("unexpected end of " + path);
-            for (auto &c : s) if (c == ',' || c == 'D' || c == 'd') c = (c == ',') ? ' ' : 'E';
-            std::istringstream ls(s);
-            std::string tok;
-            while (v.size() < n && ls >> tok) v.push_back(std::stod(tok));
-        }
-        return v;
-    }
-    std::vector<long> ints(size_t n) {
-        std::vector<double> r = reals(n);
-        std::vector<long> v(r.size());
-        for (size_t k = 0; k < r.size(); k++) v[k] = (long)std::llround(r[k]);
-        return v;
-    }
-};
-
 // ------------------------------------------------------------ PEGS4 read
 struct PegsMedium {
     std::string name;
@@ -1022,6 +992,10 @@ int tables_build(const std::string &hh, const std::string &pegs_file,
             float dqms = 0.5f / 7.0f, dqmsi = 1.0f / dqms;
             out.f["ms_const"] = {llammin, llammax, dllamb, dllambi, dqms, dqmsi};
         }
+        // init_nist_brems runs after fix_brems and mscati (HATCH,
+        // egsnrc.mortran:2261-2289): it rewrites their cross sections
+        out.i["ibr_nist"] = {0};
+        if (o.ibr_nist >= 1) nist_brems_tables(data, o.ibr_nist, out);
     } catch (const std::exception &e) {
         set_error("table build failed: %s", e.what());
         return EGSGPU_E_FORMAT;

@@ -17,7 +17,10 @@
 // "m<k>.<field>" for medium k.
 #pragma once
 #include <cstdint>
+#include <fstream>
 #include <map>
+#include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -46,11 +49,53 @@ struct BuildOptions {
     double ximax = 0.5;       // XIMAX
     int bound_compton = 3;    // IBCMP
     int rayleigh = 1;         // IRAYLR
+    int ibr_nist = 0;         // 0 Bethe-Heitler, 1 NIST brems cross sections
 };
 
 int tables_build(const std::string &hen_house, const std::string &pegs_file,
                  const std::vector<std::string> &media, const BuildOptions &o,
                  Blob &out);
+
+// ------------------------------------------------------- record reading
+// Fortran list-directed / E-format records: every READ starts on a new
+// line and drops what is left of its last line.
+struct Records {
+    std::ifstream in;
+    std::string path;
+    explicit Records(const std::string &p) : in(p), path(p) {}
+    bool ok() const { return (bool)in; }
+    bool line(std::string &s) { return (bool)std::getline(in, s); }
+    void skip(int n = 1) { std::string s; for (int k = 0; k < n; k++) std::getline(in, s); }
+    std::vector<double> reals(size_t n) {
+        std::vector<double> v;
+        std::string s;
+        while (v.size() < n) {
+            if (!std::getline(in, s)) throw std::runtime_error("unexpected end of " + path);
+            for (auto &c : s) if (c == ',' || c == 'D' || c == 'd') c = (c == ',') ? ' ' : 'E';
+            std::istringstream ls(s);
+            std::string tok;
+            while (v.size() < n && ls >> tok) v.push_back(std::stod(tok));
+        }
+        return v;
+    }
+    std::vector<long> ints(size_t n) {
+        std::vector<double> r = reals(n);
+        std::vector<long> v(r.size());
+        for (size_t k = 0; k < r.size(); k++) v[k] = (long)std::llround(r[k]);
+        return v;
+    }
+};
+
+// NIST bremsstrahlung (init_nist_brems, egsnrc.mortran:6893-7226,
+// ibr_nist = 1): replaces the total brems cross sections (esig, ebr1,
+// psig, pbr1, pbr2, esig_e, psig_e) of every medium of a built blob by the
+// integrals of data_dir/nist_brems.data and adds the photon-energy alias
+// tables nb_xdata/nb_fdata/nb_wdata/nb_idata ([kNbE][kNbX+1] / [kNbE][kNbX])
+// with nb_emin, nb_emax, nb_lemin, nb_dlei; sets "ibr_nist".
+int nist_brems_tables(const std::string &data_dir, int ibr_nist, Blob &out);
+constexpr int kNbE = 100;       // energies of the brems sampling grid ($MXBRES)
+constexpr int kNbX = 50;        // bins per alias table ($MXBRXS)
+constexpr int kNbGauss = 64;    // Gauss-Legendre points ($MXGAUSS)
 
 // mscati's default minimum step for PRESTA-I boundary crossing
 // (egsnrc.mortran:2790-2817): BLCMIN = min over `media` of log(t / log t),

@@ -48,9 +48,14 @@ struct MedScalars {
     R rho, ae, ap, te, thmoll;
     R delcm, bpar0, bpar1, zbrang, lzbrang, log_ap;
     R blcc, xcc, expeke1, esig_e, psig_e;
+    R nb_emin, nb_lemin, nb_dlei;     // NIST brems sampling grid (ibr_nist = 1)
     int nne, n_shell, meke, sig_mono_e, sig_mono_p;
     int pad;
 };
+
+// NIST brems sampling grid: energies per medium and bins per alias table
+// ($MXBRES, $MXBRXS)
+constexpr int kNbGridE = 100, kNbGridX = 50;
 
 template <typename R>
 struct DevTables {
@@ -67,6 +72,11 @@ struct DevTables {
     const R *eno;
     const int *eno_atbin, *shell;
     const R *cs_be, *cs_Jo, *cs_erfJo;
+    // NIST brems alias tables [(m*kNbGridE + j-1)*(kNbGridX+1) + k] (x, f)
+    // and [(m*kNbGridE + j-1)*kNbGridX + k] (w, bin); null with
+    // Bethe-Heitler tables
+    const R *nb_x, *nb_f, *nb_w;
+    const int *nb_i;
     const int *cs_iz, *cs_shn;   // element and shell number of Compton shell j
     // elements [m*kMaxNE + k]
     const R *z, *pz;
@@ -118,6 +128,7 @@ struct DevParams {
     int play_rr;              // Russian roulette of photon-made charged particles
     int ireject;              // DOSXYZnrc range rejection below esave
     int presta1;              // transport_algorithm PRESTA-I (msdist_pI)
+    int ibr_nist;             // 1: NIST brems photon energies (alias tables)
     double ecut, pcut, skindepth;
     double esave;             // total energy (MeV) below which range rejection applies
 };

@@ -305,6 +305,10 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
         s.zbrang = f("zbrang"); s.lzbrang = f("lzbrang"); s.log_ap = f("log_ap");
         s.blcc = f("blcc"); s.xcc = f("xcc"); s.expeke1 = f("expeke1");
         s.esig_e = f("esig_e"); s.psig_e = f("psig_e");
+        const bool nist = b.has("ibr_nist") && b.i1("ibr_nist") == 1;
+        s.nb_emin = nist ? f("nb_emin") : (R)0;
+        s.nb_lemin = nist ? f("nb_lemin") : (R)0;
+        s.nb_dlei = nist ? f("nb_dlei") : (R)0;
         s.nne = b.i1(Blob::mk(m, "nne"));
         s.n_shell = b.i1(Blob::mk(m, "n_shell"));
         s.meke = b.i1(Blob::mk(m, "meke"));
@@ -383,6 +387,16 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
         T.shell_be = T.relax_prob = nullptr;
     }
     CK(pack_med<R>(h, b, nmed, "dl", 48, &T.dl));
+    static_assert(kNbGridE == kNbE && kNbGridX == kNbX, "brems sampling grid");
+    if (b.has("ibr_nist") && b.i1("ibr_nist") == 1) {
+        CK(pack_med<R>(h, b, nmed, "nb_xdata", kNbGridE * (kNbGridX + 1), &T.nb_x));
+        CK(pack_med<R>(h, b, nmed, "nb_fdata", kNbGridE * (kNbGridX + 1), &T.nb_f));
+        CK(pack_med<R>(h, b, nmed, "nb_wdata", kNbGridE * kNbGridX, &T.nb_w));
+        CK(pack_med_i(h, b, nmed, "nb_idata", kNbGridE * kNbGridX, &T.nb_i));
+    } else {
+        T.nb_x = T.nb_f = T.nb_w = nullptr;
+        T.nb_i = nullptr;
+    }
     {
         const std::vector<std::pair<const char *, int>> ef = {
             {"esig0", kESig}, {"esig1", kESig + 1}, {"psig0", kPSig}, {"psig1", kPSig + 1},
@@ -1022,6 +1036,16 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
             "a point source needs SSD (dsource) > 0 and x0 <= x1, y0 <= y1");
     EGS_ARG(src->kind != EGSGPU_SOURCE_DOSXYZ_FRONT || src->uz > 0,
             "a front-face source must point into the grid (uz > 0)");
+    {
+        const int blob_nist = tab->blob.has("ibr_nist") ? tab->blob.i1("ibr_nist") : 0;
+        if (tp->brems_xsec != blob_nist) {
+            set_error("brems_xsec = %d but the table blob was built with %s brems cross "
+                      "sections; build it with brems_xsec = %d",
+                      (int)tp->brems_xsec, blob_nist ? "NIST" : "Bethe-Heitler",
+                      (int)tp->brems_xsec);
+            return EGSGPU_E_ARG;
+        }
+    }
     int rc = require_device();
     if (rc) return rc;
     if (tp->spin) {
@@ -1135,6 +1159,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->P.esave = tp->esave;
     h->P.ireject = tp->ireject && tp->esave > h->P.ecut ? 1 : 0;
     h->P.presta1 = tp->step_algorithm == 1;
+    h->P.ibr_nist = tp->brems_xsec;
     // mscati (egsnrc.mortran:2790-2817): skindepth_for_bca <= 1e-4 selects
     // 3 elastic mean free paths with exact boundary crossing and
     // exp(BLCMIN) at the lowest ECUT with PRESTA-I, the minimum taken over

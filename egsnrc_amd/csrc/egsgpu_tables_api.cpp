@@ -29,6 +29,9 @@ int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
         if (tp->ximax > 0 && tp->ximax < 1) o.ximax = tp->ximax;
         o.bound_compton = tp->bound_compton;
         o.rayleigh = tp->rayleigh;
+        EGS_ARG(tp->brems_xsec == 0 || tp->brems_xsec == 1,
+                "brems_xsec: 0 Bethe-Heitler or 1 NIST");
+        o.ibr_nist = tp->brems_xsec;
     }
     Blob b;
     int rc = tables_build(hen_house, pegs_file, names, o, b);

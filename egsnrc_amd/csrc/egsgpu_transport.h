@@ -2230,6 +2230,32 @@ struct Shower {
     }
 
     // BREMS (egsnrc.mortran:312-526): Bethe-Heitler, Koch-Motz 2BS angle
+    // alias_sample1 (egs_utilities.mortran:1901-1931) over the NIST table
+    // of grid energy j (1-based) of medium m: a bin by the alias method,
+    // then a position inside it from the linear density between its ends
+    __device__ R nist_alias_sample(int m, int j) {
+        const size_t row = (size_t)m * kNbGridE + (j - 1);
+        const R *xs = T.nb_x + row * (kNbGridX + 1), *fs = T.nb_f + row * (kNbGridX + 1);
+        const R *ws = T.nb_w + row * kNbGridX;
+        const int *ib = T.nb_i + row * kNbGridX;
+        R r1 = rnd(), r2 = rnd();
+        R aj = 1 + r1 * kNbGridX;
+        int k = (int)aj;
+        if (k > kNbGridX) k = kNbGridX;
+        aj = aj - k;
+        if (aj > ws[k - 1]) k = ib[k - 1];
+        R x = xs[k - 1], dx = xs[k] - x, f0 = fs[k - 1];
+        if (f0 > 0) {
+            R a = fs[k] / f0 - 1;
+            if (fabs(a) < (R)0.2) {
+                R rnno1 = (R)0.5 * (1 - r2) * a;
+                return x + r2 * dx * (1 + rnno1 * (1 - r2 * a));
+            }
+            return x - dx / a * (1 - dsqrt(1 + r2 * a * (2 + a)));
+        }
+        return x + dx * dsqrt(r2);
+    }
+
     __device__ void brems(int m, const MedScalars<R> &M, R elke) {
         const R RM = (R)0.5109989461;
         const R *dl = T.dl + m * 48;
@@ -2256,9 +2282,36 @@ struct Shower {
         const int npold = np;
         const R wph = p.wt / (R)nbr;
         R pese = peie;
+        // NIST cross sections: the two grid energies around ekin
+        // (egsnrc.mortran:404-408)
+        const bool nist = P.ibr_nist != 0;
+        int jj = 0;
+        R ajj = 0;
+        if (nist) {
+            ajj = 1 + (waux + M.log_ap - M.nb_lemin) * M.nb_dlei;
+            jj = (int)ajj;
+            ajj = ajj - jj;
+            if (jj > kNbGridE) { jj = kNbGridE; ajj = -1; }
+        }
         for (int ibr = 0; ibr < nbr; ibr++) {
         R esg, ese;
         for (;;) {
+            if (nist) {
+                // photon energy from the alias table of grid energy j
+                // (egsnrc.mortran:412-422), no rejection
+                R br;
+                if (ekin > M.nb_emin) {
+                    int j = rnd() < ajj ? jj + 1 : jj;
+                    j = j < 1 ? 1 : (j > kNbGridE ? kNbGridE : j);
+                    br = nist_alias_sample(m, j);
+                } else {
+                    br = rnd();
+                }
+                esg = M.ap * dexp(br * waux);
+                pese = peie - esg;
+                ese = pese;
+                break;
+            }
             R rnno06 = rnd(), rnno07 = rnd();
             R br = brmin * dexp(rnno06 * waux);
             esg = ekin * br;

@@ -14,7 +14,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from egsnrc_amd import _lib  # noqa: E402
 
 BLOBS = {
-    # name: (pegs4 file, media)
+    # name: (pegs4 file, media[, brems_xsec])
     "521icru.egstab": ("521icru.pegs4dat",
                        ["H2O521ICRU", "LUNG521ICRU", "ICRPBONE521ICRU",
                         "AIR521ICRU", "AL521ICRU"]),
@@ -23,16 +23,18 @@ BLOBS = {
     # high-Z media of the tutorial data set (AE 0.7, AP 0.01 MeV): tantalum
     # of tutor2, lead for the atomic relaxation cascade, and water
     "tutor.egstab": ("tutor_data.pegs4dat", ["TA", "PB", "H2O"]),
+    # the NIST bremsstrahlung cross sections (ibr_nist = 1) for water and bone
+    "521icru_nist.egstab": ("521icru.pegs4dat", ["H2O521ICRU", "ICRPBONE521ICRU"], 1),
 }
 
 
-def build(hen_house, name, pegs, media, out_dir=HERE, estepe=0.25, ximax=0.5):
+def build(hen_house, name, pegs, media, out_dir=HERE, estepe=0.25, ximax=0.5, brems_xsec=0):
     L = _lib.lib()
     L.egsgpu_tables_build.argtypes = [C.c_char_p, C.c_char_p, C.c_int32,
                                       C.POINTER(C.c_char_p), C.c_void_p,
                                       C.c_char_p]
     arr = (C.c_char_p * len(media))(*[m.encode() for m in media])
-    tp = _lib.default_transport(estepe=estepe, ximax=ximax)
+    tp = _lib.default_transport(estepe=estepe, ximax=ximax, brems_xsec=brems_xsec)
     out = os.path.join(out_dir, name)
     _lib.check(L.egsgpu_tables_build(
         hen_house.encode(), os.path.join(hen_house, "pegs4", "data", pegs).encode(),
@@ -45,8 +47,10 @@ def main():
     ap.add_argument("--hen-house", default="/root/reference/HEN_HOUSE")
     ap.add_argument("--out-dir", default=HERE)
     a = ap.parse_args()
-    for name, (pegs, media) in BLOBS.items():
-        print("wrote", build(a.hen_house, name, pegs, media, a.out_dir))
+    for name, spec in BLOBS.items():
+        pegs, media = spec[0], spec[1]
+        nist = spec[2] if len(spec) > 2 else 0
+        print("wrote", build(a.hen_house, name, pegs, media, a.out_dir, brems_xsec=nist))
 
 
 if __name__ == "__main__":

@@ -171,6 +171,12 @@ typedef struct EgsTransport {
     int32_t step_algorithm;      /* transport_algorithm: 0 PRESTA-II
                                     (msdist_pII, default), 1 PRESTA-I
                                     (msdist_pI, egsnrc.mortran:4310-4515) */
+    int32_t brems_xsec;          /* ibr_nist: 0 Bethe-Heitler (default), 1 the
+                                    NIST cross sections of nist_brems.data
+                                    (init_nist_brems, egsnrc.mortran
+                                    :6893-7226); egsgpu_tables_build makes
+                                    its tables, egsgpu_create requires a
+                                    blob built with the same value      */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */
@@ -191,8 +197,9 @@ int egsgpu_tables_load(const char *path, egsgpu_tables **out);
 /* Build a table blob from the reference data files: the PEGS4 file
  * pegs_file (media looked up by name, HATCH egsnrc.mortran:2011-2176) and
  * the EGSnrc data directory hen_house/data + hen_house/pegs4/pgs4form.dat.
- * tp supplies ESTEPE/XIMAX (mscati, egsnrc.mortran:2750-2981), IBCMP and
- * IRAYLR.  Writes the blob to out_path. */
+ * tp supplies ESTEPE/XIMAX (mscati, egsnrc.mortran:2750-2981), IBCMP,
+ * IRAYLR and the brems cross sections (brems_xsec 1: NIST tables from
+ * hen_house/data/nist_brems.data).  Writes the blob to out_path. */
 int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
                         int32_t nmedia, const char *const *media,
                         const EgsTransport *tp, const char *out_path);

@@ -85,10 +85,16 @@ struct Medium {
     std::vector<double> esig0, esig1, psig0, psig1, ededx0, ededx1, pdedx0,
         pdedx1, ebr10, ebr11, pbr10, pbr11, pbr20, pbr21, tmxs0, tmxs1,
         range_ep0, range_ep1, e_array;
+    // NIST brems sampling (init_nist_brems): 100 grid energies of 50-bin
+    // alias tables, x/f [j*51 + k], w/bin [j*50 + k]
+    double nb_emin = 0, nb_lemin = 0, nb_dlei = 0;
+    std::vector<double> nb_x, nb_f, nb_w;
+    std::vector<int32_t> nb_i;
 };
 
 struct Tables {
     int nmed;
+    int ibr_nist = 0;   // 1: the blob holds NIST brems cross sections
     std::vector<Medium> med;
     // Compton shells (init_compton)
     std::vector<int32_t> cs_iz, cs_shn;
@@ -156,7 +162,15 @@ struct Tables {
             M.tmxs0 = b.F(K("tmxs0")); M.tmxs1 = b.F(K("tmxs1"));
             M.range_ep0 = b.F(K("range_ep0")); M.range_ep1 = b.F(K("range_ep1"));
             M.e_array = b.F(K("e_array"));
+            if (b.has("ibr_nist") && b.I("ibr_nist")[0] == 1) {
+                M.nb_emin = b.F(K("nb_emin"))[0];
+                M.nb_lemin = b.F(K("nb_lemin"))[0];
+                M.nb_dlei = b.F(K("nb_dlei"))[0];
+                M.nb_x = b.F(K("nb_xdata")); M.nb_f = b.F(K("nb_fdata"));
+                M.nb_w = b.F(K("nb_wdata")); M.nb_i = b.I(K("nb_idata"));
+            }
         }
+        ibr_nist = b.has("ibr_nist") ? b.I("ibr_nist")[0] : 0;
         cs_iz = b.I("cs_iz"); cs_shn = b.I("cs_shn");
         cs_Jo = b.F("cs_Jo"); cs_be = b.F("cs_be"); cs_erfJo = b.F("cs_erfJo");
         bind = b.F("binding_energies"); iprob = b.F("interaction_prob");

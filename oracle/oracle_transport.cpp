@@ -45,6 +45,7 @@ struct Params {
     double estepe, ximax, skindepth;
     int kerma;
     int presta1;   // transport_algorithm PRESTA-I (msdist_pI)
+    int ibr_nist;  // 1: NIST brems photon energies
 };
 
 struct Problem {
@@ -1472,6 +1473,30 @@ struct Shower {
     // BREMS (egsnrc.mortran:312-526): Bethe-Heitler (ibr_nist = 0), no
     // splitting, photon angle from the Koch-Motz 2BS (ibrdst = 1) or the
     // leading term (ibrdst = 0)
+    // alias_sample1 (egs_utilities.mortran:1901-1931) on the NIST table of
+    // grid energy j (1-based)
+    double alias_sample1(const Medium &M, int j) {
+        if (j < 1 || j > 100) throw std::runtime_error("NIST brems grid index out of range");
+        const double *xs = &M.nb_x[(size_t)(j - 1) * 51], *fs = &M.nb_f[(size_t)(j - 1) * 51];
+        const double *ws = &M.nb_w[(size_t)(j - 1) * 50];
+        const int32_t *ib = &M.nb_i[(size_t)(j - 1) * 50];
+        double r1 = rnd(), r2 = rnd();
+        double aj = 1 + r1 * 50;
+        int k = (int)aj;
+        aj = aj - k;
+        if (aj > ws[k - 1]) k = ib[k - 1];
+        double x = xs[k - 1], dx = xs[k] - x;
+        if (fs[k - 1] > 0) {
+            double a = fs[k] / fs[k - 1] - 1;
+            if (std::fabs(a) < 0.2) {
+                double rnno1 = 0.5 * (1 - r2) * a;
+                return x + r2 * dx * (1 + rnno1 * (1 - r2 * a));
+            }
+            return x - dx / a * (1 - std::sqrt(1 + r2 * a * (2 + a)));
+        }
+        return x + dx * std::sqrt(r2);
+    }
+
     void brems(const Medium &M, double elke) {
         Particle &p = top();
         double peie = p.e, eie = peie;
@@ -1492,7 +1517,31 @@ struct Shower {
         double z2max = y2max + 1, z2maxi = std::sqrt(z2max);
         double esg, pesg, pese, ese;
         const double(*dl)[8] = M.dl;
+        const bool nist = pb.P.ibr_nist == 1;
+        int jj = 0;
+        double ajj = 0;
+        if (nist) {   // egsnrc.mortran:404-408
+            ajj = 1 + (waux + M.log_ap - M.nb_lemin) * M.nb_dlei;
+            jj = (int)ajj;
+            ajj = ajj - jj;
+            if (jj > 100) { jj = 100; ajj = -1; }
+        }
         for (;;) {
+            if (nist) {   // egsnrc.mortran:412-422
+                double br;
+                if (ekin > M.nb_emin) {
+                    double r1 = rnd();
+                    int j = r1 < ajj ? jj + 1 : jj;
+                    br = alias_sample1(M, j);
+                } else {
+                    br = rnd();
+                }
+                esg = M.ap * std::exp(br * waux);
+                pesg = esg;
+                pese = peie - pesg;
+                ese = pese;
+                break;
+            }
             double rnno06 = rnd(), rnno07 = rnd();
             double br = brmin * std::exp(rnno06 * waux);
             esg = ekin * br;
@@ -1906,7 +1955,7 @@ struct OTransport {
     double estepe, ximax, skin_depth;
     int32_t kerma, fp64, howfarless, n_split, nbr_split, play_rr, ireject;
     double esave;
-    int32_t step_algorithm;
+    int32_t step_algorithm, brems_xsec;
 };
 
 // mscati's default PRESTA-I skin depth (egsnrc.mortran:2790-2817) over the
@@ -1962,6 +2011,9 @@ int oracle_create(const char *blob, const OGeom *g, const OSource *s,
         P.exact_bca = tp->exact_bca; P.estepe = tp->estepe; P.ximax = tp->ximax;
         P.skindepth = tp->skin_depth; P.kerma = tp->kerma;
         P.presta1 = tp->step_algorithm == 1;
+        P.ibr_nist = tp->brems_xsec;
+        if (P.ibr_nist != pb.T.ibr_nist)
+            throw std::runtime_error("brems_xsec does not match the blob's brems tables");
         if (P.skindepth <= 1e-4) {   // over the media the voxels use
             std::vector<int> used;
             for (int m = 0; m < pb.T.nmed; m++)
@@ -2101,6 +2153,7 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
         double ecut = 0, pcut = 0;
         for (auto &m : pb.T.med) { ecut = std::max(ecut, m.ae); pcut = std::max(pcut, m.ap); }
         P = Params{ecut, pcut, 1, 3, 1, 1, 1, 1, 0, 1, 0.25, 0.5, 3.0, 0};
+        P.ibr_nist = pb.T.ibr_nist;   // the blob's brems cross sections
         if (kind == 0) P.ibcmp = (int)param;           // 0 KN, 1 bound with rejection, 3 norej
         std::vector<double> tmp(1, 0.0);
         std::vector<int> touched;
