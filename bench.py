@@ -524,6 +524,16 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, command):
                    % (k, fe[k][0], fe[k][1], wr[k][1], rb, wb, rb + wb))
         if k.startswith("k_"):
             lines.append("traffic_bytes_per_launch %s: %.6g" % (k, rb + wb))
+    # the algorithmic bytes per launch of the dominant kernel in the profiled
+    # run (its bench line, printed into <fetch_dir>.log), so that a run with
+    # another launch shape can scale the traffic to its own launches
+    log = fetch_dir.rstrip("/") + ".log"
+    if os.path.exists(log):
+        bl = [l for l in open(log) if l.startswith("{")]
+        if bl:
+            rf = json.loads(bl[-1])["roofline"]
+            lines.append("algorithmic_bytes_per_launch %s: %.6g"
+                         % (rf["kernel"], rf["algorithmic_bytes_per_launch"]))
     return "\n".join(out + [""] + lines) + "\n"
 
 
@@ -567,16 +577,15 @@ def sq_summary(sq_dir, command):
 
 def read_profile_value(kind, kernel, workload, stem):
     """A `<kind> <kernel>: <value>` line written by sq_summary or
-    pmc_summary under profiles/<stem>_<workload>.md (kernel matched by name,
-    template arguments aside), with the file it came from."""
+    pmc_summary under profiles/<stem>_<workload>.md (kernel matched by name
+    and precision), with the file it came from."""
     import re
     p = os.path.join(ROOT, "profiles", "%s_%s.md" % (stem, workload))
     if not os.path.exists(p):
         return None, None
-    base = _kernel_key(kernel).split("<")[0]
     for l in open(p):
         m = re.match(r"%s (.+): (\S+)\s*$" % kind, l)
-        if m and m.group(1).split("<")[0] == base and "double" not in m.group(1):
+        if m and _same_kernel(m.group(1), kernel):
             return float(m.group(2)), os.path.relpath(p, ROOT)
     return None, None
 
@@ -626,19 +635,25 @@ def trace_summary(trace_dir, bench_log, command):
 
 
 def read_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` (matched by name, template
-    arguments aside) recorded by pmc_summary for this workload under
-    profiles/, with the file it came from; (None, None) if there is none."""
+    """HBM bytes per launch of `kernel` (matched by name and precision)
+    recorded by pmc_summary for this workload under profiles/, the
+    algorithmic bytes per launch of the profiled run, and the file they
+    came from; Nones if there is none."""
     import re
     p = os.path.join(ROOT, "profiles", "hbm_traffic_%s.md" % workload)
     if not os.path.exists(p):
-        return None, None
-    base = _kernel_key(kernel).split("<")[0]
+        return None, None, None
+    traffic = alg = None
     for l in open(p):
-        m = re.match(r"traffic_bytes_per_launch (.+): (\S+)\s*$", l)
-        if m and m.group(1).split("<")[0] == base:
-            return float(m.group(2)), os.path.relpath(p, ROOT)
-    return None, None
+        m = re.match(r"(traffic|algorithmic)_bytes_per_launch (.+): (\S+)\s*$", l)
+        if m and _same_kernel(m.group(2), kernel):
+            if m.group(1) == "traffic":
+                traffic = float(m.group(3))
+            else:
+                alg = float(m.group(3))
+    if traffic is None:
+        return None, None, None
+    return traffic, alg, os.path.relpath(p, ROOT)
 
 
 def main():
@@ -797,7 +812,13 @@ def main():
     if tp.kerma and a.schedule in (None, "history") and not a.howfarless:
         knames[0] = "k_kerma<float>"   # the kerma-mode history kernel
     nl = max(1, int(klaunch[kdom]))
-    traffic, traffic_src = read_traffic(knames[kdom], name)
+    traffic, traffic_alg, traffic_src = read_traffic(knames[kdom], name)
+    # the profile's traffic per launch, scaled to this run's launches by the
+    # algorithmic bytes (the autotuner may choose another launch shape than
+    # the profiled run's)
+    traffic_ratio = traffic / traffic_alg if traffic and traffic_alg else None
+    if traffic_ratio:
+        traffic = traffic_ratio * alg_bytes / nl
     out = {
         "metric": METRIC,
         "value": hist_s,
@@ -824,6 +845,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
+                     "traffic_over_algorithmic": traffic_ratio,
                      "algorithmic_bytes_per_launch": alg_bytes / nl,
                      "kernel": knames[kdom],
                      "kernel_launches": int(klaunch[kdom]),
