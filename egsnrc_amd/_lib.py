@@ -46,7 +46,8 @@ class EgsTransport(C.Structure):
                 ("kerma", C.c_int32), ("fp64", C.c_int32), ("howfarless", C.c_int32),
                 ("n_split", C.c_int32), ("nbr_split", C.c_int32), ("play_rr", C.c_int32),
                 ("ireject", C.c_int32), ("esave", C.c_double),
-                ("step_algorithm", C.c_int32), ("brems_xsec", C.c_int32)]
+                ("step_algorithm", C.c_int32), ("brems_xsec", C.c_int32),
+                ("pair_xsec", C.c_int32)]
 
 
 def default_transport(**kw):
@@ -56,7 +57,7 @@ def default_transport(**kw):
                      brems_angle=1, spin=0, exact_bca=1, estepe=0.25,
                      ximax=0.5, skin_depth=3.0, kerma=0, fp64=0, howfarless=0,
                      n_split=0, nbr_split=0, play_rr=0, ireject=0, esave=0.0,
-                     step_algorithm=0, brems_xsec=0)
+                     step_algorithm=0, brems_xsec=0, pair_xsec=0)
     for k, v in kw.items():
         setattr(t, k, v)
     return t

@@ -113,6 +113,8 @@ void gauss_legendre01(int n, std::vector<double> &x, std::vector<double> &w) {
     }
 }
 
+}  // namespace
+
 // Alias table of the piecewise-linear density through (xs[i], fs[i]),
 // i = 0..n (prepare_alias_table, egs_utilities.mortran:1846-1899): bin i
 // (1..n) keeps its own mass with probability ws[i-1] and otherwise
@@ -140,8 +142,6 @@ void alias_table(int n, const double *xs, const double *fs, double *ws, int32_t 
         if (it == n - 1) ws[jh - 1] = 1;
     }
 }
-
-}  // namespace
 
 int nist_brems_tables(const std::string &data_dir, int ibr_nist, Blob &out) {
     if (ibr_nist != 1)

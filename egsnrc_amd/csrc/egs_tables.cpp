@@ -996,6 +996,8 @@ int tables_build(const std::string &hh, const std::string &pegs_file,
         // egsnrc.mortran:2261-2289): it rewrites their cross sections
         out.i["ibr_nist"] = {0};
         if (o.ibr_nist >= 1) nist_brems_tables(data, o.ibr_nist, out);
+        out.i["pair_nrc"] = {0};
+        if (o.pair_nrc) nrc_pair_tables(data, out);
     } catch (const std::exception &e) {
         set_error("table build failed: %s", e.what());
         return EGSGPU_E_FORMAT;

@@ -50,6 +50,7 @@ struct BuildOptions {
     int bound_compton = 3;    // IBCMP
     int rayleigh = 1;         // IRAYLR
     int ibr_nist = 0;         // 0 Bethe-Heitler, 1 NIST brems cross sections
+    int pair_nrc = 0;         // 1: add the NRC pair sampling tables
 };
 
 int tables_build(const std::string &hen_house, const std::string &pegs_file,
@@ -93,6 +94,19 @@ struct Records {
 // tables nb_xdata/nb_fdata/nb_wdata/nb_idata ([kNbE][kNbX+1] / [kNbE][kNbX])
 // with nb_emin, nb_emax, nb_lemin, nb_dlei; sets "ibr_nist".
 int nist_brems_tables(const std::string &data_dir, int ibr_nist, Blob &out);
+// NRC pair cross sections (init_nrc_pair, egsnrc.mortran:7256-7352,
+// pair_nrc = 1): from data_dir/pair_nrc1.data, per medium the positron
+// energy-fraction densities at kNrcpE photon energies on the kNrcpX-point
+// grid nrcp_xdata, their alias tables nrcp_fdata/nrcp_wdata/nrcp_idata and
+// the grid constants nrcp_emin, nrcp_emax, nrcp_dlei (k in units of m c^2).
+int nrc_pair_tables(const std::string &data_dir, Blob &out);
+constexpr int kNrcpE = 84;      // photon energies ($NRC-PAIR-NEE)
+constexpr int kNrcpX = 65;      // energy-fraction points ($NRC-PAIR-NXX)
+
+// prepare_alias_table (egs_utilities.mortran:1846-1899) for the linear
+// density through (xs[i], fs[i]), i = 0..n: ws[n] and 1-based ibin[n].
+void alias_table(int n, const double *xs, const double *fs, double *ws, int32_t *ibin);
+
 constexpr int kNbE = 100;       // energies of the brems sampling grid ($MXBRES)
 constexpr int kNbX = 50;        // bins per alias table ($MXBRXS)
 constexpr int kNbGauss = 64;    // Gauss-Legendre points ($MXGAUSS)

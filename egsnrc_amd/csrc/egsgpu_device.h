@@ -56,6 +56,9 @@ struct MedScalars {
 // NIST brems sampling grid: energies per medium and bins per alias table
 // ($MXBRES, $MXBRXS)
 constexpr int kNbGridE = 100, kNbGridX = 50;
+// NRC pair grid: photon energies and positron energy-fraction points
+// ($NRC-PAIR-NEE, $NRC-PAIR-NXX)
+constexpr int kNrcpGridE = 84, kNrcpGridX = 65;
 
 template <typename R>
 struct DevTables {
@@ -77,6 +80,13 @@ struct DevTables {
     // Bethe-Heitler tables
     const R *nb_x, *nb_f, *nb_w;
     const int *nb_i;
+    // NRC pair tables (pair_nrc = 1): positron energy-fraction grid
+    // [kNrcpGridX], densities [(m*kNrcpGridE + ie)*kNrcpGridX + ix], alias
+    // weights and bins [(m*kNrcpGridE + ie)*(kNrcpGridX-1) + ix]; photon
+    // energies k = E/m from nrcp_emin to nrcp_emax, log(k-2) spaced
+    const R *nrcp_x, *nrcp_f, *nrcp_w;
+    const int *nrcp_i;
+    R nrcp_emin, nrcp_emax, nrcp_dlei;
     const int *cs_iz, *cs_shn;   // element and shell number of Compton shell j
     // elements [m*kMaxNE + k]
     const R *z, *pz;
@@ -129,6 +139,7 @@ struct DevParams {
     int ireject;              // DOSXYZnrc range rejection below esave
     int presta1;              // transport_algorithm PRESTA-I (msdist_pI)
     int ibr_nist;             // 1: NIST brems photon energies (alias tables)
+    int pair_nrc;             // 1: NRC pair energy sharing (alias tables)
     double ecut, pcut, skindepth;
     double esave;             // total energy (MeV) below which range rejection applies
 };

@@ -397,6 +397,20 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
         T.nb_x = T.nb_f = T.nb_w = nullptr;
         T.nb_i = nullptr;
     }
+    static_assert(kNrcpGridE == kNrcpE && kNrcpGridX == kNrcpX, "NRC pair grid");
+    if (b.has("pair_nrc") && b.i1("pair_nrc") == 1) {
+        CK(pack_glob<R>(h, b, "nrcp_xdata", &T.nrcp_x));
+        CK(pack_med<R>(h, b, nmed, "nrcp_fdata", kNrcpGridE * kNrcpGridX, &T.nrcp_f));
+        CK(pack_med<R>(h, b, nmed, "nrcp_wdata", kNrcpGridE * (kNrcpGridX - 1), &T.nrcp_w));
+        CK(pack_med_i(h, b, nmed, "nrcp_idata", kNrcpGridE * (kNrcpGridX - 1), &T.nrcp_i));
+        T.nrcp_emin = (R)b.F("nrcp_const")[0];
+        T.nrcp_emax = (R)b.F("nrcp_const")[1];
+        T.nrcp_dlei = (R)b.F("nrcp_const")[2];
+    } else {
+        T.nrcp_x = T.nrcp_f = T.nrcp_w = nullptr;
+        T.nrcp_i = nullptr;
+        T.nrcp_emin = T.nrcp_emax = T.nrcp_dlei = 0;
+    }
     {
         const std::vector<std::pair<const char *, int>> ef = {
             {"esig0", kESig}, {"esig1", kESig + 1}, {"psig0", kPSig}, {"psig1", kPSig + 1},
@@ -1045,6 +1059,12 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
                       (int)tp->brems_xsec);
             return EGSGPU_E_ARG;
         }
+        const int blob_nrcp = tab->blob.has("pair_nrc") ? tab->blob.i1("pair_nrc") : 0;
+        if (tp->pair_xsec != 0 && (tp->pair_xsec != 1 || !blob_nrcp)) {
+            set_error("pair_xsec = %d needs a table blob built with the NRC pair tables "
+                      "(pair_xsec = 1)", (int)tp->pair_xsec);
+            return EGSGPU_E_ARG;
+        }
     }
     int rc = require_device();
     if (rc) return rc;
@@ -1160,6 +1180,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->P.ireject = tp->ireject && tp->esave > h->P.ecut ? 1 : 0;
     h->P.presta1 = tp->step_algorithm == 1;
     h->P.ibr_nist = tp->brems_xsec;
+    h->P.pair_nrc = tp->pair_xsec;
     // mscati (egsnrc.mortran:2790-2817): skindepth_for_bca <= 1e-4 selects
     // 3 elastic mean free paths with exact boundary crossing and
     // exp(BLCMIN) at the lowest ECUT with PRESTA-I, the minimum taken over

@@ -32,6 +32,8 @@ int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
         EGS_ARG(tp->brems_xsec == 0 || tp->brems_xsec == 1,
                 "brems_xsec: 0 Bethe-Heitler or 1 NIST");
         o.ibr_nist = tp->brems_xsec;
+        EGS_ARG(tp->pair_xsec == 0 || tp->pair_xsec == 1, "pair_xsec: 0 Bethe-Heitler or 1 NRC");
+        o.pair_nrc = tp->pair_xsec;
     }
     Blob b;
     int rc = tables_build(hen_house, pegs_file, names, o, b);

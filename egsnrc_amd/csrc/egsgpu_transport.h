@@ -1144,7 +1144,31 @@ struct Shower {
         R pese1, pese2;
         int iq1, iq2;
         const R *dl = T.dl + m * 48;   // dl[k*8 + l]
-        if (eig <= (R)2.1) {
+        const R kk = eig / RM;
+        if (P.pair_nrc && kk < T.nrcp_emax) {
+            // NRC energy sharing (egsnrc.mortran:4596-4620): the positron's
+            // fraction of the kinetic energy from the alias table of the
+            // photon energy bin
+            int ibin = 1;
+            if (kk > T.nrcp_emin) {
+                R abin = 1 + dlog((kk - 2) / (T.nrcp_emin - 2)) * T.nrcp_dlei;
+                ibin = (int)abin;
+                abin = abin - ibin;
+                if (rnd() < abin) ibin = ibin + 1;
+                ibin = ibin < 1 ? 1 : (ibin > kNrcpGridE ? kNrcpGridE : ibin);
+            }
+            const size_t row = (size_t)m * kNrcpGridE + (ibin - 1);
+            R xx = alias_sample(T.nrcp_x, T.nrcp_f + row * kNrcpGridX,
+                                T.nrcp_w + row * (kNrcpGridX - 1),
+                                T.nrcp_i + row * (kNrcpGridX - 1), kNrcpGridX - 1);
+            if (xx > (R)0.5) {
+                pese1 = RM * (1 + xx * (kk - 2)); iq1 = 1;
+                pese2 = peig - pese1; iq2 = -1;
+            } else {
+                pese2 = RM * (1 + xx * (kk - 2)); iq2 = 1;
+                pese1 = peig - pese2; iq1 = -1;
+            }
+        } else if (eig <= (R)2.1) {
             R rnno30 = rnd(), rnno34 = rnd();
             pese2 = RM + (R)0.5 * rnno30 * (peig - 2 * RM);
             pese1 = peig - pese2;
@@ -2230,18 +2254,14 @@ struct Shower {
     }
 
     // BREMS (egsnrc.mortran:312-526): Bethe-Heitler, Koch-Motz 2BS angle
-    // alias_sample1 (egs_utilities.mortran:1901-1931) over the NIST table
-    // of grid energy j (1-based) of medium m: a bin by the alias method,
-    // then a position inside it from the linear density between its ends
-    __device__ R nist_alias_sample(int m, int j) {
-        const size_t row = (size_t)m * kNbGridE + (j - 1);
-        const R *xs = T.nb_x + row * (kNbGridX + 1), *fs = T.nb_f + row * (kNbGridX + 1);
-        const R *ws = T.nb_w + row * kNbGridX;
-        const int *ib = T.nb_i + row * kNbGridX;
+    // alias_sample1 (egs_utilities.mortran:1901-1931) over an n-bin table:
+    // a bin by the alias method, then a position inside it from the linear
+    // density between its ends
+    __device__ R alias_sample(const R *xs, const R *fs, const R *ws, const int *ib, int n) {
         R r1 = rnd(), r2 = rnd();
-        R aj = 1 + r1 * kNbGridX;
+        R aj = 1 + r1 * n;
         int k = (int)aj;
-        if (k > kNbGridX) k = kNbGridX;
+        if (k > n) k = n;
         aj = aj - k;
         if (aj > ws[k - 1]) k = ib[k - 1];
         R x = xs[k - 1], dx = xs[k] - x, f0 = fs[k - 1];
@@ -2254,6 +2274,13 @@ struct Shower {
             return x - dx / a * (1 - dsqrt(1 + r2 * a * (2 + a)));
         }
         return x + dx * dsqrt(r2);
+    }
+
+    // the NIST brems table of grid energy j (1-based) of medium m
+    __device__ R nist_alias_sample(int m, int j) {
+        const size_t row = (size_t)m * kNbGridE + (j - 1);
+        return alias_sample(T.nb_x + row * (kNbGridX + 1), T.nb_f + row * (kNbGridX + 1),
+                            T.nb_w + row * kNbGridX, T.nb_i + row * kNbGridX, kNbGridX);
     }
 
     __device__ void brems(int m, const MedScalars<R> &M, R elke) {

@@ -23,18 +23,23 @@ BLOBS = {
     # high-Z media of the tutorial data set (AE 0.7, AP 0.01 MeV): tantalum
     # of tutor2, lead for the atomic relaxation cascade, and water
     "tutor.egstab": ("tutor_data.pegs4dat", ["TA", "PB", "H2O"]),
-    # the NIST bremsstrahlung cross sections (ibr_nist = 1) for water and bone
+    # the NIST bremsstrahlung cross sections (ibr_nist = 1) and the NRC pair
+    # tables for water and bone
     "521icru_nist.egstab": ("521icru.pegs4dat", ["H2O521ICRU", "ICRPBONE521ICRU"], 1),
 }
 
 
-def build(hen_house, name, pegs, media, out_dir=HERE, estepe=0.25, ximax=0.5, brems_xsec=0):
+def build(hen_house, name, pegs, media, out_dir=HERE, estepe=0.25, ximax=0.5, brems_xsec=0,
+          pair_xsec=None):
     L = _lib.lib()
     L.egsgpu_tables_build.argtypes = [C.c_char_p, C.c_char_p, C.c_int32,
                                       C.POINTER(C.c_char_p), C.c_void_p,
                                       C.c_char_p]
     arr = (C.c_char_p * len(media))(*[m.encode() for m in media])
-    tp = _lib.default_transport(estepe=estepe, ximax=ximax, brems_xsec=brems_xsec)
+    if pair_xsec is None:
+        pair_xsec = brems_xsec   # the NIST blob carries the NRC pair tables too
+    tp = _lib.default_transport(estepe=estepe, ximax=ximax, brems_xsec=brems_xsec,
+                                pair_xsec=pair_xsec)
     out = os.path.join(out_dir, name)
     _lib.check(L.egsgpu_tables_build(
         hen_house.encode(), os.path.join(hen_house, "pegs4", "data", pegs).encode(),

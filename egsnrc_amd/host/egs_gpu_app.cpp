@@ -507,6 +507,11 @@ class EGS_GPUApplication {
         }
         if (!t->get("Brems angular sampling", v)) tp.brems_angle = InputItem::same(v, "Simple") ? 0 : 1;
         if (!t->get("Pair angular sampling", v)) tp.pair_angle = InputItem::same(v, "Off") ? 0 : 1;
+        if (!t->get("Pair cross sections", v)) {
+            if (InputItem::same(v, "BH")) tp.pair_xsec = 0;
+            else if (InputItem::same(v, "NRC")) tp.pair_xsec = 1;
+            else throw std::runtime_error("Pair cross sections = " + v + ": BH or NRC");
+        }
         if (!t->get("Brems cross sections", v)) {
             if (InputItem::same(v, "BH")) tp.brems_xsec = 0;
             else if (InputItem::same(v, "NIST")) tp.brems_xsec = 1;
@@ -745,11 +750,11 @@ class EGS_GPUApplication {
         printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
                "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d "
                "n_split %d nbr_split %d play_rr %d ireject %d esave %.6g exact_bca %d "
-               "step_algorithm %d brems_xsec %d\n",
+               "step_algorithm %d brems_xsec %d pair_xsec %d\n",
                tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
                tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth,
                tp.howfarless, tp.n_split, tp.nbr_split, tp.play_rr, tp.ireject, tp.esave,
-               tp.exact_bca, tp.step_algorithm, tp.brems_xsec);
+               tp.exact_bca, tp.step_algorithm, tp.brems_xsec, tp.pair_xsec);
     }
 
     void init_run() {

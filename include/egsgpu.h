@@ -177,6 +177,13 @@ typedef struct EgsTransport {
                                     :6893-7226); egsgpu_tables_build makes
                                     its tables, egsgpu_create requires a
                                     blob built with the same value      */
+    int32_t pair_xsec;           /* pair_nrc: 0 Bethe-Heitler energy sharing
+                                    (default), 1 the NRC cross sections of
+                                    pair_nrc1.data (init_nrc_pair,
+                                    egsnrc.mortran:7256-7352, PAIR :4596
+                                    -4620); egsgpu_tables_build with 1 adds
+                                    the tables, which egsgpu_create then
+                                    requires                              */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */
@@ -198,8 +205,9 @@ int egsgpu_tables_load(const char *path, egsgpu_tables **out);
  * pegs_file (media looked up by name, HATCH egsnrc.mortran:2011-2176) and
  * the EGSnrc data directory hen_house/data + hen_house/pegs4/pgs4form.dat.
  * tp supplies ESTEPE/XIMAX (mscati, egsnrc.mortran:2750-2981), IBCMP,
- * IRAYLR and the brems cross sections (brems_xsec 1: NIST tables from
- * hen_house/data/nist_brems.data).  Writes the blob to out_path. */
+ * IRAYLR, the brems cross sections (brems_xsec 1: NIST tables from
+ * hen_house/data/nist_brems.data) and pair_xsec (1: add the NRC pair tables
+ * of hen_house/data/pair_nrc1.data).  Writes the blob to out_path. */
 int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
                         int32_t nmedia, const char *const *media,
                         const EgsTransport *tp, const char *out_path);

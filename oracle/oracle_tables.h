@@ -90,11 +90,19 @@ struct Medium {
     double nb_emin = 0, nb_lemin = 0, nb_dlei = 0;
     std::vector<double> nb_x, nb_f, nb_w;
     std::vector<int32_t> nb_i;
+    // NRC pair sampling (init_nrc_pair): 84 photon energies of 64-bin
+    // alias tables over the positron energy fraction, f [ie*65 + ix],
+    // w/bin [ie*64 + ix]
+    std::vector<double> nrcp_f, nrcp_w;
+    std::vector<int32_t> nrcp_i;
 };
 
 struct Tables {
     int nmed;
     int ibr_nist = 0;   // 1: the blob holds NIST brems cross sections
+    int pair_nrc = 0;   // 1: the blob holds the NRC pair tables
+    std::vector<double> nrcp_x;   // positron energy-fraction grid (65)
+    double nrcp_emin = 0, nrcp_emax = 0, nrcp_dlei = 0;
     std::vector<Medium> med;
     // Compton shells (init_compton)
     std::vector<int32_t> cs_iz, cs_shn;
@@ -171,6 +179,19 @@ struct Tables {
             }
         }
         ibr_nist = b.has("ibr_nist") ? b.I("ibr_nist")[0] : 0;
+        pair_nrc = b.has("pair_nrc") ? b.I("pair_nrc")[0] : 0;
+        if (pair_nrc) {
+            nrcp_x = b.F("nrcp_xdata");
+            nrcp_emin = b.F("nrcp_const")[0];
+            nrcp_emax = b.F("nrcp_const")[1];
+            nrcp_dlei = b.F("nrcp_const")[2];
+            for (int m = 0; m < nmed; m++) {
+                auto K = [&](const char *f) { return "m" + std::to_string(m) + "." + f; };
+                med[m].nrcp_f = b.F(K("nrcp_fdata"));
+                med[m].nrcp_w = b.F(K("nrcp_wdata"));
+                med[m].nrcp_i = b.I(K("nrcp_idata"));
+            }
+        }
         cs_iz = b.I("cs_iz"); cs_shn = b.I("cs_shn");
         cs_Jo = b.F("cs_Jo"); cs_be = b.F("cs_be"); cs_erfJo = b.F("cs_erfJo");
         bind = b.F("binding_energies"); iprob = b.F("interaction_prob");

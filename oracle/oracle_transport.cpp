@@ -46,6 +46,7 @@ struct Params {
     int kerma;
     int presta1;   // transport_algorithm PRESTA-I (msdist_pI)
     int ibr_nist;  // 1: NIST brems photon energies
+    int pair_nrc;  // 1: NRC pair energy sharing
 };
 
 struct Problem {
@@ -677,7 +678,29 @@ struct Shower {
         double peig = p.e, eig = peig;
         double pese1, pese2;
         int iq1, iq2;
-        if (eig <= 2.1) {
+        const double k = eig / RM;
+        if (pb.P.pair_nrc && k < T.nrcp_emax) {
+            // NRC energy sharing (egsnrc.mortran:4596-4620)
+            int ibin = 1;
+            if (k > T.nrcp_emin) {
+                double abin = 1 + std::log((k - 2) / (T.nrcp_emin - 2)) * T.nrcp_dlei;
+                ibin = (int)abin;
+                abin = abin - ibin;
+                double rbin = rnd();
+                if (rbin < abin) ibin = ibin + 1;
+            }
+            if (ibin < 1 || ibin > 84) throw std::runtime_error("NRC pair energy bin out of range");
+            double xx = alias_sample1(T.nrcp_x.data(), &M.nrcp_f[(size_t)(ibin - 1) * 65],
+                                      &M.nrcp_w[(size_t)(ibin - 1) * 64],
+                                      &M.nrcp_i[(size_t)(ibin - 1) * 64], 64);
+            if (xx > 0.5) {
+                pese1 = PRM * (1 + xx * (k - 2)); iq1 = 1;
+                pese2 = peig - pese1; iq2 = -1;
+            } else {
+                pese2 = PRM * (1 + xx * (k - 2)); iq2 = 1;
+                pese1 = peig - pese2; iq1 = -1;
+            }
+        } else if (eig <= 2.1) {
             double rnno30 = rnd(), rnno34 = rnd();
             pese2 = PRM + 0.5 * rnno30 * (peig - 2 * PRM);
             pese1 = peig - pese2;
@@ -1477,11 +1500,15 @@ struct Shower {
     // grid energy j (1-based)
     double alias_sample1(const Medium &M, int j) {
         if (j < 1 || j > 100) throw std::runtime_error("NIST brems grid index out of range");
-        const double *xs = &M.nb_x[(size_t)(j - 1) * 51], *fs = &M.nb_f[(size_t)(j - 1) * 51];
-        const double *ws = &M.nb_w[(size_t)(j - 1) * 50];
-        const int32_t *ib = &M.nb_i[(size_t)(j - 1) * 50];
+        return alias_sample1(&M.nb_x[(size_t)(j - 1) * 51], &M.nb_f[(size_t)(j - 1) * 51],
+                             &M.nb_w[(size_t)(j - 1) * 50], &M.nb_i[(size_t)(j - 1) * 50], 50);
+    }
+
+    // alias_sample1 over an n-bin table (xs, fs at n+1 points)
+    double alias_sample1(const double *xs, const double *fs, const double *ws,
+                         const int32_t *ib, int n) {
         double r1 = rnd(), r2 = rnd();
-        double aj = 1 + r1 * 50;
+        double aj = 1 + r1 * n;
         int k = (int)aj;
         aj = aj - k;
         if (aj > ws[k - 1]) k = ib[k - 1];
@@ -1955,7 +1982,7 @@ struct OTransport {
     double estepe, ximax, skin_depth;
     int32_t kerma, fp64, howfarless, n_split, nbr_split, play_rr, ireject;
     double esave;
-    int32_t step_algorithm, brems_xsec;
+    int32_t step_algorithm, brems_xsec, pair_xsec;
 };
 
 // mscati's default PRESTA-I skin depth (egsnrc.mortran:2790-2817) over the
@@ -2014,6 +2041,9 @@ int oracle_create(const char *blob, const OGeom *g, const OSource *s,
         P.ibr_nist = tp->brems_xsec;
         if (P.ibr_nist != pb.T.ibr_nist)
             throw std::runtime_error("brems_xsec does not match the blob's brems tables");
+        P.pair_nrc = tp->pair_xsec;
+        if (P.pair_nrc && !pb.T.pair_nrc)
+            throw std::runtime_error("pair_xsec = 1 needs the blob's NRC pair tables");
         if (P.skindepth <= 1e-4) {   // over the media the voxels use
             std::vector<int> used;
             for (int m = 0; m < pb.T.nmed; m++)
@@ -2154,6 +2184,7 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
         for (auto &m : pb.T.med) { ecut = std::max(ecut, m.ae); pcut = std::max(pcut, m.ap); }
         P = Params{ecut, pcut, 1, 3, 1, 1, 1, 1, 0, 1, 0.25, 0.5, 3.0, 0};
         P.ibr_nist = pb.T.ibr_nist;   // the blob's brems cross sections
+        P.pair_nrc = pb.T.pair_nrc;   // and pair tables, when it has them
         if (kind == 0) P.ibcmp = (int)param;           // 0 KN, 1 bound with rejection, 3 norej
         std::vector<double> tmp(1, 0.0);
         std::vector<int> touched;
