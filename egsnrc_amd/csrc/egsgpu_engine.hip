@@ -1072,6 +1072,12 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         set_error("spin effects need the spinms data, which the engine does not have");
         return EGSGPU_E_UNSUPPORTED;
     }
+    if (tp->pair_angle < 0 || tp->pair_angle > 4 || tp->brems_angle < 0 || tp->brems_angle > 1) {
+        set_error("pair_angle %d / brems_angle %d: pair angles take 0 Off, 1 Simple, 2 KM, "
+                  "3 Uniform, 4 Blend; brems angles 0 Simple, 1 KM",
+                  (int)tp->pair_angle, (int)tp->brems_angle);
+        return EGSGPU_E_ARG;
+    }
     if (tp->step_algorithm != 0 && tp->step_algorithm != 1) {
         set_error("step_algorithm = %d: 0 is PRESTA-II, 1 is PRESTA-I", (int)tp->step_algorithm);
         return EGSGPU_E_ARG;

@@ -1138,6 +1138,67 @@ struct Shower {
     }
 
     // ------------------------------------------------------------- PAIR
+    // the polar angle of a pair particle of total energy ese from a photon
+    // of energy eig, by the $SET-PAIR-ANGLE branch `use`
+    __device__ __forceinline__ void pair_angle(int use, const MedScalars<R> &M, R eig, R ese,
+                                               R &costhe, R &sinthe) {
+        const R RM = (R)0.5109989461;
+        if (use == 1) {
+            R pse = (ese - RM) * (ese + RM);
+            pse = pse > 0 ? dsqrt(pse) : (R)0;
+            costhe = 1 - 2 * rnd();
+            sinthe = RM * dsqrt((1 - costhe) * (1 + costhe)) / (pse * costhe + ese);
+            costhe = (ese * costhe + pse) / (pse * costhe + ese);
+        } else if (use == 2) {
+            const R ztarg = M.zbrang, tteig = eig / RM, ttese = ese / RM;
+            const R esedei = ttese / (tteig - ttese), eseder = 1 / esedei;
+            const R s1 = esedei + eseder;
+            const R l0 = (1 + eseder) * (1 + esedei) / (2 * tteig), l02 = l0 * l0;
+            // d G(xi) / d xi of the 3D-2003 distribution in xi = 1/(1 + (theta E/m)^2)
+            auto rej = [&](R xi) {
+                return 2 + 3 * s1 - 4 * (s1 + 1 - 4 * (xi - (R)0.5) * (xi - (R)0.5)) *
+                                       (1 + (R)0.25 * dlog(l02 + ztarg * xi * xi));
+            };
+            const R pt = (R)3.141593 * ttese;
+            const R ximin = 1 / (1 + pt * pt);
+            const R rejmin = rej(ximin);
+            const R ya = (2 / tteig) * (2 / tteig);
+            R xitry = dsqrt(ya / ztarg);
+            xitry = xitry < (R)0.5 ? xitry : (R)0.5;
+            xitry = xitry > ximin ? xitry : ximin;
+            xitry = xitry > (R)0.01 ? xitry : (R)0.01;
+            R galpha = 1 + (R)0.25 * dlog(ya + ztarg * xitry * xitry);
+            const R gbeta = (R)0.5 * ztarg * xitry / (ya + ztarg * xitry * xitry);
+            galpha = galpha - gbeta * (xitry - (R)0.5);
+            R ximid = galpha / (3 * gbeta);
+            ximid = galpha >= 0 ? (R)0.5 - ximid + dsqrt(ximid * ximid + (R)0.25)
+                                : (R)0.5 - ximid - dsqrt(ximid * ximid + (R)0.25);
+            ximid = ximid < (R)0.5 ? ximid : (R)0.5;
+            ximid = ximid > ximin ? ximid : ximin;
+            ximid = ximid > (R)0.01 ? ximid : (R)0.01;
+            const R rejmid = rej(ximid);
+            const R rejtop = (R)1.02 * (rejmin > rejmid ? rejmin : rejmid);
+            R theta;
+            for (;;) {
+                R xitst = rnd();
+                R rejtst = rej(xitst);
+                R rtest = rnd();
+                theta = dsqrt(1 / xitst - 1) / ttese;
+                if (rtest <= rejtst / rejtop && theta < (R)3.14159265358979) break;
+            }
+            sinthe = sin(theta);
+            costhe = cos(theta);
+        } else if (use == 3) {
+            costhe = 1 - 2 * rnd();
+            sinthe = (1 - costhe) * (1 + costhe);
+            sinthe = sinthe > 0 ? dsqrt(sinthe) : (R)0;
+        } else {
+            costhe = 1 - 2 * dsqrt(rnd());
+            sinthe = (1 - costhe) * (1 + costhe);
+            sinthe = sinthe > 0 ? dsqrt(sinthe) : (R)0;
+        }
+    }
+
     __device__ __forceinline__ void pair(int m, const MedScalars<R> &M) {
         const R RM = (R)0.5109989461, RMT2 = 2 * RM;
         R peig = p.e, eig = peig;
@@ -1231,14 +1292,18 @@ struct Shower {
         }
         p.e = pese1;
         if (P.iprdst > 0) {
+            // $SET-PAIR-ANGLE (egsnrc.macros:2070-2163): 1 the leading term,
+            // 2 KM (Motz, Olsen and Koch 3D-2003; the leading term below
+            // 4.14 MeV), 3 isotropic, 4 a blend of the leading term (weight
+            // E/(E+10 MeV) per particle) and 1 - 2 sqrt(r)
+            int use = P.iprdst;
+            if (use == 4) use = rnd() < pese1 / (pese1 + 10) ? 1 : 4;
+            else if (use == 2 && eig < (R)4.14) use = 1;
             for (int ichrg = 1; ichrg <= 2; ichrg++) {
                 R ese = ichrg == 1 ? pese1 : pese2;
-                R pse = (ese - RM) * (ese + RM);
-                pse = pse > 0 ? dsqrt(pse) : (R)0;
-                R costhe = rnd();
-                costhe = 1 - 2 * costhe;
-                R sinthe = RM * dsqrt((1 - costhe) * (1 + costhe)) / (pse * costhe + ese);
-                costhe = (ese * costhe + pse) / (pse * costhe + ese);
+                if (ichrg == 2 && P.iprdst == 4) use = rnd() < ese / (ese + 10) ? 1 : 4;
+                R costhe, sinthe;
+                pair_angle(use, M, eig, ese, costhe, sinthe);
                 if (ichrg == 1) {
                     uphi21(costhe, sinthe);
                 } else {

@@ -506,7 +506,15 @@ class EGS_GPUApplication {
                                           ": PRESTA-II, PRESTA-I, EGSnrc or default");
         }
         if (!t->get("Brems angular sampling", v)) tp.brems_angle = InputItem::same(v, "Simple") ? 0 : 1;
-        if (!t->get("Pair angular sampling", v)) tp.pair_angle = InputItem::same(v, "Off") ? 0 : 1;
+        // egs_advanced_application.cpp:546-551: Off, Simple, KM, Uniform, Blend
+        if (!t->get("Pair angular sampling", v)) {
+            const char *opt[] = {"Off", "Simple", "KM", "Uniform", "Blend"};
+            int k = 0;
+            while (k < 5 && !InputItem::same(v, opt[k])) k++;
+            if (k == 5) throw std::runtime_error("Pair angular sampling = " + v +
+                                                 ": Off, Simple, KM, Uniform or Blend");
+            tp.pair_angle = k;
+        }
         if (!t->get("Pair cross sections", v)) {
             if (InputItem::same(v, "BH")) tp.pair_xsec = 0;
             else if (InputItem::same(v, "NRC")) tp.pair_xsec = 1;

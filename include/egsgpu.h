@@ -119,7 +119,10 @@ typedef struct EgsTransport {
     int32_t bound_compton;       /* IBCMP 0 KN, 3 norej (default)          */
     int32_t photoelectron_angle; /* IPHTER                                 */
     int32_t atomic_relaxation;   /* IEDGFL                                 */
-    int32_t pair_angle;          /* IPRDST 0..2                            */
+    int32_t pair_angle;          /* IPRDST: 0 off, 1 leading term (default),
+                                    2 KM, 3 isotropic, 4 blend
+                                    ($SET-PAIR-ANGLE, egsnrc.macros
+                                    :2070-2163)                           */
     int32_t brems_angle;         /* IBRDST 0/1                             */
     int32_t spin;                /* spin effects (needs spinms data)       */
     int32_t exact_bca;           /* 1 exact boundary crossing (default);

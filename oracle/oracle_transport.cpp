@@ -673,6 +673,59 @@ struct Shower {
 
     // -------------------------------------------------------------- PAIR
     // egsnrc.mortran:4518-4731 (Bethe-Heitler, no NRC tables, no triplet)
+    // one $SET-PAIR-ANGLE branch: 1 leading term, 2 KM (the 3D-2003
+    // distribution by rejection in xi = 1/(1 + (theta E/m)^2)), 3 isotropic,
+    // 4 cos = 1 - 2 sqrt(r)
+    void pair_angle(int use, const Medium &M, double eig, double ese, double &costhe,
+                    double &sinthe) {
+        if (use == 1) {
+            double pse = std::sqrt(std::max(0.0, (ese - RM) * (ese + RM)));
+            costhe = 1.0 - 2.0 * rnd();
+            sinthe = RM * std::sqrt((1.0 - costhe) * (1.0 + costhe)) / (pse * costhe + ese);
+            costhe = (ese * costhe + pse) / (pse * costhe + ese);
+        } else if (use == 2) {
+            double ztarg = M.zbrang, tteig = eig / RM, ttese = ese / RM;
+            double esedei = ttese / (tteig - ttese), eseder = 1.0 / esedei;
+            auto rej = [&](double xi) {   // $SET-PAIR-REJECTION-FUNCTION
+                double a = (1.0 + eseder) * (1.0 + esedei) / (2.0 * tteig);
+                return 2.0 + 3.0 * (esedei + eseder) -
+                       4.0 * (esedei + eseder + 1.0 - 4.0 * (xi - 0.5) * (xi - 0.5)) *
+                           (1.0 + 0.25 * std::log(a * a + ztarg * xi * xi));
+            };
+            double ximin = 1.0 / (1.0 + std::pow(3.141593 * ttese, 2));
+            double rejmin = rej(ximin);
+            double ya = std::pow(2.0 / tteig, 2);
+            double xitry = std::max(0.01, std::max(ximin, std::min(0.5, std::sqrt(ya / ztarg))));
+            double galpha = 1.0 + 0.25 * std::log(ya + ztarg * xitry * xitry);
+            double gbeta = 0.5 * ztarg * xitry / (ya + ztarg * xitry * xitry);
+            galpha = galpha - gbeta * (xitry - 0.5);
+            double ximid = galpha / (3.0 * gbeta);
+            if (galpha >= 0.0) ximid = 0.5 - ximid + std::sqrt(ximid * ximid + 0.25);
+            else ximid = 0.5 - ximid - std::sqrt(ximid * ximid + 0.25);
+            ximid = std::max(0.01, std::max(ximin, std::min(0.5, ximid)));
+            double rejmid = rej(ximid);
+            double rejtop = 1.02 * std::max(rejmin, rejmid);
+            double theta;
+            for (;;) {
+                double xitst = rnd();
+                double rejtst = rej(xitst);
+                double rtest = rnd();
+                theta = std::sqrt(1.0 / xitst - 1.0) / ttese;
+                if (rtest <= rejtst / rejtop && theta < 3.14159265358979) break;
+            }
+            sinthe = std::sin(theta);
+            costhe = std::cos(theta);
+        } else if (use == 3) {
+            costhe = 1.0 - 2.0 * rnd();
+            sinthe = (1 - costhe) * (1 + costhe);
+            sinthe = sinthe > 0 ? std::sqrt(sinthe) : 0.0;
+        } else {
+            costhe = 1.0 - 2.0 * std::sqrt(rnd());
+            sinthe = (1 - costhe) * (1 + costhe);
+            sinthe = sinthe > 0 ? std::sqrt(sinthe) : 0.0;
+        }
+    }
+
     void pair(const Medium &M, int medium, double gle) {
         Particle &p = top();
         double peig = p.e, eig = peig;
@@ -760,18 +813,25 @@ struct Shower {
             rnno34 = rnd();
             if (rnno34 < 0.5) { iq1 = -1; iq2 = 1; } else { iq1 = 1; iq2 = -1; }
         }
-        // $SET-PAIR-ANGLE with iprdst = 1 (leading-order angle), the only
-        // default-path branch (egsnrc.macros:2060-2120)
+        // $SET-PAIR-ANGLE (egsnrc.macros:2070-2163)
         top().e = pese1;
         double ese2 = pese2;
         if (pb.P.iprdst > 0) {
+            int use = pb.P.iprdst;
+            if (use == 4) {
+                double rtest = rnd();
+                use = rtest < pese1 / (pese1 + 10) ? 1 : 4;
+            } else if (use == 2 && eig < 4.14) {
+                use = 1;
+            }
             for (int ichrg = 1; ichrg <= 2; ichrg++) {
                 double ese = ichrg == 1 ? pese1 : ese2;
-                double pse = std::sqrt(std::max(0.0, (ese - RM) * (ese + RM)));
-                double costhe = rnd();
-                costhe = 1.0 - 2.0 * costhe;
-                double sinthe = RM * std::sqrt((1.0 - costhe) * (1.0 + costhe)) / (pse * costhe + ese);
-                costhe = (ese * costhe + pse) / (pse * costhe + ese);
+                if (ichrg == 2 && pb.P.iprdst == 4) {
+                    double rtest = rnd();
+                    use = rtest < ese / (ese + 10) ? 1 : 4;
+                }
+                double costhe, sinthe;
+                pair_angle(use, M, eig, ese, costhe, sinthe);
                 if (ichrg == 1) {
                     uphi21(costhe, sinthe);
                 } else {
@@ -2276,6 +2336,13 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
                     if (i == 1) first = q.iq == 0 ? ek : -ek;
                 }
                 o[0] = first; o[1] = nph + 1000.0 * nel + 1e6 * sh.relax_depth; o[2] = ekin; o[3] = edep;
+            } else if (kind == 14) {
+                // PAIR with pair angular sampling `param` (0..4) along +z:
+                // the two particles' total energies and polar cosines
+                P.iprdst = (int)param;
+                p.e = energy; p.iq = 0; sh.stk.push_back(p);
+                sh.pair(M, medium, std::log(energy));
+                o[0] = sh.stk[0].e; o[1] = sh.stk[1].e; o[2] = sh.stk[0].w; o[3] = sh.stk[1].w;
             } else if (kind == 12) {
                 // one PRESTA-I step (msdist_pI) of length param without
                 // energy loss along +z from the origin
