@@ -1072,6 +1072,15 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         set_error("spin effects need the spinms data, which the engine does not have");
         return EGSGPU_E_UNSUPPORTED;
     }
+    if (tp->bound_compton < 0 || tp->bound_compton > 3 || tp->rayleigh < 0 ||
+        tp->rayleigh > 1 || tp->atomic_relaxation < 0 || tp->atomic_relaxation > 1 ||
+        tp->photoelectron_angle < 0 || tp->photoelectron_angle > 1) {
+        set_error("bound_compton %d (0 Off, 1 On, 2 Simple, 3 norej), rayleigh %d, "
+                  "atomic_relaxation %d (0 Off, 1 EADL), photoelectron_angle %d (0/1)",
+                  (int)tp->bound_compton, (int)tp->rayleigh, (int)tp->atomic_relaxation,
+                  (int)tp->photoelectron_angle);
+        return EGSGPU_E_ARG;
+    }
     if (tp->pair_angle < 0 || tp->pair_angle > 4 || tp->brems_angle < 0 || tp->brems_angle > 1) {
         set_error("pair_angle %d / brems_angle %d: pair angles take 0 Off, 1 Simple, 2 KM, "
                   "3 Uniform, 4 Blend; brems angles 0 Simple, 1 KM",

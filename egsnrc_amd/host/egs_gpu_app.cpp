@@ -36,6 +36,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <memory>
 #include <sstream>
 #include <algorithm>
@@ -478,16 +479,29 @@ class EGS_GPUApplication {
         };
         t->get("Global ECUT", tp.ecut);
         t->get("Global PCUT", tp.pcut);
-        onoff("Rayleigh scattering", tp.rayleigh);
         onoff("Spin effects", tp.spin);
         onoff("Photoelectron angular sampling", tp.photoelectron_angle);
-        onoff("Atomic relaxations", tp.atomic_relaxation);
         std::string v;
-        if (!t->get("Bound Compton scattering", v)) {
-            if (InputItem::same(v, "Off")) tp.bound_compton = 0;
-            else if (InputItem::same(v, "norej")) tp.bound_compton = 3;
-            else throw std::runtime_error("Bound Compton scattering = " + v + " not supported");
-        }
+        // the option lists of egs_advanced_application.cpp:497-531
+        auto choose = [&](const char *key, std::initializer_list<const char *> opts,
+                          std::initializer_list<int> values, int &dst) {
+            std::string val;
+            if (t->get(key, val)) return;
+            auto vi = values.begin();
+            for (const char *o : opts) {
+                if (InputItem::same(val, o)) { dst = *vi; return; }
+                ++vi;
+            }
+            throw std::runtime_error(std::string(key) + " = " + val + " is not supported");
+        };
+        // Rayleigh: custom form factors (iraylr 2) are not built
+        choose("Rayleigh scattering", {"Off", "On", "no", "yes", "0", "1"}, {0, 1, 0, 1, 0, 1},
+               tp.rayleigh);
+        // relaxation: On means EADL; the pre-EADL "simple" RELAX is not built
+        choose("Atomic relaxations", {"Off", "On", "eadl", "no", "yes", "0", "1"},
+               {0, 1, 1, 0, 1, 0, 1}, tp.atomic_relaxation);
+        choose("Bound Compton scattering", {"Off", "On", "Simple", "norej"}, {0, 1, 2, 3},
+               tp.bound_compton);
         // egs_advanced_application.cpp:552-612: the step algorithm takes
         // EGSnrc / PRESTA-I / PRESTA-II / default (all but PRESTA-I mean
         // PRESTA-II), boundary crossing Exact / PRESTA-I

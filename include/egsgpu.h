@@ -116,7 +116,8 @@ typedef struct EgsTransport {
     double ecut;                 /* total energy (MeV); <=0 -> max AE      */
     double pcut;                 /* MeV; <=0 -> max AP                     */
     int32_t rayleigh;            /* IRAYLR (egsnrc.macros:2299)            */
-    int32_t bound_compton;       /* IBCMP 0 KN, 3 norej (default)          */
+    int32_t bound_compton;       /* IBCMP 0 KN, 1 bound with rejection, 2
+                                    simple, 3 norej (default)              */
     int32_t photoelectron_angle; /* IPHTER                                 */
     int32_t atomic_relaxation;   /* IEDGFL                                 */
     int32_t pair_angle;          /* IPRDST: 0 off, 1 leading term (default),

@@ -441,9 +441,15 @@ def test_c2_electron_beam_reduced_parity(engine):
     g, keep = _lib.make_geometry(xb, yb, zb, med, rhor)
     o = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
     o.run(100000)
+    # a second oracle run calibrates the 2-sigma fraction of these
+    # heavy-tailed voxel errors (the Al cube's low-dose periphery)
+    on = oracle_lib.OracleSimulation(BLOB521, g, src[0], tp, nstreams=256, nthreads=8)
+    on.seed(*NULL_SEEDS)
+    on.run(100000)
     sim = Simulation(Tables(BLOB521), xb, yb, zb, med, rhor, src, tp, nstreams=16384)
     sim.run(100000)
-    check_parity(o.tallies(), sim.tallies(), sim.counters(), label="c2 reduced")
+    check_parity(o.tallies(), sim.tallies(), sim.counters(), label="c2 reduced",
+                 null=on.tallies())
 
 
 @pytest.mark.gpu
