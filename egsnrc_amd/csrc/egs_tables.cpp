@@ -534,48 +534,60 @@ int tables_build(const std::string &hh, const std::string &pegs_file,
                 out.f[K("ray_pmax0")] = pmax0;
                 out.f[K("ray_pmax1")] = pmax1;
             }
-            std::vector<double> g0(nge), g1(nge), b10(nge), b11(nge), b20(nge), b21(nge), c0(nge), c1(nge);
-            double gmfp_old = 0, gbr1_old = 0, gbr2_old = 0, cohe_old = 0, gmfp = 0, gbr1 = 0, gbr2 = 0, cohe = 0, gle = 0;
-            for (int i = 1; i <= nge; i++) {
-                gle = (i - ge0) / ge1;
-                double e = std::exp(gle);
-                double sig_KN = sumZ * kn_sigma0(e);
-                if (o.bound_compton > 1) {
-                    double bcf;
-                    if (e <= bc_emin) bcf = std::exp(bc_data[0]);
-                    else if (e < bc_emax) {
-                        double aj = 1 + std::log(e / bc_emin) / bc_dle;
-                        int j = (int)aj;
-                        aj -= j;
-                        bcf = std::exp(bc_data[j - 1] * (1 - aj) + bc_data[j] * aj);
-                    } else bcf = 1;
-                    sig_KN *= bcf;
+            // the photon rows with the bound-Compton total cross section
+            // (compton_sigma.data ratio, IBCMP > 1) or with Klein-Nishina
+            // (IBCMP 0 and 1): the build option's set under the plain keys,
+            // the Klein-Nishina set also under "<key>_kn" when the option
+            // is bound, so that a run may switch bound Compton off
+            auto photon_rows = [&](bool bound, const char *suffix) {
+                std::vector<double> g0(nge), g1(nge), b10(nge), b11(nge), b20(nge), b21(nge),
+                    c0(nge), c1(nge);
+                double gmfp_old = 0, gbr1_old = 0, gbr2_old = 0, cohe_old = 0, gmfp = 0, gbr1 = 0,
+                       gbr2 = 0, cohe = 0, gle = 0;
+                for (int i = 1; i <= nge; i++) {
+                    gle = (i - ge0) / ge1;
+                    double e = std::exp(gle);
+                    double sig_KN = sumZ * kn_sigma0(e);
+                    if (bound) {
+                        double bcf;
+                        if (e <= bc_emin) bcf = std::exp(bc_data[0]);
+                        else if (e < bc_emax) {
+                            double aj = 1 + std::log(e / bc_emin) / bc_dle;
+                            int j = (int)aj;
+                            aj -= j;
+                            bcf = std::exp(bc_data[j - 1] * (1 - aj) + bc_data[j] * aj);
+                        } else bcf = 1;
+                        sig_KN *= bcf;
+                    }
+                    double sig_p = sig_pair[i - 1] + sig_trip[i - 1];
+                    double sigma = sig_KN + sig_p + sig_photo[i - 1];
+                    gmfp = 1 / (sigma * con2);
+                    gbr1 = sig_p / sigma;
+                    gbr2 = gbr1 + sig_KN / sigma;
+                    cohe = sigma / (sig_rayl[i - 1] + sigma);
+                    if (i > 1) {
+                        g1[i - 2] = (gmfp - gmfp_old) * ge1; g0[i - 2] = gmfp - g1[i - 2] * gle;
+                        b11[i - 2] = (gbr1 - gbr1_old) * ge1; b10[i - 2] = gbr1 - b11[i - 2] * gle;
+                        b21[i - 2] = (gbr2 - gbr2_old) * ge1; b20[i - 2] = gbr2 - b21[i - 2] * gle;
+                        c1[i - 2] = (cohe - cohe_old) * ge1; c0[i - 2] = cohe - c1[i - 2] * gle;
+                    }
+                    gmfp_old = gmfp; gbr1_old = gbr1; gbr2_old = gbr2; cohe_old = cohe;
                 }
-                double sig_p = sig_pair[i - 1] + sig_trip[i - 1];
-                double sigma = sig_KN + sig_p + sig_photo[i - 1];
-                gmfp = 1 / (sigma * con2);
-                gbr1 = sig_p / sigma;
-                gbr2 = gbr1 + sig_KN / sigma;
-                cohe = sigma / (sig_rayl[i - 1] + sigma);
-                if (i > 1) {
-                    g1[i - 2] = (gmfp - gmfp_old) * ge1; g0[i - 2] = gmfp - g1[i - 2] * gle;
-                    b11[i - 2] = (gbr1 - gbr1_old) * ge1; b10[i - 2] = gbr1 - b11[i - 2] * gle;
-                    b21[i - 2] = (gbr2 - gbr2_old) * ge1; b20[i - 2] = gbr2 - b21[i - 2] * gle;
-                    c1[i - 2] = (cohe - cohe_old) * ge1; c0[i - 2] = cohe - c1[i - 2] * gle;
-                }
-                gmfp_old = gmfp; gbr1_old = gbr1; gbr2_old = gbr2; cohe_old = cohe;
-            }
-            g1[nge - 1] = g1[nge - 2]; g0[nge - 1] = gmfp - g1[nge - 1] * gle;
-            b11[nge - 1] = b11[nge - 2]; b10[nge - 1] = gbr1 - b11[nge - 1] * gle;
-            b21[nge - 1] = b21[nge - 2]; b20[nge - 1] = gbr2 - b21[nge - 1] * gle;
-            c1[nge - 1] = c1[nge - 2]; c0[nge - 1] = cohe - c1[nge - 1] * gle;
+                g1[nge - 1] = g1[nge - 2]; g0[nge - 1] = gmfp - g1[nge - 1] * gle;
+                b11[nge - 1] = b11[nge - 2]; b10[nge - 1] = gbr1 - b11[nge - 1] * gle;
+                b21[nge - 1] = b21[nge - 2]; b20[nge - 1] = gbr2 - b21[nge - 1] * gle;
+                c1[nge - 1] = c1[nge - 2]; c0[nge - 1] = cohe - c1[nge - 1] * gle;
+                auto KS = [&](const char *f) { return K(f) + std::string(suffix); };
+                out.f[KS("gmfp0")] = g0; out.f[KS("gmfp1")] = g1;
+                out.f[KS("gbr10")] = b10; out.f[KS("gbr11")] = b11;
+                out.f[KS("gbr20")] = b20; out.f[KS("gbr21")] = b21;
+                out.f[KS("cohe0")] = c0; out.f[KS("cohe1")] = c1;
+            };
+            photon_rows(o.bound_compton > 1, "");
+            if (o.bound_compton > 1) photon_rows(false, "_kn");
             out.i[K("mge")] = {nge};
             out.f[K("ge0")] = {ge0};
             out.f[K("ge1")] = {ge1};
-            out.f[K("gmfp0")] = g0; out.f[K("gmfp1")] = g1;
-            out.f[K("gbr10")] = b10; out.f[K("gbr11")] = b11;
-            out.f[K("gbr20")] = b20; out.f[K("gbr21")] = b21;
-            out.f[K("cohe0")] = c0; out.f[K("cohe1")] = c1;
 
             // ---- fix_brems (:2322-2408) and brems/pair angle constants
             //      ($INITIALIZE-BREMS-ANGLE, egsnrc.macros:1981-1993)

@@ -322,9 +322,17 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
     }
     CK(upload(h, ms.data(), ms.size(), &T.ms));
     {
-        const std::vector<std::pair<const char *, int>> gf = {
-            {"gmfp0", kGmfp}, {"gmfp1", kGmfp + 1}, {"gbr10", kGbr1}, {"gbr11", kGbr1 + 1},
-            {"gbr20", kGbr2}, {"gbr21", kGbr2 + 1}, {"cohe0", kCohe}, {"cohe1", kCohe + 1}};
+        // Klein-Nishina photon rows when the run has bound Compton off and
+        // the blob's main rows include the bound-Compton cross section
+        const bool kn = h->P.ibcmp <= 1 && b.i1("ibcmp") > 1;
+        const std::vector<std::pair<const char *, int>> gf = kn ?
+            std::vector<std::pair<const char *, int>>{
+                {"gmfp0_kn", kGmfp}, {"gmfp1_kn", kGmfp + 1}, {"gbr10_kn", kGbr1},
+                {"gbr11_kn", kGbr1 + 1}, {"gbr20_kn", kGbr2}, {"gbr21_kn", kGbr2 + 1},
+                {"cohe0_kn", kCohe}, {"cohe1_kn", kCohe + 1}} :
+            std::vector<std::pair<const char *, int>>{
+                {"gmfp0", kGmfp}, {"gmfp1", kGmfp + 1}, {"gbr10", kGbr1}, {"gbr11", kGbr1 + 1},
+                {"gbr20", kGbr2}, {"gbr21", kGbr2 + 1}, {"cohe0", kCohe}, {"cohe1", kCohe + 1}};
         CK(pack_rows<R>(h, b, nmed, kGE, kGRow, gf, &T.grow));
     }
     bool ray = b.has(Blob::mk(0, "ray_x"));
@@ -1057,6 +1065,21 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
                       "sections; build it with brems_xsec = %d",
                       (int)tp->brems_xsec, blob_nist ? "NIST" : "Bethe-Heitler",
                       (int)tp->brems_xsec);
+            return EGSGPU_E_ARG;
+        }
+        // egs_init_user_photon (egsnrc.mortran:8028-8274): the Compton part
+        // of the photon cross sections is bound (compton_sigma.data) for
+        // IBCMP > 1 and Klein-Nishina otherwise
+        const int blob_ibcmp = tab->blob.i1("ibcmp");
+        if (tp->bound_compton > 1 && blob_ibcmp <= 1) {
+            set_error("bound_compton = %d needs tables built with bound Compton cross "
+                      "sections (egsgpu_tables_build with bound_compton > 1)",
+                      (int)tp->bound_compton);
+            return EGSGPU_E_ARG;
+        }
+        if (tp->bound_compton <= 1 && blob_ibcmp > 1 && !tab->blob.has(Blob::mk(0, "gmfp0_kn"))) {
+            set_error("bound_compton = %d needs the Klein-Nishina photon rows, which this "
+                      "table blob lacks; rebuild it", (int)tp->bound_compton);
             return EGSGPU_E_ARG;
         }
         const int blob_nrcp = tab->blob.has("pair_nrc") ? tab->blob.i1("pair_nrc") : 0;

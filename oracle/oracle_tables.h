@@ -71,6 +71,10 @@ struct Medium {
     // photons
     double ge0, ge1;
     std::vector<double> gmfp0, gmfp1, gbr10, gbr11, gbr20, gbr21, cohe0, cohe1;
+    // the same rows with the Klein-Nishina Compton cross section (empty
+    // when the blob's main rows already are Klein-Nishina)
+    std::vector<double> kn_gmfp0, kn_gmfp1, kn_gbr10, kn_gbr11, kn_gbr20, kn_gbr21, kn_cohe0,
+        kn_cohe1;
     bool has_ray;
     std::vector<double> ray_x, ray_fcum, ray_b, ray_c, pmax0, pmax1;
     std::vector<int32_t> ray_i;
@@ -100,6 +104,23 @@ struct Medium {
 struct Tables {
     int nmed;
     int ibr_nist = 0;   // 1: the blob holds NIST brems cross sections
+    int ibcmp = 3;      // the bound-Compton option the main photon rows were built for
+
+    // a run with bound Compton off (IBCMP 0, 1) uses Klein-Nishina photon
+    // rows (egs_init_user_photon, egsnrc.mortran:8028-8274)
+    void select_compton_rows(int run_ibcmp) {
+        if (run_ibcmp > 1 && ibcmp <= 1)
+            throw std::runtime_error("bound Compton needs tables built with it");
+        if (run_ibcmp > 1 || ibcmp <= 1) return;
+        for (auto &M : med) {
+            if (M.kn_gmfp0.empty()) throw std::runtime_error("the blob lacks Klein-Nishina rows");
+            M.gmfp0 = M.kn_gmfp0; M.gmfp1 = M.kn_gmfp1;
+            M.gbr10 = M.kn_gbr10; M.gbr11 = M.kn_gbr11;
+            M.gbr20 = M.kn_gbr20; M.gbr21 = M.kn_gbr21;
+            M.cohe0 = M.kn_cohe0; M.cohe1 = M.kn_cohe1;
+        }
+        ibcmp = run_ibcmp;
+    }
     int pair_nrc = 0;   // 1: the blob holds the NRC pair tables
     std::vector<double> nrcp_x;   // positron energy-fraction grid (65)
     double nrcp_emin = 0, nrcp_emax = 0, nrcp_dlei = 0;
@@ -139,6 +160,12 @@ struct Tables {
             M.gbr10 = b.F(K("gbr10")); M.gbr11 = b.F(K("gbr11"));
             M.gbr20 = b.F(K("gbr20")); M.gbr21 = b.F(K("gbr21"));
             M.cohe0 = b.F(K("cohe0")); M.cohe1 = b.F(K("cohe1"));
+            if (b.has(K("gmfp0_kn"))) {
+                M.kn_gmfp0 = b.F(K("gmfp0_kn")); M.kn_gmfp1 = b.F(K("gmfp1_kn"));
+                M.kn_gbr10 = b.F(K("gbr10_kn")); M.kn_gbr11 = b.F(K("gbr11_kn"));
+                M.kn_gbr20 = b.F(K("gbr20_kn")); M.kn_gbr21 = b.F(K("gbr21_kn"));
+                M.kn_cohe0 = b.F(K("cohe0_kn")); M.kn_cohe1 = b.F(K("cohe1_kn"));
+            }
             M.has_ray = b.has(K("ray_x"));
             if (M.has_ray) {
                 M.ray_x = b.F(K("ray_x")); M.ray_fcum = b.F(K("ray_fcum"));
@@ -179,6 +206,7 @@ struct Tables {
             }
         }
         ibr_nist = b.has("ibr_nist") ? b.I("ibr_nist")[0] : 0;
+        ibcmp = b.I("ibcmp")[0];
         pair_nrc = b.has("pair_nrc") ? b.I("pair_nrc")[0] : 0;
         if (pair_nrc) {
             nrcp_x = b.F("nrcp_xdata");

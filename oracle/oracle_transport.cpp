@@ -2093,6 +2093,7 @@ int oracle_create(const char *blob, const OGeom *g, const OSource *s,
         P.ecut = std::max(ecut, tp->ecut);   // $adjust_rhor_ecut_pcut
         P.pcut = std::max(pcut, tp->pcut);
         P.rayleigh = tp->rayleigh; P.ibcmp = tp->bound_compton;
+        pb.T.select_compton_rows(P.ibcmp);
         P.iphter = tp->photoelectron_angle; P.iedgfl = tp->atomic_relaxation;
         P.iprdst = tp->pair_angle; P.ibrdst = tp->brems_angle; P.spin = tp->spin;
         P.exact_bca = tp->exact_bca; P.estepe = tp->estepe; P.ximax = tp->ximax;

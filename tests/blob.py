@@ -33,14 +33,15 @@ def medium_names(blob):
     return ["".join(chr(c) for c in blob["m%d.name" % m]) for m in range(n)]
 
 
-def pwlf(blob, m, field, x, grid="g"):
+def pwlf(blob, m, field, x, grid="g", suffix=""):
     """Evaluate a piecewise-linear table ($SET INTERVAL/$EVALUATE,
-    egsnrc.macros:1226-1248) at log-energy x (numpy, FP64)."""
+    egsnrc.macros:1226-1248) at log-energy x (numpy, FP64); suffix picks
+    a variant of the table ("_kn": the Klein-Nishina photon rows)."""
     if grid == "g":
         a, b = blob["m%d.ge1" % m][0], blob["m%d.ge0" % m][0]
     else:
         a, b = blob["m%d.eke1" % m][0], blob["m%d.eke0" % m][0]
     L = (a * x + b).astype(np.int64)
-    c0 = blob["m%d.%s0" % (m, field)]
-    c1 = blob["m%d.%s1" % (m, field)]
+    c0 = blob["m%d.%s0%s" % (m, field, suffix)]
+    c1 = blob["m%d.%s1%s" % (m, field, suffix)]
     return c1[L - 1] * x + c0[L - 1]

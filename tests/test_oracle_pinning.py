@@ -80,6 +80,39 @@ def test_photon_tables_match_an_independent_xcom_sum():
         assert np.all(np.abs(got - exp) <= 1e-6 + 2e-4 * exp), (name, "gbr1")
 
 
+def test_klein_nishina_photon_rows():
+    """The rows a run with bound Compton off uses (egs_init_user_photon
+    takes the Klein-Nishina Compton cross section for IBCMP <= 1) follow from
+    the independent sums of the fixture: the Compton part of the total,
+    (gbr2 - gbr1) / gmfp, divided by the fixture's bound/free ratio of
+    compton_sigma.data, with photo, pair and Rayleigh unchanged."""
+    b = blob()
+    fx = json.load(open(os.path.join(GOLDEN, "photon_xs_521icru.json")))
+    for m, name in enumerate(medium_names(b)):
+        rows = fx["media"][name]["rows"]
+        x = np.log([r["e"] for r in rows])
+        gmfp = np.array([r["gmfp"] for r in rows])
+        gbr1 = np.array([r["gbr1"] for r in rows])
+        gbr2 = np.array([r["gbr2"] for r in rows])
+        cohe = np.array([r["cohe"] for r in rows])
+        ratio = np.array([r["compton_ratio"] for r in rows])
+        sig_b = 1 / gmfp
+        comp_b = (gbr2 - gbr1) * sig_b
+        sig_kn = sig_b - comp_b + comp_b / ratio
+        rayl = sig_b * (1 / cohe - 1)
+        want = {"gmfp": 1 / sig_kn, "gbr1": gbr1 * sig_b / sig_kn,
+                "gbr2": (gbr1 * sig_b + comp_b / ratio) / sig_kn,
+                "cohe": sig_kn / (rayl + sig_kn)}
+        for f, tol in (("gmfp", 1e-4), ("gbr2", 1e-4), ("cohe", 2e-5)):
+            got = pwlf(b, m, f, x, suffix="_kn")
+            assert np.all(np.abs(got - want[f]) <= tol * want[f]), (name, f)
+        got = pwlf(b, m, "gbr1", x, suffix="_kn")
+        assert np.all(np.abs(got - want["gbr1"]) <= 1e-6 + 2e-4 * want["gbr1"]), (name, "gbr1")
+        # and they differ from the bound rows where binding matters
+        lo = x < np.log(0.05)
+        assert np.all(pwlf(b, m, "gmfp", x[lo], suffix="_kn") < pwlf(b, m, "gmfp", x[lo]))
+
+
 def test_water_attenuation_matches_nist_xcom():
     """Total attenuation (with coherent scattering) of water against NIST
     XCOM (Berger et al., XCOM: Photon Cross Sections Database): 0.8096,
