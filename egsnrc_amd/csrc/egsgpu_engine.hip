@@ -1161,10 +1161,6 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     if (tp->rayleigh && !b.has(Blob::mk(0, "ray_x"))) {
         set_error("table blob has no Rayleigh data"); return fail(EGSGPU_E_FORMAT);
     }
-    if (tp->bound_compton != 0 && tp->bound_compton != 3) {
-        set_error("bound Compton option %d not supported (0 or 3)", tp->bound_compton);
-        return fail(EGSGPU_E_UNSUPPORTED);
-    }
     // media map
     std::vector<uint8_t> med(h->nvox, 0);
     for (int i = 0; i < h->nvox; i++) {
