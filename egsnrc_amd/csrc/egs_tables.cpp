@@ -416,11 +416,18 @@ int tables_build(const std::string &hh, const std::string &pegs_file,
 
         // ---- binding energies from the XCOM photo edges
         // (egs_init_user_photon :8041-8062, eadl_relax = true)
+        // the photon data set "Photon cross sections" names
+        // (egs_init_user_photon :8031-8035): <prefix>_photo.data, ...
         ElementFile photo, rayl, pair, trip;
-        read_element_file(data + "xcom_photo.data", photo);
-        read_element_file(data + "xcom_rayleigh.data", rayl);
-        read_element_file(data + "xcom_pair.data", pair);
-        read_element_file(data + "xcom_triplet.data", trip);
+        read_element_file(data + o.photon_xsec + "_photo.data", photo);
+        read_element_file(data + o.photon_xsec + "_rayleigh.data", rayl);
+        read_element_file(data + o.photon_xsec + "_pair.data", pair);
+        read_element_file(data + o.photon_xsec + "_triplet.data", trip);
+        {
+            std::vector<int32_t> name;
+            for (char c : o.photon_xsec) name.push_back(c);
+            out.i["photon_xsec"] = name;
+        }
         std::vector<double> bind(101 * kMXSHXSEC, 0.0);   // [iz][k], k = 0..29
         for (int iz = 1; iz <= 100; iz++) {
             const auto &et = photo.le[iz];

@@ -51,6 +51,7 @@ struct BuildOptions {
     int rayleigh = 1;         // IRAYLR
     int ibr_nist = 0;         // 0 Bethe-Heitler, 1 NIST brems cross sections
     int pair_nrc = 0;         // 1: add the NRC pair sampling tables
+    std::string photon_xsec = "xcom";   // photon data set: <prefix>_photo.data, ...
 };
 
 int tables_build(const std::string &hen_house, const std::string &pegs_file,

@@ -1082,6 +1082,17 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
                       "table blob lacks; rebuild it", (int)tp->bound_compton);
             return EGSGPU_E_ARG;
         }
+        const std::string want(tp->photon_xsec, strnlen(tp->photon_xsec, sizeof tp->photon_xsec));
+        std::string have = "xcom";
+        if (tab->blob.has("photon_xsec")) {
+            have.clear();
+            for (int c : tab->blob.I("photon_xsec")) have.push_back((char)c);
+        }
+        if (!want.empty() && want != have) {
+            set_error("photon_xsec = %s but the table blob was built from the %s photon data",
+                      want.c_str(), have.c_str());
+            return EGSGPU_E_ARG;
+        }
         const int blob_nrcp = tab->blob.has("pair_nrc") ? tab->blob.i1("pair_nrc") : 0;
         if (tp->pair_xsec != 0 && (tp->pair_xsec != 1 || !blob_nrcp)) {
             set_error("pair_xsec = %d needs a table blob built with the NRC pair tables "

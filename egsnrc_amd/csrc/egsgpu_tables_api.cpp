@@ -1,6 +1,7 @@
 // egsgpu_tables_api.cpp -- C ABI for the cross-section table blob.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -34,6 +35,12 @@ int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
         o.ibr_nist = tp->brems_xsec;
         EGS_ARG(tp->pair_xsec == 0 || tp->pair_xsec == 1, "pair_xsec: 0 Bethe-Heitler or 1 NRC");
         o.pair_nrc = tp->pair_xsec;
+        const std::string px(tp->photon_xsec, strnlen(tp->photon_xsec, sizeof tp->photon_xsec));
+        if (!px.empty()) {
+            EGS_ARG(px == "xcom" || px == "epdl" || px == "si",
+                    "photon_xsec: xcom, epdl or si");
+            o.photon_xsec = px;
+        }
     }
     Blob b;
     int rc = tables_build(hen_house, pegs_file, names, o, b);

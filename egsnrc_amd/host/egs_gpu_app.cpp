@@ -31,6 +31,7 @@
 // Transport runs through the C ABI of include/egsgpu.h.
 //
 // usage: egs_gpu_app -i input.egsinp -p tables.egstab [-o output] [--parse-only]
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -502,6 +503,16 @@ class EGS_GPUApplication {
                {0, 1, 1, 0, 1, 0, 1}, tp.atomic_relaxation);
         choose("Bound Compton scattering", {"Off", "On", "Simple", "norej"}, {0, 1, 2, 3},
                tp.bound_compton);
+        // the photon data set; the table blob given with -p must have been
+        // built from it (egsgpu_create checks)
+        if (!t->get("Photon cross sections", v)) {
+            std::string px;
+            for (char c : v) px.push_back((char)std::tolower((unsigned char)c));
+            if (px != "xcom" && px != "epdl" && px != "si")
+                throw std::runtime_error("Photon cross sections = " + v + " is not supported "
+                                         "(xcom, epdl or si)");
+            std::snprintf(tp.photon_xsec, sizeof tp.photon_xsec, "%s", px.c_str());
+        }
         // egs_advanced_application.cpp:552-612: the step algorithm takes
         // EGSnrc / PRESTA-I / PRESTA-II / default (all but PRESTA-I mean
         // PRESTA-II), boundary crossing Exact / PRESTA-I
@@ -772,11 +783,12 @@ class EGS_GPUApplication {
         printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
                "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d "
                "n_split %d nbr_split %d play_rr %d ireject %d esave %.6g exact_bca %d "
-               "step_algorithm %d brems_xsec %d pair_xsec %d\n",
+               "step_algorithm %d brems_xsec %d pair_xsec %d photon_xsec %s\n",
                tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
                tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth,
                tp.howfarless, tp.n_split, tp.nbr_split, tp.play_rr, tp.ireject, tp.esave,
-               tp.exact_bca, tp.step_algorithm, tp.brems_xsec, tp.pair_xsec);
+               tp.exact_bca, tp.step_algorithm, tp.brems_xsec, tp.pair_xsec,
+               tp.photon_xsec[0] ? tp.photon_xsec : "default");
     }
 
     void init_run() {

@@ -188,6 +188,13 @@ typedef struct EgsTransport {
                                     -4620); egsgpu_tables_build with 1 adds
                                     the tables, which egsgpu_create then
                                     requires                              */
+    char photon_xsec[16];        /* "Photon cross sections": the data set
+                                    prefix xcom, epdl or si (the files
+                                    <prefix>_photo/_pair/_triplet/_rayleigh
+                                    .data, egsnrc.mortran:8031-8035); empty
+                                    = xcom for egsgpu_tables_build and "as
+                                    the blob was built" for egsgpu_create,
+                                    which refuses a blob of another set  */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */

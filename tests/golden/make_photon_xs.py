@@ -10,7 +10,8 @@ Run in a checkout that has the EGSnrc tree (default /root/reference):
 Definitions (the state egs_init_user_photon leaves behind,
 src/egsnrc.mortran:7973-8284, with IRAYLR = 1 and IBCMP = 3):
   * per element, XCOM photo-absorption, Rayleigh, pair and triplet cross
-    sections (data/xcom_*.data, barn/atom), interpolated linearly in
+    sections (data/<prefix>_*.data, barn/atom; prefix xcom unless given),
+    interpolated linearly in
     (ln E, ln sigma), the pair (triplet) data after dividing by
     (1 - 2m/E)^3 ((1 - 4m/E)^3) (egsi_get_data, :8776-8838);
   * Compton: the Klein-Nishina total cross section per electron (Heitler's
@@ -31,8 +32,14 @@ import os
 import sys
 
 HH = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/HEN_HOUSE"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "photon_xs_521icru.json")
-MEDIA = ["H2O521ICRU", "LUNG521ICRU", "ICRPBONE521ICRU", "AIR521ICRU", "AL521ICRU"]
+# the photon data set ("Photon cross sections": xcom, epdl or si); the
+# xcom fixture covers every medium of the 521icru blob, the others water
+PREFIX = sys.argv[2] if len(sys.argv) > 2 else "xcom"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                   "photon_xs_521icru.json" if PREFIX == "xcom"
+                   else "photon_xs_521icru_%s.json" % PREFIX)
+MEDIA = (["H2O521ICRU", "LUNG521ICRU", "ICRPBONE521ICRU", "AIR521ICRU", "AL521ICRU"]
+         if PREFIX == "xcom" else ["H2O521ICRU"])
 RM = 0.5109989461
 R0 = 2.8179403262e-13          # classical electron radius (cm)
 NA = 6.02214076e23             # Avogadro's number (1/mol)
@@ -111,10 +118,10 @@ def pegs_media(path):
 
 def main():
     data = os.path.join(HH, "data")
-    photo = element_tables(os.path.join(data, "xcom_photo.data"))
-    rayl = element_tables(os.path.join(data, "xcom_rayleigh.data"))
-    pair = element_tables(os.path.join(data, "xcom_pair.data"))
-    trip = element_tables(os.path.join(data, "xcom_triplet.data"))
+    photo = element_tables(os.path.join(data, PREFIX + "_photo.data"))
+    rayl = element_tables(os.path.join(data, PREFIX + "_rayleigh.data"))
+    pair = element_tables(os.path.join(data, PREFIX + "_pair.data"))
+    trip = element_tables(os.path.join(data, PREFIX + "_triplet.data"))
     cs = numbers(os.path.join(data, "compton_sigma.data"))
     cs_emin, cs_emax, cs_n = cs[0], cs[1], int(cs[2])
     cs_el = {z: cs[3 + (z - 1) * cs_n: 3 + z * cs_n] for z in range(1, 101)}

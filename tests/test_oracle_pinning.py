@@ -80,6 +80,40 @@ def test_photon_tables_match_an_independent_xcom_sum():
         assert np.all(np.abs(got - exp) <= 1e-6 + 2e-4 * exp), (name, "gbr1")
 
 
+def test_epdl_photon_tables_match_an_independent_sum():
+    """"Photon cross sections = epdl": the water rows of a blob built from
+    the EPDL files against the same independent sum over those files
+    (tests/golden/photon_xs_521icru_epdl.json, make_photon_xs.py with the
+    epdl prefix); they differ from the XCOM rows by more than the
+    tolerance somewhere, so the test tells the data sets apart."""
+    import ctypes as C
+    from egsnrc_amd import _lib
+    from egsnrc_amd.engine import Tables, make_source
+    path = os.path.join(ROOT, "egsnrc_amd", "data", "521icru_epdl.egstab")
+    b = read_blob(path)
+    assert "".join(chr(c) for c in b["photon_xsec"]) == "epdl"
+    fx = json.load(open(os.path.join(GOLDEN, "photon_xs_521icru_epdl.json")))
+    rows = fx["media"]["H2O521ICRU"]["rows"]
+    x = np.log([r["e"] for r in rows])
+    for f, tol in (("gmfp", 1e-4), ("gbr2", 1e-4), ("cohe", 2e-5)):
+        exp = np.array([r[f] for r in rows])
+        assert np.all(np.abs(pwlf(b, 0, f, x) - exp) <= tol * exp), f
+    bx = blob()
+    assert np.max(np.abs(pwlf(b, 0, "gmfp", x) / pwlf(bx, WATER, "gmfp", x) - 1)) > 1e-3
+    # the engine refuses to run another data set than the blob's
+    L = _lib.lib()
+    xb = np.array([0.0, 1.0])
+    g, keep = _lib.make_geometry(xb, xb.copy(), xb.copy(), np.zeros(1, np.int32), None)
+    src = make_source(0, 1.0)
+    tab = Tables(path)
+    tp = _lib.default_transport(photon_xsec=b"xcom")
+    h = C.c_void_p()
+    assert L.egsgpu_create(tab.h, C.byref(g), C.byref(src[0]), C.byref(tp), C.byref(h)) == -1
+    buf = C.create_string_buffer(512)
+    L.egsgpu_last_error(buf, 512)
+    assert b"epdl" in buf.value
+
+
 def test_klein_nishina_photon_rows():
     """The rows a run with bound Compton off uses (egs_init_user_photon
     takes the Klein-Nishina Compton cross section for IBCMP <= 1) follow from
