@@ -1321,9 +1321,10 @@ struct Shower {
     }
 
     // ------------------------------------------------------------ ELECTR
-    // egsnrc.mortran:985-1823: PRESTA-II condensed history with exact
-    // boundary crossing (single elastic scattering within skindepth of a
-    // boundary), spin effects off, EII off.
+    // egsnrc.mortran:985-1823: PRESTA-II or PRESTA-I condensed history with
+    // exact boundary crossing (single elastic scattering within skindepth of
+    // a boundary) or PRESTA-I boundary crossing (approximate steps with the
+    // path-length correction), spin effects off, EII off.
     __device__ __forceinline__ int lel(const MedScalars<R> &M, R elke) const {
         int L = (int)(M.eke1 * elke + M.eke0);
         return L < 1 ? 1 : (L > M.meke ? M.meke : L);

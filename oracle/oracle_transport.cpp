@@ -853,9 +853,9 @@ struct Shower {
     }
 
     // ------------------------------------------------------------ ELECTR
-    // egsnrc.mortran:985-1823 with PRESTA-II, exact boundary crossing
-    // (skin depth in elastic MFPs), spin effects off, EII off, and the
-    // egs++ C-interface callbacks.
+    // egsnrc.mortran:985-1823 with PRESTA-II or PRESTA-I steps, exact or
+    // PRESTA-I boundary crossing (skin depth in elastic MFPs), spin effects
+    // off, EII off, and the egs++ C-interface callbacks.
     int lel(const Medium &M, double elke) const {
         int L = (int)(M.eke1 * elke + M.eke0);
         return L;
