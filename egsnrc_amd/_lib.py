@@ -47,7 +47,8 @@ class EgsTransport(C.Structure):
                 ("n_split", C.c_int32), ("nbr_split", C.c_int32), ("play_rr", C.c_int32),
                 ("ireject", C.c_int32), ("esave", C.c_double),
                 ("step_algorithm", C.c_int32), ("brems_xsec", C.c_int32),
-                ("pair_xsec", C.c_int32), ("photon_xsec", C.c_char * 16)]
+                ("pair_xsec", C.c_int32), ("photon_xsec", C.c_char * 16),
+                ("eii_xsec", C.c_char * 16)]
 
 
 def default_transport(**kw):

@@ -1017,6 +1017,14 @@ int tables_build(const std::string &hh, const std::string &pegs_file,
         if (o.ibr_nist >= 1) nist_brems_tables(data, o.ibr_nist, out);
         out.i["pair_nrc"] = {0};
         if (o.pair_nrc) nrc_pair_tables(data, out);
+        // eii_init follows them (HATCH :2294) and corrects their tables
+        out.i["eii"] = {0};
+        if (!o.eii.empty()) {
+            eii_tables(data, o.eii, out);
+            std::vector<int32_t> name;
+            for (char c : o.eii) name.push_back(c);
+            out.i["eii_name"] = name;
+        }
     } catch (const std::exception &e) {
         set_error("table build failed: %s", e.what());
         return EGSGPU_E_FORMAT;

@@ -52,6 +52,7 @@ struct BuildOptions {
     int ibr_nist = 0;         // 0 Bethe-Heitler, 1 NIST brems cross sections
     int pair_nrc = 0;         // 1: add the NRC pair sampling tables
     std::string photon_xsec = "xcom";   // photon data set: <prefix>_photo.data, ...
+    std::string eii;          // "" off, else the EII data set (eii_<name>.data)
 };
 
 int tables_build(const std::string &hen_house, const std::string &pegs_file,
@@ -107,6 +108,15 @@ constexpr int kNrcpX = 65;      // energy-fraction points ($NRC-PAIR-NXX)
 // prepare_alias_table (egs_utilities.mortran:1846-1899) for the linear
 // density through (xs[i], fs[i]), i = 0..n: ws[n] and 1-based ibin[n].
 void alias_table(int n, const double *xs, const double *fs, double *ws, int32_t *ibin);
+
+// Electron impact ionisation (eii_init, egsnrc.mortran:7434-7758): the
+// shell cross sections of data_dir/eii_<name>.data for the K..LIII shells
+// above the problem's lowest threshold ("eii_*" keys) and the corrected
+// esig, ebr1, ededx, esig_e and sig_monotone of the media with such shells;
+// sets "eii" (0 when no shell qualifies).
+int eii_tables(const std::string &data_dir, const std::string &name, Blob &out);
+constexpr int kEiiBins = 250;        // $N_EII_BINS
+constexpr int kEiiMaxShells = 40;    // $MAX_EII_SHELLS
 
 constexpr int kNbE = 100;       // energies of the brems sampling grid ($MXBRES)
 constexpr int kNbX = 50;        // bins per alias table ($MXBRXS)

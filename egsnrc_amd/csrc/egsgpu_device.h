@@ -49,8 +49,9 @@ struct MedScalars {
     R delcm, bpar0, bpar1, zbrang, lzbrang, log_ap;
     R blcc, xcc, expeke1, esig_e, psig_e;
     R nb_emin, nb_lemin, nb_dlei;     // NIST brems sampling grid (ibr_nist = 1)
+    R eii_cons;                       // rho N_A / sum(PZ A) for the EII shells
     int nne, n_shell, meke, sig_mono_e, sig_mono_p;
-    int pad;
+    int eii_nsh;                      // EII shells of the medium (0: none)
 };
 
 // NIST brems sampling grid: energies per medium and bins per alias table
@@ -59,6 +60,8 @@ constexpr int kNbGridE = 100, kNbGridX = 50;
 // NRC pair grid: photon energies and positron energy-fraction points
 // ($NRC-PAIR-NEE, $NRC-PAIR-NXX)
 constexpr int kNrcpGridE = 84, kNrcpGridX = 65;
+// electron impact ionisation: bins per shell ($N_EII_BINS)
+constexpr int kEiiGridBins = 250;
 
 template <typename R>
 struct DevTables {
@@ -87,6 +90,12 @@ struct DevTables {
     const R *nrcp_x, *nrcp_f, *nrcp_w;
     const int *nrcp_i;
     R nrcp_emin, nrcp_emax, nrcp_dlei;
+    // electron impact ionisation: per (medium, element) [m*kMaxNE + e] the
+    // number of EII shells and the first (1-based) in the global list; per
+    // global shell the log-energy grid (a, b) and 250 bins of the cross
+    // section [shell*kEiiGridBins + bin] (barn/atom)
+    const int *eii_no, *eii_first;
+    const R *eii_a, *eii_b, *eii_xa, *eii_xb;
     const int *cs_iz, *cs_shn;   // element and shell number of Compton shell j
     // elements [m*kMaxNE + k]
     const R *z, *pz;
@@ -140,6 +149,7 @@ struct DevParams {
     int presta1;              // transport_algorithm PRESTA-I (msdist_pI)
     int ibr_nist;             // 1: NIST brems photon energies (alias tables)
     int pair_nrc;             // 1: NRC pair energy sharing (alias tables)
+    int eii;                  // 1: electron impact ionisation in MOLLER
     double ecut, pcut, skindepth;
     double esave;             // total energy (MeV) below which range rejection applies
 };

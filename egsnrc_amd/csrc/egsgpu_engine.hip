@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cctype>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -309,6 +310,9 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
         s.nb_emin = nist ? f("nb_emin") : (R)0;
         s.nb_lemin = nist ? f("nb_lemin") : (R)0;
         s.nb_dlei = nist ? f("nb_dlei") : (R)0;
+        const bool eii = b.has("eii") && b.i1("eii") == 1;
+        s.eii_nsh = eii ? b.i1(Blob::mk(m, "eii_nsh")) : 0;
+        s.eii_cons = eii ? f("eii_cons") : (R)0;
         s.nne = b.i1(Blob::mk(m, "nne"));
         s.n_shell = b.i1(Blob::mk(m, "n_shell"));
         s.meke = b.i1(Blob::mk(m, "meke"));
@@ -404,6 +408,18 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
     } else {
         T.nb_x = T.nb_f = T.nb_w = nullptr;
         T.nb_i = nullptr;
+    }
+    static_assert(kEiiGridBins == kEiiBins, "EII grid");
+    if (b.has("eii") && b.i1("eii") == 1) {
+        CK(pack_med_i(h, b, nmed, "eii_no", kMaxNE, &T.eii_no));
+        CK(pack_med_i(h, b, nmed, "eii_first", kMaxNE, &T.eii_first));
+        CK(pack_glob<R>(h, b, "eii_a", &T.eii_a));
+        CK(pack_glob<R>(h, b, "eii_b", &T.eii_b));
+        CK(pack_glob<R>(h, b, "eii_xa", &T.eii_xa));
+        CK(pack_glob<R>(h, b, "eii_xb", &T.eii_xb));
+    } else {
+        T.eii_no = T.eii_first = nullptr;
+        T.eii_a = T.eii_b = T.eii_xa = T.eii_xb = nullptr;
     }
     static_assert(kNrcpGridE == kNrcpE && kNrcpGridX == kNrcpX, "NRC pair grid");
     if (b.has("pair_nrc") && b.i1("pair_nrc") == 1) {
@@ -1093,6 +1109,24 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
                       want.c_str(), have.c_str());
             return EGSGPU_E_ARG;
         }
+        // EII: the blob's electron cross sections include it or not, so the
+        // run must ask for what the blob was built with
+        std::string eii_want(tp->eii_xsec, strnlen(tp->eii_xsec, sizeof tp->eii_xsec));
+        for (auto &c : eii_want) c = (char)std::tolower((unsigned char)c);
+        if (eii_want == "on") eii_want = "ik";
+        if (eii_want == "off") eii_want.clear();
+        std::string eii_have;
+        if (tab->blob.has("eii_name"))
+            for (int c : tab->blob.I("eii_name")) eii_have.push_back((char)c);
+        if (eii_want != eii_have) {
+            set_error("eii_xsec = '%s' but the table blob was built with %s%s", eii_want.c_str(),
+                      eii_have.empty() ? "EII off" : "EII data ", eii_have.c_str());
+            return EGSGPU_E_ARG;
+        }
+        if (!eii_want.empty() && !tp->atomic_relaxation) {
+            set_error("electron impact ionisation needs atomic relaxation (eii_init)");
+            return EGSGPU_E_ARG;
+        }
         const int blob_nrcp = tab->blob.has("pair_nrc") ? tab->blob.i1("pair_nrc") : 0;
         if (tp->pair_xsec != 0 && (tp->pair_xsec != 1 || !blob_nrcp)) {
             set_error("pair_xsec = %d needs a table blob built with the NRC pair tables "
@@ -1226,6 +1260,7 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
     h->P.presta1 = tp->step_algorithm == 1;
     h->P.ibr_nist = tp->brems_xsec;
     h->P.pair_nrc = tp->pair_xsec;
+    h->P.eii = b.has("eii") && b.i1("eii") == 1 ? 1 : 0;
     // mscati (egsnrc.mortran:2790-2817): skindepth_for_bca <= 1e-4 selects
     // 3 elastic mean free paths with exact boundary crossing and
     // exp(BLCMIN) at the lowest ECUT with PRESTA-I, the minimum taken over

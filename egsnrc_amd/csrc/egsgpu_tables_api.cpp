@@ -1,4 +1,5 @@
 // egsgpu_tables_api.cpp -- C ABI for the cross-section table blob.
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -35,6 +36,14 @@ int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
         o.ibr_nist = tp->brems_xsec;
         EGS_ARG(tp->pair_xsec == 0 || tp->pair_xsec == 1, "pair_xsec: 0 Bethe-Heitler or 1 NRC");
         o.pair_nrc = tp->pair_xsec;
+        std::string eii(tp->eii_xsec, strnlen(tp->eii_xsec, sizeof tp->eii_xsec));
+        for (auto &c : eii) c = (char)std::tolower((unsigned char)c);
+        if (eii == "on") eii = "ik";   // setEIIData, egs_advanced_application.cpp:841-859
+        if (eii == "off") eii.clear();
+        EGS_ARG(eii.empty() || eii == "ik" || eii == "casnati" || eii == "kolbenstvedt" ||
+                    eii == "gryzinski" || eii == "penelope",
+                "eii_xsec: off, on, ik, casnati, kolbenstvedt, gryzinski or penelope");
+        o.eii = eii;
         const std::string px(tp->photon_xsec, strnlen(tp->photon_xsec, sizeof tp->photon_xsec));
         if (!px.empty()) {
             EGS_ARG(px == "xcom" || px == "epdl" || px == "si",

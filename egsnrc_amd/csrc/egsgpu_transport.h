@@ -1865,7 +1865,7 @@ struct Shower {
         const MedScalars<R> &M = T.ms[m];
         const int npold = np;
         if (eph == kBrems) brems(m, M, e_elke);
-        else if (eph == kMoller) moller(M);
+        else if (eph == kMoller) moller(m, M, e_elke);
         else if (eph == kBhabha) bhabha(M);
         else annih();
         if constexpr (SPLIT) {   // ausgab iarg 7 and 13 (dosxyznrc.mortran:3531-3545)
@@ -2253,9 +2253,85 @@ struct Shower {
     }
 
     // MOLLER (egsnrc.mortran:2599-2746), EII off
-    __device__ void moller(const MedScalars<R> &M) {
+    // eii_sample (egsnrc.mortran:7760-7830): ionisation of shell ish
+    // (binding Uj) of element iZ; the energy transfer from the hard (Moller
+    // with the binding) or soft (Bethe-like) part by rejection, the
+    // secondary on the stack above AE (else absorbed here), then the
+    // vacancy's EADL cascade
+    __device__ __forceinline__ void eii_sample(const MedScalars<R> &M, int ish, int iZ, R Uj) {
+        const R RM = (R)0.5109989461, RMT2 = 2 * RM;
+        const R peie = p.e, Tk = peie - RM, tau = Tk / RM, tau1 = tau + 1;
+        const R tau12 = tau1 * tau1, p2 = tau * tau + 2 * tau, beta2 = p2 / tau12;
+        const R Wmax = (R)0.5 * (Tk + Uj), xmax = Uj / Wmax;
+        const R c1 = (Wmax / peie) * (Wmax / peie), c2 = (2 * tau + 1) / tau12;
+        const R fm_s = dlog(RMT2 * p2 / Uj) - beta2 - (R)0.5;
+        const R prob_s = (R)0.66666667 * fm_s * (1 + xmax + xmax * xmax);
+        R fm_h = 2 + c1 - c2;
+        if (fm_h < 1) fm_h = 1;
+        const R prob = fm_h + prob_s;
+        R wx, frej, r3;
+        do {
+            R r1 = rnd(), r2 = rnd();
+            r3 = rnd();
+            if (r1 * prob < fm_h) {   // hard collisions
+                wx = 1 / (r2 * xmax + 1 - r2);
+                R wxx = wx * xmax, aux = wxx / (2 - wxx);
+                frej = (1 + aux * (aux - c2) + c1 * wxx * wxx) / fm_h;
+            } else {                  // soft collisions
+                wx = 1 / pow(r2 * xmax * xmax * xmax + 1 - r2, (R)0.333333333);
+                frej = 1 - dlog(wx) / fm_s;
+            }
+        } while (!(r3 < frej));
+        wx = wx * Uj;
+        const R h1 = (peie + RM) / Tk, pese1 = peie - wx;
+        p.e = pese1;
+        R dcosth = h1 * (pese1 - RM) / (pese1 + RM);
+        uphi21(dsqrt(dcosth), dsqrt(1 - dcosth));
+        const R pese2 = wx - Uj + RM;
+        R edep = 0;
+        if (pese2 > M.ae) {
+            if (!push_copy()) return;
+            p.e = pese2;
+            p.iq = -1;
+            dcosth = h1 * (pese2 - RM) / (pese2 + RM);
+            uphi32(dsqrt(dcosth), -dsqrt(1 - dcosth));
+        } else {
+            edep = wx - Uj;
+        }
+        edep += eadl_relax(iZ, ish);
+        if (edep > 0) score(p.ir, edep, p.wt);
+    }
+
+    // MOLLER (egsnrc.mortran:2599-2746), with the EII shell branch first
+    // (:2634-2664) when the tables were built with EII
+    __device__ void moller(int m, const MedScalars<R> &M, R elke) {
         const R RM = (R)0.5109989461;
         R peie = p.e, pekin = peie - RM, ekin = pekin;
+        if (P.eii && M.eii_nsh > 0) {
+            const int L = lel(M, elke);
+            R rsh = ev(kESig, m, L, elke) * (1 - ev(kEBr1, m, L, elke)) * rnd();
+            for (int el = 0; el < M.nne; el++) {
+                const int nsh = T.eii_no[m * kMaxNE + el];
+                if (nsh <= 0) continue;
+                const int iZ = (int)(T.z[m * kMaxNE + el] + (R)0.5);
+                const int ifirst = T.eii_first[m * kMaxNE + el];
+                for (int ish = 1; ish <= nsh; ish++) {
+                    const R Uj = T.bind[iZ * 30 + ish - 1];
+                    if (!(ekin > Uj && (Uj > M.te || Uj > M.ap))) continue;
+                    const int jj = ifirst + ish - 1;
+                    int i = (int)(T.eii_a[jj - 1] * elke + T.eii_b[jj - 1]);
+                    i = i < 1 ? 1 : (i > kEiiGridBins ? kEiiGridBins : i);
+                    const size_t k = (size_t)(jj - 1) * kEiiGridBins + i - 1;
+                    R sig_j = (T.eii_xa[k] * elke + T.eii_xb[k]) * T.pz[m * kMaxNE + el] *
+                              M.eii_cons;
+                    rsh -= sig_j;
+                    if (rsh < 0) {
+                        eii_sample(M, ish, iZ, Uj);
+                        return;
+                    }
+                }
+            }
+        }
         if (ekin <= 2 * M.te) return;
         R t0 = ekin / RM, e0 = t0 + 1;
         R extrae = peie - M.thmoll;

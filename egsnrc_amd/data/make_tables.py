@@ -14,7 +14,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from egsnrc_amd import _lib  # noqa: E402
 
 BLOBS = {
-    # name: (pegs4 file, media[, brems_xsec[, photon data set]])
+    # name: (pegs4 file, media[, brems_xsec[, photon data set[, EII data set]]])
     "521icru.egstab": ("521icru.pegs4dat",
                        ["H2O521ICRU", "LUNG521ICRU", "ICRPBONE521ICRU",
                         "AIR521ICRU", "AL521ICRU"]),
@@ -28,11 +28,13 @@ BLOBS = {
     "521icru_nist.egstab": ("521icru.pegs4dat", ["H2O521ICRU", "ICRPBONE521ICRU"], 1),
     # water from the EPDL photon data set ("Photon cross sections = epdl")
     "521icru_epdl.egstab": ("521icru.pegs4dat", ["H2O521ICRU"], 0, "epdl"),
+    # the tutorial media with electron impact ionisation (the ik data set)
+    "tutor_eii.egstab": ("tutor_data.pegs4dat", ["TA", "PB", "H2O"], 0, "xcom", "ik"),
 }
 
 
 def build(hen_house, name, pegs, media, out_dir=HERE, estepe=0.25, ximax=0.5, brems_xsec=0,
-          pair_xsec=None, photon_xsec="xcom"):
+          pair_xsec=None, photon_xsec="xcom", eii=""):
     L = _lib.lib()
     L.egsgpu_tables_build.argtypes = [C.c_char_p, C.c_char_p, C.c_int32,
                                       C.POINTER(C.c_char_p), C.c_void_p,
@@ -41,7 +43,8 @@ def build(hen_house, name, pegs, media, out_dir=HERE, estepe=0.25, ximax=0.5, br
     if pair_xsec is None:
         pair_xsec = brems_xsec   # the NIST blob carries the NRC pair tables too
     tp = _lib.default_transport(estepe=estepe, ximax=ximax, brems_xsec=brems_xsec,
-                                pair_xsec=pair_xsec, photon_xsec=photon_xsec.encode())
+                                pair_xsec=pair_xsec, photon_xsec=photon_xsec.encode(),
+                                eii_xsec=eii.encode())
     out = os.path.join(out_dir, name)
     _lib.check(L.egsgpu_tables_build(
         hen_house.encode(), os.path.join(hen_house, "pegs4", "data", pegs).encode(),
@@ -58,8 +61,9 @@ def main():
         pegs, media = spec[0], spec[1]
         nist = spec[2] if len(spec) > 2 else 0
         px = spec[3] if len(spec) > 3 else "xcom"
+        eii = spec[4] if len(spec) > 4 else ""
         print("wrote", build(a.hen_house, name, pegs, media, a.out_dir, brems_xsec=nist,
-                             photon_xsec=px))
+                             photon_xsec=px, eii=eii))
 
 
 if __name__ == "__main__":

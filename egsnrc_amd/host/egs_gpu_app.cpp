@@ -503,6 +503,11 @@ class EGS_GPUApplication {
                {0, 1, 1, 0, 1, 0, 1}, tp.atomic_relaxation);
         choose("Bound Compton scattering", {"Off", "On", "Simple", "norej"}, {0, 1, 2, 3},
                tp.bound_compton);
+        // electron impact ionisation: Off, On (the ik data) or a data set
+        // name (setEIIData, egs_advanced_application.cpp:841-859); the blob
+        // must have been built with it (egsgpu_create checks)
+        if (!t->get("Electron impact ionization", v))
+            std::snprintf(tp.eii_xsec, sizeof tp.eii_xsec, "%s", v.c_str());
         // the photon data set; the table blob given with -p must have been
         // built from it (egsgpu_create checks)
         if (!t->get("Photon cross sections", v)) {
@@ -783,12 +788,13 @@ class EGS_GPUApplication {
         printf("transport: rayleigh %d bound_compton %d pe_angle %d relax %d pair_angle %d "
                "brems_angle %d spin %d estepe %.6g ximax %.6g skin %.6g howfarless %d "
                "n_split %d nbr_split %d play_rr %d ireject %d esave %.6g exact_bca %d "
-               "step_algorithm %d brems_xsec %d pair_xsec %d photon_xsec %s\n",
+               "step_algorithm %d brems_xsec %d pair_xsec %d photon_xsec %s eii %s\n",
                tp.rayleigh, tp.bound_compton, tp.photoelectron_angle, tp.atomic_relaxation,
                tp.pair_angle, tp.brems_angle, tp.spin, tp.estepe, tp.ximax, tp.skin_depth,
                tp.howfarless, tp.n_split, tp.nbr_split, tp.play_rr, tp.ireject, tp.esave,
                tp.exact_bca, tp.step_algorithm, tp.brems_xsec, tp.pair_xsec,
-               tp.photon_xsec[0] ? tp.photon_xsec : "default");
+               tp.photon_xsec[0] ? tp.photon_xsec : "default",
+               tp.eii_xsec[0] ? tp.eii_xsec : "off");
     }
 
     void init_run() {

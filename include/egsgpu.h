@@ -195,6 +195,16 @@ typedef struct EgsTransport {
                                     = xcom for egsgpu_tables_build and "as
                                     the blob was built" for egsgpu_create,
                                     which refuses a blob of another set  */
+    char eii_xsec[16];           /* "Electron impact ionization": empty or
+                                    "off", "on" (= ik) or a data set (ik,
+                                    casnati, kolbenstvedt, gryzinski,
+                                    penelope: eii_<name>.data; eii_init,
+                                    egsnrc.mortran:7434-7758, eii_sample
+                                    :7760); egsgpu_tables_build makes the
+                                    tables (they change the electron cross
+                                    sections), egsgpu_create requires a blob
+                                    built with the same set; needs atomic
+                                    relaxation                           */
 } EgsTransport;
 
 /* ---- tables ----------------------------------------------------------- */

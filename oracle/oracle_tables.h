@@ -99,6 +99,11 @@ struct Medium {
     // w/bin [ie*64 + ix]
     std::vector<double> nrcp_f, nrcp_w;
     std::vector<int32_t> nrcp_i;
+    // electron impact ionisation (eii_init): shells per element and the
+    // first (1-based) in the global list, rho N_A / sum(PZ A)
+    int eii_nsh = 0;
+    std::vector<int32_t> eii_no, eii_first;
+    double eii_cons = 0;
 };
 
 struct Tables {
@@ -122,6 +127,8 @@ struct Tables {
         ibcmp = run_ibcmp;
     }
     int pair_nrc = 0;   // 1: the blob holds the NRC pair tables
+    int eii = 0;        // 1: the blob holds EII shells (and corrected tables)
+    std::vector<double> eii_a, eii_b, eii_xa, eii_xb;   // per shell; 250 bins each
     std::vector<double> nrcp_x;   // positron energy-fraction grid (65)
     double nrcp_emin = 0, nrcp_emax = 0, nrcp_dlei = 0;
     std::vector<Medium> med;
@@ -208,6 +215,18 @@ struct Tables {
         ibr_nist = b.has("ibr_nist") ? b.I("ibr_nist")[0] : 0;
         ibcmp = b.I("ibcmp")[0];
         pair_nrc = b.has("pair_nrc") ? b.I("pair_nrc")[0] : 0;
+        eii = b.has("eii") ? b.I("eii")[0] : 0;
+        if (eii) {
+            eii_a = b.F("eii_a"); eii_b = b.F("eii_b");
+            eii_xa = b.F("eii_xa"); eii_xb = b.F("eii_xb");
+            for (int m = 0; m < nmed; m++) {
+                auto K = [&](const char *f) { return "m" + std::to_string(m) + "." + f; };
+                med[m].eii_nsh = b.I(K("eii_nsh"))[0];
+                med[m].eii_no = b.I(K("eii_no"));
+                med[m].eii_first = b.I(K("eii_first"));
+                med[m].eii_cons = b.F(K("eii_cons"))[0];
+            }
+        }
         if (pair_nrc) {
             nrcp_x = b.F("nrcp_xdata");
             nrcp_emin = b.F("nrcp_const")[0];

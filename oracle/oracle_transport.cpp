@@ -47,6 +47,7 @@ struct Params {
     int presta1;   // transport_algorithm PRESTA-I (msdist_pI)
     int ibr_nist;  // 1: NIST brems photon energies
     int pair_nrc;  // 1: NRC pair energy sharing
+    int eii;       // 1: electron impact ionisation in MOLLER
 };
 
 struct Problem {
@@ -1178,7 +1179,7 @@ struct Shower {
                         if (top().iq == 0) return;
                         goto newelectron;
                     }
-                    moller(*M);
+                    moller(*M, elke);
                     if (top().iq == 0) return;
                     goto newelectron;
                 }
@@ -1482,9 +1483,82 @@ struct Shower {
     }
 
     // MOLLER (egsnrc.mortran:2599-2746), EII off
-    void moller(const Medium &M) {
+    // eii_sample (egsnrc.mortran:7760-7830)
+    void eii_sample(const Medium &M, int ish, int iZ, double Uj) {
+        Particle &p = top();
+        double peie = p.e, Tk = peie - RM, tau = Tk / RM, tau1 = tau + 1;
+        double tau12 = tau1 * tau1, tau2 = tau * tau, p2 = tau2 + 2 * tau;
+        double beta2 = p2 / tau12;
+        double Wmax = 0.5 * (Tk + Uj), xmax = Uj / Wmax;
+        double c1 = std::pow(Wmax / peie, 2), c2 = (2 * tau + 1) / tau12;
+        double fm_s = std::log(RMT2 * p2 / Uj) - beta2 - 0.5;
+        double prob_s = 0.66666667 * fm_s * (1 + xmax + xmax * xmax);
+        double fm_h = 2 + c1 - c2;
+        if (fm_h < 1) fm_h = 1;
+        double prob = fm_h + prob_s;
+        double wx, frej, r3;
+        do {
+            double r1 = rnd(), r2 = rnd();
+            r3 = rnd();
+            if (r1 * prob < fm_h) {
+                wx = 1 / (r2 * xmax + 1 - r2);
+                double wxx = wx * xmax, aux = wxx / (2 - wxx);
+                frej = (1 + aux * (aux - c2) + c1 * wxx * wxx) / fm_h;
+            } else {
+                wx = 1 / std::pow(r2 * xmax * xmax * xmax + 1 - r2, 0.333333333);
+                frej = 1 - std::log(wx) / fm_s;
+            }
+        } while (!(r3 < frej));
+        wx = wx * Uj;
+        double h1 = (peie + PRM) / Tk, pese1 = peie - wx;
+        p.e = pese1;
+        double dcosth = h1 * (pese1 - PRM) / (pese1 + PRM);
+        uphi21(std::sqrt(dcosth), std::sqrt(1 - dcosth));
+        double pese2 = wx - Uj + PRM;
+        double edep = 0;
+        if (pese2 > M.ae) {
+            push_copy();
+            top().e = pese2;
+            top().iq = -1;
+            dcosth = h1 * (pese2 - PRM) / (pese2 + PRM);
+            uphi32(std::sqrt(dcosth), -std::sqrt(1 - dcosth));
+        } else {
+            edep = wx - Uj;
+        }
+        edep += eadl_relax(iZ, ish);
+        if (edep > 0) score(top().ir, edep, top().wt);
+    }
+
+    void moller(const Medium &M, double elke) {
         Particle &p = top();
         double peie = p.e, pekin = peie - PRM, ekin = pekin;
+        if (pb.P.eii && M.eii_nsh > 0) {   // egsnrc.mortran:2634-2664
+            int L = lel(M, elke);
+            double sigm = ev(M.esig0, M.esig1, L, elke) * (1 - ev(M.ebr10, M.ebr11, L, elke));
+            double rsh = rnd();
+            rsh = sigm * rsh;
+            for (int el = 0; el < M.nne; el++) {
+                int iZ = (int)(M.z[el] + 0.5);
+                int nsh = M.eii_no[el];
+                if (nsh <= 0) continue;
+                int ifirst = M.eii_first[el];
+                for (int ish = 1; ish <= nsh; ish++) {
+                    double Uj = T.bind[(size_t)iZ * 30 + ish - 1];
+                    if (!(ekin > Uj && (Uj > M.te || Uj > M.ap))) continue;
+                    int jj = ifirst + ish - 1;
+                    int i = (int)(T.eii_a[jj - 1] * elke + T.eii_b[jj - 1]) + (jj - 1) * 250;
+                    if (i < 1 || i > (int)T.eii_xa.size())
+                        throw std::runtime_error("EII cross-section index out of range");
+                    double sig_j = T.eii_xa[i - 1] * elke + T.eii_xb[i - 1];
+                    sig_j = sig_j * M.pz[el] * M.eii_cons;
+                    rsh = rsh - sig_j;
+                    if (rsh < 0) {
+                        eii_sample(M, ish, iZ, Uj);
+                        return;
+                    }
+                }
+            }
+        }
         if (ekin <= 2 * M.te) return;
         double t0 = ekin / RM, e0 = t0 + 1.0;
         double extrae = peie - M.thmoll;
@@ -2043,6 +2117,7 @@ struct OTransport {
     int32_t kerma, fp64, howfarless, n_split, nbr_split, play_rr, ireject;
     double esave;
     int32_t step_algorithm, brems_xsec, pair_xsec;
+    char photon_xsec[16], eii_xsec[16];
 };
 
 // mscati's default PRESTA-I skin depth (egsnrc.mortran:2790-2817) over the
@@ -2105,6 +2180,7 @@ int oracle_create(const char *blob, const OGeom *g, const OSource *s,
         P.pair_nrc = tp->pair_xsec;
         if (P.pair_nrc && !pb.T.pair_nrc)
             throw std::runtime_error("pair_xsec = 1 needs the blob's NRC pair tables");
+        P.eii = pb.T.eii;   // the blob's electron cross sections include EII or not
         if (P.skindepth <= 1e-4) {   // over the media the voxels use
             std::vector<int> used;
             for (int m = 0; m < pb.T.nmed; m++)
@@ -2246,6 +2322,7 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
         P = Params{ecut, pcut, 1, 3, 1, 1, 1, 1, 0, 1, 0.25, 0.5, 3.0, 0};
         P.ibr_nist = pb.T.ibr_nist;   // the blob's brems cross sections
         P.pair_nrc = pb.T.pair_nrc;   // and pair tables, when it has them
+        P.eii = pb.T.eii;
         if (kind == 0) P.ibcmp = (int)param;           // 0 KN, 1 bound with rejection, 3 norej
         std::vector<double> tmp(1, 0.0);
         std::vector<int> touched;
@@ -2270,7 +2347,7 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
                 } else { o[0] = o[1] = o[2] = o[3] = -9; }
             } else if (kind == 1 || kind == 2) {
                 p.e = energy + PRM; p.iq = kind == 1 ? -1 : 1; sh.stk.push_back(p);
-                if (kind == 1) sh.moller(M); else sh.bhabha(M);
+                if (kind == 1) sh.moller(M, std::log(energy)); else sh.bhabha(M);
                 if (sh.stk.size() == 2) {
                     o[0] = sh.stk[1].e - PRM; o[1] = sh.stk[0].e - PRM;
                     o[2] = kind == 1 ? sh.stk[0].w : sh.stk[1].iq;
@@ -2337,6 +2414,20 @@ extern "C" int oracle_sample(const char *blob, int32_t kind, int32_t medium,
                     if (i == 1) first = q.iq == 0 ? ek : -ek;
                 }
                 o[0] = first; o[1] = nph + 1000.0 * nel + 1e6 * sh.relax_depth; o[2] = ekin; o[3] = edep;
+            } else if (kind == 15) {
+                // eii_sample for shell `param` (1 K .. 4 LIII) of the first
+                // element of the medium, an electron of kinetic energy
+                // `energy`: the energy transfer, the secondary's kinetic
+                // energy (0 if absorbed) and the energy absorbed on the spot
+                int iZ = (int)(M.z[0] + 0.5), ish = (int)param;
+                double Uj = pb.T.bind[(size_t)iZ * 30 + ish - 1];
+                p.e = energy + PRM; p.iq = -1; sh.stk.push_back(p);
+                tmp[0] = 0;
+                sh.eii_sample(M, ish, iZ, Uj);
+                double wx = energy + PRM - sh.stk[0].e;
+                double sec = sh.stk.size() > 1 && sh.stk[1].iq == -1 && std::fabs(sh.stk[1].e - (wx - Uj + PRM)) < 1e-12
+                                 ? sh.stk[1].e - PRM : 0.0;
+                o[0] = wx; o[1] = sec; o[2] = tmp[0]; o[3] = Uj;
             } else if (kind == 14) {
                 // PAIR with pair angular sampling `param` (0..4) along +z:
                 // the two particles' total energies and polar cosines
