@@ -680,11 +680,12 @@ def main():
     ap.add_argument("--batching", default="",
                     help="event kernels: photon steps per turn, interaction batch, max wait, "
                          "defer below")
-    ap.add_argument("--autotune", type=int, default=500000,
+    ap.add_argument("--autotune", type=int, default=-1,
                     help="run the event-schedule autotuner (egsgpu_autotune: register budget, "
                          "queue capacity, workgroup size, deferral threshold, interaction batch, "
                          "photon steps) with this many probe histories before the warm-up, "
-                         "outside the timed region; 0 = library defaults")
+                         "outside the timed region; -1 = one step's histories, 0 = library "
+                         "defaults")
     ap.add_argument("--compare-plain", action="store_true",
                     help="batch256: also run the jobs one at a time with the plain kernel")
     ap.add_argument("--grad-hist", type=int, default=100_000_000,
@@ -778,7 +779,7 @@ def main():
         _lib.check(_lib.lib().egsgpu_set_event_graph(sim.h, 0))
     if a.woodcock is not None:
         sim.set_woodcock(a.woodcock)
-    tuned = sim.autotune(a.autotune) if a.autotune else None
+    tuned = sim.autotune(hps if a.autotune < 0 else a.autotune) if a.autotune else None
     for _ in range(a.warmup):
         sim.run(hps)
     progress("warm-up done; timing %d steps" % a.steps)

@@ -1636,7 +1636,9 @@ int egsgpu_autotune(egsgpu_handle *h, int64_t nhist_probe, int32_t *choice) {
             if ((rc = apply())) return done(rc);
             if (k.slots && (rc = probe(ms))) return done(rc);   // the new slots' first run
             if ((rc = probe(ms))) return done(rc);
-            if (ms < best) { best = ms; pick = o; }
+            // a change must gain more than 1 % to count: the probes' noise
+            // would otherwise pick settings that are no faster
+            if (ms < 0.99 * best) { best = ms; pick = o; }
         }
         *k.v = pick;
         if ((rc = apply())) return done(rc);

@@ -339,7 +339,9 @@ int egsgpu_graph_stats(egsgpu_handle *h, int32_t *launches, int32_t *iterations,
  * stream, the particle banks holding 8 per slot), the workgroup size (128
  * or 256), the deferral threshold (the number of busy lanes below which a
  * wave hands its particles to the next launch), the interaction batch and
- * the photon steps per turn, and keeps the fastest setting.  The probes
+ * the photon steps per turn, and keeps a setting when it is more than 1 %
+ * faster than the best so far (the probes' own noise is below that when
+ * nhist_probe is a production run's size).  The probes
  * advance the RNG streams but leave tallies, history count and counters as
  * they were.  choice (7 ints, may be NULL) receives waves photon, waves
  * electron, defer_below, batch, photon steps, slots per stream, workgroup
