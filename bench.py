@@ -111,7 +111,9 @@ def efficiency(sum_, sum2, n, seconds):
 
 def cpu_baseline(name, seconds_target=15.0):
     """The FP64 oracle on the host cores, run on a bounded sample.  Returns
-    the baseline record and the sample's tallies (sum, sum2, histories)."""
+    the baseline record, the sample's tallies (sum, sum2, histories) and
+    those of a second oracle run of the same size with other seeds (the
+    null for the dose agreement; not timed)."""
     from egsnrc_amd import _lib
     from tests import oracle_lib
     xb, yb, zb, med, rhor, src, tp, cfg = build_workload(name)
@@ -140,7 +142,11 @@ def cpu_baseline(name, seconds_target=15.0):
                    "egs++ needs the configure-generated egs_config1.h and the Mortran "
                    "core mortran3-generated Fortran; 16 threads = one GPU's share of the "
                    "host"}
-    return rec, o.tallies()
+    tallies = o.tallies()
+    o2 = oracle_lib.OracleSimulation(blob, g, src[0], tp, nstreams=4 * cores, nthreads=cores)
+    o2.seed(2718, 31415)
+    o2.run(n + n2)
+    return rec, tallies, o2.tallies()
 
 
 def dose_agreement(gpu, ref, frac=0.1):
@@ -901,8 +907,16 @@ def main():
                                      "SQ_WAIT_ANY / SQ_WAVE_CYCLES"}
     if not a.no_cpu:
         progress("CPU baseline")
-        out["cpu_baseline"], ref = cpu_baseline(name)
-        out["dose_agreement_vs_oracle"] = dose_agreement((s, s2, n), ref)
+        out["cpu_baseline"], ref, ref2 = cpu_baseline(name)
+        agree = dose_agreement((s, s2, n), ref)
+        # the same statistics for two oracle runs: what agreement looks like
+        # for these voxels (their errors are heavy-tailed in the low-dose
+        # regions at a few million histories)
+        null = dose_agreement(ref2, ref)
+        agree["null_oracle_vs_oracle"] = {"chi2_per_dof": null["chi2_per_dof"],
+                                          "beyond_2sigma": null["beyond_2sigma"],
+                                          "voxels": null["voxels"]}
+        out["dose_agreement_vs_oracle"] = agree
     print(json.dumps(out))
 
 
