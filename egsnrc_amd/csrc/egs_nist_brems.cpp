@@ -1,7 +1,8 @@
-// egs_nist_brems.cpp -- the NIST bremsstrahlung cross sections
-// (init_nist_brems, egsnrc.mortran:6893-7226, ibr_nist = 1): the total
+// egs_nist_brems.cpp -- the NIST or NRC bremsstrahlung cross sections
+// (init_nist_brems, egsnrc.mortran:6893-7226, ibr_nist = 1 or 2): the total
 // brems cross sections of a table blob are replaced by integrals of
-// Seltzer and Berger's scaled cross sections (data/nist_brems.data), and
+// Seltzer and Berger's scaled cross sections (data/nist_brems.data) or of
+// Tessier's (data/nrc_brems.data, the same layout on another grid), and
 // per medium a 100-energy grid of 50-bin alias tables samples the photon
 // energy.  FP64 on the host, at initialisation.
 #include <cmath>
@@ -144,14 +145,16 @@ void alias_table(int n, const double *xs, const double *fs, double *ws, int32_t 
 }
 
 int nist_brems_tables(const std::string &data_dir, int ibr_nist, Blob &out) {
-    if (ibr_nist != 1)
-        throw std::runtime_error("only the NIST brems cross sections (ibr_nist = 1) are built");
-    Records r(data_dir + "nist_brems.data");
-    if (!r.ok()) throw std::runtime_error("cannot open nist_brems.data");
+    if (ibr_nist != 1 && ibr_nist != 2)
+        throw std::runtime_error("ibr_nist: 1 NIST or 2 NRC brems cross sections");
+    // the same layout in both files (egsnrc.mortran:6933-6964)
+    const std::string file = ibr_nist == 1 ? "nist_brems.data" : "nrc_brems.data";
+    Records r(data_dir + file);
+    if (!r.ok()) throw std::runtime_error("cannot open " + file);
     r.skip(1);
     auto hdr = r.ints(2);
     const int nmix = (int)hdr[0], kmix = (int)hdr[1];
-    if (nmix > 57 || kmix > 54) throw std::runtime_error("nist_brems.data: grid too large");
+    if (nmix > 57 || kmix > 54) throw std::runtime_error(file + ": grid too large");
     const std::vector<double> energy = r.reals(nmix);
     const std::vector<double> xk = r.reals(kmix);
     r.skip(1);
@@ -312,6 +315,12 @@ int nist_brems_tables(const std::string &data_dir, int ibr_nist, Blob &out) {
                 F[ndat] = std::exp(csx[j]);
             }
             ndat++;
+            if (ndat > kNbX)
+                // the data points above AP/E alone exceed the 50 bins (the NRC
+                // grid's 54 fractions reach down to 1e-6): the reference then
+                // writes past the row and samples from its first 50 bins only
+                throw std::runtime_error(file + ": more data points above AP/E than the 50 "
+                                         "sampling bins hold (raise AP or lower UE)");
             X[ndat] = 1;
             F[ndat] = std::exp(csx[kmix - 1]);
             // add the midpoint of largest relative linear-interpolation

@@ -49,7 +49,7 @@ struct BuildOptions {
     double ximax = 0.5;       // XIMAX
     int bound_compton = 3;    // IBCMP
     int rayleigh = 1;         // IRAYLR
-    int ibr_nist = 0;         // 0 Bethe-Heitler, 1 NIST brems cross sections
+    int ibr_nist = 0;         // 0 Bethe-Heitler, 1 NIST, 2 NRC brems cross sections
     int pair_nrc = 0;         // 1: add the NRC pair sampling tables
     std::string photon_xsec = "xcom";   // photon data set: <prefix>_photo.data, ...
     std::string eii;          // "" off, else the EII data set (eii_<name>.data)

@@ -306,7 +306,7 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
         s.zbrang = f("zbrang"); s.lzbrang = f("lzbrang"); s.log_ap = f("log_ap");
         s.blcc = f("blcc"); s.xcc = f("xcc"); s.expeke1 = f("expeke1");
         s.esig_e = f("esig_e"); s.psig_e = f("psig_e");
-        const bool nist = b.has("ibr_nist") && b.i1("ibr_nist") == 1;
+        const bool nist = b.has("ibr_nist") && b.i1("ibr_nist") >= 1;
         s.nb_emin = nist ? f("nb_emin") : (R)0;
         s.nb_lemin = nist ? f("nb_lemin") : (R)0;
         s.nb_dlei = nist ? f("nb_dlei") : (R)0;
@@ -400,7 +400,7 @@ int build_tables(egsgpu_handle *h, const Blob &b, DevTables<R> &T) {
     }
     CK(pack_med<R>(h, b, nmed, "dl", 48, &T.dl));
     static_assert(kNbGridE == kNbE && kNbGridX == kNbX, "brems sampling grid");
-    if (b.has("ibr_nist") && b.i1("ibr_nist") == 1) {
+    if (b.has("ibr_nist") && b.i1("ibr_nist") >= 1) {
         CK(pack_med<R>(h, b, nmed, "nb_xdata", kNbGridE * (kNbGridX + 1), &T.nb_x));
         CK(pack_med<R>(h, b, nmed, "nb_fdata", kNbGridE * (kNbGridX + 1), &T.nb_f));
         CK(pack_med<R>(h, b, nmed, "nb_wdata", kNbGridE * kNbGridX, &T.nb_w));
@@ -1079,11 +1079,12 @@ int egsgpu_create(const egsgpu_tables *tab, const EgsXYZGeometry *geom,
         if (tp->brems_xsec != blob_nist) {
             set_error("brems_xsec = %d but the table blob was built with %s brems cross "
                       "sections; build it with brems_xsec = %d",
-                      (int)tp->brems_xsec, blob_nist ? "NIST" : "Bethe-Heitler",
+                      (int)tp->brems_xsec,
+                      blob_nist == 2 ? "NRC" : blob_nist ? "NIST" : "Bethe-Heitler",
                       (int)tp->brems_xsec);
             return EGSGPU_E_ARG;
         }
-        // egs_init_user_photon (egsnrc.mortran:8028-8274): the Compton part
+        // egs_init_user_photon (egsnrc.mortran:7973): the Compton part
         // of the photon cross sections is bound (compton_sigma.data) for
         // IBCMP > 1 and Klein-Nishina otherwise
         const int blob_ibcmp = tab->blob.i1("ibcmp");

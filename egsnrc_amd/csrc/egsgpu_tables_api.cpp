@@ -31,8 +31,8 @@ int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
         if (tp->ximax > 0 && tp->ximax < 1) o.ximax = tp->ximax;
         o.bound_compton = tp->bound_compton;
         o.rayleigh = tp->rayleigh;
-        EGS_ARG(tp->brems_xsec == 0 || tp->brems_xsec == 1,
-                "brems_xsec: 0 Bethe-Heitler or 1 NIST");
+        EGS_ARG(tp->brems_xsec >= 0 && tp->brems_xsec <= 2,
+                "brems_xsec: 0 Bethe-Heitler, 1 NIST or 2 NRC");
         o.ibr_nist = tp->brems_xsec;
         EGS_ARG(tp->pair_xsec == 0 || tp->pair_xsec == 1, "pair_xsec: 0 Bethe-Heitler or 1 NRC");
         o.pair_nrc = tp->pair_xsec;

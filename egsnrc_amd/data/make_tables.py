@@ -26,6 +26,8 @@ BLOBS = {
     # the NIST bremsstrahlung cross sections (ibr_nist = 1) and the NRC pair
     # tables for water and bone
     "521icru_nist.egstab": ("521icru.pegs4dat", ["H2O521ICRU", "ICRPBONE521ICRU"], 1),
+    # the NRC bremsstrahlung cross sections (ibr_nist = 2) for water and bone
+    "521icru_nrcbrems.egstab": ("521icru.pegs4dat", ["H2O521ICRU", "ICRPBONE521ICRU"], 2),
     # water from the EPDL photon data set ("Photon cross sections = epdl")
     "521icru_epdl.egstab": ("521icru.pegs4dat", ["H2O521ICRU"], 0, "epdl"),
     # the tutorial media with electron impact ionisation (the ik data set)
@@ -41,7 +43,7 @@ def build(hen_house, name, pegs, media, out_dir=HERE, estepe=0.25, ximax=0.5, br
                                       C.c_char_p]
     arr = (C.c_char_p * len(media))(*[m.encode() for m in media])
     if pair_xsec is None:
-        pair_xsec = brems_xsec   # the NIST blob carries the NRC pair tables too
+        pair_xsec = 1 if brems_xsec == 1 else 0   # the NIST blob carries the NRC pair tables too
     tp = _lib.default_transport(estepe=estepe, ximax=ximax, brems_xsec=brems_xsec,
                                 pair_xsec=pair_xsec, photon_xsec=photon_xsec.encode(),
                                 eii_xsec=eii.encode())

@@ -553,8 +553,9 @@ class EGS_GPUApplication {
         if (!t->get("Brems cross sections", v)) {
             if (InputItem::same(v, "BH")) tp.brems_xsec = 0;
             else if (InputItem::same(v, "NIST")) tp.brems_xsec = 1;
+            else if (InputItem::same(v, "NRC")) tp.brems_xsec = 2;
             else throw std::runtime_error("Brems cross sections = " + v +
-                                          ": BH or NIST (NRC is not supported)");
+                                          ": BH, NIST or NRC");
         }
         t->get("ESTEPE", tp.estepe);
         t->get("Ximax", tp.ximax);

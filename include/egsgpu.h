@@ -176,7 +176,8 @@ typedef struct EgsTransport {
                                     (msdist_pII, default), 1 PRESTA-I
                                     (msdist_pI, egsnrc.mortran:4310-4515) */
     int32_t brems_xsec;          /* ibr_nist: 0 Bethe-Heitler (default), 1 the
-                                    NIST cross sections of nist_brems.data
+                                    NIST cross sections of nist_brems.data,
+                                    2 the NRC ones of nrc_brems.data
                                     (init_nist_brems, egsnrc.mortran
                                     :6893-7226); egsgpu_tables_build makes
                                     its tables, egsgpu_create requires a
@@ -227,7 +228,8 @@ int egsgpu_tables_load(const char *path, egsgpu_tables **out);
  * the EGSnrc data directory hen_house/data + hen_house/pegs4/pgs4form.dat.
  * tp supplies ESTEPE/XIMAX (mscati, egsnrc.mortran:2750-2981), IBCMP,
  * IRAYLR, the brems cross sections (brems_xsec 1: NIST tables from
- * hen_house/data/nist_brems.data) and pair_xsec (1: add the NRC pair tables
+ * hen_house/data/nist_brems.data, 2: NRC tables from nrc_brems.data) and
+ * pair_xsec (1: add the NRC pair tables
  * of hen_house/data/pair_nrc1.data).  Writes the blob to out_path. */
 int egsgpu_tables_build(const char *hen_house, const char *pegs_file,
                         int32_t nmedia, const char *const *media,

@@ -108,7 +108,7 @@ struct Medium {
 
 struct Tables {
     int nmed;
-    int ibr_nist = 0;   // 1: the blob holds NIST brems cross sections
+    int ibr_nist = 0;   // 1 / 2: the blob holds NIST / NRC brems cross sections
     int ibcmp = 3;      // the bound-Compton option the main photon rows were built for
 
     // a run with bound Compton off (IBCMP 0, 1) uses Klein-Nishina photon
@@ -204,7 +204,7 @@ struct Tables {
             M.tmxs0 = b.F(K("tmxs0")); M.tmxs1 = b.F(K("tmxs1"));
             M.range_ep0 = b.F(K("range_ep0")); M.range_ep1 = b.F(K("range_ep1"));
             M.e_array = b.F(K("e_array"));
-            if (b.has("ibr_nist") && b.I("ibr_nist")[0] == 1) {
+            if (b.has("ibr_nist") && b.I("ibr_nist")[0] >= 1) {
                 M.nb_emin = b.F(K("nb_emin"))[0];
                 M.nb_lemin = b.F(K("nb_lemin"))[0];
                 M.nb_dlei = b.F(K("nb_dlei"))[0];

@@ -45,7 +45,7 @@ struct Params {
     double estepe, ximax, skindepth;
     int kerma;
     int presta1;   // transport_algorithm PRESTA-I (msdist_pI)
-    int ibr_nist;  // 1: NIST brems photon energies
+    int ibr_nist;  // 1 / 2: NIST / NRC brems photon energies
     int pair_nrc;  // 1: NRC pair energy sharing
     int eii;       // 1: electron impact ionisation in MOLLER
 };
@@ -1678,7 +1678,7 @@ struct Shower {
         double z2max = y2max + 1, z2maxi = std::sqrt(z2max);
         double esg, pesg, pese, ese;
         const double(*dl)[8] = M.dl;
-        const bool nist = pb.P.ibr_nist == 1;
+        const bool nist = pb.P.ibr_nist >= 1;   // NIST or NRC tables
         int jj = 0;
         double ajj = 0;
         if (nist) {   // egsnrc.mortran:404-408
