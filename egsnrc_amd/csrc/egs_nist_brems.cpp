@@ -308,6 +308,12 @@ int nist_brems_tables(const std::string &data_dir, int ibr_nist, Blob &out) {
             int k = 0;   // first data point strictly above AP/E (last if none)
             while (k < kmix && !(xk[k] > x0)) k++;
             if (k > kmix - 1) k = kmix - 1;
+            if (kmix - 1 - k > kNbX)
+                // the data points above AP/E alone exceed the 50 bins (the NRC
+                // grid's 54 fractions reach down to 1e-6): the reference then
+                // writes past the row and samples from its first 50 bins only
+                throw std::runtime_error(file + ": more data points above AP/E than the 50 "
+                                         "sampling bins hold (raise AP or lower UE)");
             int ndat = 0;
             for (int j = k + 1; j <= kmix - 2; j++) {
                 ndat++;
@@ -315,12 +321,6 @@ int nist_brems_tables(const std::string &data_dir, int ibr_nist, Blob &out) {
                 F[ndat] = std::exp(csx[j]);
             }
             ndat++;
-            if (ndat > kNbX)
-                // the data points above AP/E alone exceed the 50 bins (the NRC
-                // grid's 54 fractions reach down to 1e-6): the reference then
-                // writes past the row and samples from its first 50 bins only
-                throw std::runtime_error(file + ": more data points above AP/E than the 50 "
-                                         "sampling bins hold (raise AP or lower UE)");
             X[ndat] = 1;
             F[ndat] = std::exp(csx[kmix - 1]);
             // add the midpoint of largest relative linear-interpolation
